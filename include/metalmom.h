@@ -1,0 +1,186 @@
+/*
+ * libmetalmom.so -- MI355X (gfx950) implementation of MetalMom's spectral
+ * front end: STFT -> power -> mel filterbank -> power_to_db -> DCT-II (MFCC).
+ *
+ * Part 1 is a drop-in for the reference C ABI: the same status codes, struct
+ * layouts and prototypes as Sources/MetalMomCBridge/include/metalmom.h of
+ * zakkeown/MetalMom, for the hot-path exports. Part 2 is additive: complex
+ * STFT, device-resident batches (one persistent launch per step, optionally
+ * replayed from a captured hipGraph), an autotuner and a request server.
+ */
+#ifndef METALMOM_H
+#define METALMOM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------
+ * Part 1: drop-in ABI
+ * ---------------------------------------------------------------------- */
+
+/* Status codes -- reference metalmom.h:7-11. MM_ERR_METAL_UNAVAILABLE keeps
+ * its value and means "no usable GPU" here. */
+#define MM_OK 0
+#define MM_ERR_INVALID_INPUT -1
+#define MM_ERR_METAL_UNAVAILABLE -2
+#define MM_ERR_ALLOC_FAILED -3
+#define MM_ERR_INTERNAL -4
+
+/* Output buffer -- reference metalmom.h:14-20. `data` is malloc'd by the
+ * library and released by mm_buffer_free. dtype 0 = float32; dtype 1 =
+ * complex64 stored as interleaved (re, im) floats, `count` complex elements. */
+typedef struct {
+    float* data;
+    int64_t shape[8];
+    int32_t ndim;
+    int32_t dtype;
+    int64_t count;
+} MMBuffer;
+
+/* STFT parameters -- reference metalmom.h:23-28. */
+typedef struct {
+    int32_t n_fft;
+    int32_t hop_length;
+    int32_t win_length;
+    int32_t center;
+} MMSTFTParams;
+
+/* Opaque context -- reference metalmom.h:31. Not thread-safe: one per
+ * thread. Contexts on different threads run concurrently (each has its own
+ * HIP stream and workspace). */
+typedef void* mm_context;
+
+/* Replaces Bridge.swift:22-30 (mm_init). Cheap; the GPU is opened lazily. */
+mm_context mm_init(void);
+/* Replaces Bridge.swift:33-40 (mm_destroy). A second destroy is a no-op. */
+void mm_destroy(mm_context ctx);
+
+/* Replaces Bridge.swift:44-140 (mm_stft): magnitude, row-major
+ * [n_fft/2+1, n_frames], zero centre padding. n_fft must be a power of two. */
+int32_t mm_stft(mm_context ctx, const float* signal_data, int64_t signal_length,
+                int32_t sample_rate, const MMSTFTParams* params, MMBuffer* out);
+
+/* Replaces Bridge.swift:225-263 (mm_amplitude_to_db): 20 log10(max(x, amin)/ref),
+ * clipped to max - top_db when top_db > 0. 1-D output of `count` values. */
+int32_t mm_amplitude_to_db(mm_context ctx, const float* data, int64_t count,
+                           float ref, float amin, float top_db, MMBuffer* out);
+
+/* Replaces Bridge.swift:267-306 (mm_power_to_db): 10 log10(max(x, amin)/ref),
+ * clipped to max - top_db when top_db > 0. 1-D output of `count` values. */
+int32_t mm_power_to_db(mm_context ctx, const float* data, int64_t count,
+                       float ref, float amin, float top_db, MMBuffer* out);
+
+/* Replaces Bridge.swift:310-400 (mm_mel_spectrogram): [n_mels, n_frames].
+ * hop <= 0 -> n_fft/4, win <= 0 -> n_fft, f_max <= 0 -> sr/2. */
+int32_t mm_mel_spectrogram(mm_context ctx, const float* signal_data, int64_t signal_length,
+                           int32_t sample_rate, int32_t n_fft, int32_t hop_length,
+                           int32_t win_length, int32_t center, float power,
+                           int32_t n_mels, float f_min, float f_max,
+                           MMBuffer* out);
+
+/* Replaces Bridge.swift:402-493 (mm_mfcc): [min(n_mfcc, n_mels), n_frames];
+ * power 2 mel, power_to_db(ref 1, amin 1e-10, top_db 80), DCT-II ortho. */
+int32_t mm_mfcc(mm_context ctx, const float* signal_data, int64_t signal_length,
+                int32_t sample_rate, int32_t n_mfcc, int32_t n_fft,
+                int32_t hop_length, int32_t win_length,
+                int32_t n_mels, float f_min, float f_max,
+                int32_t center, MMBuffer* out);
+
+/* Replaces Bridge.swift:3695-3711 (mm_buffer_free). Null-safe; zeroes *buf. */
+void mm_buffer_free(MMBuffer* buf);
+
+/* ------------------------------------------------------------------------
+ * Part 2: additive API (no counterpart in the reference ABI)
+ * ---------------------------------------------------------------------- */
+
+/* Output kinds and precisions. */
+#define MM_KIND_STFT_MAG 0
+#define MM_KIND_STFT_POWER 1
+#define MM_KIND_STFT_COMPLEX 2
+#define MM_KIND_MEL 3
+#define MM_KIND_MFCC 4
+
+#define MM_PREC_F32 0   /* FP32 mel/DCT contraction (FP32 MFMA) */
+#define MM_PREC_BF16 1  /* BF16 MFMA, FP32 accumulation */
+#define MM_PREC_F16 2   /* FP16 MFMA, FP32 accumulation */
+
+#define MM_INPUT_F32 0
+#define MM_INPUT_F16 1
+
+#define MM_PAD_CONSTANT 0 /* zeros: the reference's centre padding */
+#define MM_PAD_REFLECT 1  /* numpy.pad(mode="reflect") */
+
+/* Full request description. Sentinels follow the drop-in calls: hop <= 0 ->
+ * n_fft/4, win <= 0 -> n_fft, f_max <= 0 -> sr/2, top_db <= 0 -> no clamp. */
+typedef struct {
+    int32_t kind;
+    int32_t sample_rate;
+    int32_t n_fft;
+    int32_t hop_length;
+    int32_t win_length;
+    int32_t center;
+    int32_t pad_mode;
+    float power;
+    int32_t n_mels;
+    float f_min;
+    float f_max;
+    int32_t n_mfcc;
+    float top_db;
+    int32_t precision;
+    int32_t input_dtype;
+} MMSpecParams;
+
+/* Complex64 STFT of one clip, [n_fft/2+1, n_frames], dtype 1 (see MMBuffer).
+ * Same framing as mm_stft (STFT.swift:512-668 computeComplex, not exported by
+ * the reference). */
+int32_t mm_stft_complex(mm_context ctx, const float* signal_data, int64_t signal_length,
+                        int32_t sample_rate, const MMSTFTParams* params, MMBuffer* out);
+
+/* One clip with every option of MMSpecParams (input must be float32 here). */
+int32_t mm_spectral(mm_context ctx, const MMSpecParams* params, const float* signal_data,
+                    int64_t signal_length, MMBuffer* out);
+
+/* Host-buffer batch: clip i -> outs[i] (each released with mm_buffer_free).
+ * All clips run in one persistent launch. */
+int32_t mm_spectral_batch(mm_context ctx, const MMSpecParams* params, const float* const* clips,
+                          const int64_t* lengths, int32_t n_clips, MMBuffer* outs);
+
+/* Device-resident batch plan. Clips are packed back to back in one device
+ * signal buffer (sample offsets from mm_batch_clip); clip i's output is the
+ * row-major [rows, frames] block at its output offset (in floats). */
+typedef void* mm_batch;
+int32_t mm_batch_create(mm_context ctx, const MMSpecParams* params, const int64_t* lengths,
+                        int32_t n_clips, mm_batch* out);
+int32_t mm_batch_info(mm_batch b, int64_t* total_input_samples, int64_t* total_output_floats,
+                      int64_t* total_frames, int32_t* n_tiles);
+int32_t mm_batch_clip(mm_batch b, int32_t clip, int64_t* input_offset, int64_t* output_offset,
+                      int64_t* rows, int64_t* frames);
+/* Enqueue one step on `hip_stream` (NULL: the context's stream). Inputs and
+ * outputs are device pointers; nothing is allocated. */
+int32_t mm_batch_run(mm_batch b, const void* d_signal, float* d_out, void* hip_stream);
+/* Same, replayed from a hipGraph captured on first use for this
+ * (d_signal, d_out) pair. */
+int32_t mm_batch_run_graph(mm_batch b, const void* d_signal, float* d_out, void* hip_stream);
+/* Persistent grid: workgroups per CU (0: autotuned or default). */
+int32_t mm_batch_set_grid(mm_batch b, int32_t wg_per_cu);
+/* Times each candidate grid on this batch, keeps the fastest and records it
+ * in the tuning file ($METALMOM_AMD_TUNE_FILE or ~/.cache/metalmom_amd_tuning.json). */
+int32_t mm_batch_autotune(mm_batch b, const void* d_signal, float* d_out, int32_t iters,
+                          int32_t* best_wg_per_cu, float* best_ms);
+/* Average device time of the fused phase-1 kernel and of the whole step over
+ * `iters` runs, from HIP events on the stream the kernels run on. */
+int32_t mm_batch_time(mm_batch b, const void* d_signal, float* d_out, void* hip_stream, int32_t iters,
+                      float* kernel_ms, float* step_ms);
+int32_t mm_batch_destroy(mm_batch b);
+
+/* Device description: name, gfx arch and compute-unit count. */
+int32_t mm_device_info(char* name, int32_t name_len, char* arch, int32_t arch_len, int32_t* n_cu);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* METALMOM_H */

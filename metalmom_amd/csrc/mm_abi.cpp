@@ -1,0 +1,284 @@
+// C ABI of libmetalmom.so (declared in include/metalmom.h).
+//
+// Validation, sentinels, ownership and empty-result conventions follow the
+// reference bridge, Sources/MetalMomBridge/Bridge.swift:
+//   mm_init/mm_destroy registry      :16-40
+//   mm_stft                          :44-140
+//   mm_amplitude_to_db               :225-263
+//   mm_power_to_db                   :267-306
+//   mm_mel_spectrogram               :310-400
+//   mm_mfcc                          :402-493
+//   mm_buffer_free                   :3695-3711
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <unordered_set>
+#include <vector>
+
+#include "../../include/metalmom.h"
+#include "mm_engine.h"
+
+using namespace mm;
+
+namespace {
+
+struct Ctx {
+  Lane* lane = nullptr;
+  Lane* get() {
+    if (!lane) {
+      Engine* e = Engine::get();
+      if (!e) return nullptr;
+      lane = e->acquire_lane();
+    }
+    return lane;
+  }
+};
+
+std::mutex g_reg_mu;
+std::unordered_set<void*>& registry() {
+  static std::unordered_set<void*> r;
+  return r;
+}
+
+bool is_live(void* p) {
+  std::lock_guard<std::mutex> g(g_reg_mu);
+  return registry().count(p) != 0;
+}
+
+bool pow2(int64_t n) { return n > 0 && (n & (n - 1)) == 0; }
+
+void fill_shape(MMBuffer* out, const std::vector<int64_t>& shape, int dtype, int64_t count, float* data) {
+  out->data = data;
+  for (int i = 0; i < 8; ++i) out->shape[i] = 0;
+  for (size_t i = 0; i < shape.size() && i < 8; ++i) out->shape[i] = shape[i];
+  out->ndim = (int32_t)shape.size();
+  out->dtype = dtype;
+  out->count = count;
+}
+
+// Runs one clip and fills `out` ([rows, frames]; empty results keep the shape
+// with data = NULL and count = 0, Bridge.swift:95-113).
+int32_t run_to_buffer(void* ctx, const SpecRequest& r, const float* sig, int64_t len, MMBuffer* out) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  const int64_t rows = out_rows(r);
+  const int64_t T = frame_count(len, r.n_fft, r.hop, r.center);
+  const int dtype = (r.out_kind == OUT_STFT_COMPLEX) ? 1 : 0;
+  if (rows * T == 0) {
+    fill_shape(out, {rows, T}, dtype, 0, nullptr);
+    return OK;
+  }
+  Lane* lane = c->get();
+  if (!lane) return ERR_NO_GPU;
+  float* data = nullptr;
+  int64_t rr = 0, cc = 0;
+  const int32_t s = run_single(lane, r, sig, len, &data, &rr, &cc);
+  if (s != OK) return s;
+  fill_shape(out, {rr, cc}, dtype, rr * cc, data);
+  return OK;
+}
+
+int32_t db_to_buffer(void* ctx, const float* data, int64_t count, float mult, float ref, float amin, float top_db,
+                     MMBuffer* out) {
+  Lane* lane = nullptr;
+  if (ctx && is_live(ctx)) lane = static_cast<Ctx*>(ctx)->get();
+  Engine* e = Engine::get();
+  if (!e) return ERR_NO_GPU;
+  bool borrowed = false;
+  if (!lane) {  // the reference does not require a context here (Bridge.swift:277-279)
+    lane = e->acquire_lane();
+    borrowed = true;
+    if (!lane) return ERR_INTERNAL;
+  }
+  float* res = nullptr;
+  const int32_t s = run_db(lane, data, count, mult, ref, amin, top_db, &res);
+  if (borrowed) e->release_lane(lane);
+  if (s != OK) return s;
+  fill_shape(out, {count}, 0, count, res);
+  return OK;
+}
+
+// Spec params -> request, applying the drop-in sentinels.
+int32_t to_request(const MMSpecParams* p, SpecRequest* r) {
+  if (!p) return ERR_INVALID;
+  if (p->kind < MM_KIND_STFT_MAG || p->kind > MM_KIND_MFCC) return ERR_INVALID;
+  if (!pow2(p->n_fft)) return ERR_INVALID;
+  r->out_kind = p->kind;
+  r->sr = p->sample_rate;
+  r->n_fft = p->n_fft;
+  r->hop = p->hop_length > 0 ? p->hop_length : p->n_fft / 4;
+  r->win = p->win_length > 0 ? p->win_length : p->n_fft;
+  r->center = p->center != 0;
+  r->pad_mode = p->pad_mode;
+  r->power = p->power;
+  r->n_mels = p->n_mels;
+  r->fmin = p->f_min;
+  r->fmax = p->f_max > 0.0f ? p->f_max : (float)p->sample_rate / 2.0f;
+  r->n_mfcc = p->n_mfcc;
+  r->top_db = p->top_db;
+  r->precision = p->precision;
+  r->in_dtype = p->input_dtype;
+  if (r->hop <= 0) return ERR_INVALID;
+  if (r->pad_mode != PAD_CONSTANT && r->pad_mode != PAD_REFLECT) return ERR_INVALID;
+  if (r->precision < PREC_F32 || r->precision > PREC_F16) return ERR_INVALID;
+  if (r->in_dtype != IN_F32 && r->in_dtype != IN_F16) return ERR_INVALID;
+  if ((r->out_kind == MM_KIND_MEL || r->out_kind == MM_KIND_MFCC) && r->n_mels < 0) return ERR_INVALID;
+  if (r->out_kind == MM_KIND_MFCC && r->n_mfcc < 0) return ERR_INVALID;
+  if (r->out_kind <= MM_KIND_STFT_COMPLEX && r->precision != PREC_F32) return ERR_INVALID;
+  return OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+mm_context mm_init(void) {
+  Ctx* c = new Ctx();
+  std::lock_guard<std::mutex> g(g_reg_mu);
+  registry().insert(c);
+  return c;
+}
+
+void mm_destroy(mm_context ctx) {
+  if (!ctx) return;
+  {
+    std::lock_guard<std::mutex> g(g_reg_mu);
+    if (registry().erase(ctx) == 0) return;  // already destroyed: no-op
+  }
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (c->lane) {
+    (void)hipStreamSynchronize(c->lane->stream);
+    Engine::get()->release_lane(c->lane);
+  }
+  delete c;
+}
+
+int32_t mm_stft(mm_context ctx, const float* signal_data, int64_t signal_length, int32_t sample_rate,
+                const MMSTFTParams* params, MMBuffer* out) {
+  if (!ctx || !signal_data || signal_length <= 0 || !params || !out) return MM_ERR_INVALID_INPUT;
+  if (!pow2(params->n_fft)) return MM_ERR_INVALID_INPUT;
+  if (params->hop_length <= 0) return MM_ERR_INVALID_INPUT;  // the reference divides by it
+  SpecRequest r;
+  r.out_kind = OUT_STFT_MAG;
+  r.sr = sample_rate;
+  r.n_fft = params->n_fft;
+  r.hop = params->hop_length;
+  r.win = params->win_length;  // taken verbatim; <= 0 gives an all-zero window
+  r.center = params->center != 0;
+  return run_to_buffer(ctx, r, signal_data, signal_length, out);
+}
+
+int32_t mm_stft_complex(mm_context ctx, const float* signal_data, int64_t signal_length, int32_t sample_rate,
+                        const MMSTFTParams* params, MMBuffer* out) {
+  if (!ctx || !signal_data || signal_length <= 0 || !params || !out) return MM_ERR_INVALID_INPUT;
+  if (!pow2(params->n_fft) || params->hop_length <= 0) return MM_ERR_INVALID_INPUT;
+  SpecRequest r;
+  r.out_kind = OUT_STFT_COMPLEX;
+  r.sr = sample_rate;
+  r.n_fft = params->n_fft;
+  r.hop = params->hop_length;
+  r.win = params->win_length;
+  r.center = params->center != 0;
+  return run_to_buffer(ctx, r, signal_data, signal_length, out);
+}
+
+int32_t mm_amplitude_to_db(mm_context ctx, const float* data, int64_t count, float ref, float amin, float top_db,
+                           MMBuffer* out) {
+  if (!data || count <= 0 || !out) return MM_ERR_INVALID_INPUT;
+  return db_to_buffer(ctx, data, count, 20.0f, ref, amin, top_db, out);
+}
+
+int32_t mm_power_to_db(mm_context ctx, const float* data, int64_t count, float ref, float amin, float top_db,
+                       MMBuffer* out) {
+  if (!data || count <= 0 || !out) return MM_ERR_INVALID_INPUT;
+  return db_to_buffer(ctx, data, count, 10.0f, ref, amin, top_db, out);
+}
+
+int32_t mm_mel_spectrogram(mm_context ctx, const float* signal_data, int64_t signal_length, int32_t sample_rate,
+                           int32_t n_fft, int32_t hop_length, int32_t win_length, int32_t center, float power,
+                           int32_t n_mels, float f_min, float f_max, MMBuffer* out) {
+  if (!ctx || !signal_data || signal_length <= 0 || !out) return MM_ERR_INVALID_INPUT;
+  if (n_fft <= 0 || n_mels < 0) return MM_ERR_INVALID_INPUT;
+  SpecRequest r;
+  r.out_kind = OUT_MEL;
+  r.sr = sample_rate;
+  r.n_fft = n_fft;
+  r.hop = hop_length > 0 ? hop_length : n_fft / 4;
+  r.win = win_length > 0 ? win_length : n_fft;
+  r.center = center != 0;
+  r.power = power;
+  r.n_mels = n_mels;
+  r.fmin = f_min;
+  r.fmax = f_max > 0.0f ? f_max : (float)sample_rate / 2.0f;
+  if (r.hop <= 0) return MM_ERR_INVALID_INPUT;
+  if (!pow2(n_fft)) {  // the reference core returns an empty [n_mels, 0] result
+    fill_shape(out, {n_mels, 0}, 0, 0, nullptr);
+    return MM_OK;
+  }
+  return run_to_buffer(ctx, r, signal_data, signal_length, out);
+}
+
+int32_t mm_mfcc(mm_context ctx, const float* signal_data, int64_t signal_length, int32_t sample_rate, int32_t n_mfcc,
+                int32_t n_fft, int32_t hop_length, int32_t win_length, int32_t n_mels, float f_min, float f_max,
+                int32_t center, MMBuffer* out) {
+  if (!ctx || !signal_data || signal_length <= 0 || !out) return MM_ERR_INVALID_INPUT;
+  if (n_fft <= 0 || n_mels < 0 || n_mfcc < 0) return MM_ERR_INVALID_INPUT;
+  SpecRequest r;
+  r.out_kind = OUT_MFCC;
+  r.sr = sample_rate;
+  r.n_fft = n_fft;
+  r.hop = hop_length > 0 ? hop_length : n_fft / 4;
+  r.win = win_length > 0 ? win_length : n_fft;
+  r.center = center != 0;
+  r.power = 2.0f;
+  r.n_mels = n_mels;
+  r.n_mfcc = n_mfcc;
+  r.fmin = f_min;
+  r.fmax = f_max > 0.0f ? f_max : (float)sample_rate / 2.0f;
+  r.ref = 1.0f;
+  r.amin = 1e-10f;
+  r.top_db = 80.0f;
+  if (r.hop <= 0) return MM_ERR_INVALID_INPUT;
+  if (!pow2(n_fft)) {
+    fill_shape(out, {std::min(n_mfcc, n_mels), 0}, 0, 0, nullptr);
+    return MM_OK;
+  }
+  return run_to_buffer(ctx, r, signal_data, signal_length, out);
+}
+
+void mm_buffer_free(MMBuffer* buf) {
+  if (!buf) return;
+  if (buf->data) std::free(buf->data);
+  buf->data = nullptr;
+  buf->ndim = 0;
+  buf->dtype = 0;
+  buf->count = 0;
+  for (int i = 0; i < 8; ++i) buf->shape[i] = 0;
+}
+
+int32_t mm_spectral(mm_context ctx, const MMSpecParams* params, const float* signal_data, int64_t signal_length,
+                    MMBuffer* out) {
+  if (!ctx || !signal_data || signal_length <= 0 || !out) return MM_ERR_INVALID_INPUT;
+  SpecRequest r;
+  int32_t s = to_request(params, &r);
+  if (s != OK) return s;
+  if (r.in_dtype != IN_F32) return MM_ERR_INVALID_INPUT;
+  return run_to_buffer(ctx, r, signal_data, signal_length, out);
+}
+
+int32_t mm_device_info(char* name, int32_t name_len, char* arch, int32_t arch_len, int32_t* n_cu) {
+  Engine* e = Engine::get();
+  if (!e) return MM_ERR_METAL_UNAVAILABLE;
+  if (name && name_len > 0) {
+    std::strncpy(name, e->name().c_str(), (size_t)name_len - 1);
+    name[name_len - 1] = 0;
+  }
+  if (arch && arch_len > 0) {
+    std::strncpy(arch, e->arch().c_str(), (size_t)arch_len - 1);
+    arch[arch_len - 1] = 0;
+  }
+  if (n_cu) *n_cu = e->num_cus();
+  return MM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,328 @@
+// Batched execution API (additive exports of include/metalmom.h): device-
+// resident batch plans with one persistent launch per step, hipGraph replay,
+// the grid autotuner and event timing, plus the host-buffer batch call.
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "../../include/metalmom.h"
+#include "mm_engine.h"
+
+using namespace mm;
+
+namespace {
+
+struct GraphEntry {
+  const void* in;
+  float* out;
+  hipStream_t st;
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+};
+
+struct Batch {
+  SpecRequest r;
+  BatchMeta m;
+  const Tables* tb = nullptr;
+  Lane* lane = nullptr;  // own stream, metadata and scratch
+  MetaPtrs mp{};
+  int wg_per_cu = 0;
+  std::vector<GraphEntry> graphs;
+  ~Batch() {
+    for (auto& g : graphs) {
+      (void)hipGraphExecDestroy(g.exec);
+      (void)hipGraphDestroy(g.graph);
+    }
+    if (lane) {
+      (void)hipStreamSynchronize(lane->stream);
+      Engine::get()->release_lane(lane);
+    }
+  }
+  int grid() const {
+    if (wg_per_cu <= 0) return 0;
+    return Engine::get()->num_cus() * wg_per_cu;
+  }
+};
+
+int32_t to_request(const MMSpecParams* p, SpecRequest* r) {
+  if (!p) return ERR_INVALID;
+  const int n_fft = p->n_fft;
+  if (n_fft <= 0 || (n_fft & (n_fft - 1))) return ERR_INVALID;
+  if (p->kind < MM_KIND_STFT_MAG || p->kind > MM_KIND_MFCC) return ERR_INVALID;
+  r->out_kind = p->kind;
+  r->sr = p->sample_rate;
+  r->n_fft = n_fft;
+  r->hop = p->hop_length > 0 ? p->hop_length : n_fft / 4;
+  r->win = p->win_length > 0 ? p->win_length : n_fft;
+  r->center = p->center != 0;
+  r->pad_mode = p->pad_mode;
+  r->power = p->power;
+  r->n_mels = p->n_mels;
+  r->fmin = p->f_min;
+  r->fmax = p->f_max > 0.0f ? p->f_max : (float)p->sample_rate / 2.0f;
+  r->n_mfcc = p->n_mfcc;
+  r->top_db = p->top_db;
+  r->precision = p->precision;
+  r->in_dtype = p->input_dtype;
+  if (r->hop <= 0) return ERR_INVALID;
+  if (r->pad_mode != PAD_CONSTANT && r->pad_mode != PAD_REFLECT) return ERR_INVALID;
+  if (r->precision < PREC_F32 || r->precision > PREC_F16) return ERR_INVALID;
+  if (r->in_dtype != IN_F32 && r->in_dtype != IN_F16) return ERR_INVALID;
+  if (r->out_kind >= OUT_MEL && r->n_mels < 0) return ERR_INVALID;
+  if (r->out_kind == OUT_MFCC && r->n_mfcc < 0) return ERR_INVALID;
+  if (r->out_kind <= OUT_STFT_COMPLEX && r->precision != PREC_F32) return ERR_INVALID;
+  if (r->precision != PREC_F32 || r->in_dtype != IN_F32) {
+    // 16-bit paths exist only in the fused kernel
+    TileLayout L;
+    if (!tile_layout(r->n_fft, r->out_kind, r->precision, r->n_mels, std::min(r->n_mfcc, r->n_mels), &L))
+      return ERR_INVALID;
+  }
+  return OK;
+}
+
+int32_t create_batch(const MMSpecParams* params, const int64_t* lengths, int32_t n_clips, Batch** out) {
+  if (!lengths || n_clips <= 0 || !out) return ERR_INVALID;
+  for (int i = 0; i < n_clips; ++i)
+    if (lengths[i] < 0) return ERR_INVALID;
+  auto b = std::make_unique<Batch>();
+  int32_t s = to_request(params, &b->r);
+  if (s != OK) return s;
+  Engine* e = Engine::get();
+  if (!e) return ERR_NO_GPU;
+  b->lane = e->acquire_lane();
+  if (!b->lane) return ERR_INTERNAL;
+  s = e->tables_for(b->r, b->lane->stream, &b->tb);
+  if (s != OK) return s;
+  TileLayout L;
+  const bool fused =
+      tile_layout(b->r.n_fft, b->r.out_kind, b->r.precision, b->r.n_mels, std::min(b->r.n_mfcc, b->r.n_mels), &L);
+  build_meta(b->r, lengths, n_clips, fused ? L.tile_frames : kTileFrames, &b->m);
+  const size_t mb = meta_bytes(b->m);
+  if (b->lane->d_meta.ensure(mb) != hipSuccess || b->lane->h_meta.ensure(mb) != hipSuccess) return ERR_ALLOC;
+  pack_meta(b->m, b->lane->h_meta.p);
+  if (hipMemcpyAsync(b->lane->d_meta.p, b->lane->h_meta.p, mb, hipMemcpyHostToDevice, b->lane->stream) !=
+          hipSuccess ||
+      hipStreamSynchronize(b->lane->stream) != hipSuccess)
+    return ERR_INTERNAL;
+  b->mp = meta_ptrs(b->m, b->lane->d_meta.p);
+  s = prepare_batch(b->r);
+  if (s != OK) return s;
+  *out = b.release();
+  return OK;
+}
+
+int32_t run_batch(Batch* b, const void* d_signal, float* d_out, hipStream_t st, hipEvent_t e0 = nullptr,
+                  hipEvent_t e1 = nullptr) {
+  return enqueue_batch(Engine::get(), b->r, b->tb, b->m, b->mp, d_signal, d_out, &b->lane->d_scratch,
+                       &b->lane->d_scratch2, b->grid(), st, e0, e1);
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t mm_batch_create(mm_context ctx, const MMSpecParams* params, const int64_t* lengths, int32_t n_clips,
+                        mm_batch* out) {
+  (void)ctx;  // plans own their stream and workspace
+  if (!out) return MM_ERR_INVALID_INPUT;
+  Batch* b = nullptr;
+  const int32_t s = create_batch(params, lengths, n_clips, &b);
+  if (s != OK) return s;
+  *out = b;
+  return MM_OK;
+}
+
+int32_t mm_batch_info(mm_batch bh, int64_t* total_input_samples, int64_t* total_output_floats,
+                      int64_t* total_frames, int32_t* n_tiles) {
+  Batch* b = static_cast<Batch*>(bh);
+  if (!b) return MM_ERR_INVALID_INPUT;
+  if (total_input_samples) *total_input_samples = b->m.total_in;
+  if (total_output_floats) *total_output_floats = b->m.total_out;
+  if (total_frames) {
+    int64_t t = 0;
+    for (int32_t f : b->m.clip_frames) t += f;
+    *total_frames = t;
+  }
+  if (n_tiles) *n_tiles = b->m.n_tiles;
+  return MM_OK;
+}
+
+int32_t mm_batch_clip(mm_batch bh, int32_t clip, int64_t* input_offset, int64_t* output_offset, int64_t* rows,
+                      int64_t* frames) {
+  Batch* b = static_cast<Batch*>(bh);
+  if (!b || clip < 0 || clip >= b->m.n_clips) return MM_ERR_INVALID_INPUT;
+  if (input_offset) *input_offset = b->m.clip_off[clip];
+  if (output_offset) *output_offset = b->m.out_off[clip] * out_elem_floats(b->r);
+  if (rows) *rows = out_rows(b->r);
+  if (frames) *frames = b->m.clip_frames[clip];
+  return MM_OK;
+}
+
+int32_t mm_batch_run(mm_batch bh, const void* d_signal, float* d_out, void* hip_stream) {
+  Batch* b = static_cast<Batch*>(bh);
+  if (!b || (b->m.total_in > 0 && !d_signal) || (b->m.total_out > 0 && !d_out)) return MM_ERR_INVALID_INPUT;
+  hipStream_t st = hip_stream ? static_cast<hipStream_t>(hip_stream) : b->lane->stream;
+  return run_batch(b, d_signal, d_out, st);
+}
+
+int32_t mm_batch_run_graph(mm_batch bh, const void* d_signal, float* d_out, void* hip_stream) {
+  Batch* b = static_cast<Batch*>(bh);
+  if (!b || (b->m.total_in > 0 && !d_signal) || (b->m.total_out > 0 && !d_out)) return MM_ERR_INVALID_INPUT;
+  hipStream_t st = hip_stream ? static_cast<hipStream_t>(hip_stream) : b->lane->stream;
+  for (auto& g : b->graphs)
+    if (g.in == d_signal && g.out == d_out && g.st == st)
+      return hipGraphLaunch(g.exec, st) == hipSuccess ? MM_OK : MM_ERR_INTERNAL;
+  // the generic path allocates scratch on the fly: size it before capture
+  GraphEntry g{d_signal, d_out, st, nullptr, nullptr};
+  if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return MM_ERR_INTERNAL;
+  const int32_t s = run_batch(b, d_signal, d_out, st);
+  hipGraph_t graph = nullptr;
+  const hipError_t ec = hipStreamEndCapture(st, &graph);
+  if (s != OK) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return s;
+  }
+  if (ec != hipSuccess || !graph) return MM_ERR_INTERNAL;
+  hipGraphExec_t exec = nullptr;
+  if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+    (void)hipGraphDestroy(graph);
+    return MM_ERR_INTERNAL;
+  }
+  g.graph = graph;
+  g.exec = exec;
+  b->graphs.push_back(g);
+  return hipGraphLaunch(exec, st) == hipSuccess ? MM_OK : MM_ERR_INTERNAL;
+}
+
+int32_t mm_batch_set_grid(mm_batch bh, int32_t wg_per_cu) {
+  Batch* b = static_cast<Batch*>(bh);
+  if (!b || wg_per_cu < 0 || wg_per_cu > 64) return MM_ERR_INVALID_INPUT;
+  b->wg_per_cu = wg_per_cu;
+  for (auto& g : b->graphs) {  // captured launches carry the old grid
+    (void)hipGraphExecDestroy(g.exec);
+    (void)hipGraphDestroy(g.graph);
+  }
+  b->graphs.clear();
+  return MM_OK;
+}
+
+int32_t mm_batch_time(mm_batch bh, const void* d_signal, float* d_out, void* hip_stream, int32_t iters,
+                      float* kernel_ms, float* step_ms) {
+  Batch* b = static_cast<Batch*>(bh);
+  if (!b || iters <= 0) return MM_ERR_INVALID_INPUT;
+  hipStream_t st = hip_stream ? static_cast<hipStream_t>(hip_stream) : b->lane->stream;
+  std::vector<hipEvent_t> ev(4 * iters, nullptr);
+  for (auto& e : ev)
+    if (hipEventCreate(&e) != hipSuccess) return MM_ERR_INTERNAL;
+  int32_t s = OK;
+  for (int i = 0; i < iters && s == OK; ++i) {
+    if (hipEventRecord(ev[4 * i], st) != hipSuccess) s = ERR_INTERNAL;
+    if (s == OK) s = run_batch(b, d_signal, d_out, st, ev[4 * i + 1], ev[4 * i + 2]);
+    if (s == OK && hipEventRecord(ev[4 * i + 3], st) != hipSuccess) s = ERR_INTERNAL;
+  }
+  if (s == OK && hipStreamSynchronize(st) != hipSuccess) s = ERR_INTERNAL;
+  double k = 0.0, t = 0.0;
+  for (int i = 0; i < iters && s == OK; ++i) {
+    float a = 0.f, c = 0.f;
+    if (hipEventElapsedTime(&a, ev[4 * i + 1], ev[4 * i + 2]) != hipSuccess ||
+        hipEventElapsedTime(&c, ev[4 * i], ev[4 * i + 3]) != hipSuccess)
+      s = ERR_INTERNAL;
+    k += a;
+    t += c;
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  if (s != OK) return s;
+  if (kernel_ms) *kernel_ms = (float)(k / iters);
+  if (step_ms) *step_ms = (float)(t / iters);
+  return MM_OK;
+}
+
+int32_t mm_batch_autotune(mm_batch bh, const void* d_signal, float* d_out, int32_t iters, int32_t* best_wg_per_cu,
+                          float* best_ms) {
+  Batch* b = static_cast<Batch*>(bh);
+  if (!b || iters <= 0) return MM_ERR_INVALID_INPUT;
+  const int cands[] = {1, 2, 3, 4, 6, 8};
+  const int saved = b->wg_per_cu;
+  int best = 0;
+  float best_t = 1e30f;
+  for (int c : cands) {
+    b->wg_per_cu = c;
+    float k = 0.f, t = 0.f;
+    int32_t s = mm_batch_time(b, d_signal, d_out, nullptr, 1, &k, &t);  // warm-up
+    if (s == OK) s = mm_batch_time(b, d_signal, d_out, nullptr, iters, &k, &t);
+    if (s != OK) {
+      b->wg_per_cu = saved;
+      return s;
+    }
+    if (t < best_t) {
+      best_t = t;
+      best = c;
+    }
+  }
+  mm_batch_set_grid(b, best);
+  Engine::get()->record_tuning(b->r, best);
+  if (best_wg_per_cu) *best_wg_per_cu = best;
+  if (best_ms) *best_ms = best_t;
+  return MM_OK;
+}
+
+int32_t mm_batch_destroy(mm_batch bh) {
+  Batch* b = static_cast<Batch*>(bh);
+  if (!b) return MM_ERR_INVALID_INPUT;
+  delete b;
+  return MM_OK;
+}
+
+int32_t mm_spectral_batch(mm_context ctx, const MMSpecParams* params, const float* const* clips,
+                          const int64_t* lengths, int32_t n_clips, MMBuffer* outs) {
+  if (!ctx || !params || !clips || !lengths || n_clips <= 0 || !outs) return MM_ERR_INVALID_INPUT;
+  if (params->input_dtype != MM_INPUT_F32) return MM_ERR_INVALID_INPUT;
+  for (int i = 0; i < n_clips; ++i)
+    if (!clips[i] || lengths[i] <= 0) return MM_ERR_INVALID_INPUT;
+  Batch* b = nullptr;
+  int32_t s = create_batch(params, lengths, n_clips, &b);
+  if (s != OK) return s;
+  std::unique_ptr<Batch> hold(b);
+  Lane* L = b->lane;
+  const size_t in_bytes = (size_t)b->m.total_in * 4, out_bytes = (size_t)b->m.total_out * 4;
+  if (L->h_in.ensure(in_bytes) != hipSuccess || L->d_in.ensure(in_bytes) != hipSuccess ||
+      L->d_out.ensure(out_bytes) != hipSuccess || L->h_out.ensure(out_bytes) != hipSuccess)
+    return MM_ERR_ALLOC_FAILED;
+  for (int i = 0; i < n_clips; ++i)
+    std::memcpy(static_cast<float*>(L->h_in.p) + b->m.clip_off[i], clips[i], (size_t)lengths[i] * 4);
+  if (hipMemcpyAsync(L->d_in.p, L->h_in.p, in_bytes, hipMemcpyHostToDevice, L->stream) != hipSuccess)
+    return MM_ERR_INTERNAL;
+  s = run_batch(b, L->d_in.p, static_cast<float*>(L->d_out.p), L->stream);
+  if (s != OK) {
+    (void)hipStreamSynchronize(L->stream);
+    return s;
+  }
+  if (hipMemcpyAsync(L->h_out.p, L->d_out.p, out_bytes, hipMemcpyDeviceToHost, L->stream) != hipSuccess ||
+      hipStreamSynchronize(L->stream) != hipSuccess)
+    return MM_ERR_INTERNAL;
+  const int ef = out_elem_floats(b->r);
+  const int64_t rows = out_rows(b->r);
+  for (int i = 0; i < n_clips; ++i) {
+    const int64_t T = b->m.clip_frames[i];
+    const int64_t cnt = rows * T;
+    MMBuffer* o = &outs[i];
+    std::memset(o, 0, sizeof(*o));
+    o->ndim = 2;
+    o->shape[0] = rows;
+    o->shape[1] = T;
+    o->dtype = (ef == 2) ? 1 : 0;
+    o->count = cnt;
+    if (cnt > 0) {
+      o->data = static_cast<float*>(std::malloc((size_t)cnt * ef * 4));
+      if (!o->data) {
+        for (int j = 0; j <= i; ++j) mm_buffer_free(&outs[j]);
+        return MM_ERR_ALLOC_FAILED;
+      }
+      std::memcpy(o->data, static_cast<float*>(L->h_out.p) + b->m.out_off[i] * ef, (size_t)cnt * ef * 4);
+    }
+  }
+  return MM_OK;
+}
+
+}  // extern "C"

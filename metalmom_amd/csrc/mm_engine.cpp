@@ -1,0 +1,615 @@
+// Host engine of libmetalmom.so: lazy device singleton, device-resident table
+// cache, pooled stream/workspace lanes, batch metadata, fused-kernel and
+// generic-kernel execution, and the autotuner's persisted choices.
+#include "mm_engine.h"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+
+#include <sys/stat.h>
+
+namespace mm {
+
+#define MM_HIP(expr)                                 \
+  do {                                               \
+    hipError_t e_ = (expr);                          \
+    if (e_ != hipSuccess) return hip_status(e_);     \
+  } while (0)
+
+static int32_t hip_status(hipError_t e) {
+  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return ERR_ALLOC;
+  if (e == hipErrorNoDevice || e == hipErrorInvalidDevice) return ERR_NO_GPU;
+  return ERR_INTERNAL;
+}
+
+// ---------------------------------------------------------------------------
+// shapes
+// ---------------------------------------------------------------------------
+
+// 1 + (padded - n_fft) / hop, or 0 when the padded signal is shorter than one
+// frame (STFT.swift:240-254).
+int64_t frame_count(int64_t n, int n_fft, int hop, int center) {
+  const int64_t padded = n + (center ? 2 * (int64_t)(n_fft / 2) : 0);
+  if (padded < n_fft || hop <= 0) return 0;
+  return 1 + (padded - n_fft) / hop;
+}
+
+int64_t out_rows(const SpecRequest& r) {
+  switch (r.out_kind) {
+    case OUT_MEL: return r.n_mels;
+    case OUT_MFCC: return std::min(r.n_mfcc, r.n_mels);
+    default: return r.n_fft / 2 + 1;
+  }
+}
+
+int out_elem_floats(const SpecRequest& r) { return r.out_kind == OUT_STFT_COMPLEX ? 2 : 1; }
+
+// ---------------------------------------------------------------------------
+// buffers
+// ---------------------------------------------------------------------------
+
+hipError_t DevBuf::ensure(size_t bytes) {
+  if (bytes <= n && p) return hipSuccess;
+  release();
+  size_t want = std::max<size_t>(bytes, 256);
+  want = (want + (1 << 20) - 1) & ~((size_t)(1 << 20) - 1);
+  hipError_t e = hipMalloc(&p, want);
+  if (e != hipSuccess) {
+    p = nullptr;
+    n = 0;
+    return e;
+  }
+  n = want;
+  return hipSuccess;
+}
+void DevBuf::release() {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  n = 0;
+}
+hipError_t HostBuf::ensure(size_t bytes) {
+  if (bytes <= n && p) return hipSuccess;
+  release();
+  size_t want = std::max<size_t>(bytes, 4096);
+  want = (want + (1 << 20) - 1) & ~((size_t)(1 << 20) - 1);
+  hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    p = nullptr;
+    n = 0;
+    return e;
+  }
+  n = want;
+  return hipSuccess;
+}
+void HostBuf::release() {
+  if (p) (void)hipHostFree(p);
+  p = nullptr;
+  n = 0;
+}
+Lane::~Lane() {
+  d_in.release();
+  d_out.release();
+  d_meta.release();
+  d_scratch.release();
+  d_scratch2.release();
+  h_in.release();
+  h_out.release();
+  h_meta.release();
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+// ---------------------------------------------------------------------------
+// tables
+// ---------------------------------------------------------------------------
+
+struct Tables {
+  TableKey key;
+  DeviceTables t;
+  void* mem = nullptr;
+  ~Tables() {
+    if (mem) (void)hipFree(mem);
+  }
+};
+
+static TableKey key_of(const SpecRequest& r) {
+  TableKey k;
+  k.sr = r.sr;
+  k.n_fft = r.n_fft;
+  k.win = r.win;
+  const bool mel = (r.out_kind == OUT_MEL || r.out_kind == OUT_MFCC);
+  k.n_mels = mel ? r.n_mels : 0;
+  k.n_mfcc = (r.out_kind == OUT_MFCC) ? std::min(r.n_mfcc, r.n_mels) : 0;
+  k.fmin = mel ? r.fmin : 0.0f;
+  k.fmax = mel ? r.fmax : 0.0f;
+  return k;
+}
+
+static int32_t make_tables(const TableKey& key, hipStream_t st, std::unique_ptr<Tables>* out) {
+  auto T = std::make_unique<Tables>();
+  T->key = key;
+  DeviceTables& t = T->t;
+  const int M = std::max(key.n_fft / 2, 1);
+  const int F = key.n_fft / 2 + 1;
+  t.fbs = ((F + 31) / 32) * 32 + 64;
+  t.nmb = (key.n_mels + 15) / 16;
+  t.ndt = (key.n_mfcc + 15) / 16;
+  const int rows = t.nmb * 16;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    size_t o = off;
+    off += (bytes + 255) & ~(size_t)255;
+    return o;
+  };
+  const size_t o_win = take((size_t)key.n_fft * 4);
+  const size_t o_tw = take((size_t)M * 8);
+  const size_t o_twn = take((size_t)(M / 2 + 1) * 8);
+  const size_t o_fb32 = take((size_t)rows * t.fbs * 4);
+  const size_t o_fb16 = take((size_t)rows * t.fbs * 2);
+  const size_t o_fbh = take((size_t)rows * t.fbs * 2);
+  const size_t o_b32 = take((size_t)t.nmb * 8);
+  const size_t o_b16 = take((size_t)t.nmb * 8);
+  const size_t nd = (size_t)t.ndt * t.nmb * 256;
+  const size_t o_d32 = take(nd * 4);
+  const size_t o_dbh = take(nd * 2);
+  const size_t o_dbl = take(nd * 2);
+  const size_t o_dhh = take(nd * 2);
+  const size_t o_dhl = take(nd * 2);
+  const size_t o_dd = take((size_t)key.n_mfcc * key.n_mels * 4);
+  hipError_t e = hipMalloc(&T->mem, off);
+  if (e != hipSuccess) return ERR_ALLOC;
+  char* b = static_cast<char*>(T->mem);
+  t.window = reinterpret_cast<float*>(b + o_win);
+  t.tw = reinterpret_cast<float2*>(b + o_tw);
+  t.twN = reinterpret_cast<float2*>(b + o_twn);
+  t.fb32 = reinterpret_cast<float*>(b + o_fb32);
+  t.fb16 = reinterpret_cast<uint16_t*>(b + o_fb16);
+  t.fbh = reinterpret_cast<uint16_t*>(b + o_fbh);
+  t.band32 = reinterpret_cast<int2*>(b + o_b32);
+  t.band16 = reinterpret_cast<int2*>(b + o_b16);
+  t.dct32 = reinterpret_cast<float*>(b + o_d32);
+  t.dctb_hi = reinterpret_cast<uint16_t*>(b + o_dbh);
+  t.dctb_lo = reinterpret_cast<uint16_t*>(b + o_dbl);
+  t.dcth_hi = reinterpret_cast<uint16_t*>(b + o_dhh);
+  t.dcth_lo = reinterpret_cast<uint16_t*>(b + o_dhl);
+  t.dct_dense = reinterpret_cast<float*>(b + o_dd);
+  MM_HIP(gen_tables(key, t, st));
+  MM_HIP(hipStreamSynchronize(st));
+  *out = std::move(T);
+  return OK;
+}
+
+// ---------------------------------------------------------------------------
+// engine singleton
+// ---------------------------------------------------------------------------
+
+static Engine* g_engine = nullptr;
+static int32_t g_status = ERR_NO_GPU;
+static std::once_flag g_once;
+
+bool Engine::init() {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return false;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return false;
+  hipDeviceProp_t p;
+  if (hipGetDeviceProperties(&p, dev) != hipSuccess) return false;
+  n_cu_ = p.multiProcessorCount;
+  arch_ = p.gcnArchName;
+  name_ = p.name;
+  int lds = 0;
+  if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && lds > 0)
+    set_lds_limit_bytes((size_t)lds);
+  return true;
+}
+
+Engine* Engine::get() {
+  std::call_once(g_once, [] {
+    Engine* e = new Engine();
+    if (e->init()) {
+      g_engine = e;
+      g_status = OK;
+    } else {
+      delete e;
+      g_status = ERR_NO_GPU;
+    }
+  });
+  return g_engine;
+}
+
+int32_t Engine::status() {
+  get();
+  return g_status;
+}
+
+Lane* Engine::acquire_lane() {
+  {
+    std::lock_guard<std::mutex> g(mu_);
+    if (!free_lanes_.empty()) {
+      Lane* l = free_lanes_.back();
+      free_lanes_.pop_back();
+      return l;
+    }
+  }
+  Lane* l = new Lane();
+  if (hipStreamCreateWithFlags(&l->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete l;
+    return nullptr;
+  }
+  return l;
+}
+
+void Engine::release_lane(Lane* l) {
+  if (!l) return;
+  std::lock_guard<std::mutex> g(mu_);
+  free_lanes_.push_back(l);
+}
+
+int32_t Engine::tables_for(const SpecRequest& r, hipStream_t st, const Tables** out) {
+  const TableKey k = key_of(r);
+  std::lock_guard<std::mutex> g(mu_);
+  for (auto& t : tables_)
+    if (t->key == k) {
+      *out = t.get();
+      return OK;
+    }
+  std::unique_ptr<Tables> T;
+  int32_t s = make_tables(k, st, &T);
+  if (s != OK) return s;
+  *out = T.get();
+  tables_.push_back(std::move(T));
+  return OK;
+}
+
+std::string tune_key(const SpecRequest& r, const std::string& arch) {
+  std::ostringstream o;
+  o << arch << ":k" << r.out_kind << ":n" << r.n_fft << ":h" << r.hop << ":m" << r.n_mels << ":c"
+    << ((r.out_kind == OUT_MFCC) ? r.n_mfcc : 0) << ":p" << r.precision << ":i" << r.in_dtype;
+  return o.str();
+}
+
+std::string Engine::tuning_path() const {
+  const char* p = std::getenv("METALMOM_AMD_TUNE_FILE");
+  if (p && *p) return p;
+  const char* home = std::getenv("HOME");
+  return std::string(home ? home : "/tmp") + "/.cache/metalmom_amd_tuning.json";
+}
+
+// The file is a flat JSON object {"key": wg_per_cu, ...}.
+void Engine::load_tuning() {
+  if (tune_loaded_) return;
+  tune_loaded_ = true;
+  std::ifstream f(tuning_path());
+  if (!f) return;
+  std::stringstream ss;
+  ss << f.rdbuf();
+  const std::string s = ss.str();
+  size_t i = 0;
+  while ((i = s.find('"', i)) != std::string::npos) {
+    const size_t j = s.find('"', i + 1);
+    if (j == std::string::npos) break;
+    const std::string key = s.substr(i + 1, j - i - 1);
+    const size_t c = s.find(':', j);
+    if (c == std::string::npos) break;
+    const int v = std::atoi(s.c_str() + c + 1);
+    if (v > 0) tune_.push_back({key, v});
+    i = s.find_first_of(",}", c);
+    if (i == std::string::npos) break;
+  }
+}
+
+void Engine::save_tuning() {
+  const std::string path = tuning_path();
+  const size_t slash = path.find_last_of('/');
+  if (slash != std::string::npos && slash > 0) (void)mkdir(path.substr(0, slash).c_str(), 0755);
+  std::ofstream f(path);
+  if (!f) return;
+  f << "{";
+  for (size_t i = 0; i < tune_.size(); ++i) f << (i ? ", " : "") << "\"" << tune_[i].key << "\": " << tune_[i].wg_per_cu;
+  f << "}\n";
+}
+
+int Engine::tuned_wg_per_cu(const SpecRequest& r) {
+  std::lock_guard<std::mutex> g(mu_);
+  load_tuning();
+  const std::string k = tune_key(r, arch_);
+  for (auto& e : tune_)
+    if (e.key == k) return e.wg_per_cu;
+  return 0;
+}
+
+void Engine::record_tuning(const SpecRequest& r, int wg_per_cu) {
+  std::lock_guard<std::mutex> g(mu_);
+  load_tuning();
+  const std::string k = tune_key(r, arch_);
+  bool found = false;
+  for (auto& e : tune_)
+    if (e.key == k) {
+      e.wg_per_cu = wg_per_cu;
+      found = true;
+    }
+  if (!found) tune_.push_back({k, wg_per_cu});
+  save_tuning();
+}
+
+int Engine::grid_for(const SpecRequest& r, int n_tiles, const TileLayout& L) {
+  int per_cu = tuned_wg_per_cu(r);
+  if (per_cu <= 0) {
+    // default: as many workgroups as fit by LDS (at most 4 per CU)
+    per_cu = (int)std::max<size_t>(1, std::min<size_t>(4, lds_limit_bytes() / std::max<size_t>(L.lds_bytes, 1)));
+  }
+  const int g = std::max(1, n_cu_ * per_cu);
+  return std::min(g, std::max(n_tiles, 1));
+}
+
+// ---------------------------------------------------------------------------
+// batch metadata
+// ---------------------------------------------------------------------------
+
+void build_meta(const SpecRequest& r, const int64_t* lens, int n, int tile_frames, BatchMeta* m) {
+  m->n_clips = n;
+  m->clip_off.resize(n);
+  m->clip_len.resize(n);
+  m->out_off.resize(n);
+  m->clip_frames.resize(n);
+  m->tile_clip.clear();
+  m->tile_f0.clear();
+  const int64_t rows = out_rows(r);
+  const int ef = out_elem_floats(r);
+  int64_t in = 0, out = 0;
+  for (int c = 0; c < n; ++c) {
+    const int64_t T = frame_count(lens[c], r.n_fft, r.hop, r.center);
+    m->clip_off[c] = in;
+    m->clip_len[c] = lens[c];
+    m->clip_frames[c] = (int32_t)T;
+    m->out_off[c] = out;  // in output entries (complex entries for complex STFT)
+    in += lens[c];
+    out += rows * T;
+    for (int64_t f = 0; f < T; f += tile_frames) {
+      m->tile_clip.push_back(c);
+      m->tile_f0.push_back((int32_t)f);
+    }
+  }
+  m->total_in = in;
+  m->total_out = out * ef;
+  m->n_tiles = (int)m->tile_clip.size();
+}
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+size_t meta_bytes(const BatchMeta& m) {
+  const size_t c = (size_t)std::max(m.n_clips, 1), t = (size_t)std::max(m.n_tiles, 1);
+  return align256(c * 8) * 3 + align256(c * 4) + align256(t * 4) * 2 + align256(c * 4) + align256(t * 4);
+}
+
+MetaPtrs meta_ptrs(const BatchMeta& m, void* dev) {
+  const size_t c = (size_t)std::max(m.n_clips, 1), t = (size_t)std::max(m.n_tiles, 1);
+  char* b = static_cast<char*>(dev);
+  MetaPtrs p;
+  size_t o = 0;
+  p.clip_off = reinterpret_cast<const int64_t*>(b + o); o += align256(c * 8);
+  p.clip_len = reinterpret_cast<const int64_t*>(b + o); o += align256(c * 8);
+  p.out_off = reinterpret_cast<const int64_t*>(b + o); o += align256(c * 8);
+  p.clip_frames = reinterpret_cast<const int32_t*>(b + o); o += align256(c * 4);
+  p.tile_clip = reinterpret_cast<const int32_t*>(b + o); o += align256(t * 4);
+  p.tile_f0 = reinterpret_cast<const int32_t*>(b + o); o += align256(t * 4);
+  p.clip_max_key = reinterpret_cast<unsigned*>(b + o); o += align256(c * 4);
+  p.tile_min = reinterpret_cast<float*>(b + o);
+  return p;
+}
+
+void pack_meta(const BatchMeta& m, void* host) {
+  const size_t c = (size_t)std::max(m.n_clips, 1), t = (size_t)std::max(m.n_tiles, 1);
+  char* b = static_cast<char*>(host);
+  std::memset(b, 0, meta_bytes(m));
+  size_t o = 0;
+  if (m.n_clips) std::memcpy(b + o, m.clip_off.data(), (size_t)m.n_clips * 8);
+  o += align256(c * 8);
+  if (m.n_clips) std::memcpy(b + o, m.clip_len.data(), (size_t)m.n_clips * 8);
+  o += align256(c * 8);
+  if (m.n_clips) std::memcpy(b + o, m.out_off.data(), (size_t)m.n_clips * 8);
+  o += align256(c * 8);
+  if (m.n_clips) std::memcpy(b + o, m.clip_frames.data(), (size_t)m.n_clips * 4);
+  o += align256(c * 4);
+  if (m.n_tiles) std::memcpy(b + o, m.tile_clip.data(), (size_t)m.n_tiles * 4);
+  o += align256(t * 4);
+  if (m.n_tiles) std::memcpy(b + o, m.tile_f0.data(), (size_t)m.n_tiles * 4);
+}
+
+// ---------------------------------------------------------------------------
+// execution
+// ---------------------------------------------------------------------------
+
+static TileArgs make_args(const SpecRequest& r, const Tables* tb, const BatchMeta& m, const MetaPtrs& mp,
+                          const void* d_signal, void* d_out, const TileLayout& L, int grid) {
+  const DeviceTables& t = tb->t;
+  TileArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.signal = d_signal;
+  a.clip_off = mp.clip_off;
+  a.clip_len = mp.clip_len;
+  a.clip_frames = mp.clip_frames;
+  a.out_off = mp.out_off;
+  a.tile_clip = mp.tile_clip;
+  a.tile_f0 = mp.tile_f0;
+  a.n_tiles = m.n_tiles;
+  a.tiles_per_wg = (m.n_tiles + grid - 1) / std::max(grid, 1);
+  a.n_fft = r.n_fft;
+  a.hop = r.hop;
+  a.center = r.center;
+  a.pad_mode = r.pad_mode;
+  a.in_dtype = r.in_dtype;
+  a.window = t.window;
+  a.tw = t.tw;
+  a.twN = t.twN;
+  a.fb32 = t.fb32;
+  a.fb16 = (r.precision == PREC_F16) ? t.fbh : t.fb16;
+  a.band = (r.precision == PREC_F32) ? t.band32 : t.band16;
+  a.fbs = t.fbs;
+  a.nmb = t.nmb;
+  a.n_mels = r.n_mels;
+  a.dct32 = t.dct32;
+  a.dct_hi = (r.precision == PREC_F16) ? t.dcth_hi : t.dctb_hi;
+  a.dct_lo = (r.precision == PREC_F16) ? t.dcth_lo : t.dctb_lo;
+  a.ndt = t.ndt;
+  a.n_mfcc = std::min(r.n_mfcc, r.n_mels);
+  a.power = r.power;
+  a.ref = r.ref;
+  a.amin = r.amin;
+  a.top_db = r.top_db;
+  a.use_top_db = (r.out_kind == OUT_MFCC && r.top_db > 0.0f) ? 1 : 0;
+  a.phase = 1;
+  a.clip_max_key = mp.clip_max_key;
+  a.tile_min = mp.tile_min;
+  a.out = d_out;
+  a.out_kind = r.out_kind;
+  a.lds_row_stride = L.row_stride;
+  a.lds_region_bytes = L.region_bytes;
+  return a;
+}
+
+static int32_t enqueue_generic(const SpecRequest& r, const Tables* tb, const BatchMeta& m, const void* d_signal,
+                               void* d_out, DevBuf* scratch, DevBuf* scratch2, hipStream_t st) {
+  if (r.in_dtype != IN_F32) return ERR_INVALID;
+  if (r.n_fft > 16384) return ERR_INVALID;
+  const DeviceTables& t = tb->t;
+  const int F = r.n_fft / 2 + 1;
+  for (int c = 0; c < m.n_clips; ++c) {
+    const int T = m.clip_frames[c];
+    if (T == 0) continue;
+    const float* x = static_cast<const float*>(d_signal) + m.clip_off[c];
+    MM_HIP(scratch->ensure((size_t)T * F * 8));
+    float2* spec = static_cast<float2*>(scratch->p);
+    MM_HIP(launch_generic_fft(x, m.clip_len[c], r.n_fft, r.hop, r.center, r.pad_mode, t.window, t.tw, t.twN, T,
+                              spec, st));
+    if (r.out_kind <= OUT_STFT_COMPLEX) {
+      void* o = static_cast<char*>(d_out) + m.out_off[c] * out_elem_floats(r) * 4;
+      MM_HIP(launch_generic_spec_out(spec, F, T, r.out_kind, o, st));
+    } else if (r.out_kind == OUT_MEL) {
+      float* o = static_cast<float*>(d_out) + m.out_off[c];
+      MM_HIP(launch_generic_mel(spec, F, T, t.fb32, t.fbs, r.n_mels, r.power, o, st));
+    } else {
+      const int nm = std::min(r.n_mfcc, r.n_mels);
+      MM_HIP(scratch2->ensure((size_t)r.n_mels * T * 4 + 256));
+      float* mel = static_cast<float*>(scratch2->p);
+      unsigned* key = reinterpret_cast<unsigned*>(static_cast<char*>(scratch2->p) + (size_t)r.n_mels * T * 4);
+      MM_HIP(launch_generic_mel(spec, F, T, t.fb32, t.fbs, r.n_mels, r.power, mel, st));
+      MM_HIP(launch_fill_u32(key, 1, float_to_key(-INFINITY), st));
+      MM_HIP(launch_db(mel, (int64_t)r.n_mels * T, 10.0f, r.ref, r.amin, mel, key, st));
+      if (r.top_db > 0.0f) MM_HIP(launch_db_clamp(mel, (int64_t)r.n_mels * T, key, r.top_db, st));
+      float* o = static_cast<float*>(d_out) + m.out_off[c];
+      MM_HIP(launch_generic_dct(mel, r.n_mels, T, t.dct_dense, nm, o, st));
+    }
+  }
+  return OK;
+}
+
+int32_t prepare_batch(const SpecRequest& r) {
+  TileLayout L;
+  if (!tile_layout(r.n_fft, r.out_kind, r.precision, r.n_mels, std::min(r.n_mfcc, r.n_mels), &L)) return OK;
+  TileArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.n_fft = r.n_fft;
+  a.out_kind = r.out_kind;
+  MM_HIP(launch_tile(a, r.precision, L, 0, nullptr));
+  return OK;
+}
+
+int32_t enqueue_batch(Engine* eng, const SpecRequest& r, const Tables* tb, const BatchMeta& m, const MetaPtrs& mp,
+                      const void* d_signal, void* d_out, DevBuf* scratch, DevBuf* scratch2, int grid_override,
+                      hipStream_t st, hipEvent_t ev_p1_begin, hipEvent_t ev_p1_end) {
+  if (m.n_tiles == 0) return OK;
+  TileLayout L;
+  const bool fused = tile_layout(r.n_fft, r.out_kind, r.precision, r.n_mels, std::min(r.n_mfcc, r.n_mels), &L);
+  if (!fused) {
+    if (r.precision != PREC_F32) return ERR_INVALID;
+    return enqueue_generic(r, tb, m, d_signal, d_out, scratch, scratch2, st);
+  }
+  const int grid = grid_override > 0 ? std::min(grid_override, m.n_tiles) : eng->grid_for(r, m.n_tiles, L);
+  TileArgs a = make_args(r, tb, m, mp, d_signal, d_out, L, grid);
+  const int g = (m.n_tiles + a.tiles_per_wg - 1) / a.tiles_per_wg;
+  if (a.use_top_db) MM_HIP(launch_fill_u32(mp.clip_max_key, m.n_clips, float_to_key(-INFINITY), st));
+  if (ev_p1_begin) MM_HIP(hipEventRecord(ev_p1_begin, st));
+  MM_HIP(launch_tile(a, r.precision, L, g, st));
+  if (ev_p1_end) MM_HIP(hipEventRecord(ev_p1_end, st));
+  if (a.use_top_db) {
+    a.phase = 2;
+    MM_HIP(launch_tile(a, r.precision, L, g, st));
+  }
+  return OK;
+}
+
+int32_t run_single(Lane* lane, const SpecRequest& r, const float* signal, int64_t len, float** out, int64_t* rows,
+                   int64_t* cols) {
+  Engine* eng = Engine::get();
+  if (!eng) return ERR_NO_GPU;
+  hipStream_t st = lane->stream;
+  const Tables* tb = nullptr;
+  int32_t s = eng->tables_for(r, st, &tb);
+  if (s != OK) return s;
+  TileLayout L;
+  const bool fused = tile_layout(r.n_fft, r.out_kind, r.precision, r.n_mels, std::min(r.n_mfcc, r.n_mels), &L);
+  BatchMeta m;
+  build_meta(r, &len, 1, fused ? L.tile_frames : kTileFrames, &m);
+  *rows = out_rows(r);
+  *cols = m.clip_frames[0];
+  *out = nullptr;
+  const size_t out_bytes = (size_t)m.total_out * 4;
+  if (out_bytes == 0) return OK;
+  const size_t in_bytes = (size_t)len * 4;
+  const size_t mb = meta_bytes(m);
+  MM_HIP(lane->h_in.ensure(in_bytes + mb));
+  MM_HIP(lane->d_in.ensure(in_bytes));
+  MM_HIP(lane->d_meta.ensure(mb));
+  MM_HIP(lane->d_out.ensure(out_bytes));
+  MM_HIP(lane->h_out.ensure(out_bytes));
+  std::memcpy(lane->h_in.p, signal, in_bytes);
+  pack_meta(m, static_cast<char*>(lane->h_in.p) + in_bytes);
+  MM_HIP(hipMemcpyAsync(lane->d_in.p, lane->h_in.p, in_bytes, hipMemcpyHostToDevice, st));
+  MM_HIP(hipMemcpyAsync(lane->d_meta.p, static_cast<char*>(lane->h_in.p) + in_bytes, mb, hipMemcpyHostToDevice, st));
+  const MetaPtrs mp = meta_ptrs(m, lane->d_meta.p);
+  s = enqueue_batch(eng, r, tb, m, mp, lane->d_in.p, lane->d_out.p, &lane->d_scratch, &lane->d_scratch2, 0, st);
+  if (s != OK) {
+    (void)hipStreamSynchronize(st);
+    return s;
+  }
+  MM_HIP(hipMemcpyAsync(lane->h_out.p, lane->d_out.p, out_bytes, hipMemcpyDeviceToHost, st));
+  MM_HIP(hipStreamSynchronize(st));
+  float* res = static_cast<float*>(std::malloc(out_bytes));
+  if (!res) return ERR_ALLOC;
+  std::memcpy(res, lane->h_out.p, out_bytes);
+  *out = res;
+  return OK;
+}
+
+int32_t run_db(Lane* lane, const float* data, int64_t n, float mult, float ref, float amin, float top_db,
+               float** out) {
+  Engine* eng = Engine::get();
+  if (!eng) return ERR_NO_GPU;
+  hipStream_t st = lane->stream;
+  const size_t bytes = (size_t)n * 4;
+  const size_t key_off = (bytes + 255) & ~(size_t)255;
+  MM_HIP(lane->h_in.ensure(bytes));
+  MM_HIP(lane->d_in.ensure(key_off + 256));
+  MM_HIP(lane->d_out.ensure(bytes));
+  MM_HIP(lane->h_out.ensure(bytes));
+  std::memcpy(lane->h_in.p, data, bytes);
+  unsigned* key = reinterpret_cast<unsigned*>(static_cast<char*>(lane->d_in.p) + key_off);
+  MM_HIP(hipMemcpyAsync(lane->d_in.p, lane->h_in.p, bytes, hipMemcpyHostToDevice, st));
+  MM_HIP(launch_fill_u32(key, 1, float_to_key(-INFINITY), st));
+  float* d = static_cast<float*>(lane->d_out.p);
+  MM_HIP(launch_db(static_cast<const float*>(lane->d_in.p), n, mult, ref, amin, d, key, st));
+  if (top_db > 0.0f) MM_HIP(launch_db_clamp(d, n, key, top_db, st));
+  MM_HIP(hipMemcpyAsync(lane->h_out.p, d, bytes, hipMemcpyDeviceToHost, st));
+  MM_HIP(hipStreamSynchronize(st));
+  float* res = static_cast<float*>(std::malloc(bytes));
+  if (!res) return ERR_ALLOC;
+  std::memcpy(res, lane->h_out.p, bytes);
+  *out = res;
+  return OK;
+}
+
+}  // namespace mm

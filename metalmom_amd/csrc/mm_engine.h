@@ -1,0 +1,153 @@
+// Host engine: device singleton, table cache, per-context stream/workspace
+// lanes, single-clip and batched execution. Internal to libmetalmom.so.
+#pragma once
+#include <stdint.h>
+
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "mm_common.h"
+#include "mm_kernels.h"
+
+namespace mm {
+
+// Status codes (values of the public header).
+enum Status : int32_t { OK = 0, ERR_INVALID = -1, ERR_NO_GPU = -2, ERR_ALLOC = -3, ERR_INTERNAL = -4 };
+
+// Full description of one spectral request.
+struct SpecRequest {
+  int out_kind = OUT_MFCC;
+  int sr = 22050;
+  int n_fft = 2048;
+  int hop = 512;
+  int win = 2048;
+  int center = 1;
+  int pad_mode = PAD_CONSTANT;
+  float power = 2.0f;
+  int n_mels = 128;
+  float fmin = 0.0f;
+  float fmax = 11025.0f;
+  int n_mfcc = 13;
+  float ref = 1.0f;
+  float amin = 1e-10f;
+  float top_db = 80.0f;   // <= 0: no clamp
+  int precision = PREC_F32;
+  int in_dtype = IN_F32;
+};
+
+int64_t frame_count(int64_t n, int n_fft, int hop, int center);
+// rows of the output matrix for a request (bins, mels or coefficients)
+int64_t out_rows(const SpecRequest& r);
+// float32 elements per output entry (2 for complex)
+int out_elem_floats(const SpecRequest& r);
+
+struct Tables;  // device-resident tables (cached, process lifetime)
+
+// Grow-only device / pinned host buffers.
+struct DevBuf {
+  void* p = nullptr;
+  size_t n = 0;
+  hipError_t ensure(size_t bytes);
+  void release();
+};
+struct HostBuf {
+  void* p = nullptr;
+  size_t n = 0;
+  hipError_t ensure(size_t bytes);
+  void release();
+};
+
+// One stream plus its workspaces. Pooled: contexts borrow one on first use.
+struct Lane {
+  hipStream_t stream = nullptr;
+  DevBuf d_in, d_out, d_meta, d_scratch, d_scratch2;
+  HostBuf h_in, h_out, h_meta;
+  ~Lane();
+};
+
+class Engine {
+ public:
+  static Engine* get();          // nullptr when no GPU is usable
+  static int32_t status();       // OK or ERR_NO_GPU
+
+  int num_cus() const { return n_cu_; }
+  const std::string& arch() const { return arch_; }
+  const std::string& name() const { return name_; }
+
+  Lane* acquire_lane();
+  void release_lane(Lane* l);
+
+  // Tables for a request (generated on first use, then shared).
+  int32_t tables_for(const SpecRequest& r, hipStream_t st, const Tables** out);
+
+  // Persistent grid size: autotuned if known, else a default.
+  int grid_for(const SpecRequest& r, int n_tiles, const TileLayout& L);
+  void record_tuning(const SpecRequest& r, int wg_per_cu);
+  int tuned_wg_per_cu(const SpecRequest& r);
+  std::string tuning_path() const;
+  void load_tuning();
+  void save_tuning();
+
+ private:
+  Engine() = default;
+  bool init();
+  int n_cu_ = 0;
+  std::string arch_, name_;
+  std::mutex mu_;
+  std::vector<Lane*> free_lanes_;
+  std::vector<std::unique_ptr<Tables>> tables_;
+  struct TuneEntry {
+    std::string key;
+    int wg_per_cu;
+  };
+  std::vector<TuneEntry> tune_;
+  bool tune_loaded_ = false;
+};
+
+std::string tune_key(const SpecRequest& r, const std::string& arch);
+
+// Device-side batch description (shared by single-clip and batched paths).
+struct BatchMeta {
+  int n_clips = 0;
+  int n_tiles = 0;
+  int64_t total_in = 0;     // samples
+  int64_t total_out = 0;    // float32 elements
+  std::vector<int64_t> clip_off, clip_len, out_off;
+  std::vector<int32_t> clip_frames, tile_clip, tile_f0;
+};
+void build_meta(const SpecRequest& r, const int64_t* lens, int n, int tile_frames, BatchMeta* m);
+
+// Device execution of a whole batch whose signal and output already live in
+// HBM. `d_meta` must hold the uploaded metadata (see meta_bytes/pack_meta).
+size_t meta_bytes(const BatchMeta& m);
+void pack_meta(const BatchMeta& m, void* host);
+struct MetaPtrs {
+  const int64_t* clip_off;
+  const int64_t* clip_len;
+  const int32_t* clip_frames;
+  const int64_t* out_off;
+  const int32_t* tile_clip;
+  const int32_t* tile_f0;
+  unsigned* clip_max_key;
+  float* tile_min;
+};
+MetaPtrs meta_ptrs(const BatchMeta& m, void* dev);
+
+int32_t enqueue_batch(Engine* eng, const SpecRequest& r, const Tables* t, const BatchMeta& m, const MetaPtrs& mp,
+                      const void* d_signal, void* d_out, DevBuf* scratch, DevBuf* scratch2, int grid_override,
+                      hipStream_t st, hipEvent_t ev_p1_begin = nullptr, hipEvent_t ev_p1_end = nullptr);
+
+// Sets the fused kernel's LDS attribute ahead of any stream capture.
+int32_t prepare_batch(const SpecRequest& r);
+
+// Single clip through pinned staging on the lane's stream (synchronous).
+// On success *out is malloc'd (caller frees) and rows/cols are set.
+int32_t run_single(Lane* lane, const SpecRequest& r, const float* signal, int64_t len, float** out, int64_t* rows,
+                   int64_t* cols);
+
+int32_t run_db(Lane* lane, const float* data, int64_t n, float mult, float ref, float amin, float top_db,
+               float** out);
+
+}  // namespace mm

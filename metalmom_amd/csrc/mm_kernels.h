@@ -1,0 +1,50 @@
+// Launchers for the HIP kernels in mm_kernels.hip (host-callable, internal).
+#pragma once
+#include "mm_common.h"
+
+namespace mm {
+
+// Key of one set of device-resident tables.
+struct TableKey {
+  int sr, n_fft, win, n_mels, n_mfcc;
+  float fmin, fmax;
+  bool operator==(const TableKey& o) const {
+    return sr == o.sr && n_fft == o.n_fft && win == o.win && n_mels == o.n_mels && n_mfcc == o.n_mfcc &&
+           fmin == o.fmin && fmax == o.fmax;
+  }
+};
+
+// LDS layout of the fused tile kernel for one configuration.
+struct TileLayout {
+  int fpw;           // frames per wavefront
+  int tile_frames;   // frames per tile (8 * fpw)
+  int row_stride;    // staging row stride, elements
+  int region_bytes;  // LDS bytes per wavefront region
+  size_t lds_bytes;  // dynamic LDS per workgroup
+};
+
+void set_lds_limit_bytes(size_t n);
+size_t lds_limit_bytes();
+
+hipError_t gen_tables(const TableKey& key, const DeviceTables& t, hipStream_t st);
+
+// Returns false when the fused kernel does not cover this configuration
+// (then the generic kernels are used).
+bool tile_layout(int n_fft, int out_kind, int prec, int n_mels, int n_mfcc, TileLayout* L);
+hipError_t launch_tile(const TileArgs& a, int prec, const TileLayout& L, int grid, hipStream_t st);
+
+hipError_t launch_fill_u32(unsigned* p, int64_t n, unsigned v, hipStream_t st);
+
+hipError_t launch_generic_fft(const float* x, int64_t len, int n_fft, int hop, int center, int pad_mode,
+                              const float* window, const float2* tw, const float2* twN, int T, float2* spec,
+                              hipStream_t st);
+hipError_t launch_generic_spec_out(const float2* spec, int F, int T, int kind, void* out, hipStream_t st);
+hipError_t launch_generic_mel(const float2* spec, int F, int T, const float* fb, int fbs, int n_mels, float power,
+                              float* mel, hipStream_t st);
+hipError_t launch_db(const float* in, int64_t n, float mult, float ref, float amin, float* out, unsigned* max_key,
+                     hipStream_t st);
+hipError_t launch_db_clamp(float* x, int64_t n, const unsigned* max_key, float top_db, hipStream_t st);
+hipError_t launch_generic_dct(const float* logmel, int n_mels, int T, const float* dct, int n_mfcc, float* out,
+                              hipStream_t st);
+
+}  // namespace mm

@@ -1,0 +1,1086 @@
+// HIP kernels for gfx950: device-side FP64 table generation, the fused
+// frame-tile kernel (framing -> Hann -> real FFT -> |X|^p -> banded mel on
+// MFMA -> dB -> DCT-II on MFMA), generic fallbacks for FFT sizes outside the
+// fused range, and the standalone dB conversion.
+//
+// Reference semantics followed (zakkeown/MetalMom):
+//   framing / centre padding / frame count  Sources/MetalMomCore/Spectral/STFT.swift:240-254
+//   periodic Hann + centred pad/truncate     Filters/Windows.swift:14-48, STFT.swift:44-57
+//   real FFT, DC/Nyquist, power, magnitude   STFT.swift:287-329, 140-197
+//   Slaney mel filterbank                    Filters/FilterBank.swift:23-73, Convert/Units.swift:55-85
+//   FB @ |X|^power                           Spectral/MelSpectrogram.swift:28-142
+//   power_to_db with a whole-clip top_db     Spectral/Scaling.swift:73-118
+//   DCT-II ortho basis product               Features/MFCC.swift:79-106
+#include "mm_common.h"
+#include "mm_kernels.h"
+
+#include <algorithm>
+
+#ifndef MM_TILE_OCC
+#define MM_TILE_OCC
+#endif
+
+namespace mm {
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef short s16x4 __attribute__((ext_vector_type(4)));
+typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------
+// small helpers
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 w) {
+  return make_float2(fmaf(a.x, w.x, -a.y * w.y), fmaf(a.x, w.y, a.y * w.x));
+}
+// a * (-i)
+__device__ __forceinline__ float2 mul_mi(float2 a) { return make_float2(a.y, -a.x); }
+
+// Round-to-nearest-even float -> bf16 bits (NaN stays NaN).
+__host__ __device__ inline uint16_t f2bf(float f) {
+  unsigned u;
+  __builtin_memcpy(&u, &f, 4);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__host__ __device__ inline float bf2f(uint16_t h) {
+  unsigned u = ((unsigned)h) << 16;
+  float f;
+  __builtin_memcpy(&f, &u, 4);
+  return f;
+}
+__device__ inline uint16_t f2h(float f) {
+  _Float16 h = (_Float16)f;
+  uint16_t b;
+  __builtin_memcpy(&b, &h, 2);
+  return b;
+}
+__device__ inline float h2f(uint16_t b) {
+  _Float16 h;
+  __builtin_memcpy(&h, &b, 2);
+  return (float)h;
+}
+
+// Keeps the compiler from moving LDS accesses across this point. Within one
+// wavefront LDS instructions execute in issue order, so no hardware wait is
+// needed for wave-private scratch.
+__device__ __forceinline__ void wave_lds_fence() { __asm__ __volatile__("" ::: "memory"); }
+
+// Padded LDS index: one spare complex slot per 16 keeps the strided radix-16
+// accesses of a Stockham pass on distinct banks.
+__device__ __forceinline__ int pidx(int i) { return i + (i >> 4); }
+
+// Index of sample `i` of a clip of `len` samples after centre padding.
+// Returns -1 for a zero (constant) pad sample.
+__device__ __forceinline__ int64_t pad_index(int64_t i, int64_t len, int pad_mode) {
+  if (i >= 0 && i < len) return i;
+  if (pad_mode == PAD_CONSTANT || len <= 0) return -1;
+  if (len == 1) return 0;
+  // numpy.pad(mode="reflect"): mirror without repeating the edge sample,
+  // periodic with period 2*(len-1) for pads longer than the signal.
+  const int64_t period = 2 * (len - 1);
+  int64_t m = i % period;
+  if (m < 0) m += period;
+  return (m < len) ? m : period - m;
+}
+
+// ---------------------------------------------------------------------------
+// forward DFTs of size 2, 4, 8, 16 in registers (W = exp(-2 pi i / R))
+// ---------------------------------------------------------------------------
+
+template <int R> struct Dft;
+
+template <> struct Dft<1> {
+  __device__ __forceinline__ static void run(float2*) {}
+};
+
+template <> struct Dft<2> {
+  __device__ __forceinline__ static void run(float2* v) {
+    float2 a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  }
+};
+
+template <> struct Dft<4> {
+  __device__ __forceinline__ static void run(float2* v) {
+    float2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+    float2 t2 = cadd(v[1], v[3]), t3 = csub(v[1], v[3]);
+    v[0] = cadd(t0, t2);
+    v[2] = csub(t0, t2);
+    v[1] = make_float2(t1.x + t3.y, t1.y - t3.x);  // t1 - i t3
+    v[3] = make_float2(t1.x - t3.y, t1.y + t3.x);  // t1 + i t3
+  }
+};
+
+template <> struct Dft<8> {
+  __device__ __forceinline__ static void run(float2* v) {
+    constexpr float r = 0.70710678118654752440f;
+    float2 e[4] = {v[0], v[2], v[4], v[6]};
+    float2 o[4] = {v[1], v[3], v[5], v[7]};
+    Dft<4>::run(e);
+    Dft<4>::run(o);
+    // o[k] *= W8^k
+    float2 o1 = make_float2(r * (o[1].x + o[1].y), r * (o[1].y - o[1].x));
+    float2 o2 = mul_mi(o[2]);
+    float2 o3 = make_float2(r * (o[3].y - o[3].x), -r * (o[3].x + o[3].y));
+    v[0] = cadd(e[0], o[0]); v[4] = csub(e[0], o[0]);
+    v[1] = cadd(e[1], o1);   v[5] = csub(e[1], o1);
+    v[2] = cadd(e[2], o2);   v[6] = csub(e[2], o2);
+    v[3] = cadd(e[3], o3);   v[7] = csub(e[3], o3);
+  }
+};
+
+template <> struct Dft<16> {
+  // 16 = 4 x 4: x[n1*4 + n2]; Y[n2][k1] = DFT4_n1; *= W16^(n2 k1); X[k1 + 4 k2] = DFT4_n2
+  __device__ __forceinline__ static void run(float2* v) {
+    constexpr float c = 0.92387953251128675613f;  // cos(pi/8)
+    constexpr float s = 0.38268343236508977173f;  // sin(pi/8)
+    constexpr float r = 0.70710678118654752440f;
+    float2 y[4][4];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2) {
+      float2 t[4] = {v[n2], v[n2 + 4], v[n2 + 8], v[n2 + 12]};
+      Dft<4>::run(t);
+#pragma unroll
+      for (int k1 = 0; k1 < 4; ++k1) y[n2][k1] = t[k1];
+    }
+    // twiddles W16^(n2*k1)
+    y[1][1] = cmul(y[1][1], make_float2(c, -s));
+    y[1][2] = make_float2(r * (y[1][2].x + y[1][2].y), r * (y[1][2].y - y[1][2].x));
+    y[1][3] = cmul(y[1][3], make_float2(s, -c));
+    y[2][1] = make_float2(r * (y[2][1].x + y[2][1].y), r * (y[2][1].y - y[2][1].x));
+    y[2][2] = mul_mi(y[2][2]);
+    y[2][3] = make_float2(r * (y[2][3].y - y[2][3].x), -r * (y[2][3].x + y[2][3].y));
+    y[3][1] = cmul(y[3][1], make_float2(s, -c));
+    y[3][2] = make_float2(r * (y[3][2].y - y[3][2].x), -r * (y[3][2].x + y[3][2].y));
+    y[3][3] = cmul(y[3][3], make_float2(-c, s));
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+      float2 t[4] = {y[0][k1], y[1][k1], y[2][k1], y[3][k1]};
+      Dft<4>::run(t);
+#pragma unroll
+      for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = t[k2];
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// wave-level Stockham FFT of M complex points held in LDS scratch
+// ---------------------------------------------------------------------------
+
+template <int LOG2M> struct FftShape {
+  static constexpr int M = 1 << LOG2M;
+  static constexpr int NP16 = LOG2M / 4;          // radix-16 passes
+  static constexpr int TAIL = 1 << (LOG2M % 4);   // final radix (1 = none)
+  static constexpr int R0 = (NP16 > 0) ? 16 : TAIL;
+  static constexpr int B0 = M / R0;               // butterflies of pass 0
+  static constexpr int Q0 = (B0 + 63) / 64;
+};
+
+// One Stockham pass reading and writing LDS. NS = product of earlier radices.
+template <int M, int R, int NS>
+__device__ __forceinline__ void stockham_pass(float2* scr, const float2* __restrict__ tw, int lane) {
+  constexpr int B = M / R;
+  constexpr int QB = (B + 63) / 64;
+  float2 v[QB][R];
+#pragma unroll
+  for (int q = 0; q < QB; ++q) {
+    const int j = lane + 64 * q;
+    if ((B % 64 == 0) || j < B) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[q][r] = scr[pidx(j + r * B)];
+    }
+  }
+  wave_lds_fence();
+#pragma unroll
+  for (int q = 0; q < QB; ++q) {
+    const int j = lane + 64 * q;
+    if ((B % 64 == 0) || j < B) {
+      const int k = j % NS;
+      constexpr int STRIDE = M / (NS * R);
+#pragma unroll
+      for (int r = 1; r < R; ++r) v[q][r] = cmul(v[q][r], tw[(k * r) * STRIDE]);
+      Dft<R>::run(v[q]);
+      const int base = (j / NS) * NS * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) scr[pidx(base + r * NS)] = v[q][r];
+    }
+  }
+  wave_lds_fence();
+}
+
+template <int M, int P, int NPASS16, int TAIL>
+struct PassChain {
+  // pass P (P >= 1) with NS = 16^P
+  __device__ __forceinline__ static void run(float2* scr, const float2* tw, int lane) {
+    constexpr int NS = 1 << (4 * P);
+    if constexpr (P < NPASS16) {
+      stockham_pass<M, 16, NS>(scr, tw, lane);
+      PassChain<M, P + 1, NPASS16, TAIL>::run(scr, tw, lane);
+    } else if constexpr (TAIL > 1) {
+      stockham_pass<M, TAIL, NS>(scr, tw, lane);
+    }
+  }
+};
+
+// Loads frame `fr` of a clip (pass-0 inputs, windowed) into registers, runs
+// pass 0 from registers and the remaining passes in LDS. On return scr holds
+// Z[k] = DFT_M(z)[k] in natural order (padded index), z[n] = x[2n] + i x[2n+1].
+// Interior frames load straight from global memory; frames that touch the
+// centre padding are first staged into the scratch by a plain loop.
+template <typename TIn>
+__device__ __forceinline__ float load_sample(const TIn* x, int64_t i) { return (float)x[i]; }
+
+template <int LOG2M, typename TIn>
+__device__ __forceinline__ void frame_load(const TileArgs& a, const TIn* __restrict__ x, int64_t len, int64_t s0,
+                                           bool interior, float2* scr, int lane,
+                                           float2 (&v)[FftShape<LOG2M>::Q0][FftShape<LOG2M>::R0]) {
+  using S = FftShape<LOG2M>;
+  constexpr int M = S::M, R0 = S::R0, B0 = S::B0, Q0 = S::Q0;
+  if (interior) {
+    const TIn* xf = x + s0;
+#pragma unroll
+    for (int q = 0; q < Q0; ++q) {
+      const int j = lane + 64 * q;
+      if ((B0 % 64 == 0) || j < B0) {
+#pragma unroll
+        for (int r = 0; r < R0; ++r) {
+          const int n = j + r * B0;
+          const float2 w = *reinterpret_cast<const float2*>(a.window + 2 * n);
+          v[q][r] = make_float2(load_sample(xf, 2 * n) * w.x, load_sample(xf, 2 * n + 1) * w.y);
+        }
+      }
+    }
+  } else {
+#pragma unroll 1
+    for (int n = lane; n < M; n += 64) {
+      const int64_t i0 = pad_index(s0 + 2 * n, len, a.pad_mode);
+      const int64_t i1 = pad_index(s0 + 2 * n + 1, len, a.pad_mode);
+      const float x0 = (i0 >= 0) ? load_sample(x, i0) : 0.0f;
+      const float x1 = (i1 >= 0) ? load_sample(x, i1) : 0.0f;
+      scr[pidx(n)] = make_float2(x0 * a.window[2 * n], x1 * a.window[2 * n + 1]);
+    }
+    wave_lds_fence();
+#pragma unroll
+    for (int q = 0; q < Q0; ++q) {
+      const int j = lane + 64 * q;
+      if ((B0 % 64 == 0) || j < B0) {
+#pragma unroll
+        for (int r = 0; r < R0; ++r) v[q][r] = scr[pidx(j + r * B0)];
+      }
+    }
+    wave_lds_fence();
+  }
+}
+
+template <int LOG2M>
+__device__ __forceinline__ void frame_fft(const TileArgs& a, int clip, int fr, float2* scr, int lane) {
+  using S = FftShape<LOG2M>;
+  constexpr int M = S::M, R0 = S::R0, B0 = S::B0, Q0 = S::Q0;
+  const int n_fft = 2 * M;
+  const int64_t len = a.clip_len[clip];
+  const int64_t s0 = (int64_t)fr * a.hop - (a.center ? (n_fft / 2) : 0);
+  const bool interior = (s0 >= 0) && (s0 + n_fft <= len);
+  float2 v[Q0][R0];
+  if (a.in_dtype == IN_F32)
+    frame_load<LOG2M>(a, static_cast<const float*>(a.signal) + a.clip_off[clip], len, s0, interior, scr, lane, v);
+  else
+    frame_load<LOG2M>(a, static_cast<const _Float16*>(a.signal) + a.clip_off[clip], len, s0, interior, scr, lane, v);
+  // pass 0 (NS = 1: no twiddles), written in Stockham order
+#pragma unroll
+  for (int q = 0; q < Q0; ++q) {
+    const int j = lane + 64 * q;
+    if ((B0 % 64 == 0) || j < B0) {
+      Dft<R0>::run(v[q]);
+#pragma unroll
+      for (int r = 0; r < R0; ++r) scr[pidx(j * R0 + r)] = v[q][r];
+    }
+  }
+  wave_lds_fence();
+  if constexpr (S::NP16 > 0) PassChain<M, 1, S::NP16, S::TAIL>::run(scr, a.tw, lane);
+}
+
+// Real-FFT split: from Z (M complex) to X[k] and X[M-k] for k = lane + 64 i.
+// X[k] = E - i W_N^k O,  X[M-k] = conj(E + i W_N^k O),
+// E = (Z[k] + conj Z[M-k]) / 2,  O = (Z[k] - conj Z[M-k]) / 2.
+template <int LOG2M>
+__device__ __forceinline__ void real_split(const float2* scr, const float2* __restrict__ twN, int k,
+                                           float2& xk, float2& xmk) {
+  constexpr int M = 1 << LOG2M;
+  const float2 A = scr[pidx(k)];
+  const float2 Bz = scr[pidx((M - k) & (M - 1))];
+  const float2 Bc = make_float2(Bz.x, -Bz.y);
+  const float2 E = make_float2(0.5f * (A.x + Bc.x), 0.5f * (A.y + Bc.y));
+  const float2 O = make_float2(0.5f * (A.x - Bc.x), 0.5f * (A.y - Bc.y));
+  const float2 T = cmul(O, twN[k]);
+  xk = make_float2(E.x + T.y, E.y - T.x);
+  xmk = make_float2(E.x - T.y, -(E.y + T.x));
+}
+
+__device__ __forceinline__ float spec_value(float2 X, float power) {
+  const float p2 = fmaf(X.x, X.x, X.y * X.y);
+  if (power == 2.0f) return p2;
+  const float mag = sqrtf(p2);
+  if (power == 1.0f) return mag;
+  return powf(mag, power);
+}
+
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ float wave_min(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// the fused tile kernel
+// ---------------------------------------------------------------------------
+//
+// One workgroup = 8 wavefronts; one tile = 16 consecutive frames of one clip
+// (8 for complex STFT output at n_fft 4096: FPW = 1). Wave w transforms frames
+// w*FPW .. w*FPW+FPW-1 of the tile, keeping their spectra in registers until
+// its FFT scratch (which lives in its own LDS region) is free, then writes
+// them into its staging rows. After a workgroup barrier:
+//   * STFT outputs are written transposed ([bins, frames]) with 16-frame runs;
+//   * mel: wave w takes mel blocks w, w+8, ... (16 mels each) and contracts
+//     the banded filterbank with the staged spectra on MFMA
+//     (mel^T[16 mels x 16 frames] = FB[16 x K] * P^T[K x 16]);
+//   * MFCC: dB in registers, then DCT-II on MFMA with the mel registers fed
+//     straight in as the B operand (the DCT table is stored in the matching
+//     K order), partial sums reduced across waves in LDS in fixed order.
+// Persistent: workgroup g processes tiles [g*C, (g+1)*C).
+template <int LOG2M, int MODE, int PREC, int FPW>
+__global__ __launch_bounds__(kWgThreads) MM_TILE_OCC void k_tile(TileArgs a) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  constexpr int M = 1 << LOG2M;
+  constexpr int NI = (M / 2 + 1 + 63) / 64;       // split iterations per lane
+  constexpr int TF = kWaves * FPW;                 // frames per tile
+  constexpr bool SPEC = (MODE == OUT_STFT_MAG || MODE == OUT_STFT_POWER || MODE == OUT_STFT_COMPLEX);
+  constexpr bool CPLX = (MODE == OUT_STFT_COMPLEX);
+  constexpr int F = M + 1;
+  constexpr int MAXB = 4;                          // mel blocks per wave (n_mels <= 512)
+
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int RB = a.lds_region_bytes;
+  const int S = a.lds_row_stride;                  // staging row stride (elements)
+  unsigned char* region = lds + wave * RB;
+  float2* scr = reinterpret_cast<float2*>(region);
+  int* sc_exp = reinterpret_cast<int*>(lds + kWaves * RB);   // [16]
+  float* red_max = reinterpret_cast<float*>(sc_exp + 16);    // [8]
+  float* red_min = red_max + kWaves;                          // [8]
+  constexpr int ESZ = SPEC ? (CPLX ? 8 : 4) : (PREC == PREC_F32 ? 4 : 2);
+  const int rowb = S * ESZ;
+  auto row_ptr = [&](int f) -> unsigned char* {   // staging row of tile frame f
+    return lds + (f / FPW) * RB + (f % FPW) * rowb;
+  };
+
+  const int t_begin = blockIdx.x * a.tiles_per_wg;
+  const int t_end = min(t_begin + a.tiles_per_wg, a.n_tiles);
+  for (int tile = t_begin; tile < t_end; ++tile) {
+    const int clip = a.tile_clip[tile];
+    const int f0 = a.tile_f0[tile];
+    const int T = a.clip_frames[clip];
+    float thr = -INFINITY;
+    if constexpr (MODE == OUT_MFCC) {
+      if (a.use_top_db) {
+        if (a.phase == 2) {
+          thr = key_to_float(a.clip_max_key[clip]) - a.top_db;
+          if (!(a.tile_min[tile] < thr)) continue;   // clamp inactive: phase 1 result stands
+        }
+      }
+    }
+
+    // ---- FFT of this wave's frames; spectra kept in registers ----
+    float hk[FPW][NI][CPLX ? 2 : 1];
+    float hm[FPW][NI][CPLX ? 2 : 1];
+    int fexp[FPW];
+#pragma unroll
+    for (int q = 0; q < FPW; ++q) {
+      fexp[q] = 0;
+      const int fl = wave * FPW + q;
+      const int fr = f0 + fl;
+      if (fr < T) {
+        frame_fft<LOG2M>(a, clip, fr, scr, lane);
+        float vmax = 0.0f;
+#pragma unroll
+        for (int i = 0; i < NI; ++i) {
+          const int k = lane + 64 * i;
+          if ((M / 2 + 1) % 64 == 0 || k <= M / 2) {
+            float2 xk, xmk;
+            real_split<LOG2M>(scr, a.twN, k, xk, xmk);
+            if constexpr (CPLX) {
+              hk[q][i][0] = xk.x; hk[q][i][1] = xk.y;
+              hm[q][i][0] = xmk.x; hm[q][i][1] = xmk.y;
+            } else if constexpr (MODE == OUT_STFT_MAG) {
+              hk[q][i][0] = sqrtf(fmaf(xk.x, xk.x, xk.y * xk.y));
+              hm[q][i][0] = sqrtf(fmaf(xmk.x, xmk.x, xmk.y * xmk.y));
+            } else if constexpr (MODE == OUT_STFT_POWER) {
+              hk[q][i][0] = fmaf(xk.x, xk.x, xk.y * xk.y);
+              hm[q][i][0] = fmaf(xmk.x, xmk.x, xmk.y * xmk.y);
+            } else {
+              hk[q][i][0] = spec_value(xk, a.power);
+              hm[q][i][0] = spec_value(xmk, a.power);
+              vmax = fmaxf(vmax, fmaxf(hk[q][i][0], hm[q][i][0]));
+            }
+          }
+        }
+        wave_lds_fence();
+        if constexpr (!SPEC && PREC != PREC_F32) {
+          // per-frame power-of-two scale so the 16-bit operand keeps the
+          // frame's top range (exactly undone after the contraction)
+          vmax = wave_max(vmax);
+          int e = 0;
+          if (vmax > 0.0f && vmax <= 3.0e38f) e = ilogbf(vmax);
+          fexp[q] = e;
+        }
+      }
+    }
+
+    // ---- staging rows (this wave's region; its scratch is no longer used) ----
+#pragma unroll
+    for (int q = 0; q < FPW; ++q) {
+      const int fl = wave * FPW + q;
+      const int fr = f0 + fl;
+      unsigned char* row = row_ptr(fl);
+      if (fr < T) {
+#pragma unroll
+        for (int i = 0; i < NI; ++i) {
+          const int k = lane + 64 * i;
+          if ((M / 2 + 1) % 64 == 0 || k <= M / 2) {
+            if constexpr (CPLX) {
+              reinterpret_cast<float2*>(row)[k] = make_float2(hk[q][i][0], hk[q][i][1]);
+              reinterpret_cast<float2*>(row)[M - k] = make_float2(hm[q][i][0], hm[q][i][1]);
+            } else if constexpr (SPEC || PREC == PREC_F32) {
+              reinterpret_cast<float*>(row)[k] = hk[q][i][0];
+              reinterpret_cast<float*>(row)[M - k] = hm[q][i][0];
+            } else {
+              const float a0 = ldexpf(hk[q][i][0], -fexp[q]);
+              const float a1 = ldexpf(hm[q][i][0], -fexp[q]);
+              uint16_t* r16 = reinterpret_cast<uint16_t*>(row);
+              r16[k] = (PREC == PREC_BF16) ? f2bf(a0) : f2h(a0);
+              r16[M - k] = (PREC == PREC_BF16) ? f2bf(a1) : f2h(a1);
+            }
+          }
+        }
+        if constexpr (!SPEC) {
+          // bins F .. S-1 must read as zero in the banded contraction
+          for (int k = F + lane; k < S; k += 64) {
+            if constexpr (PREC == PREC_F32) reinterpret_cast<float*>(row)[k] = 0.0f;
+            else reinterpret_cast<uint16_t*>(row)[k] = 0;
+          }
+          if (lane == 0) sc_exp[fl] = fexp[q];
+        }
+      } else if constexpr (!SPEC) {
+        if (lane == 0) sc_exp[fl] = 0;
+      }
+    }
+    __syncthreads();
+
+    if constexpr (SPEC) {
+      // ---- transposed write-out: [F, T], runs of TF frames per bin ----
+      const int nf = min(TF, T - f0);
+      const int64_t base = a.out_off[clip];
+      for (int idx = threadIdx.x; idx < F * TF; idx += kWgThreads) {
+        const int f = idx % TF;
+        const int k = idx / TF;
+        if (f < nf) {
+          const unsigned char* row = row_ptr(f);
+          if constexpr (CPLX) {
+            float2 v = reinterpret_cast<const float2*>(row)[k];
+            reinterpret_cast<float2*>(a.out)[base + (int64_t)k * T + f0 + f] = v;
+          } else {
+            float v = reinterpret_cast<const float*>(row)[k];
+            reinterpret_cast<float*>(a.out)[base + (int64_t)k * T + f0 + f] = v;
+          }
+        }
+      }
+      __syncthreads();
+      continue;
+    } else {
+      // ---- banded mel contraction on MFMA ----
+      const int ri = lane & 15;          // MFMA row/col index within 16
+      const int g = lane >> 4;           // lane group (K slice)
+      const unsigned char* prow = row_ptr(ri);
+      const int myexp = sc_exp[ri];
+      const bool frame_ok = (f0 + ri) < T;
+      f32x4 dbv[MAXB];
+      float lmax = -INFINITY, lmin = INFINITY;
+      int nb = 0;
+#pragma unroll
+      for (int bi = 0; bi < MAXB; ++bi) {
+        const int b = wave + kWaves * bi;
+        if (b < a.nmb) {
+          nb = bi + 1;
+          const int2 bd = a.band[b];
+          f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+          if constexpr (PREC == PREC_F32) {
+            const float* fbrow = a.fb32 + (int64_t)(16 * b + ri) * a.fbs;
+            const float* pr = reinterpret_cast<const float*>(prow);
+            for (int c = 0; c < bd.y; ++c) {
+              const int k0 = bd.x + 16 * c + 4 * g;
+              const float4 av = *reinterpret_cast<const float4*>(fbrow + k0);
+              const float4 bv = *reinterpret_cast<const float4*>(pr + k0);
+              acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av.x, bv.x, acc, 0, 0, 0);
+              acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av.y, bv.y, acc, 0, 0, 0);
+              acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av.z, bv.z, acc, 0, 0, 0);
+              acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av.w, bv.w, acc, 0, 0, 0);
+            }
+          } else {
+            const uint16_t* fbrow = a.fb16 + (int64_t)(16 * b + ri) * a.fbs;
+            const uint16_t* pr = reinterpret_cast<const uint16_t*>(prow);
+            for (int c = 0; c < bd.y; ++c) {
+              const int k0 = bd.x + 32 * c + 8 * g;
+              const uint4 av = *reinterpret_cast<const uint4*>(fbrow + k0);
+              const uint4 bv = *reinterpret_cast<const uint4*>(pr + k0);
+              if constexpr (PREC == PREC_BF16) {
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, av),
+                                                              __builtin_bit_cast(bf16x8, bv), acc, 0, 0, 0);
+              } else {
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8, av),
+                                                             __builtin_bit_cast(f16x8, bv), acc, 0, 0, 0);
+              }
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[j] = ldexpf(acc[j], myexp);
+          }
+          // acc[j] = mel[16b + 4g + j] of tile frame ri
+          if constexpr (MODE == OUT_MEL) {
+            if (frame_ok) {
+              float* o = reinterpret_cast<float*>(a.out) + a.out_off[clip];
+#pragma unroll
+              for (int j = 0; j < 4; ++j) {
+                const int m = 16 * b + 4 * g + j;
+                if (m < a.n_mels) o[(int64_t)m * T + f0 + ri] = acc[j];
+              }
+            }
+          } else {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+              const int m = 16 * b + 4 * g + j;
+              float d = 10.0f * log10f(fmaxf(acc[j], a.amin) / a.ref);
+              d = fmaxf(d, thr);
+              if (m < a.n_mels && frame_ok) {
+                lmax = fmaxf(lmax, d);
+                lmin = fminf(lmin, d);
+              }
+              dbv[bi][j] = d;
+            }
+          }
+        }
+      }
+
+      if constexpr (MODE == OUT_MFCC) {
+        lmax = wave_max(lmax);
+        lmin = wave_min(lmin);
+        if (lane == 0) { red_max[wave] = lmax; red_min[wave] = lmin; }
+        __syncthreads();   // staging reads done; region memory is reused below
+        float* red = reinterpret_cast<float*>(lds);   // [ndt][8 waves][64 lanes][4]
+        for (int ct = 0; ct < a.ndt; ++ct) {
+          f32x4 accd = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+          for (int bi = 0; bi < MAXB; ++bi) {
+            if (bi < nb) {
+              const int b = wave + kWaves * bi;
+              if constexpr (PREC == PREC_F32) {
+                const float* dt = a.dct32 + ((int64_t)(ct * a.nmb + b) * 4) * 64;
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                  accd = __builtin_amdgcn_mfma_f32_16x16x4f32(dt[j * 64 + lane], dbv[bi][j], accd, 0, 0, 0);
+              } else {
+                const int64_t off = ((int64_t)(ct * a.nmb + b) * 64 + lane) * 4;
+                const uint2 hi2 = *reinterpret_cast<const uint2*>(a.dct_hi + off);
+                const uint2 lo2 = *reinterpret_cast<const uint2*>(a.dct_lo + off);
+                uint16_t bh[4], bl[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                  const float d = dbv[bi][j];
+                  if constexpr (PREC == PREC_BF16) {
+                    bh[j] = f2bf(d);
+                    bl[j] = f2bf(d - bf2f(bh[j]));
+                  } else {
+                    bh[j] = f2h(d);
+                    bl[j] = f2h(d - h2f(bh[j]));
+                  }
+                }
+                uint2 bhi2 = make_uint2(bh[0] | ((unsigned)bh[1] << 16), bh[2] | ((unsigned)bh[3] << 16));
+                uint2 blo2 = make_uint2(bl[0] | ((unsigned)bl[1] << 16), bl[2] | ((unsigned)bl[3] << 16));
+                if constexpr (PREC == PREC_BF16) {
+                  const s16x4 Ah = __builtin_bit_cast(s16x4, hi2), Al = __builtin_bit_cast(s16x4, lo2);
+                  const s16x4 Bh = __builtin_bit_cast(s16x4, bhi2), Bl = __builtin_bit_cast(s16x4, blo2);
+                  accd = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Ah, Bh, accd, 0, 0, 0);
+                  accd = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Al, Bh, accd, 0, 0, 0);
+                  accd = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Ah, Bl, accd, 0, 0, 0);
+                } else {
+                  const f16x4 Ah = __builtin_bit_cast(f16x4, hi2), Al = __builtin_bit_cast(f16x4, lo2);
+                  const f16x4 Bh = __builtin_bit_cast(f16x4, bhi2), Bl = __builtin_bit_cast(f16x4, blo2);
+                  accd = __builtin_amdgcn_mfma_f32_16x16x16f16(Ah, Bh, accd, 0, 0, 0);
+                  accd = __builtin_amdgcn_mfma_f32_16x16x16f16(Al, Bh, accd, 0, 0, 0);
+                  accd = __builtin_amdgcn_mfma_f32_16x16x16f16(Ah, Bl, accd, 0, 0, 0);
+                }
+              }
+            }
+          }
+          float* dst = red + ((int64_t)(ct * kWaves + wave) * 64 + lane) * 4;
+          *reinterpret_cast<float4*>(dst) = make_float4(accd[0], accd[1], accd[2], accd[3]);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0 && a.phase == 1 && a.use_top_db) {
+          float bm = -INFINITY, bn = INFINITY;
+          for (int w = 0; w < kWaves; ++w) { bm = fmaxf(bm, red_max[w]); bn = fminf(bn, red_min[w]); }
+          if (bm > -INFINITY) atomicMax(&a.clip_max_key[clip], float_to_key(bm));
+          a.tile_min[tile] = bn;
+        }
+        // fixed-order cross-wave sum; wave ct writes DCT rows 16ct .. 16ct+15
+        for (int ct = wave; ct < a.ndt; ct += kWaves) {
+          float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
+          for (int w = 0; w < kWaves; ++w) {
+            const float4 p = *reinterpret_cast<const float4*>(red + ((int64_t)(ct * kWaves + w) * 64 + lane) * 4);
+            s.x += p.x; s.y += p.y; s.z += p.z; s.w += p.w;
+          }
+          if (frame_ok) {
+            float* o = reinterpret_cast<float*>(a.out) + a.out_off[clip];
+            const float sv[4] = {s.x, s.y, s.z, s.w};
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+              const int c = 16 * ct + 4 * g + j;
+              if (c < a.n_mfcc) o[(int64_t)c * T + f0 + ri] = sv[j];
+            }
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// generic path: any power-of-two n_fft up to 16384, one clip, plain kernels
+// ---------------------------------------------------------------------------
+
+// One workgroup per frame: radix-2 Stockham on M complex points in LDS
+// (ping-pong), then the real split. Writes frame-major spectra spec[t][F]
+// (complex) into a scratch buffer.
+__global__ __launch_bounds__(256) void k_generic_fft(const float* __restrict__ x, int64_t len, int n_fft,
+                                                     int hop, int center, int pad_mode,
+                                                     const float* __restrict__ window,
+                                                     const float2* __restrict__ tw,
+                                                     const float2* __restrict__ twN, int n_frames,
+                                                     float2* __restrict__ spec) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  const int fr = blockIdx.x;
+  if (fr >= n_frames) return;
+  const int F = n_fft / 2 + 1;
+  const int64_t s0 = (int64_t)fr * hop - (center ? (n_fft / 2) : 0);
+  float2* out = spec + (int64_t)fr * F;
+  if (n_fft == 1) {
+    if (threadIdx.x == 0) {
+      const int64_t i0 = pad_index(s0, len, pad_mode);
+      out[0] = make_float2((i0 >= 0 ? x[i0] : 0.0f) * window[0], 0.0f);
+    }
+    return;
+  }
+  const int M = n_fft / 2;
+  float2* A = reinterpret_cast<float2*>(lds);
+  float2* B = A + M;
+  for (int n = threadIdx.x; n < M; n += blockDim.x) {
+    const int64_t i0 = pad_index(s0 + 2 * n, len, pad_mode);
+    const int64_t i1 = pad_index(s0 + 2 * n + 1, len, pad_mode);
+    const float x0 = (i0 >= 0) ? x[i0] : 0.0f, x1 = (i1 >= 0) ? x[i1] : 0.0f;
+    A[n] = make_float2(x0 * window[2 * n], x1 * window[2 * n + 1]);
+  }
+  __syncthreads();
+  // radix-2 Stockham: Ns = 1, 2, 4, ...
+  for (int Ns = 1; Ns < M; Ns <<= 1) {
+    for (int j = threadIdx.x; j < M / 2; j += blockDim.x) {
+      const int k = j % Ns;
+      float2 u = A[j], v = A[j + M / 2];
+      v = cmul(v, tw[k * (M / (2 * Ns))]);   // W_{2Ns}^k
+      const int base = (j / Ns) * Ns * 2 + k;
+      B[base] = cadd(u, v);
+      B[base + Ns] = csub(u, v);
+    }
+    __syncthreads();
+    float2* t = A; A = B; B = t;
+  }
+  for (int k = threadIdx.x; k <= M / 2; k += blockDim.x) {
+    const float2 Az = A[k];
+    const float2 Bz = A[(M - k) % M];
+    const float2 Bc = make_float2(Bz.x, -Bz.y);
+    const float2 E = make_float2(0.5f * (Az.x + Bc.x), 0.5f * (Az.y + Bc.y));
+    const float2 O = make_float2(0.5f * (Az.x - Bc.x), 0.5f * (Az.y - Bc.y));
+    const float2 T = cmul(O, twN[k]);
+    out[k] = make_float2(E.x + T.y, E.y - T.x);
+    out[M - k] = make_float2(E.x - T.y, -(E.y + T.x));
+  }
+}
+
+// spec [T][F] complex -> out [F, T] in the requested STFT form
+__global__ void k_generic_spec_out(const float2* __restrict__ spec, int F, int T, int kind, void* out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)F * T) return;
+  const int k = (int)(i / T), t = (int)(i % T);
+  const float2 X = spec[(int64_t)t * F + k];
+  if (kind == OUT_STFT_COMPLEX) reinterpret_cast<float2*>(out)[i] = X;
+  else if (kind == OUT_STFT_POWER) reinterpret_cast<float*>(out)[i] = fmaf(X.x, X.x, X.y * X.y);
+  else reinterpret_cast<float*>(out)[i] = sqrtf(fmaf(X.x, X.x, X.y * X.y));
+}
+
+// mel[m][t] = sum_k FB[m][k] * |X[t][k]|^power (dense, plain FP32)
+__global__ void k_generic_mel(const float2* __restrict__ spec, int F, int T, const float* __restrict__ fb,
+                              int fbs, int n_mels, float power, float* __restrict__ mel) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)n_mels * T) return;
+  const int m = (int)(i / T), t = (int)(i % T);
+  const float2* X = spec + (int64_t)t * F;
+  const float* w = fb + (int64_t)m * fbs;
+  float s = 0.0f;
+  for (int k = 0; k < F; ++k) {
+    const float wk = w[k];
+    if (wk != 0.0f) s = fmaf(wk, spec_value(X[k], power), s);
+  }
+  mel[i] = s;
+}
+
+// dB conversion with a running maximum (ordered-int atomic)
+__global__ void k_db(const float* __restrict__ in, int64_t n, float mult, float ref, float amin,
+                     float* __restrict__ out, unsigned* max_key) {
+  float lmax = -INFINITY;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const float d = mult * log10f(fmaxf(in[i], amin) / ref);
+    out[i] = d;
+    lmax = fmaxf(lmax, d);
+  }
+  lmax = wave_max(lmax);
+  if ((threadIdx.x & 63) == 0 && lmax > -INFINITY) atomicMax(max_key, float_to_key(lmax));
+}
+
+__global__ void k_db_clamp(float* __restrict__ x, int64_t n, const unsigned* max_key, float top_db) {
+  const float thr = key_to_float(*max_key) - top_db;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    x[i] = fmaxf(x[i], thr);
+}
+
+// out[c][t] = sum_m D[c][m] * logmel[m][t]
+__global__ void k_generic_dct(const float* __restrict__ logmel, int n_mels, int T, const float* __restrict__ dct,
+                              int n_mfcc, float* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)n_mfcc * T) return;
+  const int c = (int)(i / T), t = (int)(i % T);
+  float s = 0.0f;
+  for (int m = 0; m < n_mels; ++m) s = fmaf(dct[(int64_t)c * n_mels + m], logmel[(int64_t)m * T + t], s);
+  out[i] = s;
+}
+
+__global__ void k_fill_u32(unsigned* p, int64_t n, unsigned v) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// table generation in FP64 on the device, rounded once
+// ---------------------------------------------------------------------------
+
+__device__ inline double hz_to_mel_d(double hz) {
+  const double f_sp = 200.0 / 3.0, min_log_hz = 1000.0, min_log_mel = min_log_hz / f_sp;
+  const double logstep = log(6.4) / 27.0;
+  return (hz >= min_log_hz) ? min_log_mel + log(hz / min_log_hz) / logstep : hz / f_sp;
+}
+__device__ inline double mel_to_hz_d(double mel) {
+  const double f_sp = 200.0 / 3.0, min_log_hz = 1000.0, min_log_mel = min_log_hz / f_sp;
+  const double logstep = log(6.4) / 27.0;
+  return (mel < min_log_mel) ? mel * f_sp : min_log_hz * exp((mel - min_log_mel) * logstep);
+}
+
+__global__ void k_tab_window(float* w, int n_fft, int win) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= n_fft) return;
+  int m;
+  if (win < n_fft) m = n - (n_fft - win) / 2;
+  else m = n + (win - n_fft) / 2;
+  double v = 0.0;
+  if (win == 1) v = (m == 0) ? 1.0 : 0.0;
+  else if (win > 1 && m >= 0 && m < win) v = 0.5 - 0.5 * cospi(2.0 * (double)m / (double)win);
+  w[n] = (float)v;
+}
+
+__global__ void k_tab_twiddle(float2* tw, int M, float2* twN, int n_fft) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < M) tw[t] = make_float2((float)cospi(2.0 * t / M), (float)(-sinpi(2.0 * t / M)));
+  if (t <= M / 2) twN[t] = make_float2((float)cospi(2.0 * t / n_fft), (float)(-sinpi(2.0 * t / n_fft)));
+}
+
+__global__ void k_tab_mel(float* fb32, uint16_t* fbb, uint16_t* fbh, int fbs, int rows, int sr, int n_fft,
+                          int n_mels, double fmin, double fmax) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)rows * fbs) return;
+  const int m = (int)(i / fbs), k = (int)(i % fbs);
+  const int F = n_fft / 2 + 1;
+  double w = 0.0;
+  if (m < n_mels && k < F) {
+    const double mlo = hz_to_mel_d(fmin), mhi = hz_to_mel_d(fmax);
+    const double step = (mhi - mlo) / (double)(n_mels + 1);
+    const double fl = mel_to_hz_d(mlo + m * step), fc = mel_to_hz_d(mlo + (m + 1) * step),
+                 fr = mel_to_hz_d(mlo + (m + 2) * step);
+    const double f = (double)k * (double)sr / (double)n_fft;
+    if (f >= fl && f <= fc && fc != fl) w = (f - fl) / (fc - fl);
+    else if (f > fc && f <= fr && fr != fc) w = (fr - f) / (fr - fc);
+    w *= 2.0 / (fr - fl);
+  }
+  const float wf = (float)w;
+  fb32[i] = wf;
+  fbb[i] = f2bf(wf);
+  fbh[i] = f2h(wf);
+}
+
+// per block of 16 mel rows: first nonzero bin and chunk counts (16 and 32 bins)
+__global__ void k_tab_bands(const float* fb32, int fbs, int F, int nmb, int2* band32, int2* band16) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nmb) return;
+  int lo = F, hi = -1;
+  for (int r = 0; r < 16; ++r) {
+    const float* row = fb32 + (int64_t)(16 * b + r) * fbs;
+    for (int k = 0; k < F; ++k)
+      if (row[k] != 0.0f) { lo = min(lo, k); hi = max(hi, k); }
+  }
+  if (hi < 0) { band32[b] = make_int2(0, 0); band16[b] = make_int2(0, 0); return; }
+  const int s32 = lo & ~3, s16 = lo & ~7;
+  band32[b] = make_int2(s32, (hi + 1 - s32 + 15) / 16);
+  band16[b] = make_int2(s16, (hi + 1 - s16 + 31) / 32);
+}
+
+__device__ inline double dct_val(int c, int m, int n_mfcc, int n_mels) {
+  if (c >= n_mfcc || m >= n_mels) return 0.0;
+  const double norm = (c == 0) ? sqrt(1.0 / n_mels) : sqrt(2.0 / n_mels);
+  return cospi((double)c * (2.0 * m + 1.0) / (2.0 * n_mels)) * norm;
+}
+
+// dct32 [ndt][nmb][4][64]: lane l, step j -> D[16ct + l%16][16b + 4(l/16) + j]
+// dct16 [ndt][nmb][64][4]: lane l, elem j -> same entry (hi / lo split)
+__global__ void k_tab_dct(float* dct32, uint16_t* bhi, uint16_t* blo, uint16_t* hhi, uint16_t* hlo, float* dense,
+                          int ndt, int nmb, int n_mfcc, int n_mels) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t n = (int64_t)ndt * nmb * 256;
+  if (i < n) {
+    const int lane = (int)(i % 64);
+    const int j = (int)((i / 64) % 4);
+    const int b = (int)((i / 256) % nmb);
+    const int ct = (int)(i / (256 * (int64_t)nmb));
+    const int c = 16 * ct + (lane & 15), m = 16 * b + 4 * (lane >> 4) + j;
+    const double d = dct_val(c, m, n_mfcc, n_mels);
+    dct32[i] = (float)d;
+    const int64_t o = (((int64_t)ct * nmb + b) * 64 + lane) * 4 + j;
+    const uint16_t h = f2bf((float)d);
+    bhi[o] = h;
+    blo[o] = f2bf((float)(d - (double)bf2f(h)));
+    const uint16_t hh = f2h((float)d);
+    hhi[o] = hh;
+    hlo[o] = f2h((float)(d - (double)h2f(hh)));
+  }
+  if (i < (int64_t)n_mfcc * n_mels) {
+    const int c = (int)(i / n_mels), m = (int)(i % n_mels);
+    dense[i] = (float)dct_val(c, m, n_mfcc, n_mels);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host-side launchers
+// ---------------------------------------------------------------------------
+
+static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+hipError_t gen_tables(const TableKey& key, const DeviceTables& t, hipStream_t st) {
+  const int M = key.n_fft / 2;
+  const int F = key.n_fft / 2 + 1;
+  hipLaunchKernelGGL(k_tab_window, dim3(nblk(key.n_fft, 256)), dim3(256), 0, st, t.window, key.n_fft, key.win);
+  if (M >= 1)
+    hipLaunchKernelGGL(k_tab_twiddle, dim3(nblk(M, 256)), dim3(256), 0, st, t.tw, M, t.twN, key.n_fft);
+  const int rows = t.nmb * 16;
+  if (rows > 0) {
+    hipLaunchKernelGGL(k_tab_mel, dim3(nblk((int64_t)rows * t.fbs, 256)), dim3(256), 0, st, t.fb32, t.fb16, t.fbh,
+                       t.fbs, rows, key.sr, key.n_fft, key.n_mels, (double)key.fmin, (double)key.fmax);
+    hipLaunchKernelGGL(k_tab_bands, dim3(nblk(t.nmb, 64)), dim3(64), 0, st, t.fb32, t.fbs, F, t.nmb, t.band32,
+                       t.band16);
+  }
+  const int64_t nd = std::max<int64_t>((int64_t)t.ndt * t.nmb * 256, (int64_t)key.n_mfcc * key.n_mels);
+  if (nd > 0)
+    hipLaunchKernelGGL(k_tab_dct, dim3(nblk(nd, 256)), dim3(256), 0, st, t.dct32, t.dctb_hi, t.dctb_lo, t.dcth_hi,
+                       t.dcth_lo, t.dct_dense, t.ndt, t.nmb, key.n_mfcc, key.n_mels);
+  return hipGetLastError();
+}
+
+// ---- fused kernel dispatch ----
+
+template <int LOG2M, int MODE, int PREC, int FPW>
+static hipError_t launch_tile_t(const TileArgs& a, int grid, size_t lds, hipStream_t st) {
+  auto fn = k_tile<LOG2M, MODE, PREC, FPW>;
+  // set once per instantiation (thread-safe static init); grid == 0 only
+  // prepares the attribute, so stream capture never sees the first call
+  static const hipError_t attr = hipFuncSetAttribute(
+      reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_limit_bytes());
+  if (attr != hipSuccess) return attr;
+  if (grid <= 0) return hipSuccess;
+  hipLaunchKernelGGL(fn, dim3(grid), dim3(kWgThreads), lds, st, a);
+  return hipGetLastError();
+}
+
+template <int LOG2M>
+static hipError_t launch_tile_m(const TileArgs& a, int prec, int fpw, int grid, size_t lds, hipStream_t st) {
+  switch (a.out_kind) {
+    case OUT_STFT_MAG:
+      return fpw == 2 ? launch_tile_t<LOG2M, OUT_STFT_MAG, PREC_F32, 2>(a, grid, lds, st)
+                      : launch_tile_t<LOG2M, OUT_STFT_MAG, PREC_F32, 1>(a, grid, lds, st);
+    case OUT_STFT_POWER:
+      return fpw == 2 ? launch_tile_t<LOG2M, OUT_STFT_POWER, PREC_F32, 2>(a, grid, lds, st)
+                      : launch_tile_t<LOG2M, OUT_STFT_POWER, PREC_F32, 1>(a, grid, lds, st);
+    case OUT_STFT_COMPLEX:
+      return fpw == 2 ? launch_tile_t<LOG2M, OUT_STFT_COMPLEX, PREC_F32, 2>(a, grid, lds, st)
+                      : launch_tile_t<LOG2M, OUT_STFT_COMPLEX, PREC_F32, 1>(a, grid, lds, st);
+    case OUT_MEL:
+      if (prec == PREC_BF16) return launch_tile_t<LOG2M, OUT_MEL, PREC_BF16, 2>(a, grid, lds, st);
+      if (prec == PREC_F16) return launch_tile_t<LOG2M, OUT_MEL, PREC_F16, 2>(a, grid, lds, st);
+      return launch_tile_t<LOG2M, OUT_MEL, PREC_F32, 2>(a, grid, lds, st);
+    case OUT_MFCC:
+      if (prec == PREC_BF16) return launch_tile_t<LOG2M, OUT_MFCC, PREC_BF16, 2>(a, grid, lds, st);
+      if (prec == PREC_F16) return launch_tile_t<LOG2M, OUT_MFCC, PREC_F16, 2>(a, grid, lds, st);
+      return launch_tile_t<LOG2M, OUT_MFCC, PREC_F32, 2>(a, grid, lds, st);
+  }
+  return hipErrorInvalidValue;
+}
+
+static size_t g_lds_limit = 64 * 1024;
+void set_lds_limit_bytes(size_t n) { g_lds_limit = n; }
+size_t lds_limit_bytes() { return g_lds_limit; }
+
+bool tile_layout(int n_fft, int out_kind, int prec, int n_mels, int n_mfcc, TileLayout* L) {
+  if (n_fft < 256 || n_fft > 4096 || (n_fft & (n_fft - 1))) return false;
+  const int M = n_fft / 2, F = M + 1;
+  const bool spec = out_kind <= OUT_STFT_COMPLEX;
+  if (!spec) {
+    if (n_mels < 1 || n_mels > 16 * kWaves * 4) return false;
+    if (out_kind == OUT_MFCC && (n_mfcc < 1 || n_mfcc > 128)) return false;
+  }
+  const int scr = (M + M / 16) * 8;
+  int esz, S;
+  if (spec) {
+    esz = (out_kind == OUT_STFT_COMPLEX) ? 8 : 4;
+    S = F + ((out_kind == OUT_STFT_COMPLEX) ? 1 : 1);
+  } else if (prec == PREC_F32) {
+    esz = 4;
+    const int need = ((F + 31) / 32) * 32 + 64;
+    S = need + ((4 - need % 64) + 64) % 64;          // S = 4 (mod 64) dwords
+  } else {
+    esz = 2;
+    const int need = ((F + 31) / 32) * 32 + 64;
+    S = need + ((8 - need % 128) + 128) % 128;       // S/2 = 4 (mod 64) dwords
+  }
+  int fpw = 2;
+  auto region = [&](int f) {
+    int rb = std::max(f * S * esz, scr);
+    return (rb + 15) / 16 * 16;
+  };
+  int rb = region(fpw);
+  if (spec && (size_t)kWaves * rb + 256 > lds_limit_bytes()) {
+    fpw = 1;
+    rb = region(fpw);
+  }
+  const size_t total = (size_t)kWaves * rb + 16 * 4 + 2 * kWaves * 4;
+  if (total > lds_limit_bytes()) return false;
+  // the MFCC cross-wave reduction reuses the regions
+  if (out_kind == OUT_MFCC) {
+    const int ndt = (n_mfcc + 15) / 16;
+    if ((size_t)ndt * kWaves * 256 * 4 > (size_t)kWaves * rb) return false;
+  }
+  L->fpw = fpw;
+  L->row_stride = S;
+  L->region_bytes = rb;
+  L->lds_bytes = total;
+  L->tile_frames = kWaves * fpw;
+  return true;
+}
+
+hipError_t launch_tile(const TileArgs& a, int prec, const TileLayout& L, int grid, hipStream_t st) {
+  const int log2m = __builtin_ctz(a.n_fft) - 1;
+#ifdef MM_DEV_ONLY_LOG2M  // development builds: one FFT size only
+  if (log2m == MM_DEV_ONLY_LOG2M) return launch_tile_m<MM_DEV_ONLY_LOG2M>(a, prec, L.fpw, grid, L.lds_bytes, st);
+  return hipErrorInvalidValue;
+#else
+  switch (log2m) {
+    case 7: return launch_tile_m<7>(a, prec, L.fpw, grid, L.lds_bytes, st);
+    case 8: return launch_tile_m<8>(a, prec, L.fpw, grid, L.lds_bytes, st);
+    case 9: return launch_tile_m<9>(a, prec, L.fpw, grid, L.lds_bytes, st);
+    case 10: return launch_tile_m<10>(a, prec, L.fpw, grid, L.lds_bytes, st);
+    case 11: return launch_tile_m<11>(a, prec, L.fpw, grid, L.lds_bytes, st);
+  }
+  return hipErrorInvalidValue;
+#endif
+}
+
+hipError_t launch_fill_u32(unsigned* p, int64_t n, unsigned v, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_u32, dim3(std::min<unsigned>(nblk(n, 256), 1024)), dim3(256), 0, st, p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_generic_fft(const float* x, int64_t len, int n_fft, int hop, int center, int pad_mode,
+                              const float* window, const float2* tw, const float2* twN, int T, float2* spec,
+                              hipStream_t st) {
+  if (T <= 0) return hipSuccess;
+  const size_t lds = (n_fft >= 2) ? (size_t)n_fft * 8 : 16;
+  static const hipError_t attr = hipFuncSetAttribute(
+      reinterpret_cast<const void*>(k_generic_fft), hipFuncAttributeMaxDynamicSharedMemorySize,
+      (int)lds_limit_bytes());
+  if (attr != hipSuccess) return attr;
+  hipLaunchKernelGGL(k_generic_fft, dim3(T), dim3(256), lds, st, x, len, n_fft, hop, center, pad_mode, window, tw,
+                     twN, T, spec);
+  return hipGetLastError();
+}
+
+hipError_t launch_generic_spec_out(const float2* spec, int F, int T, int kind, void* out, hipStream_t st) {
+  const int64_t n = (int64_t)F * T;
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_generic_spec_out, dim3(nblk(n, 256)), dim3(256), 0, st, spec, F, T, kind, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_generic_mel(const float2* spec, int F, int T, const float* fb, int fbs, int n_mels, float power,
+                              float* mel, hipStream_t st) {
+  const int64_t n = (int64_t)n_mels * T;
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_generic_mel, dim3(nblk(n, 256)), dim3(256), 0, st, spec, F, T, fb, fbs, n_mels, power, mel);
+  return hipGetLastError();
+}
+
+hipError_t launch_db(const float* in, int64_t n, float mult, float ref, float amin, float* out, unsigned* max_key,
+                     hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_db, dim3(std::min<unsigned>(nblk(n, 256), 2048)), dim3(256), 0, st, in, n, mult, ref, amin,
+                     out, max_key);
+  return hipGetLastError();
+}
+
+hipError_t launch_db_clamp(float* x, int64_t n, const unsigned* max_key, float top_db, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_db_clamp, dim3(std::min<unsigned>(nblk(n, 256), 2048)), dim3(256), 0, st, x, n, max_key,
+                     top_db);
+  return hipGetLastError();
+}
+
+hipError_t launch_generic_dct(const float* logmel, int n_mels, int T, const float* dct, int n_mfcc, float* out,
+                              hipStream_t st) {
+  const int64_t n = (int64_t)n_mfcc * T;
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_generic_dct, dim3(nblk(n, 256)), dim3(256), 0, st, logmel, n_mels, T, dct, n_mfcc, out);
+  return hipGetLastError();
+}
+
+}  // namespace mm

@@ -1,0 +1,105 @@
+"""Feature entry points, mirroring the reference ``python/metalmom/feature.py``.
+
+``melspectrogram`` (``feature.py:8-91``), ``mfcc`` (``:94-167``),
+``amplitude_to_db`` (``:170-220``) and ``power_to_db`` (``:223-270``) keep the
+reference signatures, defaults (hop 512 for mel/MFCC, ``n_mfcc=20``) and error
+behaviour (``RuntimeError("mm_... failed with status s")``). Audio input runs
+on the GPU through libmetalmom.so. When a precomputed ``S`` is given, the
+reference applies the filterbank / DCT in NumPy/SciPy (``feature.py:50-57,
+130-134``) and so does this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._buffer import buffer_to_numpy
+from ._native import MMBuffer, float_ptr, lib
+
+
+def melspectrogram(y=None, sr=22050, S=None, n_fft=2048, hop_length=None, win_length=None,
+                   center=True, power=2.0, n_mels=128, fmin=0.0, fmax=None, **kwargs):
+    """Mel spectrogram ``(n_mels, n_frames)``: ``FB @ |STFT|^power``."""
+    if S is not None:
+        from ._mel_fb import mel_filterbank
+        S = np.ascontiguousarray(S, dtype=np.float32)
+        fb = mel_filterbank(sr=sr, n_fft=n_fft, n_mels=n_mels, fmin=fmin, fmax=fmax)
+        return fb @ S
+    if y is None:
+        raise ValueError("Either y or S must be provided")
+    y = np.ascontiguousarray(y, dtype=np.float32)
+    hop = hop_length if hop_length is not None else 512
+    win = win_length if win_length is not None else n_fft
+    c_fmax = float(fmax) if fmax is not None else 0.0
+    ctx = lib.mm_init()
+    if not ctx:
+        raise RuntimeError("Failed to initialize MetalMom context")
+    try:
+        out = MMBuffer()
+        status = lib.mm_mel_spectrogram(ctx, float_ptr(y), len(y), sr, n_fft, hop, win,
+                                        1 if center else 0, power, n_mels, fmin, c_fmax,
+                                        C.byref(out))
+        if status != 0:
+            raise RuntimeError(f"mm_mel_spectrogram failed with status {status}")
+        return buffer_to_numpy(out)
+    finally:
+        lib.mm_destroy(ctx)
+
+
+def mfcc(y=None, sr=22050, S=None, n_mfcc=20, n_fft=2048, hop_length=None, win_length=None,
+         n_mels=128, fmin=0.0, fmax=None, center=True, **kwargs):
+    """MFCCs ``(n_mfcc, n_frames)``: DCT-II (ortho) of ``power_to_db(mel)``."""
+    if S is not None:
+        from scipy.fftpack import dct
+        S = np.ascontiguousarray(S, dtype=np.float32)
+        return dct(S, axis=0, type=2, norm="ortho")[:n_mfcc]
+    if y is None:
+        raise ValueError("Either y or S must be provided")
+    y = np.ascontiguousarray(y, dtype=np.float32)
+    hop = hop_length if hop_length is not None else 512
+    win = win_length if win_length is not None else n_fft
+    c_fmax = float(fmax) if fmax is not None else 0.0
+    ctx = lib.mm_init()
+    if not ctx:
+        raise RuntimeError("Failed to initialize MetalMom context")
+    try:
+        out = MMBuffer()
+        status = lib.mm_mfcc(ctx, float_ptr(y), len(y), sr, n_mfcc, n_fft, hop, win, n_mels, fmin,
+                             c_fmax, 1 if center else 0, C.byref(out))
+        if status != 0:
+            raise RuntimeError(f"mm_mfcc failed with status {status}")
+        return buffer_to_numpy(out)
+    finally:
+        lib.mm_destroy(ctx)
+
+
+def _db(fn_name, S, ref, amin, top_db):
+    S = np.ascontiguousarray(S, dtype=np.float32)
+    shape = S.shape
+    if callable(ref):
+        ref = float(ref(S))
+    flat = S.ravel()
+    ctx = lib.mm_init()
+    if not ctx:
+        raise RuntimeError("Failed to initialize MetalMom context")
+    try:
+        out = MMBuffer()
+        c_top_db = float(top_db) if top_db is not None else 0.0
+        status = getattr(lib, fn_name)(ctx, float_ptr(flat), len(flat), float(ref), float(amin),
+                                       c_top_db, C.byref(out))
+        if status != 0:
+            raise RuntimeError(f"{fn_name} failed with status {status}")
+        return buffer_to_numpy(out).reshape(shape)
+    finally:
+        lib.mm_destroy(ctx)
+
+
+def amplitude_to_db(S, ref=1.0, amin=1e-5, top_db=80.0):
+    """``20 log10(max(S, amin) / ref)``, clipped to ``max - top_db``."""
+    return _db("mm_amplitude_to_db", S, ref, amin, top_db)
+
+
+def power_to_db(S, ref=1.0, amin=1e-10, top_db=80.0):
+    """``10 log10(max(S, amin) / ref)``, clipped to ``max - top_db``."""
+    return _db("mm_power_to_db", S, ref, amin, top_db)

@@ -1,0 +1,295 @@
+"""Float64 host oracle for the spectral front end (STFT -> power -> mel -> dB -> DCT-II).
+
+TEST INFRASTRUCTURE ONLY. Nothing in the shipped library imports this module:
+only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` may use it, and only as the checker / the timed CPU restatement.
+
+Every function restates the reference algorithm (zakkeown/MetalMom) in float64
+NumPy and cites the Swift source it follows. How each function is pinned
+(golden vectors, closed forms, brute force) is listed in ``PINS`` at the end of
+this file and exercised by ``tests/test_oracle.py``.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+__all__ = [
+    "hann_periodic", "prepare_window", "frame_count", "pad_signal",
+    "stft_complex", "stft_magnitude", "power_spectrogram",
+    "hz_to_mel", "mel_to_hz", "mel_filterbank", "melspectrogram",
+    "linear_to_db", "power_to_db", "amplitude_to_db",
+    "dct_ortho_basis", "mfcc", "mfcc_from_logmel",
+]
+
+
+# ---------------------------------------------------------------------------
+# Window and framing
+# ---------------------------------------------------------------------------
+
+def hann_periodic(length: int) -> np.ndarray:
+    """Periodic Hann window, ``w[n] = 0.5 - 0.5 cos(2 pi n / length)``.
+
+    Follows ``Sources/MetalMomCore/Filters/Windows.swift:14-48``: a symmetric
+    window of length ``length + 1`` with the last sample dropped; length 0 gives
+    an empty window and length 1 gives ``[1.0]``.
+    """
+    if length <= 0:
+        return np.zeros(0, dtype=np.float64)
+    if length == 1:
+        return np.ones(1, dtype=np.float64)
+    n = np.arange(length, dtype=np.float64)
+    return 0.5 - 0.5 * np.cos(2.0 * np.pi * n / length)
+
+
+def prepare_window(win_length: int, n_fft: int) -> np.ndarray:
+    """Full ``n_fft``-sample analysis window.
+
+    Follows ``Sources/MetalMomCore/Spectral/STFT.swift:44-57``: a shorter window
+    is zero-padded with ``(n_fft - win)//2`` zeros in front, a longer one is cut
+    to its centre ``n_fft`` samples. A non-positive ``win_length`` yields an
+    all-zero window (``Windows.hann`` returns ``[]`` there and the padding fills
+    the frame with zeros).
+    """
+    w = hann_periodic(win_length)
+    if win_length < n_fft:
+        before = (n_fft - win_length) // 2
+        out = np.zeros(n_fft, dtype=np.float64)
+        if win_length > 0:
+            out[before:before + win_length] = w
+        return out
+    if win_length > n_fft:
+        start = (win_length - n_fft) // 2
+        return w[start:start + n_fft].copy()
+    return w
+
+
+def frame_count(n_samples: int, n_fft: int, hop: int, center: bool) -> int:
+    """Number of STFT frames, ``1 + (padded - n_fft) // hop`` or 0.
+
+    Follows ``STFT.swift:240-254`` (also ``:79-98``): with ``center`` the signal
+    is padded by ``n_fft//2`` on both sides; a padded length below ``n_fft``
+    gives no frame.
+    """
+    padded = n_samples + (2 * (n_fft // 2) if center else 0)
+    if padded < n_fft:
+        return 0
+    return 1 + (padded - n_fft) // hop
+
+
+def pad_signal(y: np.ndarray, n_fft: int, center: bool, pad_mode: str = "constant") -> np.ndarray:
+    """Centre padding by ``n_fft//2`` samples per side.
+
+    ``constant`` (zeros) is the reference's behaviour (``STFT.swift:81-85,
+    242-246``). ``reflect`` follows ``numpy.pad(mode="reflect")`` and is an
+    addition of this build (the reference has no reflect framing).
+    """
+    y = np.asarray(y, dtype=np.float64)
+    if not center:
+        return y.copy()
+    p = n_fft // 2
+    if pad_mode == "constant":
+        return np.pad(y, (p, p), mode="constant")
+    if pad_mode == "reflect":
+        return np.pad(y, (p, p), mode="reflect")
+    raise ValueError(f"unknown pad_mode {pad_mode!r}")
+
+
+# ---------------------------------------------------------------------------
+# STFT
+# ---------------------------------------------------------------------------
+
+def stft_complex(y, n_fft=2048, hop=None, win_length=None, center=True,
+                 pad_mode="constant") -> np.ndarray:
+    """Complex STFT, shape ``[n_fft//2+1, n_frames]`` (row-major, bins x frames).
+
+    Restates ``STFT.swift:512-668`` (``computeComplex``): frame ``f`` is
+    ``pad(y)[f*hop : f*hop+n_fft] * window`` and bin ``k`` is the plain forward
+    DFT ``sum_n x[n] exp(-2 pi i k n / n_fft)`` (vDSP's factor 2 is undone at
+    ``STFT.swift:308-310``). Computed here with ``numpy.fft.rfft`` in float64.
+    """
+    hop = n_fft // 4 if hop is None else hop
+    win_length = n_fft if win_length is None else win_length
+    y = np.asarray(y, dtype=np.float64)
+    n_freqs = n_fft // 2 + 1
+    n_frames = frame_count(len(y), n_fft, hop, center)
+    if n_frames == 0:
+        return np.zeros((n_freqs, 0), dtype=np.complex128)
+    x = pad_signal(y, n_fft, center, pad_mode)
+    w = prepare_window(win_length, n_fft)
+    idx = np.arange(n_frames)[:, None] * hop + np.arange(n_fft)[None, :]
+    frames = x[idx] * w[None, :]
+    return np.fft.rfft(frames, n=n_fft, axis=1).T.copy()
+
+
+def stft_magnitude(y, n_fft=2048, hop=None, win_length=None, center=True,
+                   pad_mode="constant") -> np.ndarray:
+    """``|STFT|``, the output of ``mm_stft`` (``STFT.swift:73-215``)."""
+    return np.abs(stft_complex(y, n_fft, hop, win_length, center, pad_mode))
+
+
+def power_spectrogram(y, n_fft=2048, hop=None, win_length=None, center=True,
+                      pad_mode="constant") -> np.ndarray:
+    """``re^2 + im^2`` per bin (``STFT.swift:227-340``)."""
+    X = stft_complex(y, n_fft, hop, win_length, center, pad_mode)
+    return X.real ** 2 + X.imag ** 2
+
+
+# ---------------------------------------------------------------------------
+# Mel filterbank
+# ---------------------------------------------------------------------------
+
+_F_SP = 200.0 / 3.0
+_MIN_LOG_HZ = 1000.0
+_MIN_LOG_MEL = _MIN_LOG_HZ / _F_SP
+_LOGSTEP = np.log(6.4) / 27.0
+
+
+def hz_to_mel(hz):
+    """Slaney mel scale, ``Convert/Units.swift:55-67`` (float64)."""
+    hz = np.asarray(hz, dtype=np.float64)
+    lin = hz / _F_SP
+    with np.errstate(divide="ignore", invalid="ignore"):
+        log = _MIN_LOG_MEL + np.log(hz / _MIN_LOG_HZ) / _LOGSTEP
+    return np.where(hz >= _MIN_LOG_HZ, log, lin)
+
+
+def mel_to_hz(mel):
+    """Inverse Slaney mel scale, ``Convert/Units.swift:74-85`` (float64)."""
+    mel = np.asarray(mel, dtype=np.float64)
+    lin = mel * _F_SP
+    log = _MIN_LOG_HZ * np.exp((mel - _MIN_LOG_MEL) * _LOGSTEP)
+    return np.where(mel < _MIN_LOG_MEL, lin, log)
+
+
+def mel_filterbank(sr, n_fft, n_mels=128, fmin=0.0, fmax=None) -> np.ndarray:
+    """Slaney-normalised triangular mel filterbank ``[n_mels, n_fft//2+1]``.
+
+    Follows ``Filters/FilterBank.swift:23-73``: ``n_mels+2`` mel points equally
+    spaced between ``hz_to_mel(fmin)`` and ``hz_to_mel(fmax)``; bin ``k`` at
+    ``k*sr/n_fft`` Hz gets the rising edge ``(f-fl)/(fc-fl)`` for
+    ``fl <= f <= fc`` (when ``fc != fl``), else the falling edge
+    ``(fr-f)/(fr-fc)`` for ``fc < f <= fr``; each row is scaled by
+    ``2/(fr-fl)``. Everything here is float64 (the reference rounds the mel
+    points and bin frequencies to float32 first; librosa, which made the
+    goldens, stays in float64).
+    """
+    fmax = sr / 2.0 if fmax is None else float(fmax)
+    n_freqs = n_fft // 2 + 1
+    if n_mels <= 0:
+        return np.zeros((0, n_freqs), dtype=np.float64)
+    mel_lo, mel_hi = hz_to_mel(float(fmin)), hz_to_mel(fmax)
+    pts = mel_to_hz(mel_lo + np.arange(n_mels + 2, dtype=np.float64) * (mel_hi - mel_lo) / (n_mels + 1))
+    freqs = np.arange(n_freqs, dtype=np.float64) * sr / n_fft
+    fb = np.zeros((n_mels, n_freqs), dtype=np.float64)
+    for m in range(n_mels):
+        fl, fc, fr = pts[m], pts[m + 1], pts[m + 2]
+        up = (freqs >= fl) & (freqs <= fc)
+        down = (freqs > fc) & (freqs <= fr)
+        if fc != fl:
+            fb[m, up] = (freqs[up] - fl) / (fc - fl)
+        if fr != fc:
+            # the reference tests the rising edge first, so a bin on it never
+            # takes the falling value (the two masks are disjoint here anyway)
+            fb[m, down] = (fr - freqs[down]) / (fr - fc)
+        fb[m] *= 2.0 / (fr - fl)
+    return fb
+
+
+def melspectrogram(y, sr=22050, n_fft=2048, hop=512, win_length=None, center=True,
+                   power=2.0, n_mels=128, fmin=0.0, fmax=None, pad_mode="constant") -> np.ndarray:
+    """``FB @ |STFT|^power``, shape ``[n_mels, n_frames]``.
+
+    Follows ``Spectral/MelSpectrogram.swift:28-142``: power 2 uses
+    ``re^2+im^2`` directly, power 1 the magnitude, anything else
+    ``magnitude ** power``.
+    """
+    X = stft_complex(y, n_fft, hop, win_length, center, pad_mode)
+    if power == 2.0:
+        S = X.real ** 2 + X.imag ** 2
+    elif power == 1.0:
+        S = np.abs(X)
+    else:
+        S = np.abs(X) ** power
+    return mel_filterbank(sr, n_fft, n_mels, fmin, fmax) @ S
+
+
+# ---------------------------------------------------------------------------
+# dB scaling
+# ---------------------------------------------------------------------------
+
+def linear_to_db(S, multiplier, ref=1.0, amin=1e-10, top_db=80.0) -> np.ndarray:
+    """``multiplier*log10(max(S, amin)/ref)``, then ``max(x, max(x) - top_db)``.
+
+    Follows ``Spectral/Scaling.swift:73-118``: the clamp floor is the maximum
+    over the whole array (one scalar per call); ``top_db=None`` skips it.
+    """
+    S = np.asarray(S, dtype=np.float64)
+    out = multiplier * np.log10(np.maximum(S, amin) / ref)
+    if top_db is not None and out.size:
+        out = np.maximum(out, out.max() - top_db)
+    return out
+
+
+def power_to_db(S, ref=1.0, amin=1e-10, top_db=80.0) -> np.ndarray:
+    """``Scaling.powerToDb`` (``Spectral/Scaling.swift:39-42``)."""
+    return linear_to_db(S, 10.0, ref, amin, top_db)
+
+
+def amplitude_to_db(S, ref=1.0, amin=1e-5, top_db=80.0) -> np.ndarray:
+    """``Scaling.amplitudeToDb`` (``Spectral/Scaling.swift:23-26``)."""
+    return linear_to_db(S, 20.0, ref, amin, top_db)
+
+
+# ---------------------------------------------------------------------------
+# MFCC
+# ---------------------------------------------------------------------------
+
+def dct_ortho_basis(n_mfcc: int, n_mels: int) -> np.ndarray:
+    """Orthonormal DCT-II basis ``[n_mfcc, n_mels]``.
+
+    ``basis[k, n] = cos(pi k (2n+1) / (2N)) * (sqrt(1/N) if k == 0 else sqrt(2/N))``
+    as in ``Features/MFCC.swift:79-88`` (float64 here).
+    """
+    N = n_mels
+    k = np.arange(n_mfcc, dtype=np.float64)[:, None]
+    n = np.arange(N, dtype=np.float64)[None, :]
+    basis = np.cos(np.pi * k * (2.0 * n + 1.0) / (2.0 * N))
+    norm = np.full((n_mfcc, 1), np.sqrt(2.0 / N))
+    if n_mfcc:
+        norm[0, 0] = np.sqrt(1.0 / N)
+    return basis * norm
+
+
+def mfcc_from_logmel(logmel: np.ndarray, n_mfcc: int) -> np.ndarray:
+    """DCT-II (ortho) along the mel axis, truncated to ``min(n_mfcc, n_mels)``."""
+    n_mels = logmel.shape[0]
+    k = min(n_mfcc, n_mels)
+    return dct_ortho_basis(k, n_mels) @ logmel
+
+
+def mfcc(y, sr=22050, n_mfcc=20, n_fft=2048, hop=512, win_length=None, n_mels=128,
+         fmin=0.0, fmax=None, center=True, top_db=80.0, pad_mode="constant") -> np.ndarray:
+    """MFCC ``[min(n_mfcc, n_mels), n_frames]``.
+
+    Follows ``Features/MFCC.swift:28-110``: mel spectrogram with power 2,
+    ``powerToDb`` with ``ref=1, amin=1e-10, top_db=80`` over the whole clip,
+    then the orthonormal DCT-II basis product.
+    """
+    M = melspectrogram(y, sr, n_fft, hop, win_length, center, 2.0, n_mels, fmin, fmax, pad_mode)
+    logmel = power_to_db(M, 1.0, 1e-10, top_db)
+    return mfcc_from_logmel(logmel, n_mfcc)
+
+
+# How each oracle function is pinned (checked by tests/test_oracle.py).
+PINS = {
+    "hann_periodic": "known answer hann(4) = [0, .5, 1, .5] (WindowTests.swift:60-66); closed form",
+    "prepare_window": "centre zero-pad / centre truncation indices checked by hand on small cases",
+    "frame_count": "22050 -> 44 (STFTTests.swift:8-24), 661500 -> 1292, empty -> 1/0 (EdgeCaseTests.swift:69-81)",
+    "stft_complex": "golden stft_440hz_default_magnitude.npy; O(N^2) brute-force DFT for n_fft <= 256; Parseval",
+    "mel_filterbank": "goldens mel_filterbank_{128_2048,40_1024,64_2048_300_8000}.npy",
+    "melspectrogram": "goldens mel_spectrogram_440hz_*.npy",
+    "power_to_db": "golden stft_440hz_power_db.npy; 0.5 -> -3.0103 dB, 100 -> 20 dB (ScalingTests.swift:39-56)",
+    "amplitude_to_db": "goldens stft_440hz_amplitude_db{,_ref1}.npy",
+    "dct_ortho_basis": "scipy.fft.dct(type=2, norm='ortho'); orthonormality",
+    "mfcc": "goldens mfcc_440hz_*.npy",
+}

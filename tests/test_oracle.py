@@ -1,0 +1,189 @@
+"""Pins the float64 oracle before anything is checked against it.
+
+Each oracle function is checked against the reference's librosa golden vectors
+(Tests/golden, copied to tests/golden) and against what the mathematics fixes
+(known answers, brute-force DFT, Parseval, scipy's DCT, invariants), chosen so
+that a dropped term, wrong sign, wrong index or transposed operand fails.
+"""
+import numpy as np
+import pytest
+import scipy.fft
+
+from conftest import load_golden
+from oracle import spectral as o
+
+
+# -- window and framing ------------------------------------------------------
+
+def test_hann_known_answer():
+    # WindowTests.swift:60-66
+    np.testing.assert_allclose(o.hann_periodic(4), [0.0, 0.5, 1.0, 0.5], atol=1e-15)
+    assert o.hann_periodic(0).size == 0
+    np.testing.assert_array_equal(o.hann_periodic(1), [1.0])
+
+
+def test_prepare_window_pad_and_truncate():
+    w = o.prepare_window(4, 8)          # centred: 2 zeros, hann(4), 2 zeros
+    np.testing.assert_allclose(w, [0, 0, 0, 0.5, 1, 0.5, 0, 0], atol=1e-15)
+    w = o.prepare_window(5, 8)          # (8-5)//2 = 1 zero in front
+    np.testing.assert_allclose(w[:1], [0]); np.testing.assert_allclose(w[1:6], o.hann_periodic(5))
+    w = o.prepare_window(8, 4)          # centre 4 of hann(8): indices 2..5
+    np.testing.assert_allclose(w, o.hann_periodic(8)[2:6])
+    assert not o.prepare_window(0, 8).any() and not o.prepare_window(-3, 8).any()
+
+
+def test_frame_counts_known_answers():
+    assert o.frame_count(22050, 2048, 512, True) == 44        # STFTTests.swift:8-24
+    assert o.frame_count(661500, 2048, 512, True) == 1292     # 30 s at 22.05 kHz
+    assert o.frame_count(0, 2048, 512, True) == 1             # EdgeCaseTests.swift:69-81
+    assert o.frame_count(0, 2048, 512, False) == 0
+    assert o.frame_count(2047, 2048, 512, False) == 0
+    assert o.frame_count(2048, 2048, 512, False) == 1
+    assert o.frame_count(441000, 4096, 1024, True) == 431
+
+
+def test_reflect_pad_matches_numpy():
+    y = np.arange(5.0)
+    np.testing.assert_array_equal(o.pad_signal(y, 4, True, "reflect"), np.pad(y, 2, mode="reflect"))
+
+
+# -- STFT --------------------------------------------------------------------
+
+def test_stft_golden(signal_440):
+    G = load_golden("stft_440hz_default_magnitude")
+    M = o.stft_magnitude(signal_440, 2048, 512)
+    assert M.shape == G.shape == (1025, 44)
+    np.testing.assert_allclose(M, G, rtol=1e-4, atol=1e-4)   # test_parity_stft.py tier
+    assert np.abs(M - G).max() / G.max() < 1e-6
+
+
+@pytest.mark.parametrize("n_fft", [2, 4, 8, 32, 256])
+def test_stft_bruteforce_dft(n_fft):
+    rng = np.random.default_rng(n_fft)
+    y = rng.standard_normal(3 * n_fft + 5)
+    hop = max(1, n_fft // 4)
+    X = o.stft_complex(y, n_fft, hop, n_fft, True)
+    xp = np.pad(y, n_fft // 2)
+    w = 0.5 - 0.5 * np.cos(2 * np.pi * np.arange(n_fft) / n_fft)
+    k = np.arange(n_fft // 2 + 1)[:, None]
+    n = np.arange(n_fft)[None, :]
+    D = np.exp(-2j * np.pi * k * n / n_fft)
+    for f in range(X.shape[1]):
+        ref = D @ (xp[f * hop:f * hop + n_fft] * w)
+        np.testing.assert_allclose(X[:, f], ref, atol=1e-10 * max(1, np.abs(ref).max()))
+
+
+def test_stft_parseval_and_symmetries():
+    rng = np.random.default_rng(7)
+    y = rng.standard_normal(5000)
+    n = 512
+    X = o.stft_complex(y, n, 128, n, True)
+    xp = np.pad(y, n // 2)
+    w = o.hann_periodic(n)
+    for f in (0, 5, X.shape[1] - 1):
+        fr = xp[f * 128:f * 128 + n] * w
+        full = np.concatenate([X[:, f], np.conj(X[1:-1, f][::-1])])
+        assert np.isclose(np.sum(np.abs(full) ** 2) / n, np.sum(fr ** 2))
+    assert np.allclose(X[0].imag, 0) and np.allclose(X[-1].imag, 0)
+
+
+def test_stft_tone_peak_bin():
+    sr, n = 22050, 2048
+    y = np.sin(2 * np.pi * 440 * np.arange(sr) / sr)
+    M = o.stft_magnitude(y, n, 512)
+    assert np.argmax(M[:, 20]) == round(440 * n / sr)          # STFTTests.swift:28-66
+
+
+# -- mel ---------------------------------------------------------------------
+
+@pytest.mark.parametrize("name,args", [
+    ("mel_filterbank_128_2048", (22050, 2048, 128)),
+    ("mel_filterbank_40_1024", (22050, 1024, 40)),
+    ("mel_filterbank_64_2048_300_8000", (22050, 2048, 64, 300, 8000)),
+])
+def test_mel_filterbank_golden(name, args):
+    G = load_golden(name)
+    fb = o.mel_filterbank(*args)
+    assert fb.shape == G.shape
+    np.testing.assert_allclose(fb, G, rtol=1e-5, atol=1e-6)   # test_parity_mel_filterbank.py tier
+    assert np.abs(fb - G).max() < 1e-8
+
+
+def test_mel_scale_closed_form():
+    assert np.isclose(o.hz_to_mel(1000.0), 15.0)
+    assert np.isclose(o.hz_to_mel(200.0), 3.0)
+    assert np.isclose(o.mel_to_hz(o.hz_to_mel(3456.0)), 3456.0)
+
+
+@pytest.mark.parametrize("name,kw", [
+    ("mel_spectrogram_440hz_default", {}),
+    ("mel_spectrogram_440hz_40_1024", dict(n_fft=1024, n_mels=40)),
+    ("mel_spectrogram_440hz_64_fmin300_fmax8000", dict(n_mels=64, fmin=300, fmax=8000)),
+])
+def test_melspectrogram_golden(signal_440, name, kw):
+    G = load_golden(name)
+    M = o.melspectrogram(signal_440, 22050, **kw)
+    np.testing.assert_allclose(M, G, rtol=1e-4, atol=1e-4)
+    assert np.abs(M - G).max() / G.max() < 1e-6
+
+
+# -- dB ----------------------------------------------------------------------
+
+def test_db_known_answers():
+    # ScalingTests.swift:39-56
+    np.testing.assert_allclose(o.power_to_db(np.array([0.5]), top_db=None), [-3.0102999566], atol=1e-9)
+    np.testing.assert_allclose(o.power_to_db(np.array([100.0]), top_db=None), [20.0], atol=1e-12)
+    d = o.power_to_db(np.array([1.0, 1e-12]), top_db=80.0)
+    np.testing.assert_allclose(d, [0.0, -80.0])
+
+
+def test_db_goldens(signal_440):
+    M = o.stft_magnitude(signal_440, 2048, 512)
+    P = M ** 2
+    np.testing.assert_allclose(o.power_to_db(P, ref=P.max()), load_golden("stft_440hz_power_db"), atol=1e-4)
+    np.testing.assert_allclose(o.amplitude_to_db(M, ref=M.max()), load_golden("stft_440hz_amplitude_db"),
+                               atol=1e-4)
+    np.testing.assert_allclose(o.amplitude_to_db(M, ref=1.0), load_golden("stft_440hz_amplitude_db_ref1"),
+                               atol=1e-4)
+
+
+# -- DCT / MFCC ---------------------------------------------------------------
+
+def test_dct_basis_matches_scipy_and_is_orthonormal():
+    B = o.dct_ortho_basis(128, 128)
+    x = np.random.default_rng(3).standard_normal(128)
+    np.testing.assert_allclose(B @ x, scipy.fft.dct(x, type=2, norm="ortho"), atol=1e-12)
+    np.testing.assert_allclose(B @ B.T, np.eye(128), atol=1e-12)
+    c = o.mfcc_from_logmel(np.full((40, 3), 2.0), 13)          # constant column: c0 only
+    assert np.allclose(c[1:], 0) and np.allclose(c[0], 2.0 * np.sqrt(40))
+
+
+@pytest.mark.parametrize("name,kw,tol", [
+    ("mfcc_440hz_default", dict(n_mfcc=20), 1e-4),
+    ("mfcc_440hz_13_1024", dict(n_mfcc=13, n_fft=1024, n_mels=40), 1e-4),
+    ("mfcc_440hz_20_64_fmin300_fmax8000", dict(n_mfcc=20, n_mels=64, fmin=300, fmax=8000), 5e-4),
+])
+def test_mfcc_golden(signal_440, name, kw, tol):
+    G = load_golden(name)
+    C = o.mfcc(signal_440, 22050, **kw)
+    np.testing.assert_allclose(C, G, rtol=tol, atol=tol)       # test_parity_mfcc.py tiers
+
+
+def test_mfcc_caps_n_mfcc_at_n_mels():
+    y = np.random.default_rng(0).standard_normal(4000)
+    assert o.mfcc(y, n_mfcc=50, n_mels=20).shape[0] == 20       # EdgeCaseTests.swift:185-213
+
+
+def test_planted_mistakes_are_caught(signal_440):
+    """A wrong sign, dropped Slaney factor or transposed DCT fails the goldens."""
+    G = load_golden("mfcc_440hz_default")
+    M = o.melspectrogram(signal_440, 22050)
+    good = o.mfcc_from_logmel(o.power_to_db(M), 20)
+    assert np.abs(good - G).max() < 1e-3
+    bad_sign = o.dct_ortho_basis(20, 128) @ (-o.power_to_db(M))
+    bad_T = o.dct_ortho_basis(128, 128)[:, :20].T @ o.power_to_db(M)
+    fb = o.mel_filterbank(22050, 2048, 128)
+    lo = np.argmax(fb > 0, axis=1)
+    assert np.abs(bad_sign - G).max() > 1.0 and np.abs(bad_T - G).max() > 1.0
+    assert np.abs(fb - load_golden("mel_filterbank_128_2048")).max() < 1e-8
+    assert lo[0] >= 1  # first triangle starts above DC
