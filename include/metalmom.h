@@ -176,6 +176,26 @@ int32_t mm_batch_time(mm_batch b, const void* d_signal, float* d_out, void* hip_
                       float* kernel_ms, float* step_ms);
 int32_t mm_batch_destroy(mm_batch b);
 
+/* Request server with continuous batching. Requests of any kind/config are
+ * admitted FIFO into the next device step while the HBM budget (bytes of
+ * device input + output per request) and a free stream allow it; up to three
+ * steps are in flight, each one persistent launch per configuration group.
+ * Results are bitwise identical to the same clip run alone. Thread-safe. */
+typedef void* mm_server;
+int32_t mm_server_create(mm_context ctx, int64_t hbm_budget_bytes, int32_t max_batch, mm_server* out);
+/* Copies the clip and queues it; returns the request id (> 0) or a status. */
+int64_t mm_server_submit(mm_server s, const MMSpecParams* params, const float* signal_data,
+                         int64_t signal_length);
+/* 1: done (out filled, latency = completion - submission in seconds, id
+ * released), 0: pending, < 0: the request's error status. */
+int32_t mm_server_poll(mm_server s, int64_t request_id, MMBuffer* out, double* latency_s);
+/* Like mm_server_poll but blocks until the request is done. */
+int32_t mm_server_wait(mm_server s, int64_t request_id, MMBuffer* out, double* latency_s);
+int32_t mm_server_stats(mm_server s, int64_t* submitted, int64_t* completed, int64_t* steps,
+                        int64_t* frames);
+/* Finishes queued work, then releases the server. */
+int32_t mm_server_destroy(mm_server s);
+
 /* Device description: name, gfx arch and compute-unit count. */
 int32_t mm_device_info(char* name, int32_t name_len, char* arch, int32_t arch_len, int32_t* n_cu);
 

@@ -82,11 +82,37 @@ SIGNATURES = {
     "mm_batch_time": (_I32, [_P, _P, _P, _P, _I32, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "mm_batch_destroy": (_I32, [_P]),
     "mm_device_info": (_I32, [C.c_char_p, _I32, C.c_char_p, _I32, C.POINTER(_I32)]),
+    "mm_server_create": (_I32, [_P, _I64, _I32, C.POINTER(_P)]),
+    "mm_server_submit": (_I64, [_P, C.POINTER(MMSpecParams), _F, _I64]),
+    "mm_server_poll": (_I32, [_P, _I64, _BUF, C.POINTER(C.c_double)]),
+    "mm_server_wait": (_I32, [_P, _I64, _BUF, C.POINTER(C.c_double)]),
+    "mm_server_stats": (_I32, [_P, C.POINTER(_I64), C.POINTER(_I64), C.POINTER(_I64), C.POINTER(_I64)]),
+    "mm_server_destroy": (_I32, [_P]),
 }
+
+
+def _share_torch_hip_runtime():
+    """One HIP runtime per process.
+
+    libmetalmom.so links ``libamdhip64.so.7``. PyTorch-ROCm ships its own copy
+    whose SONAME is also ``libamdhip64.so.7`` but which its libraries request
+    by the file name ``libamdhip64.so``. If this library were loaded first,
+    a later ``import torch`` would map a second HIP runtime and lose the GPU.
+    Importing torch first makes the dynamic linker resolve this library's
+    dependency to torch's already-loaded runtime. Set
+    ``METALMOM_AMD_STANDALONE_RUNTIME=1`` to skip this in torch-free processes.
+    """
+    if os.environ.get("METALMOM_AMD_STANDALONE_RUNTIME") == "1":
+        return
+    try:
+        import torch  # noqa: F401
+    except Exception:
+        pass
 
 
 def load(path: str = LIB_PATH) -> C.CDLL:
     """Loads the library and declares every entry point; raises if absent."""
+    _share_torch_hip_runtime()
     if not os.path.exists(path):
         raise RuntimeError(
             f"libmetalmom.so not found at {path}. Build it with `python -m metalmom_amd.build`.")

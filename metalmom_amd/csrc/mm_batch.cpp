@@ -85,11 +85,12 @@ int32_t create_batch(const MMSpecParams* params, const int64_t* lengths, int32_t
   if (!lengths || n_clips <= 0 || !out) return ERR_INVALID;
   for (int i = 0; i < n_clips; ++i)
     if (lengths[i] < 0) return ERR_INVALID;
+  // the engine sets the device's LDS limit, which the layout check reads
+  Engine* e = Engine::get();
+  if (!e) return ERR_NO_GPU;
   auto b = std::make_unique<Batch>();
   int32_t s = to_request(params, &b->r);
   if (s != OK) return s;
-  Engine* e = Engine::get();
-  if (!e) return ERR_NO_GPU;
   b->lane = e->acquire_lane();
   if (!b->lane) return ERR_INTERNAL;
   s = e->tables_for(b->r, b->lane->stream, &b->tb);
@@ -173,7 +174,11 @@ int32_t mm_batch_run_graph(mm_batch bh, const void* d_signal, float* d_out, void
   for (auto& g : b->graphs)
     if (g.in == d_signal && g.out == d_out && g.st == st)
       return hipGraphLaunch(g.exec, st) == hipSuccess ? MM_OK : MM_ERR_INTERNAL;
-  // the generic path allocates scratch on the fly: size it before capture
+  // the generic (non-fused) path grows scratch buffers on the fly, which
+  // stream capture does not allow: it runs eagerly
+  TileLayout L;
+  if (!tile_layout(b->r.n_fft, b->r.out_kind, b->r.precision, b->r.n_mels, std::min(b->r.n_mfcc, b->r.n_mels), &L))
+    return run_batch(b, d_signal, d_out, st);
   GraphEntry g{d_signal, d_out, st, nullptr, nullptr};
   if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return MM_ERR_INTERNAL;
   const int32_t s = run_batch(b, d_signal, d_out, st);

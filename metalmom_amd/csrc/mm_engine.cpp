@@ -200,9 +200,19 @@ bool Engine::init() {
   n_cu_ = p.multiProcessorCount;
   arch_ = p.gcnArchName;
   name_ = p.name;
-  int lds = 0;
-  if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && lds > 0)
-    set_lds_limit_bytes((size_t)lds);
+  // per-block LDS a kernel may request: the larger of the default and the
+  // opt-in limit (some runtimes report only 64 KiB as the default)
+  int lds = 0, optin = 0;
+  (void)hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+  (void)hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, dev);
+  const int limit = std::max(lds, optin);
+  if (limit > 0) set_lds_limit_bytes((size_t)limit);
+  if (const char* v = std::getenv("METALMOM_AMD_VERBOSE"); v && *v == '1') {
+    int rt = 0;
+    (void)hipRuntimeGetVersion(&rt);
+    std::fprintf(stderr, "[metalmom_amd] %s (%s), %d CUs, LDS/block %d (opt-in %d), HIP runtime %d\n",
+                 name_.c_str(), arch_.c_str(), n_cu_, lds, optin, rt);
+  }
   return true;
 }
 

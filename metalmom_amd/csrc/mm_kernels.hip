@@ -439,9 +439,12 @@ __global__ __launch_bounds__(kWgThreads) MM_TILE_OCC void k_tile(TileArgs a) {
         if constexpr (!SPEC && PREC != PREC_F32) {
           // per-frame power-of-two scale so the 16-bit operand keeps the
           // frame's top range (exactly undone after the contraction)
+          // bf16 keeps the FP32 exponent range, so any scale works; FP16
+          // maps the frame maximum into [2^14, 2^15) so ~84 dB below it
+          // stay normal numbers
           vmax = wave_max(vmax);
           int e = 0;
-          if (vmax > 0.0f && vmax <= 3.0e38f) e = ilogbf(vmax);
+          if (vmax > 0.0f && vmax <= 3.0e38f) e = ilogbf(vmax) - (PREC == PREC_F16 ? 14 : 0);
           fexp[q] = e;
         }
       }

@@ -1,0 +1,439 @@
+// Request server with continuous batching (additive exports mm_server_*).
+//
+// Clients submit single clips of any request type from any thread. A
+// scheduler thread admits queued requests into the next device step while the
+// HBM budget and a free lane allow it, groups the step's requests by
+// configuration (one persistent fused launch per group), and keeps up to
+// kLanes steps in flight on separate streams, so requests that arrive while
+// others run on the GPU join the next step and overlap with them (H2D of step
+// n+1 and D2H of step n-1 run next to the kernels of step n). Device state
+// (tables, work lists, lane buffers) stays resident between steps.
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstdlib>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/metalmom.h"
+#include "mm_engine.h"
+
+using namespace mm;
+
+namespace {
+
+constexpr int kLanes = 3;
+
+double now_s() {
+  using namespace std::chrono;
+  return duration_cast<duration<double>>(steady_clock::now().time_since_epoch()).count();
+}
+
+struct Req {
+  int64_t id = 0;
+  SpecRequest r;
+  std::vector<float> sig;
+  double t_submit = 0, t_done = 0;
+  int state = 0;  // 0 queued, 1 on device, 2 done
+  int32_t status = OK;
+  float* out = nullptr;
+  int64_t rows = 0, cols = 0;
+  size_t dev_bytes = 0;
+};
+
+struct Group {
+  SpecRequest r;
+  std::vector<Req*> reqs;
+  BatchMeta m;
+  size_t in_off = 0, out_off = 0, meta_off = 0;  // byte offsets in the lane buffers
+};
+
+struct Step {
+  Lane* lane = nullptr;
+  hipEvent_t done = nullptr;
+  std::vector<Group> groups;
+  size_t in_bytes = 0, out_bytes = 0, meta_bytes_ = 0, dev_bytes = 0;
+  int32_t status = OK;
+};
+
+bool same_plan(const SpecRequest& a, const SpecRequest& b) {
+  return a.out_kind == b.out_kind && a.sr == b.sr && a.n_fft == b.n_fft && a.hop == b.hop && a.win == b.win &&
+         a.center == b.center && a.pad_mode == b.pad_mode && a.power == b.power && a.n_mels == b.n_mels &&
+         a.fmin == b.fmin && a.fmax == b.fmax && a.n_mfcc == b.n_mfcc && a.top_db == b.top_db &&
+         a.precision == b.precision && a.in_dtype == b.in_dtype;
+}
+
+size_t a256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+size_t request_bytes(const SpecRequest& r, int64_t len) {
+  const int64_t T = frame_count(len, r.n_fft, r.hop, r.center);
+  return a256((size_t)len * 4) + a256((size_t)(out_rows(r) * T * out_elem_floats(r)) * 4) + 1024;
+}
+
+class Server {
+ public:
+  Server(size_t budget, int max_batch) : budget_(budget), max_batch_(std::max(1, max_batch)) {}
+  ~Server() { stop(); }
+
+  int32_t start() {
+    Engine* e = Engine::get();
+    if (!e) return ERR_NO_GPU;
+    for (int i = 0; i < kLanes; ++i) {
+      Lane* l = e->acquire_lane();
+      if (!l) return ERR_INTERNAL;
+      free_lanes_.push_back(l);
+    }
+    th_ = std::thread([this] { loop(); });
+    return OK;
+  }
+
+  void stop() {
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      if (stopping_) return;
+      stopping_ = true;
+    }
+    cv_.notify_all();
+    if (th_.joinable()) th_.join();
+    for (auto& s : inflight_) {
+      (void)hipEventSynchronize(s.done);
+      (void)hipEventDestroy(s.done);
+      free_lanes_.push_back(s.lane);
+    }
+    inflight_.clear();
+    for (Lane* l : free_lanes_) Engine::get()->release_lane(l);
+    free_lanes_.clear();
+    for (auto& kv : reqs_) {
+      if (kv.second->out) std::free(kv.second->out);
+      delete kv.second;
+    }
+    reqs_.clear();
+  }
+
+  int64_t submit(const SpecRequest& r, const float* sig, int64_t len) {
+    auto* q = new Req();
+    q->r = r;
+    q->sig.assign(sig, sig + len);
+    q->dev_bytes = request_bytes(r, len);
+    q->t_submit = now_s();
+    std::lock_guard<std::mutex> g(mu_);
+    q->id = next_id_++;
+    if (q->dev_bytes > budget_) {  // can never be admitted
+      q->status = ERR_ALLOC;
+      q->state = 2;
+      q->t_done = q->t_submit;
+    } else {
+      queue_.push_back(q);
+    }
+    reqs_[q->id] = q;
+    ++submitted_;
+    cv_.notify_all();
+    return q->id;
+  }
+
+  // 1: finished (out filled, request forgotten), 0: pending, <0: error/unknown
+  int32_t poll(int64_t id, MMBuffer* out, double* latency, bool wait) {
+    std::unique_lock<std::mutex> g(mu_);
+    auto it = reqs_.find(id);
+    if (it == reqs_.end()) return ERR_INVALID;
+    Req* q = it->second;
+    if (wait) done_cv_.wait(g, [&] { return q->state == 2; });
+    if (q->state != 2) return 0;
+    reqs_.erase(it);
+    g.unlock();
+    if (latency) *latency = q->t_done - q->t_submit;
+    int32_t s = q->status;
+    if (s == OK && out) {
+      std::memset(out, 0, sizeof(*out));
+      out->ndim = 2;
+      out->shape[0] = q->rows;
+      out->shape[1] = q->cols;
+      out->dtype = (q->r.out_kind == OUT_STFT_COMPLEX) ? 1 : 0;
+      out->count = q->rows * q->cols;
+      out->data = q->out;
+      q->out = nullptr;
+    }
+    if (q->out) std::free(q->out);
+    delete q;
+    return s == OK ? 1 : s;
+  }
+
+  void stats(int64_t* submitted, int64_t* completed, int64_t* steps, int64_t* frames) {
+    std::lock_guard<std::mutex> g(mu_);
+    if (submitted) *submitted = submitted_;
+    if (completed) *completed = completed_;
+    if (steps) *steps = steps_;
+    if (frames) *frames = frames_;
+  }
+
+ private:
+  void loop() {
+    while (true) {
+      retire(false);
+      std::vector<Req*> take;
+      {
+        std::unique_lock<std::mutex> g(mu_);
+        if (stopping_ && queue_.empty()) break;
+        if (queue_.empty() || free_lanes_.empty()) {
+          if (inflight_.empty()) {
+            cv_.wait(g, [&] { return stopping_ || !queue_.empty(); });
+            continue;
+          }
+          g.unlock();
+          std::this_thread::sleep_for(std::chrono::microseconds(20));
+          continue;
+        }
+        // admission: FIFO while the HBM budget allows
+        size_t step_bytes = 0;
+        while (!queue_.empty() && (int)take.size() < max_batch_) {
+          Req* q = queue_.front();
+          if (in_use_ + step_bytes + q->dev_bytes > budget_) break;
+          step_bytes += q->dev_bytes;
+          q->state = 1;
+          take.push_back(q);
+          queue_.pop_front();
+        }
+        if (take.empty()) {  // budget full: wait for a step to retire
+          g.unlock();
+          retire(true);
+          continue;
+        }
+        in_use_ += step_bytes;
+      }
+      launch(take);
+    }
+    while (!inflight_.empty()) retire(true);
+  }
+
+  void launch(std::vector<Req*>& take) {
+    Step s;
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      s.lane = free_lanes_.back();
+      free_lanes_.pop_back();
+    }
+    Engine* eng = Engine::get();
+    for (Req* q : take) {
+      s.dev_bytes += q->dev_bytes;
+      Group* grp = nullptr;
+      for (auto& G : s.groups)
+        if (same_plan(G.r, q->r)) grp = &G;
+      if (!grp) {
+        s.groups.push_back(Group());
+        grp = &s.groups.back();
+        grp->r = q->r;
+      }
+      grp->reqs.push_back(q);
+    }
+    Lane* L = s.lane;
+    hipStream_t st = L->stream;
+    // layout of inputs, outputs and work lists in the lane buffers
+    for (auto& G : s.groups) {
+      std::vector<int64_t> lens;
+      for (Req* q : G.reqs) lens.push_back((int64_t)q->sig.size());
+      TileLayout TL;
+      const bool fused =
+          tile_layout(G.r.n_fft, G.r.out_kind, G.r.precision, G.r.n_mels, std::min(G.r.n_mfcc, G.r.n_mels), &TL);
+      build_meta(G.r, lens.data(), (int)lens.size(), fused ? TL.tile_frames : kTileFrames, &G.m);
+      G.in_off = s.in_bytes;
+      s.in_bytes += a256((size_t)G.m.total_in * 4);
+      G.out_off = s.out_bytes;
+      s.out_bytes += a256((size_t)G.m.total_out * 4);
+      G.meta_off = s.meta_bytes_;
+      s.meta_bytes_ += a256(meta_bytes(G.m));
+    }
+    int32_t st_code = OK;
+    auto chk = [&](hipError_t e) {
+      if (e != hipSuccess && st_code == OK) st_code = ERR_INTERNAL;
+    };
+    chk(L->h_in.ensure(s.in_bytes + s.meta_bytes_));
+    chk(L->d_in.ensure(s.in_bytes));
+    chk(L->d_meta.ensure(s.meta_bytes_));
+    chk(L->d_out.ensure(s.out_bytes));
+    chk(L->h_out.ensure(s.out_bytes));
+    if (st_code == OK) {
+      char* hin = static_cast<char*>(L->h_in.p);
+      for (auto& G : s.groups) {
+        for (size_t i = 0; i < G.reqs.size(); ++i)
+          std::memcpy(hin + G.in_off + (size_t)G.m.clip_off[i] * 4, G.reqs[i]->sig.data(), G.reqs[i]->sig.size() * 4);
+        pack_meta(G.m, hin + s.in_bytes + G.meta_off);
+      }
+      chk(hipMemcpyAsync(L->d_in.p, hin, s.in_bytes, hipMemcpyHostToDevice, st));
+      chk(hipMemcpyAsync(L->d_meta.p, hin + s.in_bytes, s.meta_bytes_, hipMemcpyHostToDevice, st));
+      for (auto& G : s.groups) {
+        if (st_code != OK) break;
+        const Tables* tb = nullptr;
+        int32_t r = eng->tables_for(G.r, st, &tb);
+        if (r == OK) {
+          const MetaPtrs mp = meta_ptrs(G.m, static_cast<char*>(L->d_meta.p) + G.meta_off);
+          r = enqueue_batch(eng, G.r, tb, G.m, mp, static_cast<char*>(L->d_in.p) + G.in_off,
+                            static_cast<char*>(L->d_out.p) + G.out_off, &L->d_scratch, &L->d_scratch2, 0, st);
+        }
+        if (r != OK) st_code = r;
+      }
+      chk(hipMemcpyAsync(L->h_out.p, L->d_out.p, s.out_bytes, hipMemcpyDeviceToHost, st));
+    }
+    chk(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
+    if (s.done) chk(hipEventRecord(s.done, st));
+    s.status = st_code;
+    std::lock_guard<std::mutex> g(mu_);
+    ++steps_;
+    inflight_.push_back(std::move(s));
+  }
+
+  // Completes finished steps (the oldest first). block: wait for the oldest.
+  void retire(bool block) {
+    while (true) {
+      Step* s = nullptr;
+      {
+        std::lock_guard<std::mutex> g(mu_);
+        if (inflight_.empty()) return;
+        s = &inflight_.front();
+      }
+      if (block) {
+        (void)hipEventSynchronize(s->done);
+        block = false;
+      } else if (hipEventQuery(s->done) == hipErrorNotReady) {
+        return;
+      }
+      const char* hout = static_cast<const char*>(s->lane->h_out.p);
+      const double t = now_s();
+      int64_t frames = 0;
+      for (auto& G : s->groups) {
+        const int ef = out_elem_floats(G.r);
+        const int64_t rows = out_rows(G.r);
+        for (size_t i = 0; i < G.reqs.size(); ++i) {
+          Req* q = G.reqs[i];
+          q->rows = rows;
+          q->cols = G.m.clip_frames[i];
+          frames += q->cols;
+          q->status = s->status;
+          const size_t n = (size_t)(rows * q->cols * ef) * 4;
+          if (q->status == OK && n > 0) {
+            q->out = static_cast<float*>(std::malloc(n));
+            if (!q->out) q->status = ERR_ALLOC;
+            else std::memcpy(q->out, hout + G.out_off + (size_t)G.m.out_off[i] * ef * 4, n);
+          }
+          std::vector<float>().swap(q->sig);
+        }
+      }
+      (void)hipEventDestroy(s->done);
+      {
+        std::lock_guard<std::mutex> g(mu_);
+        for (auto& G : s->groups)
+          for (Req* q : G.reqs) {
+            q->t_done = t;
+            q->state = 2;
+            ++completed_;
+          }
+        frames_ += frames;
+        in_use_ -= s->dev_bytes;
+        free_lanes_.push_back(s->lane);
+        inflight_.pop_front();
+      }
+      done_cv_.notify_all();
+      cv_.notify_all();
+    }
+  }
+
+  size_t budget_;
+  int max_batch_;
+  std::mutex mu_;
+  std::condition_variable cv_, done_cv_;
+  std::deque<Req*> queue_;
+  std::unordered_map<int64_t, Req*> reqs_;
+  std::deque<Step> inflight_;
+  std::vector<Lane*> free_lanes_;
+  size_t in_use_ = 0;
+  int64_t next_id_ = 1, submitted_ = 0, completed_ = 0, steps_ = 0, frames_ = 0;
+  bool stopping_ = false;
+  std::thread th_;
+};
+
+int32_t spec_to_request(const MMSpecParams* p, SpecRequest* r) {
+  if (!p) return ERR_INVALID;
+  const int n_fft = p->n_fft;
+  if (n_fft <= 0 || (n_fft & (n_fft - 1))) return ERR_INVALID;
+  if (p->kind < MM_KIND_STFT_MAG || p->kind > MM_KIND_MFCC) return ERR_INVALID;
+  if (p->input_dtype != MM_INPUT_F32) return ERR_INVALID;
+  r->out_kind = p->kind;
+  r->sr = p->sample_rate;
+  r->n_fft = n_fft;
+  r->hop = p->hop_length > 0 ? p->hop_length : n_fft / 4;
+  r->win = p->win_length > 0 ? p->win_length : n_fft;
+  r->center = p->center != 0;
+  r->pad_mode = p->pad_mode;
+  r->power = p->power;
+  r->n_mels = p->n_mels;
+  r->fmin = p->f_min;
+  r->fmax = p->f_max > 0.0f ? p->f_max : (float)p->sample_rate / 2.0f;
+  r->n_mfcc = p->n_mfcc;
+  r->top_db = p->top_db;
+  r->precision = p->precision;
+  r->in_dtype = IN_F32;
+  if (r->hop <= 0 || r->pad_mode < 0 || r->pad_mode > 1 || r->precision < 0 || r->precision > 2) return ERR_INVALID;
+  if (r->out_kind >= OUT_MEL && r->n_mels < 0) return ERR_INVALID;
+  if (r->out_kind == OUT_MFCC && r->n_mfcc < 0) return ERR_INVALID;
+  if (r->out_kind <= OUT_STFT_COMPLEX && r->precision != PREC_F32) return ERR_INVALID;
+  if (r->precision != PREC_F32) {
+    TileLayout L;
+    if (!tile_layout(r->n_fft, r->out_kind, r->precision, r->n_mels, std::min(r->n_mfcc, r->n_mels), &L))
+      return ERR_INVALID;
+  }
+  return OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t mm_server_create(mm_context ctx, int64_t hbm_budget_bytes, int32_t max_batch, mm_server* out) {
+  (void)ctx;
+  if (!out || hbm_budget_bytes <= 0) return MM_ERR_INVALID_INPUT;
+  auto* s = new Server((size_t)hbm_budget_bytes, max_batch > 0 ? max_batch : 256);
+  const int32_t r = s->start();
+  if (r != OK) {
+    delete s;
+    return r;
+  }
+  *out = s;
+  return MM_OK;
+}
+
+int64_t mm_server_submit(mm_server srv, const MMSpecParams* params, const float* signal_data, int64_t signal_length) {
+  if (!srv || !signal_data || signal_length <= 0) return MM_ERR_INVALID_INPUT;
+  SpecRequest r;
+  const int32_t s = spec_to_request(params, &r);
+  if (s != OK) return s;
+  return static_cast<Server*>(srv)->submit(r, signal_data, signal_length);
+}
+
+int32_t mm_server_poll(mm_server srv, int64_t request_id, MMBuffer* out, double* latency_s) {
+  if (!srv) return MM_ERR_INVALID_INPUT;
+  return static_cast<Server*>(srv)->poll(request_id, out, latency_s, false);
+}
+
+int32_t mm_server_wait(mm_server srv, int64_t request_id, MMBuffer* out, double* latency_s) {
+  if (!srv) return MM_ERR_INVALID_INPUT;
+  return static_cast<Server*>(srv)->poll(request_id, out, latency_s, true);
+}
+
+int32_t mm_server_stats(mm_server srv, int64_t* submitted, int64_t* completed, int64_t* steps, int64_t* frames) {
+  if (!srv) return MM_ERR_INVALID_INPUT;
+  static_cast<Server*>(srv)->stats(submitted, completed, steps, frames);
+  return MM_OK;
+}
+
+int32_t mm_server_destroy(mm_server srv) {
+  if (!srv) return MM_ERR_INVALID_INPUT;
+  delete static_cast<Server*>(srv);
+  return MM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,84 @@
+"""Serving: continuous batching returns, for every request, exactly what the
+same clip gives alone; admission respects the HBM budget; oversized requests
+fail cleanly without disturbing the others."""
+import numpy as np
+import pytest
+
+from oracle import spectral as o
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def sources():
+    from metalmom_amd.serve import make_sources
+    return make_sources(seconds=40.0, seed=9)
+
+
+def _alone(kind, sr, y):
+    import metalmom_amd
+    return metalmom_amd.spectral(y, kind=kind, sr=sr, n_fft=2048, hop_length=512, n_mels=128,
+                                 n_mfcc=13)
+
+
+def test_mixed_requests_equal_alone(sources):
+    from metalmom_amd.serve import SpectralServer
+    rng = np.random.default_rng(5)
+    reqs = []
+    for i in range(40):
+        sr = (22050, 44100)[i % 2]
+        kind = ("stft", "mel", "mfcc")[i % 3]
+        n = int(rng.integers(2000, 8 * sr))
+        off = int(rng.integers(0, len(sources[sr]) - n))
+        reqs.append((kind, sr, sources[sr][off:off + n]))
+    srv = SpectralServer(hbm_budget_bytes=2 << 30)
+    ids = [srv.submit(y, srv.params(kind, sr=sr)) for kind, sr, y in reqs]    # burst: all in flight
+    outs = [srv.wait(i) for i in ids]
+    st = srv.stats()
+    srv.close()
+    assert st["completed"] == len(reqs) and st["steps"] >= 1
+    for (kind, sr, y), (arr, lat) in zip(reqs, outs):
+        ref = _alone(kind, sr, y)
+        assert arr.shape == ref.shape
+        assert np.array_equal(arr, ref), (kind, sr, len(y))
+        assert lat > 0
+
+
+def test_results_match_oracle(sources):
+    from metalmom_amd.serve import SpectralServer
+    y = sources[22050][:3 * 22050]
+    srv = SpectralServer()
+    a, _ = srv.wait(srv.submit(y, srv.params("mfcc", sr=22050)))
+    b, _ = srv.wait(srv.submit(y, srv.params("mel", sr=22050)))
+    srv.close()
+    O = o.mfcc(y, 22050, 13, 2048, 512)
+    assert np.abs(a - O).max() / np.abs(O).max() <= 1e-4
+    M = o.melspectrogram(y, 22050, 2048, 512)
+    assert np.abs(b - M).max() / M.max() <= 1e-5
+
+
+def test_small_budget_serialises_and_rejects_oversized(sources):
+    from metalmom_amd.serve import SpectralServer
+    y = sources[22050][:5 * 22050]
+    big = sources[22050][:30 * 22050]
+    srv = SpectralServer(hbm_budget_bytes=4 << 20, max_batch=4)   # fits ~1-2 small requests
+    p = srv.params("mfcc", sr=22050)
+    ids = [srv.submit(y, p) for _ in range(6)]
+    bad = srv.submit(big, srv.params("stft", sr=22050))          # needs > 4 MiB: rejected
+    outs = [srv.wait(i)[0] for i in ids]
+    with pytest.raises(RuntimeError, match="status -3"):
+        srv.wait(bad)
+    srv.close()
+    ref = _alone("mfcc", 22050, y)
+    for a in outs:
+        assert np.array_equal(a, ref)
+
+
+def test_open_loop_latency_report(sources):
+    from metalmom_amd.serve import SpectralServer, open_loop
+    srv = SpectralServer()
+    r = open_loop(srv, rate=200.0, duration=0.5, seed=3)
+    srv.close()
+    assert r["n_requests"] >= 50
+    assert 0 < r["p50"] <= r["p99"] < 5.0
+    assert r["frames"] > 0
