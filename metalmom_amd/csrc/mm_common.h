@@ -23,14 +23,12 @@ enum InDtype : int { IN_F32 = 0, IN_F16 = 1 };
 enum PadMode : int { PAD_CONSTANT = 0, PAD_REFLECT = 1 };
 
 constexpr int kTileFrames = 16;   // frames per tile = MFMA N dimension
-constexpr int kWaves = 8;         // wavefronts per workgroup of the fused kernel
-constexpr int kWgThreads = kWaves * 64;
 
 // Device-resident tables for one (sr, n_fft, win, n_mels, fmin, fmax, n_mfcc) key.
 struct DeviceTables {
   float* window = nullptr;      // [n_fft]
   float2* tw = nullptr;         // [M] W_M^t, M = n_fft/2
-  float2* twN = nullptr;        // [M/2+1] W_N^k, N = n_fft
+  float2* twN = nullptr;        // [M] W_N^k, N = n_fft
   float* fb32 = nullptr;        // [nmb*16][fbs] mel weights, zero padded
   uint16_t* fb16 = nullptr;     // same in bf16 or f16 (see fb16_kind)
   uint16_t* fbh = nullptr;      // f16 copy

@@ -146,7 +146,7 @@ static int32_t make_tables(const TableKey& key, hipStream_t st, std::unique_ptr<
   };
   const size_t o_win = take((size_t)key.n_fft * 4);
   const size_t o_tw = take((size_t)M * 8);
-  const size_t o_twn = take((size_t)(M / 2 + 1) * 8);
+  const size_t o_twn = take((size_t)M * 8);
   const size_t o_fb32 = take((size_t)rows * t.fbs * 4);
   const size_t o_fb16 = take((size_t)rows * t.fbs * 2);
   const size_t o_fbh = take((size_t)rows * t.fbs * 2);
@@ -524,6 +524,7 @@ int32_t prepare_batch(const SpecRequest& r) {
   std::memset(&a, 0, sizeof(a));
   a.n_fft = r.n_fft;
   a.out_kind = r.out_kind;
+  a.in_dtype = r.in_dtype;
   MM_HIP(launch_tile(a, r.precision, L, 0, nullptr));
   return OK;
 }

@@ -16,8 +16,9 @@ struct TableKey {
 
 // LDS layout of the fused tile kernel for one configuration.
 struct TileLayout {
+  int nw;            // wavefronts per workgroup
   int fpw;           // frames per wavefront
-  int tile_frames;   // frames per tile (8 * fpw)
+  int tile_frames;   // frames per tile (nw * fpw)
   int row_stride;    // staging row stride, elements
   int region_bytes;  // LDS bytes per wavefront region
   size_t lds_bytes;  // dynamic LDS per workgroup

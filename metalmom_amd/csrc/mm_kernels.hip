@@ -181,6 +181,13 @@ template <int LOG2M> struct FftShape {
   static constexpr int R0 = (NP16 > 0) ? 16 : TAIL;
   static constexpr int B0 = M / R0;               // butterflies of pass 0
   static constexpr int Q0 = (B0 + 63) / 64;
+  static constexpr int NPASS = NP16 + (TAIL > 1 ? 1 : 0);
+  static constexpr int RL = (TAIL > 1) ? TAIL : 16;   // radix of the last pass
+  static constexpr int BL = M / RL;                   // its butterflies (= its output stride)
+  static constexpr int QL = BL / 64;                  // butterflies per lane in the last pass
+  // the last pass can stay in registers (and pair k with M-k by a lane
+  // permutation) when every lane owns whole butterflies of it
+  static constexpr bool REG_SPLIT = (NPASS >= 2) && (BL % 64 == 0);
 };
 
 // One Stockham pass reading and writing LDS. NS = product of earlier radices.
@@ -215,35 +222,77 @@ __device__ __forceinline__ void stockham_pass(float2* scr, const float2* __restr
   wave_lds_fence();
 }
 
-template <int M, int P, int NPASS16, int TAIL>
+template <int M, int P, int NPASS16, int TAIL, int PEND>
 struct PassChain {
-  // pass P (P >= 1) with NS = 16^P
+  // passes P .. PEND-1 (P >= 1), pass p with NS = 16^p
   __device__ __forceinline__ static void run(float2* scr, const float2* tw, int lane) {
-    constexpr int NS = 1 << (4 * P);
-    if constexpr (P < NPASS16) {
-      stockham_pass<M, 16, NS>(scr, tw, lane);
-      PassChain<M, P + 1, NPASS16, TAIL>::run(scr, tw, lane);
-    } else if constexpr (TAIL > 1) {
-      stockham_pass<M, TAIL, NS>(scr, tw, lane);
+    if constexpr (P < PEND) {
+      constexpr int NS = 1 << (4 * P);
+      if constexpr (P < NPASS16) {
+        stockham_pass<M, 16, NS>(scr, tw, lane);
+      } else {
+        stockham_pass<M, TAIL, NS>(scr, tw, lane);
+      }
+      PassChain<M, P + 1, NPASS16, TAIL, PEND>::run(scr, tw, lane);
     }
   }
 };
+
+// Last Stockham pass kept in registers: z[q][r] = Z[j + BL*r], j = lane + 64 q
+// (its natural-order outputs are exactly the butterfly outputs).
+template <int M, int R, int NS>
+__device__ __forceinline__ void stockham_last_regs(const float2* scr, const float2* __restrict__ tw, int lane,
+                                                   float2 (&z)[M / R / 64][R]) {
+  constexpr int B = M / R;
+  constexpr int QB = B / 64;
+#pragma unroll
+  for (int q = 0; q < QB; ++q) {
+    const int j = lane + 64 * q;
+#pragma unroll
+    for (int r = 0; r < R; ++r) z[q][r] = scr[pidx(j + r * B)];
+  }
+  wave_lds_fence();
+#pragma unroll
+  for (int q = 0; q < QB; ++q) {
+    const int k = (lane + 64 * q) % NS;
+    constexpr int STRIDE = M / (NS * R);
+#pragma unroll
+    for (int r = 1; r < R; ++r) z[q][r] = cmul(z[q][r], tw[(k * r) * STRIDE]);
+    Dft<R>::run(z[q]);
+  }
+}
 
 // Loads frame `fr` of a clip (pass-0 inputs, windowed) into registers, runs
 // pass 0 from registers and the remaining passes in LDS. On return scr holds
 // Z[k] = DFT_M(z)[k] in natural order (padded index), z[n] = x[2n] + i x[2n+1].
 // Interior frames load straight from global memory; frames that touch the
 // centre padding are first staged into the scratch by a plain loop.
-template <typename TIn>
-__device__ __forceinline__ float load_sample(const TIn* x, int64_t i) { return (float)x[i]; }
+typedef _Float16 h2v __attribute__((ext_vector_type(2)));
 
+template <typename TIn> struct SamplePair;
+template <> struct SamplePair<float> {
+  __device__ __forceinline__ static float2 load(const float* p) { return *reinterpret_cast<const float2*>(p); }
+};
+template <> struct SamplePair<_Float16> {
+  __device__ __forceinline__ static float2 load(const _Float16* p) {
+    const h2v h = *reinterpret_cast<const h2v*>(p);
+    return make_float2((float)h.x, (float)h.y);
+  }
+};
+
+// Pass-0 inputs of one frame, windowed, in registers: v[q][r] = z[j + r*B0]
+// with z[n] = (x[2n] w[2n], x[2n+1] w[2n+1]), j = lane + 64 q. Interior frames
+// load straight from global memory (one 8-byte / 4-byte pair load per complex
+// value when the frame start is even); frames that touch the centre padding
+// are staged through the scratch by a plain loop.
 template <int LOG2M, typename TIn>
 __device__ __forceinline__ void frame_load(const TileArgs& a, const TIn* __restrict__ x, int64_t len, int64_t s0,
-                                           bool interior, float2* scr, int lane,
+                                           float2* scr, int lane,
                                            float2 (&v)[FftShape<LOG2M>::Q0][FftShape<LOG2M>::R0]) {
   using S = FftShape<LOG2M>;
   constexpr int M = S::M, R0 = S::R0, B0 = S::B0, Q0 = S::Q0;
-  if (interior) {
+  const bool interior = (s0 >= 0) && (s0 + 2 * M <= len);
+  if (interior && ((s0 & 1) == 0)) {
     const TIn* xf = x + s0;
 #pragma unroll
     for (int q = 0; q < Q0; ++q) {
@@ -253,7 +302,22 @@ __device__ __forceinline__ void frame_load(const TileArgs& a, const TIn* __restr
         for (int r = 0; r < R0; ++r) {
           const int n = j + r * B0;
           const float2 w = *reinterpret_cast<const float2*>(a.window + 2 * n);
-          v[q][r] = make_float2(load_sample(xf, 2 * n) * w.x, load_sample(xf, 2 * n + 1) * w.y);
+          const float2 xs = SamplePair<TIn>::load(xf + 2 * n);
+          v[q][r] = make_float2(xs.x * w.x, xs.y * w.y);
+        }
+      }
+    }
+  } else if (interior) {
+    const TIn* xf = x + s0;
+#pragma unroll
+    for (int q = 0; q < Q0; ++q) {
+      const int j = lane + 64 * q;
+      if ((B0 % 64 == 0) || j < B0) {
+#pragma unroll
+        for (int r = 0; r < R0; ++r) {
+          const int n = j + r * B0;
+          const float2 w = *reinterpret_cast<const float2*>(a.window + 2 * n);
+          v[q][r] = make_float2((float)xf[2 * n] * w.x, (float)xf[2 * n + 1] * w.y);
         }
       }
     }
@@ -262,8 +326,8 @@ __device__ __forceinline__ void frame_load(const TileArgs& a, const TIn* __restr
     for (int n = lane; n < M; n += 64) {
       const int64_t i0 = pad_index(s0 + 2 * n, len, a.pad_mode);
       const int64_t i1 = pad_index(s0 + 2 * n + 1, len, a.pad_mode);
-      const float x0 = (i0 >= 0) ? load_sample(x, i0) : 0.0f;
-      const float x1 = (i1 >= 0) ? load_sample(x, i1) : 0.0f;
+      const float x0 = (i0 >= 0) ? (float)x[i0] : 0.0f;
+      const float x1 = (i1 >= 0) ? (float)x[i1] : 0.0f;
       scr[pidx(n)] = make_float2(x0 * a.window[2 * n], x1 * a.window[2 * n + 1]);
     }
     wave_lds_fence();
@@ -279,19 +343,16 @@ __device__ __forceinline__ void frame_load(const TileArgs& a, const TIn* __restr
   }
 }
 
-template <int LOG2M>
+// FFT of frame `fr`: on return scr holds Z = DFT_M(z) in natural order
+// (padded index), z[n] = x[2n] + i x[2n+1] windowed.
+template <int LOG2M, typename TIn>
 __device__ __forceinline__ void frame_fft(const TileArgs& a, int clip, int fr, float2* scr, int lane) {
   using S = FftShape<LOG2M>;
   constexpr int M = S::M, R0 = S::R0, B0 = S::B0, Q0 = S::Q0;
-  const int n_fft = 2 * M;
   const int64_t len = a.clip_len[clip];
-  const int64_t s0 = (int64_t)fr * a.hop - (a.center ? (n_fft / 2) : 0);
-  const bool interior = (s0 >= 0) && (s0 + n_fft <= len);
+  const int64_t s0 = (int64_t)fr * a.hop - (a.center ? M : 0);
   float2 v[Q0][R0];
-  if (a.in_dtype == IN_F32)
-    frame_load<LOG2M>(a, static_cast<const float*>(a.signal) + a.clip_off[clip], len, s0, interior, scr, lane, v);
-  else
-    frame_load<LOG2M>(a, static_cast<const _Float16*>(a.signal) + a.clip_off[clip], len, s0, interior, scr, lane, v);
+  frame_load<LOG2M>(a, static_cast<const TIn*>(a.signal) + a.clip_off[clip], len, s0, scr, lane, v);
   // pass 0 (NS = 1: no twiddles), written in Stockham order
 #pragma unroll
   for (int q = 0; q < Q0; ++q) {
@@ -303,7 +364,68 @@ __device__ __forceinline__ void frame_fft(const TileArgs& a, int clip, int fr, f
     }
   }
   wave_lds_fence();
-  if constexpr (S::NP16 > 0) PassChain<M, 1, S::NP16, S::TAIL>::run(scr, a.tw, lane);
+  if constexpr (S::NP16 > 0) PassChain<M, 1, S::NP16, S::TAIL, S::NPASS>::run(scr, a.tw, lane);
+}
+
+// Same FFT, but the last pass stays in registers: z[q][r] = Z[lane + 64 q + BL r].
+template <int LOG2M, typename TIn>
+__device__ __forceinline__ void frame_fft_regs(const TileArgs& a, int clip, int fr, float2* scr, int lane,
+                                               float2 (&z)[FftShape<LOG2M>::QL][FftShape<LOG2M>::RL]) {
+  using S = FftShape<LOG2M>;
+  constexpr int M = S::M, R0 = S::R0, B0 = S::B0, Q0 = S::Q0;
+  const int64_t len = a.clip_len[clip];
+  const int64_t s0 = (int64_t)fr * a.hop - (a.center ? M : 0);
+  {
+    float2 v[Q0][R0];
+    frame_load<LOG2M>(a, static_cast<const TIn*>(a.signal) + a.clip_off[clip], len, s0, scr, lane, v);
+#pragma unroll
+    for (int q = 0; q < Q0; ++q) {
+      const int j = lane + 64 * q;
+      if ((B0 % 64 == 0) || j < B0) {
+        Dft<R0>::run(v[q]);
+#pragma unroll
+        for (int r = 0; r < R0; ++r) scr[pidx(j * R0 + r)] = v[q][r];
+      }
+    }
+  }
+  wave_lds_fence();
+  PassChain<M, 1, S::NP16, S::TAIL, S::NPASS - 1>::run(scr, a.tw, lane);
+  constexpr int NSL = M / S::RL;
+  stockham_last_regs<M, S::RL, NSL>(scr, a.tw, lane, z);
+  wave_lds_fence();
+}
+
+// Real-FFT split in registers. Lane l owns Z[k], k = l + 64 q + BL r; its
+// mirror Z[M-k] sits in lane (64-l) mod 64 at register (QL-1-q, RL-1-r)
+// (lane 0 pairs with itself at (QL-q, RL-1-r), or (0, (RL-r) mod RL) for
+// q = 0), fetched with ds_bpermute. X[k] = E - i W_N^k O as below; lane 0
+// also returns the Nyquist bin X[M] = Re Z[0] - Im Z[0].
+template <int LOG2M>
+__device__ __forceinline__ void real_split_regs(const float2 (&z)[FftShape<LOG2M>::QL][FftShape<LOG2M>::RL],
+                                                const float2* __restrict__ twN, int lane,
+                                                float2 (&x)[FftShape<LOG2M>::QL][FftShape<LOG2M>::RL],
+                                                float& nyq_re) {
+  using S = FftShape<LOG2M>;
+  constexpr int QL = S::QL, RL = S::RL, BL = S::BL;
+  const int paddr = ((64 - lane) & 63) << 2;
+#pragma unroll
+  for (int q = 0; q < QL; ++q) {
+#pragma unroll
+    for (int r = 0; r < RL; ++r) {
+      const float2 src = z[QL - 1 - q][RL - 1 - r];
+      float2 bz = make_float2(__int_as_float(__builtin_amdgcn_ds_bpermute(paddr, __float_as_int(src.x))),
+                              __int_as_float(__builtin_amdgcn_ds_bpermute(paddr, __float_as_int(src.y))));
+      const float2 own = (q == 0) ? z[0][(RL - r) % RL] : z[(QL - q) % QL][RL - 1 - r];
+      if (lane == 0) bz = own;
+      const float2 A = z[q][r];
+      const float2 E = make_float2(0.5f * (A.x + bz.x), 0.5f * (A.y - bz.y));
+      const float2 O = make_float2(0.5f * (A.x - bz.x), 0.5f * (A.y + bz.y));
+      const int k = lane + 64 * q + BL * r;
+      const float2 T = cmul(O, twN[k]);
+      x[q][r] = make_float2(E.x + T.y, E.y - T.x);
+    }
+  }
+  nyq_re = z[0][0].x - z[0][0].y;
 }
 
 // Real-FFT split: from Z (M complex) to X[k] and X[M-k] for k = lane + 64 i.
@@ -323,13 +445,16 @@ __device__ __forceinline__ void real_split(const float2* scr, const float2* __re
   xmk = make_float2(E.x - T.y, -(E.y + T.x));
 }
 
-__device__ __forceinline__ float spec_value(float2 X, float power) {
-  const float p2 = fmaf(X.x, X.x, X.y * X.y);
+// |X|^power from |X|^2: power 2 and 1 exactly; otherwise 2^(p/2 log2 |X|^2)
+// with the hardware log2/exp2.
+__device__ __forceinline__ float pow_from_p2(float p2, float power) {
   if (power == 2.0f) return p2;
-  const float mag = sqrtf(p2);
-  if (power == 1.0f) return mag;
-  return powf(mag, power);
+  if (power == 1.0f) return sqrtf(p2);
+  return (p2 > 0.0f) ? __builtin_amdgcn_exp2f(0.5f * power * __builtin_amdgcn_logf(p2)) : 0.0f;
 }
+
+// 10 log10(v) = 10 log10(2) * log2(v), hardware log2 (v >= amin > 0 here)
+__device__ __forceinline__ float db10(float v) { return 3.0102999566398120f * __builtin_amdgcn_logf(v); }
 
 __device__ __forceinline__ float wave_max(float v) {
 #pragma unroll
@@ -346,39 +471,42 @@ __device__ __forceinline__ float wave_min(float v) {
 // the fused tile kernel
 // ---------------------------------------------------------------------------
 //
-// One workgroup = 8 wavefronts; one tile = 16 consecutive frames of one clip
-// (8 for complex STFT output at n_fft 4096: FPW = 1). Wave w transforms frames
-// w*FPW .. w*FPW+FPW-1 of the tile, keeping their spectra in registers until
-// its FFT scratch (which lives in its own LDS region) is free, then writes
-// them into its staging rows. After a workgroup barrier:
-//   * STFT outputs are written transposed ([bins, frames]) with 16-frame runs;
-//   * mel: wave w takes mel blocks w, w+8, ... (16 mels each) and contracts
+// One workgroup = NW wavefronts; one tile = NW*FPW consecutive frames of one
+// clip (16 for the mel/MFCC outputs: the MFMA N dimension). Wave w
+// transforms frames w*FPW .. w*FPW+FPW-1 of the tile in its own LDS region,
+// keeping each spectrum in registers until the FFT scratch under it is free,
+// then writes it into its staging row(s) in the same region. After a
+// workgroup barrier:
+//   * STFT outputs are written transposed ([bins, frames], TF-frame runs);
+//   * mel: wave w takes mel blocks w, w+NW, ... (16 mels each) and contracts
 //     the banded filterbank with the staged spectra on MFMA
 //     (mel^T[16 mels x 16 frames] = FB[16 x K] * P^T[K x 16]);
 //   * MFCC: dB in registers, then DCT-II on MFMA with the mel registers fed
 //     straight in as the B operand (the DCT table is stored in the matching
 //     K order), partial sums reduced across waves in LDS in fixed order.
 // Persistent: workgroup g processes tiles [g*C, (g+1)*C).
-template <int LOG2M, int MODE, int PREC, int FPW>
-__global__ __launch_bounds__(kWgThreads) MM_TILE_OCC void k_tile(TileArgs a) {
+template <int LOG2M, int MODE, int PREC, int NW, int FPW, typename TIn>
+__global__ __launch_bounds__(NW * 64) MM_TILE_OCC void k_tile(TileArgs a) {
   extern __shared__ __align__(16) unsigned char lds[];
   constexpr int M = 1 << LOG2M;
   constexpr int NI = (M / 2 + 1 + 63) / 64;       // split iterations per lane
-  constexpr int TF = kWaves * FPW;                 // frames per tile
+  constexpr int TF = NW * FPW;                     // frames per tile
+  constexpr int NT = NW * 64;
   constexpr bool SPEC = (MODE == OUT_STFT_MAG || MODE == OUT_STFT_POWER || MODE == OUT_STFT_COMPLEX);
   constexpr bool CPLX = (MODE == OUT_STFT_COMPLEX);
   constexpr int F = M + 1;
-  constexpr int MAXB = 4;                          // mel blocks per wave (n_mels <= 512)
+  constexpr int MAXB = (32 + NW - 1) / NW;         // mel blocks per wave (n_mels <= 512)
+  static_assert(TF <= 16, "tile larger than the MFMA N dimension");
 
-  const int lane = threadIdx.x & 63;
+  const int lane0 = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int RB = a.lds_region_bytes;
   const int S = a.lds_row_stride;                  // staging row stride (elements)
   unsigned char* region = lds + wave * RB;
   float2* scr = reinterpret_cast<float2*>(region);
-  int* sc_exp = reinterpret_cast<int*>(lds + kWaves * RB);   // [16]
-  float* red_max = reinterpret_cast<float*>(sc_exp + 16);    // [8]
-  float* red_min = red_max + kWaves;                          // [8]
+  int* sc_exp = reinterpret_cast<int*>(lds + NW * RB);       // [16]
+  float* red_max = reinterpret_cast<float*>(sc_exp + 16);    // [NW]
+  float* red_min = red_max + NW;                              // [NW]
   constexpr int ESZ = SPEC ? (CPLX ? 8 : 4) : (PREC == PREC_F32 ? 4 : 2);
   const int rowb = S * ESZ;
   auto row_ptr = [&](int f) -> unsigned char* {   // staging row of tile frame f
@@ -388,91 +516,147 @@ __global__ __launch_bounds__(kWgThreads) MM_TILE_OCC void k_tile(TileArgs a) {
   const int t_begin = blockIdx.x * a.tiles_per_wg;
   const int t_end = min(t_begin + a.tiles_per_wg, a.n_tiles);
   for (int tile = t_begin; tile < t_end; ++tile) {
+    // keep lane-derived table addresses inside the loop: hoisted out of it
+    // they would stay live as 64-bit register pairs across the whole FFT
+    int lane = lane0;
+    __asm__ __volatile__("" : "+v"(lane));
     const int clip = a.tile_clip[tile];
     const int f0 = a.tile_f0[tile];
     const int T = a.clip_frames[clip];
     float thr = -INFINITY;
     if constexpr (MODE == OUT_MFCC) {
-      if (a.use_top_db) {
-        if (a.phase == 2) {
-          thr = key_to_float(a.clip_max_key[clip]) - a.top_db;
-          if (!(a.tile_min[tile] < thr)) continue;   // clamp inactive: phase 1 result stands
-        }
+      if (a.use_top_db && a.phase == 2) {
+        thr = key_to_float(a.clip_max_key[clip]) - a.top_db;
+        if (!(a.tile_min[tile] < thr)) continue;   // clamp inactive: phase 1 result stands
       }
     }
 
     // ---- FFT of this wave's frames; spectra kept in registers ----
-    float hk[FPW][NI][CPLX ? 2 : 1];
-    float hm[FPW][NI][CPLX ? 2 : 1];
+    using SH = FftShape<LOG2M>;
+    constexpr bool RS = SH::REG_SPLIT;
+    constexpr int QL = RS ? SH::QL : 1, RL = RS ? SH::RL : 1;
+    constexpr int NH = RS ? 1 : FPW;               // register-split frames hold nothing across FFTs
+    constexpr int EC = CPLX ? 2 : 1;
+    float hk[NH][NI][EC];
+    float hm[NH][NI][EC];
+    float hv[FPW][QL][RL][EC];
+    float hn[FPW][EC];
     int fexp[FPW];
 #pragma unroll
     for (int q = 0; q < FPW; ++q) {
       fexp[q] = 0;
-      const int fl = wave * FPW + q;
-      const int fr = f0 + fl;
+      const int fr = f0 + wave * FPW + q;
       if (fr < T) {
-        frame_fft<LOG2M>(a, clip, fr, scr, lane);
         float vmax = 0.0f;
+        if constexpr (RS) {
+          float2 z[QL][RL];
+          frame_fft_regs<LOG2M, TIn>(a, clip, fr, scr, lane, z);
+          float2 x[QL][RL];
+          float nyq;
+          real_split_regs<LOG2M>(z, a.twN, lane, x, nyq);
 #pragma unroll
-        for (int i = 0; i < NI; ++i) {
-          const int k = lane + 64 * i;
-          if ((M / 2 + 1) % 64 == 0 || k <= M / 2) {
-            float2 xk, xmk;
-            real_split<LOG2M>(scr, a.twN, k, xk, xmk);
-            if constexpr (CPLX) {
-              hk[q][i][0] = xk.x; hk[q][i][1] = xk.y;
-              hm[q][i][0] = xmk.x; hm[q][i][1] = xmk.y;
-            } else if constexpr (MODE == OUT_STFT_MAG) {
-              hk[q][i][0] = sqrtf(fmaf(xk.x, xk.x, xk.y * xk.y));
-              hm[q][i][0] = sqrtf(fmaf(xmk.x, xmk.x, xmk.y * xmk.y));
-            } else if constexpr (MODE == OUT_STFT_POWER) {
-              hk[q][i][0] = fmaf(xk.x, xk.x, xk.y * xk.y);
-              hm[q][i][0] = fmaf(xmk.x, xmk.x, xmk.y * xmk.y);
-            } else {
-              hk[q][i][0] = spec_value(xk, a.power);
-              hm[q][i][0] = spec_value(xmk, a.power);
-              vmax = fmaxf(vmax, fmaxf(hk[q][i][0], hm[q][i][0]));
+          for (int b = 0; b < QL; ++b)
+#pragma unroll
+            for (int r = 0; r < RL; ++r) {
+              const float2 X = x[b][r];
+              if constexpr (CPLX) {
+                hv[q][b][r][0] = X.x;
+                hv[q][b][r][EC - 1] = X.y;
+              } else {
+                const float p2 = fmaf(X.x, X.x, X.y * X.y);
+                float v;
+                if constexpr (MODE == OUT_STFT_MAG) v = sqrtf(p2);
+                else if constexpr (MODE == OUT_STFT_POWER) v = p2;
+                else v = pow_from_p2(p2, a.power);
+                hv[q][b][r][0] = v;
+                vmax = fmaxf(vmax, v);
+              }
+            }
+          if constexpr (CPLX) {
+            hn[q][0] = nyq;
+            hn[q][EC - 1] = 0.0f;
+          } else {
+            const float p2 = nyq * nyq;
+            float v;
+            if constexpr (MODE == OUT_STFT_MAG) v = fabsf(nyq);
+            else if constexpr (MODE == OUT_STFT_POWER) v = p2;
+            else v = pow_from_p2(p2, a.power);
+            hn[q][0] = v;
+            if (lane == 0) vmax = fmaxf(vmax, v);
+          }
+        } else {
+          frame_fft<LOG2M, TIn>(a, clip, fr, scr, lane);
+#pragma unroll
+          for (int i = 0; i < NI; ++i) {
+            const int k = lane + 64 * i;
+            if ((M / 2 + 1) % 64 == 0 || k <= M / 2) {
+              float2 xk, xmk;
+              real_split<LOG2M>(scr, a.twN, k, xk, xmk);
+              if constexpr (CPLX) {
+                hk[q][i][0] = xk.x; hk[q][i][EC - 1] = xk.y;
+                hm[q][i][0] = xmk.x; hm[q][i][EC - 1] = xmk.y;
+              } else {
+                const float pk = fmaf(xk.x, xk.x, xk.y * xk.y);
+                const float pm = fmaf(xmk.x, xmk.x, xmk.y * xmk.y);
+                if constexpr (MODE == OUT_STFT_MAG) {
+                  hk[q][i][0] = sqrtf(pk);
+                  hm[q][i][0] = sqrtf(pm);
+                } else if constexpr (MODE == OUT_STFT_POWER) {
+                  hk[q][i][0] = pk;
+                  hm[q][i][0] = pm;
+                } else {
+                  hk[q][i][0] = pow_from_p2(pk, a.power);
+                  hm[q][i][0] = pow_from_p2(pm, a.power);
+                  vmax = fmaxf(vmax, fmaxf(hk[q][i][0], hm[q][i][0]));
+                }
+              }
             }
           }
         }
         wave_lds_fence();
         if constexpr (!SPEC && PREC != PREC_F32) {
           // per-frame power-of-two scale so the 16-bit operand keeps the
-          // frame's top range (exactly undone after the contraction)
-          // bf16 keeps the FP32 exponent range, so any scale works; FP16
-          // maps the frame maximum into [2^14, 2^15) so ~84 dB below it
-          // stay normal numbers
+          // frame's top range (exactly undone after the contraction): bf16
+          // keeps the FP32 exponent range, FP16 maps the frame maximum into
+          // [2^14, 2^15) so ~84 dB below it stay normal numbers
           vmax = wave_max(vmax);
           int e = 0;
-          if (vmax > 0.0f && vmax <= 3.0e38f) e = ilogbf(vmax) - (PREC == PREC_F16 ? 14 : 0);
+          if (vmax > 0.0f && vmax <= 3.0e38f) e = __builtin_amdgcn_frexp_expf(vmax) - 1 - (PREC == PREC_F16 ? 14 : 0);
           fexp[q] = e;
         }
       }
     }
 
     // ---- staging rows (this wave's region; its scratch is no longer used) ----
+    auto put = [&](unsigned char* row, int k, const float* v, int e) {
+      if constexpr (CPLX) {
+        reinterpret_cast<float2*>(row)[k] = make_float2(v[0], v[EC - 1]);
+      } else if constexpr (SPEC || PREC == PREC_F32) {
+        reinterpret_cast<float*>(row)[k] = v[0];
+      } else {
+        const float sv = ldexpf(v[0], -e);
+        reinterpret_cast<uint16_t*>(row)[k] = (PREC == PREC_BF16) ? f2bf(sv) : f2h(sv);
+      }
+    };
 #pragma unroll
     for (int q = 0; q < FPW; ++q) {
       const int fl = wave * FPW + q;
       const int fr = f0 + fl;
       unsigned char* row = row_ptr(fl);
       if (fr < T) {
+        if constexpr (RS) {
 #pragma unroll
-        for (int i = 0; i < NI; ++i) {
-          const int k = lane + 64 * i;
-          if ((M / 2 + 1) % 64 == 0 || k <= M / 2) {
-            if constexpr (CPLX) {
-              reinterpret_cast<float2*>(row)[k] = make_float2(hk[q][i][0], hk[q][i][1]);
-              reinterpret_cast<float2*>(row)[M - k] = make_float2(hm[q][i][0], hm[q][i][1]);
-            } else if constexpr (SPEC || PREC == PREC_F32) {
-              reinterpret_cast<float*>(row)[k] = hk[q][i][0];
-              reinterpret_cast<float*>(row)[M - k] = hm[q][i][0];
-            } else {
-              const float a0 = ldexpf(hk[q][i][0], -fexp[q]);
-              const float a1 = ldexpf(hm[q][i][0], -fexp[q]);
-              uint16_t* r16 = reinterpret_cast<uint16_t*>(row);
-              r16[k] = (PREC == PREC_BF16) ? f2bf(a0) : f2h(a0);
-              r16[M - k] = (PREC == PREC_BF16) ? f2bf(a1) : f2h(a1);
+          for (int b = 0; b < QL; ++b)
+#pragma unroll
+            for (int r = 0; r < RL; ++r) put(row, lane + 64 * b + SH::BL * r, hv[q][b][r], fexp[q]);
+          if (lane == 0) put(row, M, hn[q], fexp[q]);
+        } else {
+#pragma unroll
+          for (int i = 0; i < NI; ++i) {
+            const int k = lane + 64 * i;
+            if ((M / 2 + 1) % 64 == 0 || k <= M / 2) {
+              put(row, k, hk[q][i], fexp[q]);
+              put(row, M - k, hm[q][i], fexp[q]);
             }
           }
         }
@@ -482,10 +666,10 @@ __global__ __launch_bounds__(kWgThreads) MM_TILE_OCC void k_tile(TileArgs a) {
             if constexpr (PREC == PREC_F32) reinterpret_cast<float*>(row)[k] = 0.0f;
             else reinterpret_cast<uint16_t*>(row)[k] = 0;
           }
-          if (lane == 0) sc_exp[fl] = fexp[q];
         }
-      } else if constexpr (!SPEC) {
-        if (lane == 0) sc_exp[fl] = 0;
+      }
+      if constexpr (!SPEC) {
+        if (lane == 0) sc_exp[fl] = (fr < T) ? fexp[q] : 0;
       }
     }
     __syncthreads();
@@ -494,17 +678,17 @@ __global__ __launch_bounds__(kWgThreads) MM_TILE_OCC void k_tile(TileArgs a) {
       // ---- transposed write-out: [F, T], runs of TF frames per bin ----
       const int nf = min(TF, T - f0);
       const int64_t base = a.out_off[clip];
-      for (int idx = threadIdx.x; idx < F * TF; idx += kWgThreads) {
+      for (int idx = threadIdx.x; idx < F * TF; idx += NT) {
         const int f = idx % TF;
         const int k = idx / TF;
         if (f < nf) {
           const unsigned char* row = row_ptr(f);
           if constexpr (CPLX) {
-            float2 v = reinterpret_cast<const float2*>(row)[k];
-            reinterpret_cast<float2*>(a.out)[base + (int64_t)k * T + f0 + f] = v;
+            reinterpret_cast<float2*>(a.out)[base + (int64_t)k * T + f0 + f] =
+                reinterpret_cast<const float2*>(row)[k];
           } else {
-            float v = reinterpret_cast<const float*>(row)[k];
-            reinterpret_cast<float*>(a.out)[base + (int64_t)k * T + f0 + f] = v;
+            reinterpret_cast<float*>(a.out)[base + (int64_t)k * T + f0 + f] =
+                reinterpret_cast<const float*>(row)[k];
           }
         }
       }
@@ -514,15 +698,16 @@ __global__ __launch_bounds__(kWgThreads) MM_TILE_OCC void k_tile(TileArgs a) {
       // ---- banded mel contraction on MFMA ----
       const int ri = lane & 15;          // MFMA row/col index within 16
       const int g = lane >> 4;           // lane group (K slice)
-      const unsigned char* prow = row_ptr(ri);
-      const int myexp = sc_exp[ri];
-      const bool frame_ok = (f0 + ri) < T;
+      // columns past an 8-frame tile read row 0 and are never stored
+      const unsigned char* prow = row_ptr(ri < TF ? ri : 0);
+      const int myexp = (ri < TF) ? sc_exp[ri] : 0;
+      const bool frame_ok = (ri < TF) && (f0 + ri) < T;
       f32x4 dbv[MAXB];
       float lmax = -INFINITY, lmin = INFINITY;
       int nb = 0;
 #pragma unroll
       for (int bi = 0; bi < MAXB; ++bi) {
-        const int b = wave + kWaves * bi;
+        const int b = wave + NW * bi;
         if (b < a.nmb) {
           nb = bi + 1;
           const int2 bd = a.band[b];
@@ -571,7 +756,7 @@ __global__ __launch_bounds__(kWgThreads) MM_TILE_OCC void k_tile(TileArgs a) {
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
               const int m = 16 * b + 4 * g + j;
-              float d = 10.0f * log10f(fmaxf(acc[j], a.amin) / a.ref);
+              float d = db10(fmaxf(acc[j], a.amin) / a.ref);
               d = fmaxf(d, thr);
               if (m < a.n_mels && frame_ok) {
                 lmax = fmaxf(lmax, d);
@@ -584,71 +769,74 @@ __global__ __launch_bounds__(kWgThreads) MM_TILE_OCC void k_tile(TileArgs a) {
       }
 
       if constexpr (MODE == OUT_MFCC) {
+        const int nact = min(NW, a.nmb);   // waves holding mel blocks
         lmax = wave_max(lmax);
         lmin = wave_min(lmin);
         if (lane == 0) { red_max[wave] = lmax; red_min[wave] = lmin; }
         __syncthreads();   // staging reads done; region memory is reused below
-        float* red = reinterpret_cast<float*>(lds);   // [ndt][8 waves][64 lanes][4]
-        for (int ct = 0; ct < a.ndt; ++ct) {
-          f32x4 accd = {0.0f, 0.0f, 0.0f, 0.0f};
+        float* red = reinterpret_cast<float*>(lds);   // [ndt][nact][64 lanes][4]
+        if (wave < nact) {
+          for (int ct = 0; ct < a.ndt; ++ct) {
+            f32x4 accd = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-          for (int bi = 0; bi < MAXB; ++bi) {
-            if (bi < nb) {
-              const int b = wave + kWaves * bi;
-              if constexpr (PREC == PREC_F32) {
-                const float* dt = a.dct32 + ((int64_t)(ct * a.nmb + b) * 4) * 64;
+            for (int bi = 0; bi < MAXB; ++bi) {
+              if (bi < nb) {
+                const int b = wave + NW * bi;
+                if constexpr (PREC == PREC_F32) {
+                  const float* dt = a.dct32 + ((int64_t)(ct * a.nmb + b) * 4) * 64;
 #pragma unroll
-                for (int j = 0; j < 4; ++j)
-                  accd = __builtin_amdgcn_mfma_f32_16x16x4f32(dt[j * 64 + lane], dbv[bi][j], accd, 0, 0, 0);
-              } else {
-                const int64_t off = ((int64_t)(ct * a.nmb + b) * 64 + lane) * 4;
-                const uint2 hi2 = *reinterpret_cast<const uint2*>(a.dct_hi + off);
-                const uint2 lo2 = *reinterpret_cast<const uint2*>(a.dct_lo + off);
-                uint16_t bh[4], bl[4];
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                  const float d = dbv[bi][j];
-                  if constexpr (PREC == PREC_BF16) {
-                    bh[j] = f2bf(d);
-                    bl[j] = f2bf(d - bf2f(bh[j]));
-                  } else {
-                    bh[j] = f2h(d);
-                    bl[j] = f2h(d - h2f(bh[j]));
-                  }
-                }
-                uint2 bhi2 = make_uint2(bh[0] | ((unsigned)bh[1] << 16), bh[2] | ((unsigned)bh[3] << 16));
-                uint2 blo2 = make_uint2(bl[0] | ((unsigned)bl[1] << 16), bl[2] | ((unsigned)bl[3] << 16));
-                if constexpr (PREC == PREC_BF16) {
-                  const s16x4 Ah = __builtin_bit_cast(s16x4, hi2), Al = __builtin_bit_cast(s16x4, lo2);
-                  const s16x4 Bh = __builtin_bit_cast(s16x4, bhi2), Bl = __builtin_bit_cast(s16x4, blo2);
-                  accd = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Ah, Bh, accd, 0, 0, 0);
-                  accd = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Al, Bh, accd, 0, 0, 0);
-                  accd = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Ah, Bl, accd, 0, 0, 0);
+                  for (int j = 0; j < 4; ++j)
+                    accd = __builtin_amdgcn_mfma_f32_16x16x4f32(dt[j * 64 + lane], dbv[bi][j], accd, 0, 0, 0);
                 } else {
-                  const f16x4 Ah = __builtin_bit_cast(f16x4, hi2), Al = __builtin_bit_cast(f16x4, lo2);
-                  const f16x4 Bh = __builtin_bit_cast(f16x4, bhi2), Bl = __builtin_bit_cast(f16x4, blo2);
-                  accd = __builtin_amdgcn_mfma_f32_16x16x16f16(Ah, Bh, accd, 0, 0, 0);
-                  accd = __builtin_amdgcn_mfma_f32_16x16x16f16(Al, Bh, accd, 0, 0, 0);
-                  accd = __builtin_amdgcn_mfma_f32_16x16x16f16(Ah, Bl, accd, 0, 0, 0);
+                  const int64_t off = ((int64_t)(ct * a.nmb + b) * 64 + lane) * 4;
+                  const uint2 hi2 = *reinterpret_cast<const uint2*>(a.dct_hi + off);
+                  const uint2 lo2 = *reinterpret_cast<const uint2*>(a.dct_lo + off);
+                  uint16_t bh[4], bl[4];
+#pragma unroll
+                  for (int j = 0; j < 4; ++j) {
+                    const float d = dbv[bi][j];
+                    if constexpr (PREC == PREC_BF16) {
+                      bh[j] = f2bf(d);
+                      bl[j] = f2bf(d - bf2f(bh[j]));
+                    } else {
+                      bh[j] = f2h(d);
+                      bl[j] = f2h(d - h2f(bh[j]));
+                    }
+                  }
+                  const uint2 bhi2 = make_uint2(bh[0] | ((unsigned)bh[1] << 16), bh[2] | ((unsigned)bh[3] << 16));
+                  const uint2 blo2 = make_uint2(bl[0] | ((unsigned)bl[1] << 16), bl[2] | ((unsigned)bl[3] << 16));
+                  if constexpr (PREC == PREC_BF16) {
+                    const s16x4 Ah = __builtin_bit_cast(s16x4, hi2), Al = __builtin_bit_cast(s16x4, lo2);
+                    const s16x4 Bh = __builtin_bit_cast(s16x4, bhi2), Bl = __builtin_bit_cast(s16x4, blo2);
+                    accd = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Ah, Bh, accd, 0, 0, 0);
+                    accd = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Al, Bh, accd, 0, 0, 0);
+                    accd = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Ah, Bl, accd, 0, 0, 0);
+                  } else {
+                    const f16x4 Ah = __builtin_bit_cast(f16x4, hi2), Al = __builtin_bit_cast(f16x4, lo2);
+                    const f16x4 Bh = __builtin_bit_cast(f16x4, bhi2), Bl = __builtin_bit_cast(f16x4, blo2);
+                    accd = __builtin_amdgcn_mfma_f32_16x16x16f16(Ah, Bh, accd, 0, 0, 0);
+                    accd = __builtin_amdgcn_mfma_f32_16x16x16f16(Al, Bh, accd, 0, 0, 0);
+                    accd = __builtin_amdgcn_mfma_f32_16x16x16f16(Ah, Bl, accd, 0, 0, 0);
+                  }
                 }
               }
             }
+            float* dst = red + ((int64_t)(ct * nact + wave) * 64 + lane) * 4;
+            *reinterpret_cast<float4*>(dst) = make_float4(accd[0], accd[1], accd[2], accd[3]);
           }
-          float* dst = red + ((int64_t)(ct * kWaves + wave) * 64 + lane) * 4;
-          *reinterpret_cast<float4*>(dst) = make_float4(accd[0], accd[1], accd[2], accd[3]);
         }
         __syncthreads();
         if (threadIdx.x == 0 && a.phase == 1 && a.use_top_db) {
           float bm = -INFINITY, bn = INFINITY;
-          for (int w = 0; w < kWaves; ++w) { bm = fmaxf(bm, red_max[w]); bn = fminf(bn, red_min[w]); }
+          for (int w = 0; w < NW; ++w) { bm = fmaxf(bm, red_max[w]); bn = fminf(bn, red_min[w]); }
           if (bm > -INFINITY) atomicMax(&a.clip_max_key[clip], float_to_key(bm));
           a.tile_min[tile] = bn;
         }
         // fixed-order cross-wave sum; wave ct writes DCT rows 16ct .. 16ct+15
-        for (int ct = wave; ct < a.ndt; ct += kWaves) {
+        for (int ct = wave; ct < a.ndt; ct += NW) {
           float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
-          for (int w = 0; w < kWaves; ++w) {
-            const float4 p = *reinterpret_cast<const float4*>(red + ((int64_t)(ct * kWaves + w) * 64 + lane) * 4);
+          for (int w = 0; w < nact; ++w) {
+            const float4 p = *reinterpret_cast<const float4*>(red + ((int64_t)(ct * nact + w) * 64 + lane) * 4);
             s.x += p.x; s.y += p.y; s.z += p.z; s.w += p.w;
           }
           if (frame_ok) {
@@ -750,7 +938,7 @@ __global__ void k_generic_mel(const float2* __restrict__ spec, int F, int T, con
   float s = 0.0f;
   for (int k = 0; k < F; ++k) {
     const float wk = w[k];
-    if (wk != 0.0f) s = fmaf(wk, spec_value(X[k], power), s);
+    if (wk != 0.0f) s = fmaf(wk, pow_from_p2(fmaf(X[k].x, X[k].x, X[k].y * X[k].y), power), s);
   }
   mel[i] = s;
 }
@@ -819,7 +1007,7 @@ __global__ void k_tab_window(float* w, int n_fft, int win) {
 __global__ void k_tab_twiddle(float2* tw, int M, float2* twN, int n_fft) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < M) tw[t] = make_float2((float)cospi(2.0 * t / M), (float)(-sinpi(2.0 * t / M)));
-  if (t <= M / 2) twN[t] = make_float2((float)cospi(2.0 * t / n_fft), (float)(-sinpi(2.0 * t / n_fft)));
+  if (t < M || t == 0) twN[t] = make_float2((float)cospi(2.0 * t / n_fft), (float)(-sinpi(2.0 * t / n_fft)));
 }
 
 __global__ void k_tab_mel(float* fb32, uint16_t* fbb, uint16_t* fbh, int fbs, int rows, int sr, int n_fft,
@@ -923,39 +1111,54 @@ hipError_t gen_tables(const TableKey& key, const DeviceTables& t, hipStream_t st
 
 // ---- fused kernel dispatch ----
 
-template <int LOG2M, int MODE, int PREC, int FPW>
+template <int LOG2M, int MODE, int PREC, int NW, int FPW, typename TIn>
 static hipError_t launch_tile_t(const TileArgs& a, int grid, size_t lds, hipStream_t st) {
-  auto fn = k_tile<LOG2M, MODE, PREC, FPW>;
+  auto fn = k_tile<LOG2M, MODE, PREC, NW, FPW, TIn>;
   // set once per instantiation (thread-safe static init); grid == 0 only
   // prepares the attribute, so stream capture never sees the first call
   static const hipError_t attr = hipFuncSetAttribute(
       reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_limit_bytes());
   if (attr != hipSuccess) return attr;
   if (grid <= 0) return hipSuccess;
-  hipLaunchKernelGGL(fn, dim3(grid), dim3(kWgThreads), lds, st, a);
+  hipLaunchKernelGGL(fn, dim3(grid), dim3(NW * 64), lds, st, a);
   return hipGetLastError();
 }
 
+// (NW, FPW): 16 x 1 up to n_fft 2048; 8 x 1 (8-frame tiles) at 4096
+template <int LOG2M, int MODE, int PREC, typename TIn>
+static hipError_t launch_tile_s(const TileArgs& a, const TileLayout& L, int grid, hipStream_t st) {
+  if constexpr (LOG2M <= 10) {
+    return launch_tile_t<LOG2M, MODE, PREC, 16, 1, TIn>(a, grid, L.lds_bytes, st);
+  } else {
+    return launch_tile_t<LOG2M, MODE, PREC, 8, 1, TIn>(a, grid, L.lds_bytes, st);
+  }
+}
+
 template <int LOG2M>
-static hipError_t launch_tile_m(const TileArgs& a, int prec, int fpw, int grid, size_t lds, hipStream_t st) {
+static hipError_t launch_tile_m(const TileArgs& a, int prec, const TileLayout& L, int grid, hipStream_t st) {
+  const bool h = (a.in_dtype == IN_F16);
   switch (a.out_kind) {
-    case OUT_STFT_MAG:
-      return fpw == 2 ? launch_tile_t<LOG2M, OUT_STFT_MAG, PREC_F32, 2>(a, grid, lds, st)
-                      : launch_tile_t<LOG2M, OUT_STFT_MAG, PREC_F32, 1>(a, grid, lds, st);
-    case OUT_STFT_POWER:
-      return fpw == 2 ? launch_tile_t<LOG2M, OUT_STFT_POWER, PREC_F32, 2>(a, grid, lds, st)
-                      : launch_tile_t<LOG2M, OUT_STFT_POWER, PREC_F32, 1>(a, grid, lds, st);
-    case OUT_STFT_COMPLEX:
-      return fpw == 2 ? launch_tile_t<LOG2M, OUT_STFT_COMPLEX, PREC_F32, 2>(a, grid, lds, st)
-                      : launch_tile_t<LOG2M, OUT_STFT_COMPLEX, PREC_F32, 1>(a, grid, lds, st);
+    case OUT_STFT_MAG: return launch_tile_s<LOG2M, OUT_STFT_MAG, PREC_F32, float>(a, L, grid, st);
+    case OUT_STFT_POWER: return launch_tile_s<LOG2M, OUT_STFT_POWER, PREC_F32, float>(a, L, grid, st);
+    case OUT_STFT_COMPLEX: return launch_tile_s<LOG2M, OUT_STFT_COMPLEX, PREC_F32, float>(a, L, grid, st);
     case OUT_MEL:
-      if (prec == PREC_BF16) return launch_tile_t<LOG2M, OUT_MEL, PREC_BF16, 2>(a, grid, lds, st);
-      if (prec == PREC_F16) return launch_tile_t<LOG2M, OUT_MEL, PREC_F16, 2>(a, grid, lds, st);
-      return launch_tile_t<LOG2M, OUT_MEL, PREC_F32, 2>(a, grid, lds, st);
+      if (prec == PREC_BF16)
+        return h ? launch_tile_s<LOG2M, OUT_MEL, PREC_BF16, _Float16>(a, L, grid, st)
+                 : launch_tile_s<LOG2M, OUT_MEL, PREC_BF16, float>(a, L, grid, st);
+      if (prec == PREC_F16)
+        return h ? launch_tile_s<LOG2M, OUT_MEL, PREC_F16, _Float16>(a, L, grid, st)
+                 : launch_tile_s<LOG2M, OUT_MEL, PREC_F16, float>(a, L, grid, st);
+      return h ? launch_tile_s<LOG2M, OUT_MEL, PREC_F32, _Float16>(a, L, grid, st)
+               : launch_tile_s<LOG2M, OUT_MEL, PREC_F32, float>(a, L, grid, st);
     case OUT_MFCC:
-      if (prec == PREC_BF16) return launch_tile_t<LOG2M, OUT_MFCC, PREC_BF16, 2>(a, grid, lds, st);
-      if (prec == PREC_F16) return launch_tile_t<LOG2M, OUT_MFCC, PREC_F16, 2>(a, grid, lds, st);
-      return launch_tile_t<LOG2M, OUT_MFCC, PREC_F32, 2>(a, grid, lds, st);
+      if (prec == PREC_BF16)
+        return h ? launch_tile_s<LOG2M, OUT_MFCC, PREC_BF16, _Float16>(a, L, grid, st)
+                 : launch_tile_s<LOG2M, OUT_MFCC, PREC_BF16, float>(a, L, grid, st);
+      if (prec == PREC_F16)
+        return h ? launch_tile_s<LOG2M, OUT_MFCC, PREC_F16, _Float16>(a, L, grid, st)
+                 : launch_tile_s<LOG2M, OUT_MFCC, PREC_F16, float>(a, L, grid, st);
+      return h ? launch_tile_s<LOG2M, OUT_MFCC, PREC_F32, _Float16>(a, L, grid, st)
+               : launch_tile_s<LOG2M, OUT_MFCC, PREC_F32, float>(a, L, grid, st);
   }
   return hipErrorInvalidValue;
 }
@@ -969,14 +1172,15 @@ bool tile_layout(int n_fft, int out_kind, int prec, int n_mels, int n_mfcc, Tile
   const int M = n_fft / 2, F = M + 1;
   const bool spec = out_kind <= OUT_STFT_COMPLEX;
   if (!spec) {
-    if (n_mels < 1 || n_mels > 16 * kWaves * 4) return false;
+    if (n_mels < 1 || n_mels > 512) return false;
     if (out_kind == OUT_MFCC && (n_mfcc < 1 || n_mfcc > 128)) return false;
   }
+  const int nw = (M > 1024) ? 8 : 16, fpw = 1;
   const int scr = (M + M / 16) * 8;
   int esz, S;
   if (spec) {
     esz = (out_kind == OUT_STFT_COMPLEX) ? 8 : 4;
-    S = F + ((out_kind == OUT_STFT_COMPLEX) ? 1 : 1);
+    S = F + 1;
   } else if (prec == PREC_F32) {
     esz = 4;
     const int need = ((F + 31) / 32) * 32 + 64;
@@ -986,43 +1190,37 @@ bool tile_layout(int n_fft, int out_kind, int prec, int n_mels, int n_mfcc, Tile
     const int need = ((F + 31) / 32) * 32 + 64;
     S = need + ((8 - need % 128) + 128) % 128;       // S/2 = 4 (mod 64) dwords
   }
-  int fpw = 2;
-  auto region = [&](int f) {
-    int rb = std::max(f * S * esz, scr);
-    return (rb + 15) / 16 * 16;
-  };
-  int rb = region(fpw);
-  if (spec && (size_t)kWaves * rb + 256 > lds_limit_bytes()) {
-    fpw = 1;
-    rb = region(fpw);
-  }
-  const size_t total = (size_t)kWaves * rb + 16 * 4 + 2 * kWaves * 4;
+  int rb = std::max(fpw * S * esz, scr);
+  rb = (rb + 15) / 16 * 16;
+  const size_t total = (size_t)nw * rb + 16 * 4 + 2 * nw * 4;
   if (total > lds_limit_bytes()) return false;
   // the MFCC cross-wave reduction reuses the regions
   if (out_kind == OUT_MFCC) {
     const int ndt = (n_mfcc + 15) / 16;
-    if ((size_t)ndt * kWaves * 256 * 4 > (size_t)kWaves * rb) return false;
+    const int nact = std::min(nw, (n_mels + 15) / 16);
+    if ((size_t)ndt * nact * 256 * 4 > (size_t)nw * rb) return false;
   }
+  L->nw = nw;
   L->fpw = fpw;
   L->row_stride = S;
   L->region_bytes = rb;
   L->lds_bytes = total;
-  L->tile_frames = kWaves * fpw;
+  L->tile_frames = nw * fpw;
   return true;
 }
 
 hipError_t launch_tile(const TileArgs& a, int prec, const TileLayout& L, int grid, hipStream_t st) {
   const int log2m = __builtin_ctz(a.n_fft) - 1;
 #ifdef MM_DEV_ONLY_LOG2M  // development builds: one FFT size only
-  if (log2m == MM_DEV_ONLY_LOG2M) return launch_tile_m<MM_DEV_ONLY_LOG2M>(a, prec, L.fpw, grid, L.lds_bytes, st);
+  if (log2m == MM_DEV_ONLY_LOG2M) return launch_tile_m<MM_DEV_ONLY_LOG2M>(a, prec, L, grid, st);
   return hipErrorInvalidValue;
 #else
   switch (log2m) {
-    case 7: return launch_tile_m<7>(a, prec, L.fpw, grid, L.lds_bytes, st);
-    case 8: return launch_tile_m<8>(a, prec, L.fpw, grid, L.lds_bytes, st);
-    case 9: return launch_tile_m<9>(a, prec, L.fpw, grid, L.lds_bytes, st);
-    case 10: return launch_tile_m<10>(a, prec, L.fpw, grid, L.lds_bytes, st);
-    case 11: return launch_tile_m<11>(a, prec, L.fpw, grid, L.lds_bytes, st);
+    case 7: return launch_tile_m<7>(a, prec, L, grid, st);
+    case 8: return launch_tile_m<8>(a, prec, L, grid, st);
+    case 9: return launch_tile_m<9>(a, prec, L, grid, st);
+    case 10: return launch_tile_m<10>(a, prec, L, grid, st);
+    case 11: return launch_tile_m<11>(a, prec, L, grid, st);
   }
   return hipErrorInvalidValue;
 #endif
