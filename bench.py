@@ -71,6 +71,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain single-clip leg")
     ap.add_argument("--profile", action="store_true", help="short run for a profiler")
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="profiling aid: also run one device-to-device copy of this many bytes, "
+                         "so HBM counters can be scaled by a kernel of known traffic")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the batch shards by clip with no exchange, "
@@ -115,6 +118,15 @@ def main():
     value = frames * steps / dt
     ms_step = dt * 1e3 / steps
     log(f"bf16 batched: {value / 1e6:.1f} M frames/s ({ms_step:.3f} ms/step)")
+
+    if args.copy_bytes > 0:
+        src = torch.empty(args.copy_bytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {args.copy_bytes} bytes read, {args.copy_bytes} bytes written")
+        del src, dst
 
     # fused kernel time from HIP events on its own stream (the plan's stream)
     k_ms, s_ms = B.time(d_in, out, iters=5 if args.profile else 20)

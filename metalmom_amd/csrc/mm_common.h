@@ -29,6 +29,7 @@ struct DeviceTables {
   float* window = nullptr;      // [n_fft]
   float2* tw = nullptr;         // [M] W_M^t, M = n_fft/2
   float2* twN = nullptr;        // [M] W_N^k, N = n_fft
+  float2* twp = nullptr;        // per-pass Stockham twiddles of the fused FFT (< M entries)
   float* fb32 = nullptr;        // [nmb*16][fbs] mel weights, zero padded
   uint16_t* fb16 = nullptr;     // same in bf16 or f16 (see fb16_kind)
   uint16_t* fbh = nullptr;      // f16 copy
@@ -63,6 +64,7 @@ struct TileArgs {
   const float* window;
   const float2* tw;
   const float2* twN;
+  const float2* twp;
   const float* fb32;
   const uint16_t* fb16;
   const int2* band;
@@ -73,6 +75,7 @@ struct TileArgs {
   int ndt, n_mfcc;
   // value transforms
   float power, ref, amin, top_db;
+  float post_scale;            // 2^-power: the fused kernel contracts (2|X|)^power
   int use_top_db;
   int phase;                   // 1: unclamped pass, 2: recompute tiles whose clamp is active
   unsigned* clip_max_key;      // per clip, ordered-int encoding of the max dB

@@ -4,6 +4,7 @@
 #include "mm_engine.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -147,6 +148,7 @@ static int32_t make_tables(const TableKey& key, hipStream_t st, std::unique_ptr<
   const size_t o_win = take((size_t)key.n_fft * 4);
   const size_t o_tw = take((size_t)M * 8);
   const size_t o_twn = take((size_t)M * 8);
+  const size_t o_twp = take((size_t)M * 8);   // per-pass tables total < M entries
   const size_t o_fb32 = take((size_t)rows * t.fbs * 4);
   const size_t o_fb16 = take((size_t)rows * t.fbs * 2);
   const size_t o_fbh = take((size_t)rows * t.fbs * 2);
@@ -165,6 +167,7 @@ static int32_t make_tables(const TableKey& key, hipStream_t st, std::unique_ptr<
   t.window = reinterpret_cast<float*>(b + o_win);
   t.tw = reinterpret_cast<float2*>(b + o_tw);
   t.twN = reinterpret_cast<float2*>(b + o_twn);
+  t.twp = reinterpret_cast<float2*>(b + o_twp);
   t.fb32 = reinterpret_cast<float*>(b + o_fb32);
   t.fb16 = reinterpret_cast<uint16_t*>(b + o_fb16);
   t.fbh = reinterpret_cast<uint16_t*>(b + o_fbh);
@@ -455,6 +458,7 @@ static TileArgs make_args(const SpecRequest& r, const Tables* tb, const BatchMet
   a.window = t.window;
   a.tw = t.tw;
   a.twN = t.twN;
+  a.twp = t.twp;
   a.fb32 = t.fb32;
   a.fb16 = (r.precision == PREC_F16) ? t.fbh : t.fb16;
   a.band = (r.precision == PREC_F32) ? t.band32 : t.band16;
@@ -467,6 +471,7 @@ static TileArgs make_args(const SpecRequest& r, const Tables* tb, const BatchMet
   a.ndt = t.ndt;
   a.n_mfcc = std::min(r.n_mfcc, r.n_mels);
   a.power = r.power;
+  a.post_scale = std::exp2(-r.power);
   a.ref = r.ref;
   a.amin = r.amin;
   a.top_db = r.top_db;

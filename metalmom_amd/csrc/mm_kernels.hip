@@ -40,13 +40,21 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 w) {
 // a * (-i)
 __device__ __forceinline__ float2 mul_mi(float2 a) { return make_float2(a.y, -a.x); }
 
-// Round-to-nearest-even float -> bf16 bits (NaN stays NaN).
+// Round-to-nearest-even float -> bf16 bits (NaN stays NaN); one hardware
+// conversion on gfx950.
 __host__ __device__ inline uint16_t f2bf(float f) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const __bf16 h = (__bf16)f;
+  uint16_t b;
+  __builtin_memcpy(&b, &h, 2);
+  return b;
+#else
   unsigned u;
   __builtin_memcpy(&u, &f, 4);
   if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40);
   u += 0x7fffu + ((u >> 16) & 1u);
   return (uint16_t)(u >> 16);
+#endif
 }
 __host__ __device__ inline float bf2f(uint16_t h) {
   unsigned u = ((unsigned)h) << 16;
@@ -70,6 +78,13 @@ __device__ inline float h2f(uint16_t b) {
 // wavefront LDS instructions execute in issue order, so no hardware wait is
 // needed for wave-private scratch.
 __device__ __forceinline__ void wave_lds_fence() { __asm__ __volatile__("" ::: "memory"); }
+
+// A copy of v the compiler cannot see through: computations derived from it
+// stay after this point instead of being scheduled (and kept live) earlier.
+__device__ __forceinline__ int opaque(int v) {
+  __asm__ __volatile__("" : "+v"(v));
+  return v;
+}
 
 // Padded LDS index: one spare complex slot per 16 keeps the strided radix-16
 // accesses of a Stockham pass on distinct banks.
@@ -190,9 +205,27 @@ template <int LOG2M> struct FftShape {
   static constexpr bool REG_SPLIT = (NPASS >= 2) && (BL % 64 == 0);
 };
 
+// Per-pass twiddle tables: pass p (p >= 1, NS = 16^p, radix R_p) owns
+// (R_p - 1) * NS entries laid out [r-1][k] = W_{NS R_p}^{k r}, so one lane
+// reaches all of its pass twiddles from one base address plus constants.
+template <int M, int P> struct PassTw {
+  static constexpr int LOG2M = __builtin_ctz(M);
+  static constexpr int NP16 = LOG2M / 4;
+  static constexpr int TAIL = 1 << (LOG2M % 4);
+  static constexpr int R = (P <= NP16 - 1) ? 16 : TAIL;   // pass P's radix (P >= 1)
+  static constexpr int NS = 1 << (4 * P);
+  static constexpr int OFF = PassTw<M, P - 1>::OFF + PassTw<M, P - 1>::SIZE;
+  static constexpr int SIZE = (R - 1) * NS;
+};
+template <int M> struct PassTw<M, 0> {
+  static constexpr int OFF = 0;
+  static constexpr int SIZE = 0;
+};
+
 // One Stockham pass reading and writing LDS. NS = product of earlier radices.
 template <int M, int R, int NS>
-__device__ __forceinline__ void stockham_pass(float2* scr, const float2* __restrict__ tw, int lane) {
+__device__ __forceinline__ void stockham_pass(float2* scr, const float2* __restrict__ tw, int lane_in) {
+  const int lane = opaque(lane_in);
   constexpr int B = M / R;
   constexpr int QB = (B + 63) / 64;
   float2 v[QB][R];
@@ -209,10 +242,11 @@ __device__ __forceinline__ void stockham_pass(float2* scr, const float2* __restr
   for (int q = 0; q < QB; ++q) {
     const int j = lane + 64 * q;
     if ((B % 64 == 0) || j < B) {
-      const int k = j % NS;
-      constexpr int STRIDE = M / (NS * R);
+      const unsigned k = (unsigned)j % NS;
+      constexpr int P = __builtin_ctz(NS) / 4;
+      const float2* t = tw + PassTw<M, P>::OFF + k;
 #pragma unroll
-      for (int r = 1; r < R; ++r) v[q][r] = cmul(v[q][r], tw[(k * r) * STRIDE]);
+      for (int r = 1; r < R; ++r) v[q][r] = cmul(v[q][r], t[(r - 1) * NS]);
       Dft<R>::run(v[q]);
       const int base = (j / NS) * NS * R + k;
 #pragma unroll
@@ -241,8 +275,9 @@ struct PassChain {
 // Last Stockham pass kept in registers: z[q][r] = Z[j + BL*r], j = lane + 64 q
 // (its natural-order outputs are exactly the butterfly outputs).
 template <int M, int R, int NS>
-__device__ __forceinline__ void stockham_last_regs(const float2* scr, const float2* __restrict__ tw, int lane,
+__device__ __forceinline__ void stockham_last_regs(const float2* scr, const float2* __restrict__ tw, int lane_in,
                                                    float2 (&z)[M / R / 64][R]) {
+  const int lane = opaque(lane_in);
   constexpr int B = M / R;
   constexpr int QB = B / 64;
 #pragma unroll
@@ -254,10 +289,11 @@ __device__ __forceinline__ void stockham_last_regs(const float2* scr, const floa
   wave_lds_fence();
 #pragma unroll
   for (int q = 0; q < QB; ++q) {
-    const int k = (lane + 64 * q) % NS;
-    constexpr int STRIDE = M / (NS * R);
+    const unsigned k = (unsigned)(lane + 64 * q) % NS;
+    constexpr int P = __builtin_ctz(NS) / 4;
+    const float2* t = tw + PassTw<M, P>::OFF + k;
 #pragma unroll
-    for (int r = 1; r < R; ++r) z[q][r] = cmul(z[q][r], tw[(k * r) * STRIDE]);
+    for (int r = 1; r < R; ++r) z[q][r] = cmul(z[q][r], t[(r - 1) * NS]);
     Dft<R>::run(z[q]);
   }
 }
@@ -287,8 +323,9 @@ template <> struct SamplePair<_Float16> {
 // are staged through the scratch by a plain loop.
 template <int LOG2M, typename TIn>
 __device__ __forceinline__ void frame_load(const TileArgs& a, const TIn* __restrict__ x, int64_t len, int64_t s0,
-                                           float2* scr, int lane,
+                                           float2* scr, int lane_in,
                                            float2 (&v)[FftShape<LOG2M>::Q0][FftShape<LOG2M>::R0]) {
+  const int lane = opaque(lane_in);
   using S = FftShape<LOG2M>;
   constexpr int M = S::M, R0 = S::R0, B0 = S::B0, Q0 = S::Q0;
   const bool interior = (s0 >= 0) && (s0 + 2 * M <= len);
@@ -300,7 +337,7 @@ __device__ __forceinline__ void frame_load(const TileArgs& a, const TIn* __restr
       if ((B0 % 64 == 0) || j < B0) {
 #pragma unroll
         for (int r = 0; r < R0; ++r) {
-          const int n = j + r * B0;
+          const unsigned n = (unsigned)(j + r * B0);
           const float2 w = *reinterpret_cast<const float2*>(a.window + 2 * n);
           const float2 xs = SamplePair<TIn>::load(xf + 2 * n);
           v[q][r] = make_float2(xs.x * w.x, xs.y * w.y);
@@ -315,7 +352,7 @@ __device__ __forceinline__ void frame_load(const TileArgs& a, const TIn* __restr
       if ((B0 % 64 == 0) || j < B0) {
 #pragma unroll
         for (int r = 0; r < R0; ++r) {
-          const int n = j + r * B0;
+          const unsigned n = (unsigned)(j + r * B0);
           const float2 w = *reinterpret_cast<const float2*>(a.window + 2 * n);
           v[q][r] = make_float2((float)xf[2 * n] * w.x, (float)xf[2 * n + 1] * w.y);
         }
@@ -364,7 +401,7 @@ __device__ __forceinline__ void frame_fft(const TileArgs& a, int clip, int fr, f
     }
   }
   wave_lds_fence();
-  if constexpr (S::NP16 > 0) PassChain<M, 1, S::NP16, S::TAIL, S::NPASS>::run(scr, a.tw, lane);
+  if constexpr (S::NP16 > 0) PassChain<M, 1, S::NP16, S::TAIL, S::NPASS>::run(scr, a.twp, lane);
 }
 
 // Same FFT, but the last pass stays in registers: z[q][r] = Z[lane + 64 q + BL r].
@@ -377,6 +414,7 @@ __device__ __forceinline__ void frame_fft_regs(const TileArgs& a, int clip, int 
   const int64_t s0 = (int64_t)fr * a.hop - (a.center ? M : 0);
   {
     float2 v[Q0][R0];
+    lane = opaque(lane);
     frame_load<LOG2M>(a, static_cast<const TIn*>(a.signal) + a.clip_off[clip], len, s0, scr, lane, v);
 #pragma unroll
     for (int q = 0; q < Q0; ++q) {
@@ -389,22 +427,24 @@ __device__ __forceinline__ void frame_fft_regs(const TileArgs& a, int clip, int 
     }
   }
   wave_lds_fence();
-  PassChain<M, 1, S::NP16, S::TAIL, S::NPASS - 1>::run(scr, a.tw, lane);
+  PassChain<M, 1, S::NP16, S::TAIL, S::NPASS - 1>::run(scr, a.twp, lane);
   constexpr int NSL = M / S::RL;
-  stockham_last_regs<M, S::RL, NSL>(scr, a.tw, lane, z);
+  stockham_last_regs<M, S::RL, NSL>(scr, a.twp, lane, z);
   wave_lds_fence();
 }
 
 // Real-FFT split in registers. Lane l owns Z[k], k = l + 64 q + BL r; its
 // mirror Z[M-k] sits in lane (64-l) mod 64 at register (QL-1-q, RL-1-r)
 // (lane 0 pairs with itself at (QL-q, RL-1-r), or (0, (RL-r) mod RL) for
-// q = 0), fetched with ds_bpermute. X[k] = E - i W_N^k O as below; lane 0
-// also returns the Nyquist bin X[M] = Re Z[0] - Im Z[0].
+// q = 0), fetched with ds_bpermute. Returns 2 X[k] with X[k] = E - i W_N^k O
+// as below (the factor 2 saves the halvings and is removed by the caller);
+// lane 0 also returns 2 X[M] = 2 (Re Z[0] - Im Z[0]).
 template <int LOG2M>
 __device__ __forceinline__ void real_split_regs(const float2 (&z)[FftShape<LOG2M>::QL][FftShape<LOG2M>::RL],
-                                                const float2* __restrict__ twN, int lane,
+                                                const float2* __restrict__ twN, int lane_in,
                                                 float2 (&x)[FftShape<LOG2M>::QL][FftShape<LOG2M>::RL],
                                                 float& nyq_re) {
+  const int lane = opaque(lane_in);
   using S = FftShape<LOG2M>;
   constexpr int QL = S::QL, RL = S::RL, BL = S::BL;
   const int paddr = ((64 - lane) & 63) << 2;
@@ -418,14 +458,14 @@ __device__ __forceinline__ void real_split_regs(const float2 (&z)[FftShape<LOG2M
       const float2 own = (q == 0) ? z[0][(RL - r) % RL] : z[(QL - q) % QL][RL - 1 - r];
       if (lane == 0) bz = own;
       const float2 A = z[q][r];
-      const float2 E = make_float2(0.5f * (A.x + bz.x), 0.5f * (A.y - bz.y));
-      const float2 O = make_float2(0.5f * (A.x - bz.x), 0.5f * (A.y + bz.y));
-      const int k = lane + 64 * q + BL * r;
+      const float2 E = make_float2(A.x + bz.x, A.y - bz.y);   // 2E
+      const float2 O = make_float2(A.x - bz.x, A.y + bz.y);   // 2O
+      const unsigned k = (unsigned)(lane + 64 * q + BL * r);
       const float2 T = cmul(O, twN[k]);
-      x[q][r] = make_float2(E.x + T.y, E.y - T.x);
+      x[q][r] = make_float2(E.x + T.y, E.y - T.x);            // 2 X[k]
     }
   }
-  nyq_re = z[0][0].x - z[0][0].y;
+  nyq_re = 2.0f * (z[0][0].x - z[0][0].y);                   // 2 X[M]
 }
 
 // Real-FFT split: from Z (M complex) to X[k] and X[M-k] for k = lane + 64 i.
@@ -558,28 +598,28 @@ __global__ __launch_bounds__(NW * 64) MM_TILE_OCC void k_tile(TileArgs a) {
           for (int b = 0; b < QL; ++b)
 #pragma unroll
             for (int r = 0; r < RL; ++r) {
-              const float2 X = x[b][r];
+              const float2 X = x[b][r];   // 2 X[k]
               if constexpr (CPLX) {
-                hv[q][b][r][0] = X.x;
-                hv[q][b][r][EC - 1] = X.y;
+                hv[q][b][r][0] = 0.5f * X.x;
+                hv[q][b][r][EC - 1] = 0.5f * X.y;
               } else {
-                const float p2 = fmaf(X.x, X.x, X.y * X.y);
+                const float p2 = fmaf(X.x, X.x, X.y * X.y);   // 4 |X[k]|^2
                 float v;
-                if constexpr (MODE == OUT_STFT_MAG) v = sqrtf(p2);
-                else if constexpr (MODE == OUT_STFT_POWER) v = p2;
-                else v = pow_from_p2(p2, a.power);
+                if constexpr (MODE == OUT_STFT_MAG) v = 0.5f * sqrtf(p2);
+                else if constexpr (MODE == OUT_STFT_POWER) v = 0.25f * p2;
+                else v = pow_from_p2(p2, a.power);   // 2^p |X|^p: removed after the mel product
                 hv[q][b][r][0] = v;
                 vmax = fmaxf(vmax, v);
               }
             }
           if constexpr (CPLX) {
-            hn[q][0] = nyq;
+            hn[q][0] = 0.5f * nyq;
             hn[q][EC - 1] = 0.0f;
           } else {
             const float p2 = nyq * nyq;
             float v;
-            if constexpr (MODE == OUT_STFT_MAG) v = fabsf(nyq);
-            else if constexpr (MODE == OUT_STFT_POWER) v = p2;
+            if constexpr (MODE == OUT_STFT_MAG) v = 0.5f * fabsf(nyq);
+            else if constexpr (MODE == OUT_STFT_POWER) v = 0.25f * p2;
             else v = pow_from_p2(p2, a.power);
             hn[q][0] = v;
             if (lane == 0) vmax = fmaxf(vmax, v);
@@ -605,8 +645,9 @@ __global__ __launch_bounds__(NW * 64) MM_TILE_OCC void k_tile(TileArgs a) {
                   hk[q][i][0] = pk;
                   hm[q][i][0] = pm;
                 } else {
-                  hk[q][i][0] = pow_from_p2(pk, a.power);
-                  hm[q][i][0] = pow_from_p2(pm, a.power);
+                  // same 2^p scale as the register split (removed after the mel product)
+                  hk[q][i][0] = pow_from_p2(4.0f * pk, a.power);
+                  hm[q][i][0] = pow_from_p2(4.0f * pm, a.power);
                   vmax = fmaxf(vmax, fmaxf(hk[q][i][0], hm[q][i][0]));
                 }
               }
@@ -742,6 +783,8 @@ __global__ __launch_bounds__(NW * 64) MM_TILE_OCC void k_tile(TileArgs a) {
 #pragma unroll
             for (int j = 0; j < 4; ++j) acc[j] = ldexpf(acc[j], myexp);
           }
+#pragma unroll
+          for (int j = 0; j < 4; ++j) acc[j] *= a.post_scale;   // 2^-power
           // acc[j] = mel[16b + 4g + j] of tile frame ri
           if constexpr (MODE == OUT_MEL) {
             if (frame_ok) {
@@ -1010,6 +1053,15 @@ __global__ void k_tab_twiddle(float2* tw, int M, float2* twN, int n_fft) {
   if (t < M || t == 0) twN[t] = make_float2((float)cospi(2.0 * t / n_fft), (float)(-sinpi(2.0 * t / n_fft)));
 }
 
+// entry [(r-1) NS + k] = W_{NS R}^{k r} of one Stockham pass
+__global__ void k_tab_twiddle_pass(float2* twp, int NS, int R) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (R - 1) * NS) return;
+  const int r = i / NS + 1, k = i % NS;
+  const double e = 2.0 * (double)k * (double)r / ((double)NS * (double)R);
+  twp[i] = make_float2((float)cospi(e), (float)(-sinpi(e)));
+}
+
 __global__ void k_tab_mel(float* fb32, uint16_t* fbb, uint16_t* fbh, int fbs, int rows, int sr, int n_fft,
                           int n_mels, double fmin, double fmax) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1095,6 +1147,18 @@ hipError_t gen_tables(const TableKey& key, const DeviceTables& t, hipStream_t st
   hipLaunchKernelGGL(k_tab_window, dim3(nblk(key.n_fft, 256)), dim3(256), 0, st, t.window, key.n_fft, key.win);
   if (M >= 1)
     hipLaunchKernelGGL(k_tab_twiddle, dim3(nblk(M, 256)), dim3(256), 0, st, t.tw, M, t.twN, key.n_fft);
+  {  // per-pass tables of the fused FFT: radix-16 passes then the tail radix
+    const int log2m = (M >= 1) ? __builtin_ctz(M) : 0;
+    const int np16 = log2m / 4, tail = 1 << (log2m % 4);
+    const int npass = np16 + (tail > 1 ? 1 : 0);
+    int off = 0;
+    for (int p = 1; p < npass; ++p) {
+      const int R = (p <= np16 - 1) ? 16 : tail, NS = 1 << (4 * p);
+      hipLaunchKernelGGL(k_tab_twiddle_pass, dim3(nblk((int64_t)(R - 1) * NS, 256)), dim3(256), 0, st, t.twp + off,
+                         NS, R);
+      off += (R - 1) * NS;
+    }
+  }
   const int rows = t.nmb * 16;
   if (rows > 0) {
     hipLaunchKernelGGL(k_tab_mel, dim3(nblk((int64_t)rows * t.fbs, 256)), dim3(256), 0, st, t.fb32, t.fb16, t.fbh,
