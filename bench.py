@@ -71,6 +71,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain single-clip leg")
     ap.add_argument("--profile", action="store_true", help="short run for a profiler")
+    ap.add_argument("--serve-rates", default="1000,5000",
+                    help="Poisson request rates (req/s) for the serving leg; empty to skip")
+    ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="profiling aid: also run one device-to-device copy of this many bytes, "
                          "so HBM counters can be scaled by a kernel of known traffic")
@@ -174,6 +177,25 @@ def main():
             for s in singles:
                 s.close()
 
+    serving = []
+    if args.serve_rates and not args.profile:
+        # continuous batching server: open-loop Poisson arrivals, mixed request
+        # types (STFT / mel 128 / MFCC 13), sr in {22050, 44100}, log-normal
+        # lengths (median 5 s); latency = submission to result on the host
+        from metalmom_amd.serve import SpectralServer, make_sources, open_loop
+        sources = make_sources()
+        for rate in [float(r) for r in args.serve_rates.split(",") if r]:
+            srv = SpectralServer(hbm_budget_bytes=32 << 30)
+            open_loop(srv, rate=min(rate, 200.0), duration=0.25, seed=99, sources=sources)  # warm-up
+            r = open_loop(srv, rate=rate, duration=args.serve_seconds, seed=4, sources=sources)
+            st = srv.stats()
+            srv.close()
+            serving.append({"rate_req_s": rate, "achieved_rate": r["achieved_rate"],
+                            "requests": r["n_requests"], "p50_ms": r["p50"] * 1e3, "p99_ms": r["p99"] * 1e3,
+                            "frames_per_s": r["frames_per_s"], "device_steps": st["steps"]})
+            log(f"serving @{rate:.0f} req/s: p50 {r['p50'] * 1e3:.2f} ms, p99 {r['p99'] * 1e3:.2f} ms, "
+                f"{r['frames_per_s'] / 1e6:.2f} M frames/s, {st['steps']} steps for {r['n_requests']} requests")
+
     cpu = None
     if not args.no_cpu and not args.profile:
         log("cpu baseline ...")
@@ -204,6 +226,8 @@ def main():
         "cpu_baseline": cpu,
     }
     rec.update(extra)
+    if serving:
+        rec["serving"] = serving
     print(json.dumps(rec), flush=True)
 
 

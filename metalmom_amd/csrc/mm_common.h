@@ -85,6 +85,7 @@ struct TileArgs {
   int out_kind;
   int lds_row_stride;          // staging row stride in elements
   int lds_region_bytes;        // per-wave LDS region
+  int lds_part_off;            // MFCC: LDS offset of the DCT partials (0: over the regions)
 };
 
 // Ordered encoding of a float so that unsigned comparison matches float order.

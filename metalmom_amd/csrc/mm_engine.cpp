@@ -210,6 +210,7 @@ bool Engine::init() {
   (void)hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, dev);
   const int limit = std::max(lds, optin);
   if (limit > 0) set_lds_limit_bytes((size_t)limit);
+  if (const char* v = std::getenv("METALMOM_AMD_TILE_WAVES")) set_tile_waves(std::atoi(v));
   if (const char* v = std::getenv("METALMOM_AMD_VERBOSE"); v && *v == '1') {
     int rt = 0;
     (void)hipRuntimeGetVersion(&rt);
@@ -483,6 +484,7 @@ static TileArgs make_args(const SpecRequest& r, const Tables* tb, const BatchMet
   a.out_kind = r.out_kind;
   a.lds_row_stride = L.row_stride;
   a.lds_region_bytes = L.region_bytes;
+  a.lds_part_off = L.part_off;
   return a;
 }
 

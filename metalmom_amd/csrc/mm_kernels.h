@@ -22,9 +22,14 @@ struct TileLayout {
   int row_stride;    // staging row stride, elements
   int region_bytes;  // LDS bytes per wavefront region
   size_t lds_bytes;  // dynamic LDS per workgroup
+  int part_off;      // LDS offset of the DCT partial area (0: reuse the staging regions)
 };
 
 void set_lds_limit_bytes(size_t n);
+// Workgroup shape of the fused kernel for n_fft <= 2048: 16 or 8 waves (one
+// frame each); 0 restores the default (16).
+void set_tile_waves(int nw);
+int tile_waves();
 size_t lds_limit_bytes();
 
 hipError_t gen_tables(const TableKey& key, const DeviceTables& t, hipStream_t st);

@@ -746,68 +746,98 @@ __global__ __launch_bounds__(NW * 64) MM_TILE_OCC void k_tile(TileArgs a) {
       f32x4 dbv[MAXB];
       float lmax = -INFINITY, lmin = INFINITY;
       int nb = 0;
+      // mel^T block b over its bin chunks [c0, c1): operands for up to UC
+      // chunks are loaded before their MFMAs so the filterbank loads overlap
+      auto mel_range = [&](int b, int c0, int c1) -> f32x4 {
+        const int2 bd = a.band[b];
+        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        if constexpr (PREC == PREC_F32) {
+          constexpr int UC = 2;
+          const float* fbrow = a.fb32 + (int64_t)(16 * b + ri) * a.fbs + bd.x + 4 * g;
+          const float* pr = reinterpret_cast<const float*>(prow) + bd.x + 4 * g;
+          for (int c = c0; c < c1; c += UC) {
+            float4 av[UC], bv[UC];
+#pragma unroll
+            for (int u = 0; u < UC; ++u) {
+              const int cc = min(c + u, c1 - 1);
+              av[u] = *reinterpret_cast<const float4*>(fbrow + 16 * cc);
+              bv[u] = *reinterpret_cast<const float4*>(pr + 16 * cc);
+            }
+#pragma unroll
+            for (int u = 0; u < UC; ++u) {
+              if (c + u < c1) {
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[u].x, bv[u].x, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[u].y, bv[u].y, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[u].z, bv[u].z, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[u].w, bv[u].w, acc, 0, 0, 0);
+              }
+            }
+          }
+        } else {
+          constexpr int UC = 4;
+          const uint16_t* fbrow = a.fb16 + (int64_t)(16 * b + ri) * a.fbs + bd.x + 8 * g;
+          const uint16_t* pr = reinterpret_cast<const uint16_t*>(prow) + bd.x + 8 * g;
+          for (int c = c0; c < c1; c += UC) {
+            uint4 av[UC], bv[UC];
+#pragma unroll
+            for (int u = 0; u < UC; ++u) {
+              const int cc = min(c + u, c1 - 1);
+              av[u] = *reinterpret_cast<const uint4*>(fbrow + 32 * cc);
+              bv[u] = *reinterpret_cast<const uint4*>(pr + 32 * cc);
+            }
+#pragma unroll
+            for (int u = 0; u < UC; ++u) {
+              if (c + u < c1) {
+                if constexpr (PREC == PREC_BF16)
+                  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, av[u]),
+                                                                __builtin_bit_cast(bf16x8, bv[u]), acc, 0, 0, 0);
+                else
+                  acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8, av[u]),
+                                                               __builtin_bit_cast(f16x8, bv[u]), acc, 0, 0, 0);
+              }
+            }
+          }
+        }
+        return acc;
+      };
+      // acc (of block b, all chunks) -> mel values, then the MEL store or dB
+      auto finish_block = [&](int b, int bi, f32x4 acc) {
+        if constexpr (PREC != PREC_F32) {
+#pragma unroll
+          for (int j = 0; j < 4; ++j) acc[j] = ldexpf(acc[j], myexp);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[j] *= a.post_scale;   // 2^-power
+        // acc[j] = mel[16b + 4g + j] of tile frame ri
+        if constexpr (MODE == OUT_MEL) {
+          if (frame_ok) {
+            float* o = reinterpret_cast<float*>(a.out) + a.out_off[clip];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+              const int m = 16 * b + 4 * g + j;
+              if (m < a.n_mels) o[(int64_t)m * T + f0 + ri] = acc[j];
+            }
+          }
+        } else {
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const int m = 16 * b + 4 * g + j;
+            float d = db10(fmaxf(acc[j], a.amin) / a.ref);
+            d = fmaxf(d, thr);
+            if (m < a.n_mels && frame_ok) {
+              lmax = fmaxf(lmax, d);
+              lmin = fminf(lmin, d);
+            }
+            dbv[bi][j] = d;
+          }
+        }
+      };
 #pragma unroll
       for (int bi = 0; bi < MAXB; ++bi) {
         const int b = wave + NW * bi;
         if (b < a.nmb) {
           nb = bi + 1;
-          const int2 bd = a.band[b];
-          f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-          if constexpr (PREC == PREC_F32) {
-            const float* fbrow = a.fb32 + (int64_t)(16 * b + ri) * a.fbs;
-            const float* pr = reinterpret_cast<const float*>(prow);
-            for (int c = 0; c < bd.y; ++c) {
-              const int k0 = bd.x + 16 * c + 4 * g;
-              const float4 av = *reinterpret_cast<const float4*>(fbrow + k0);
-              const float4 bv = *reinterpret_cast<const float4*>(pr + k0);
-              acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av.x, bv.x, acc, 0, 0, 0);
-              acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av.y, bv.y, acc, 0, 0, 0);
-              acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av.z, bv.z, acc, 0, 0, 0);
-              acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av.w, bv.w, acc, 0, 0, 0);
-            }
-          } else {
-            const uint16_t* fbrow = a.fb16 + (int64_t)(16 * b + ri) * a.fbs;
-            const uint16_t* pr = reinterpret_cast<const uint16_t*>(prow);
-            for (int c = 0; c < bd.y; ++c) {
-              const int k0 = bd.x + 32 * c + 8 * g;
-              const uint4 av = *reinterpret_cast<const uint4*>(fbrow + k0);
-              const uint4 bv = *reinterpret_cast<const uint4*>(pr + k0);
-              if constexpr (PREC == PREC_BF16) {
-                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, av),
-                                                              __builtin_bit_cast(bf16x8, bv), acc, 0, 0, 0);
-              } else {
-                acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8, av),
-                                                             __builtin_bit_cast(f16x8, bv), acc, 0, 0, 0);
-              }
-            }
-#pragma unroll
-            for (int j = 0; j < 4; ++j) acc[j] = ldexpf(acc[j], myexp);
-          }
-#pragma unroll
-          for (int j = 0; j < 4; ++j) acc[j] *= a.post_scale;   // 2^-power
-          // acc[j] = mel[16b + 4g + j] of tile frame ri
-          if constexpr (MODE == OUT_MEL) {
-            if (frame_ok) {
-              float* o = reinterpret_cast<float*>(a.out) + a.out_off[clip];
-#pragma unroll
-              for (int j = 0; j < 4; ++j) {
-                const int m = 16 * b + 4 * g + j;
-                if (m < a.n_mels) o[(int64_t)m * T + f0 + ri] = acc[j];
-              }
-            }
-          } else {
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-              const int m = 16 * b + 4 * g + j;
-              float d = db10(fmaxf(acc[j], a.amin) / a.ref);
-              d = fmaxf(d, thr);
-              if (m < a.n_mels && frame_ok) {
-                lmax = fmaxf(lmax, d);
-                lmin = fminf(lmin, d);
-              }
-              dbv[bi][j] = d;
-            }
-          }
+          finish_block(b, bi, mel_range(b, 0, a.band[b].y));
         }
       }
 
@@ -816,8 +846,12 @@ __global__ __launch_bounds__(NW * 64) MM_TILE_OCC void k_tile(TileArgs a) {
         lmax = wave_max(lmax);
         lmin = wave_min(lmin);
         if (lane == 0) { red_max[wave] = lmax; red_min[wave] = lmin; }
-        __syncthreads();   // staging reads done; region memory is reused below
-        float* red = reinterpret_cast<float*>(lds);   // [ndt][nact][64 lanes][4]
+        // DCT partials [ndt][nact][64 lanes][4]: in their own LDS area when
+        // the layout has one (2 barriers per tile), else over the staging
+        // regions once every wave has read them (4 barriers)
+        const bool sep = a.lds_part_off > 0;
+        float* red = reinterpret_cast<float*>(lds + (sep ? a.lds_part_off : 0));
+        if (!sep) __syncthreads();   // staging reads done; region memory is reused below
         if (wave < nact) {
           for (int ct = 0; ct < a.ndt; ++ct) {
             f32x4 accd = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -868,7 +902,7 @@ __global__ __launch_bounds__(NW * 64) MM_TILE_OCC void k_tile(TileArgs a) {
             *reinterpret_cast<float4*>(dst) = make_float4(accd[0], accd[1], accd[2], accd[3]);
           }
         }
-        __syncthreads();
+        __syncthreads();   // partials and per-wave extrema are in LDS
         if (threadIdx.x == 0 && a.phase == 1 && a.use_top_db) {
           float bm = -INFINITY, bn = INFINITY;
           for (int w = 0; w < NW; ++w) { bm = fmaxf(bm, red_max[w]); bn = fminf(bn, red_min[w]); }
@@ -876,15 +910,16 @@ __global__ __launch_bounds__(NW * 64) MM_TILE_OCC void k_tile(TileArgs a) {
           a.tile_min[tile] = bn;
         }
         // fixed-order cross-wave sum; wave ct writes DCT rows 16ct .. 16ct+15
+        // (with a separate area, the other waves already start the next tile)
         for (int ct = wave; ct < a.ndt; ct += NW) {
-          float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
+          float4 sm = make_float4(0.f, 0.f, 0.f, 0.f);
           for (int w = 0; w < nact; ++w) {
             const float4 p = *reinterpret_cast<const float4*>(red + ((int64_t)(ct * nact + w) * 64 + lane) * 4);
-            s.x += p.x; s.y += p.y; s.z += p.z; s.w += p.w;
+            sm.x += p.x; sm.y += p.y; sm.z += p.z; sm.w += p.w;
           }
           if (frame_ok) {
             float* o = reinterpret_cast<float*>(a.out) + a.out_off[clip];
-            const float sv[4] = {s.x, s.y, s.z, s.w};
+            const float sv[4] = {sm.x, sm.y, sm.z, sm.w};
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
               const int c = 16 * ct + 4 * g + j;
@@ -892,6 +927,7 @@ __global__ __launch_bounds__(NW * 64) MM_TILE_OCC void k_tile(TileArgs a) {
             }
           }
         }
+        if (sep) continue;
       }
       __syncthreads();
     }
@@ -1175,6 +1211,8 @@ hipError_t gen_tables(const TableKey& key, const DeviceTables& t, hipStream_t st
 
 // ---- fused kernel dispatch ----
 
+static int g_tile_waves;
+
 template <int LOG2M, int MODE, int PREC, int NW, int FPW, typename TIn>
 static hipError_t launch_tile_t(const TileArgs& a, int grid, size_t lds, hipStream_t st) {
   auto fn = k_tile<LOG2M, MODE, PREC, NW, FPW, TIn>;
@@ -1188,15 +1226,17 @@ static hipError_t launch_tile_t(const TileArgs& a, int grid, size_t lds, hipStre
   return hipGetLastError();
 }
 
-// (NW, FPW): 16 x 1 up to n_fft 2048; 8 x 1 (8-frame tiles) at 4096
+// (NW, FPW): 16 x 1 or 8 x 1 (8-frame tiles) up to n_fft 2048; 8 x 1 at 4096
 template <int LOG2M, int MODE, int PREC, typename TIn>
 static hipError_t launch_tile_s(const TileArgs& a, const TileLayout& L, int grid, hipStream_t st) {
   if constexpr (LOG2M <= 10) {
-    return launch_tile_t<LOG2M, MODE, PREC, 16, 1, TIn>(a, grid, L.lds_bytes, st);
-  } else {
-    return launch_tile_t<LOG2M, MODE, PREC, 8, 1, TIn>(a, grid, L.lds_bytes, st);
+    if (L.nw == 16) return launch_tile_t<LOG2M, MODE, PREC, 16, 1, TIn>(a, grid, L.lds_bytes, st);
   }
+  return launch_tile_t<LOG2M, MODE, PREC, 8, 1, TIn>(a, grid, L.lds_bytes, st);
 }
+
+void set_tile_waves(int nw) { g_tile_waves = (nw == 8 || nw == 16) ? nw : 0; }
+int tile_waves() { return g_tile_waves; }
 
 template <int LOG2M>
 static hipError_t launch_tile_m(const TileArgs& a, int prec, const TileLayout& L, int grid, hipStream_t st) {
@@ -1239,7 +1279,7 @@ bool tile_layout(int n_fft, int out_kind, int prec, int n_mels, int n_mfcc, Tile
     if (n_mels < 1 || n_mels > 512) return false;
     if (out_kind == OUT_MFCC && (n_mfcc < 1 || n_mfcc > 128)) return false;
   }
-  const int nw = (M > 1024) ? 8 : 16, fpw = 1;
+  const int nw = (M > 1024) ? 8 : (g_tile_waves ? g_tile_waves : 16), fpw = 1;
   const int scr = (M + M / 16) * 8;
   int esz, S;
   if (spec) {
@@ -1254,21 +1294,31 @@ bool tile_layout(int n_fft, int out_kind, int prec, int n_mels, int n_mfcc, Tile
     const int need = ((F + 31) / 32) * 32 + 64;
     S = need + ((8 - need % 128) + 128) % 128;       // S/2 = 4 (mod 64) dwords
   }
+  // region: FFT scratch, or staging row(s) + a 1 KiB hand-over slot
   int rb = std::max(fpw * S * esz, scr);
   rb = (rb + 15) / 16 * 16;
-  const size_t total = (size_t)nw * rb + 16 * 4 + 2 * nw * 4;
+  const size_t extras = 16 * 4 + 2 * nw * 4;
+  size_t total = (size_t)nw * rb + extras;
   if (total > lds_limit_bytes()) return false;
-  // the MFCC cross-wave reduction reuses the regions
+  int part_off = 0;
   if (out_kind == OUT_MFCC) {
     const int ndt = (n_mfcc + 15) / 16;
     const int nact = std::min(nw, (n_mels + 15) / 16);
-    if ((size_t)ndt * nact * 256 * 4 > (size_t)nw * rb) return false;
+    const size_t part = (size_t)ndt * nact * 256 * 4;
+    if (total + part + 16 <= lds_limit_bytes()) {
+      // DCT partials in their own area: two barriers per tile
+      part_off = (int)(((total + 15) / 16) * 16);
+      total = part_off + part;
+    } else if (part > (size_t)nw * rb) {
+      return false;   // partials would not even fit over the staging regions
+    }
   }
   L->nw = nw;
   L->fpw = fpw;
   L->row_stride = S;
   L->region_bytes = rb;
   L->lds_bytes = total;
+  L->part_off = part_off;
   L->tile_frames = nw * fpw;
   return true;
 }
