@@ -196,6 +196,11 @@ int32_t mm_server_stats(mm_server s, int64_t* submitted, int64_t* completed, int
 /* Finishes queued work, then releases the server. */
 int32_t mm_server_destroy(mm_server s);
 
+/* Instrumented builds (-DMM_PHASE_TIMING) only: device buffer receiving
+ * per-wave cycle sums of the fused kernel's phases, [grid][waves][6] int64.
+ * A no-op for the kernels of a normal build. */
+int32_t mm_debug_set_phase_buffer(void* d_buf);
+
 /* Device description: name, gfx arch and compute-unit count. */
 int32_t mm_device_info(char* name, int32_t name_len, char* arch, int32_t arch_len, int32_t* n_cu);
 

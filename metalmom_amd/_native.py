@@ -82,6 +82,7 @@ SIGNATURES = {
     "mm_batch_time": (_I32, [_P, _P, _P, _P, _I32, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "mm_batch_destroy": (_I32, [_P]),
     "mm_device_info": (_I32, [C.c_char_p, _I32, C.c_char_p, _I32, C.POINTER(_I32)]),
+    "mm_debug_set_phase_buffer": (_I32, [_P]),
     "mm_server_create": (_I32, [_P, _I64, _I32, C.POINTER(_P)]),
     "mm_server_submit": (_I64, [_P, C.POINTER(MMSpecParams), _F, _I64]),
     "mm_server_poll": (_I32, [_P, _I64, _BUF, C.POINTER(C.c_double)]),

@@ -272,6 +272,12 @@ int32_t mm_batch_autotune(mm_batch bh, const void* d_signal, float* d_out, int32
   return MM_OK;
 }
 
+// Instrumented builds only: device buffer for per-wave phase cycle sums.
+int32_t mm_debug_set_phase_buffer(void* d_buf) {
+  set_phase_cycles_buffer(static_cast<long long*>(d_buf));
+  return MM_OK;
+}
+
 int32_t mm_batch_destroy(mm_batch bh) {
   Batch* b = static_cast<Batch*>(bh);
   if (!b) return MM_ERR_INVALID_INPUT;

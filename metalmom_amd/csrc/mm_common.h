@@ -23,6 +23,16 @@ enum InDtype : int { IN_F32 = 0, IN_F16 = 1 };
 enum PadMode : int { PAD_CONSTANT = 0, PAD_REFLECT = 1 };
 
 constexpr int kTileFrames = 16;   // frames per tile = MFMA N dimension
+constexpr int kMelBlock = 8;      // mel rows per filterbank block of the fused contraction
+constexpr int kMaxMelBlocks = 32; // n_mels <= 256 on the fused path
+
+// Per tile: where its frames start in the packed signal and which of them can
+// be fetched by the plain interior path (used to prefetch the next tile).
+struct TilePf {
+  long long base;   // packed-signal index of frame f0's first sample (may be < clip start)
+  unsigned mask;    // bit w: frame f0 + w exists, lies inside its clip and starts at an even offset
+  int pad;
+};
 
 // Device-resident tables for one (sr, n_fft, win, n_mels, fmin, fmax, n_mfcc) key.
 struct DeviceTables {
@@ -33,16 +43,18 @@ struct DeviceTables {
   float* fb32 = nullptr;        // [nmb*16][fbs] mel weights, zero padded
   uint16_t* fb16 = nullptr;     // same in bf16 or f16 (see fb16_kind)
   uint16_t* fbh = nullptr;      // f16 copy
-  int2* band32 = nullptr;       // [nmb] (k_start, n_chunks of 16 bins)
-  int2* band16 = nullptr;       // [nmb] (k_start, n_chunks of 32 bins)
-  float* dct32 = nullptr;       // [ndt][nmb][4][64] permuted DCT-II rows (A operand)
-  uint16_t* dctb_hi = nullptr;  // [ndt][nmb][64][4] bf16 hi part
+  int2* band32 = nullptr;       // [nmh] (k_start, n_chunks of 16 bins) per block of 8 mels
+  int2* band16 = nullptr;       // [nmh] (k_start, n_chunks of 32 bins)
+  int* assign = nullptr;        // [2 chunk widths][2 wave counts (8, 16)][64]: blocks per wave
+  float* dct32 = nullptr;       // [ndt][nmh][4][64] permuted DCT-II rows (A operand)
+  uint16_t* dctb_hi = nullptr;  // [ndt][nmh][64][4] bf16 hi part
   uint16_t* dctb_lo = nullptr;  // bf16 lo part
   uint16_t* dcth_hi = nullptr;  // f16 hi part
   uint16_t* dcth_lo = nullptr;  // f16 lo part
   float* dct_dense = nullptr;   // [n_mfcc][n_mels] plain basis (generic path)
   int fbs = 0;                  // fb row stride (elements)
-  int nmb = 0;                  // mel blocks of 16
+  int nmb = 0;                  // filterbank rows / 16 (rows are zero padded to 16 nmb)
+  int nmh = 0;                  // mel blocks of 8 (the contraction's unit of work)
   int ndt = 0;                  // DCT row tiles of 16
 };
 
@@ -56,6 +68,7 @@ struct TileArgs {
   const int64_t* out_off;      // first output element of each clip
   const int32_t* tile_clip;    // clip of each tile
   const int32_t* tile_f0;      // first frame of each tile
+  const TilePf* tile_pf;       // prefetch record of each tile
   int n_tiles;
   int tiles_per_wg;            // contiguous chunk of tiles per persistent workgroup
   // framing
@@ -67,8 +80,9 @@ struct TileArgs {
   const float2* twp;
   const float* fb32;
   const uint16_t* fb16;
-  const int2* band;
-  int fbs, nmb, n_mels;
+  const int2* band;            // per block of 8 mels
+  const int* assign;           // [waves][64 / waves] block ids of each wave (-1: none)
+  int fbs, nmh, n_mels;
   const float* dct32;
   const uint16_t* dct_hi;
   const uint16_t* dct_lo;
@@ -86,7 +100,38 @@ struct TileArgs {
   int lds_row_stride;          // staging row stride in elements
   int lds_region_bytes;        // per-wave LDS region
   int lds_part_off;            // MFCC: LDS offset of the DCT partials (0: over the regions)
+  long long* phase_cycles;     // instrumented builds only: [grid][waves][6] cycle sums
 };
+
+// Mel blocks per wavefront of the fused contraction, longest processing time
+// first: blocks in order of decreasing cost (chunks + 1; highest index first on
+// ties) each go to the least loaded wave that still has a free slot (lowest
+// index on ties). The first min(nw, nmh) blocks land on waves 0 .. nw-1, so
+// the waves holding blocks are always 0 .. min(nw, nmh)-1. Writes
+// out[w * (kMaxMelBlocks / nw) + i] = block | first bin << 6 | chunks << 19,
+// -1 for an empty slot (all slots when nmh is 0 or above kMaxMelBlocks).
+// band[b] = (first bin, chunks) of block b.
+__host__ __device__ inline void mel_block_assign(const int2* band, int nmh, int nw, int* out) {
+  const int slots = kMaxMelBlocks / nw;
+  for (int i = 0; i < kMaxMelBlocks; ++i) out[i] = -1;
+  if (nmh < 1 || nmh > kMaxMelBlocks || nw < 1 || nw > 16) return;
+  int load[16], cnt[16];
+  for (int w = 0; w < 16; ++w) { load[w] = 0; cnt[w] = 0; }
+  bool taken[kMaxMelBlocks];
+  for (int b = 0; b < kMaxMelBlocks; ++b) taken[b] = false;
+  for (int n = 0; n < nmh; ++n) {
+    int best = -1;
+    for (int b = 0; b < nmh; ++b)
+      if (!taken[b] && (best < 0 || band[b].y >= band[best].y)) best = b;
+    taken[best] = true;
+    int wb = -1;   // nw * slots >= nmh, so a wave with a free slot exists
+    for (int w = 0; w < nw; ++w)
+      if (cnt[w] < slots && (wb < 0 || load[w] < load[wb])) wb = w;
+    out[wb * slots + cnt[wb]] = best | (band[best].x << 6) | (band[best].y << 19);
+    cnt[wb] += 1;
+    load[wb] += band[best].y + 1;
+  }
+}
 
 // Ordered encoding of a float so that unsigned comparison matches float order.
 __host__ __device__ inline unsigned float_to_key(float x) {

@@ -137,6 +137,7 @@ static int32_t make_tables(const TableKey& key, hipStream_t st, std::unique_ptr<
   const int F = key.n_fft / 2 + 1;
   t.fbs = ((F + 31) / 32) * 32 + 64;
   t.nmb = (key.n_mels + 15) / 16;
+  t.nmh = (key.n_mels + kMelBlock - 1) / kMelBlock;
   t.ndt = (key.n_mfcc + 15) / 16;
   const int rows = t.nmb * 16;
   size_t off = 0;
@@ -152,9 +153,10 @@ static int32_t make_tables(const TableKey& key, hipStream_t st, std::unique_ptr<
   const size_t o_fb32 = take((size_t)rows * t.fbs * 4);
   const size_t o_fb16 = take((size_t)rows * t.fbs * 2);
   const size_t o_fbh = take((size_t)rows * t.fbs * 2);
-  const size_t o_b32 = take((size_t)t.nmb * 8);
-  const size_t o_b16 = take((size_t)t.nmb * 8);
-  const size_t nd = (size_t)t.ndt * t.nmb * 256;
+  const size_t o_b32 = take((size_t)t.nmh * 8);
+  const size_t o_b16 = take((size_t)t.nmh * 8);
+  const size_t o_asg = take((size_t)4 * kMaxMelBlocks * 4);
+  const size_t nd = (size_t)t.ndt * t.nmh * 256;
   const size_t o_d32 = take(nd * 4);
   const size_t o_dbh = take(nd * 2);
   const size_t o_dbl = take(nd * 2);
@@ -173,6 +175,7 @@ static int32_t make_tables(const TableKey& key, hipStream_t st, std::unique_ptr<
   t.fbh = reinterpret_cast<uint16_t*>(b + o_fbh);
   t.band32 = reinterpret_cast<int2*>(b + o_b32);
   t.band16 = reinterpret_cast<int2*>(b + o_b16);
+  t.assign = reinterpret_cast<int*>(b + o_asg);
   t.dct32 = reinterpret_cast<float*>(b + o_d32);
   t.dctb_hi = reinterpret_cast<uint16_t*>(b + o_dbh);
   t.dctb_lo = reinterpret_cast<uint16_t*>(b + o_dbl);
@@ -371,6 +374,7 @@ void build_meta(const SpecRequest& r, const int64_t* lens, int n, int tile_frame
   m->clip_frames.resize(n);
   m->tile_clip.clear();
   m->tile_f0.clear();
+  m->tile_pf.clear();
   const int64_t rows = out_rows(r);
   const int ef = out_elem_floats(r);
   int64_t in = 0, out = 0;
@@ -382,9 +386,19 @@ void build_meta(const SpecRequest& r, const int64_t* lens, int n, int tile_frame
     m->out_off[c] = out;  // in output entries (complex entries for complex STFT)
     in += lens[c];
     out += rows * T;
+    const int64_t pad = r.center ? r.n_fft / 2 : 0;
     for (int64_t f = 0; f < T; f += tile_frames) {
       m->tile_clip.push_back(c);
       m->tile_f0.push_back((int32_t)f);
+      TilePf p;
+      p.base = (long long)(m->clip_off[c] + f * r.hop - pad);
+      p.mask = 0u;
+      p.pad = 0;
+      for (int w = 0; w < tile_frames && w < 32; ++w) {
+        const int64_t s0 = (f + w) * r.hop - pad;   // frame start within the clip
+        if (f + w < T && s0 >= 0 && s0 + r.n_fft <= lens[c] && (s0 & 1) == 0) p.mask |= 1u << w;
+      }
+      m->tile_pf.push_back(p);
     }
   }
   m->total_in = in;
@@ -396,7 +410,8 @@ static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 size_t meta_bytes(const BatchMeta& m) {
   const size_t c = (size_t)std::max(m.n_clips, 1), t = (size_t)std::max(m.n_tiles, 1);
-  return align256(c * 8) * 3 + align256(c * 4) + align256(t * 4) * 2 + align256(c * 4) + align256(t * 4);
+  return align256(c * 8) * 3 + align256(c * 4) + align256(t * 4) * 2 + align256(c * 4) + align256(t * 4) +
+         align256(t * sizeof(TilePf));
 }
 
 MetaPtrs meta_ptrs(const BatchMeta& m, void* dev) {
@@ -411,7 +426,8 @@ MetaPtrs meta_ptrs(const BatchMeta& m, void* dev) {
   p.tile_clip = reinterpret_cast<const int32_t*>(b + o); o += align256(t * 4);
   p.tile_f0 = reinterpret_cast<const int32_t*>(b + o); o += align256(t * 4);
   p.clip_max_key = reinterpret_cast<unsigned*>(b + o); o += align256(c * 4);
-  p.tile_min = reinterpret_cast<float*>(b + o);
+  p.tile_min = reinterpret_cast<float*>(b + o); o += align256(t * 4);
+  p.tile_pf = reinterpret_cast<const TilePf*>(b + o);
   return p;
 }
 
@@ -431,11 +447,20 @@ void pack_meta(const BatchMeta& m, void* host) {
   if (m.n_tiles) std::memcpy(b + o, m.tile_clip.data(), (size_t)m.n_tiles * 4);
   o += align256(t * 4);
   if (m.n_tiles) std::memcpy(b + o, m.tile_f0.data(), (size_t)m.n_tiles * 4);
+  o += align256(t * 4);
+  o += align256(c * 4);   // clip_max_key (device-written)
+  o += align256(t * 4);   // tile_min (device-written)
+  if (m.n_tiles) std::memcpy(b + o, m.tile_pf.data(), (size_t)m.n_tiles * sizeof(TilePf));
 }
 
 // ---------------------------------------------------------------------------
 // execution
 // ---------------------------------------------------------------------------
+
+// Instrumented builds (-DMM_PHASE_TIMING) write per-wave phase cycles here.
+static long long* g_phase_cycles = nullptr;
+long long* phase_cycles_buffer() { return g_phase_cycles; }
+void set_phase_cycles_buffer(long long* p) { g_phase_cycles = p; }
 
 static TileArgs make_args(const SpecRequest& r, const Tables* tb, const BatchMeta& m, const MetaPtrs& mp,
                           const void* d_signal, void* d_out, const TileLayout& L, int grid) {
@@ -449,6 +474,7 @@ static TileArgs make_args(const SpecRequest& r, const Tables* tb, const BatchMet
   a.out_off = mp.out_off;
   a.tile_clip = mp.tile_clip;
   a.tile_f0 = mp.tile_f0;
+  a.tile_pf = mp.tile_pf;
   a.n_tiles = m.n_tiles;
   a.tiles_per_wg = (m.n_tiles + grid - 1) / std::max(grid, 1);
   a.n_fft = r.n_fft;
@@ -463,8 +489,11 @@ static TileArgs make_args(const SpecRequest& r, const Tables* tb, const BatchMet
   a.fb32 = t.fb32;
   a.fb16 = (r.precision == PREC_F16) ? t.fbh : t.fb16;
   a.band = (r.precision == PREC_F32) ? t.band32 : t.band16;
+  // block lists for this chunk width and workgroup shape (8 or 16 waves)
+  a.assign = t.assign ? t.assign + (((r.precision == PREC_F32) ? 0 : 2) + (L.nw == 16 ? 1 : 0)) * kMaxMelBlocks
+                      : nullptr;
   a.fbs = t.fbs;
-  a.nmb = t.nmb;
+  a.nmh = t.nmh;
   a.n_mels = r.n_mels;
   a.dct32 = t.dct32;
   a.dct_hi = (r.precision == PREC_F16) ? t.dcth_hi : t.dctb_hi;
@@ -485,6 +514,7 @@ static TileArgs make_args(const SpecRequest& r, const Tables* tb, const BatchMet
   a.lds_row_stride = L.row_stride;
   a.lds_region_bytes = L.region_bytes;
   a.lds_part_off = L.part_off;
+  a.phase_cycles = phase_cycles_buffer();
   return a;
 }
 

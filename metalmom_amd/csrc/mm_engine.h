@@ -116,6 +116,7 @@ struct BatchMeta {
   int64_t total_out = 0;    // float32 elements
   std::vector<int64_t> clip_off, clip_len, out_off;
   std::vector<int32_t> clip_frames, tile_clip, tile_f0;
+  std::vector<TilePf> tile_pf;
 };
 void build_meta(const SpecRequest& r, const int64_t* lens, int n, int tile_frames, BatchMeta* m);
 
@@ -132,12 +133,17 @@ struct MetaPtrs {
   const int32_t* tile_f0;
   unsigned* clip_max_key;
   float* tile_min;
+  const TilePf* tile_pf;
 };
 MetaPtrs meta_ptrs(const BatchMeta& m, void* dev);
 
 int32_t enqueue_batch(Engine* eng, const SpecRequest& r, const Tables* t, const BatchMeta& m, const MetaPtrs& mp,
                       const void* d_signal, void* d_out, DevBuf* scratch, DevBuf* scratch2, int grid_override,
                       hipStream_t st, hipEvent_t ev_p1_begin = nullptr, hipEvent_t ev_p1_end = nullptr);
+
+// Device buffer for the phase timing of instrumented builds (null: off).
+long long* phase_cycles_buffer();
+void set_phase_cycles_buffer(long long* p);
 
 // Sets the fused kernel's LDS attribute ahead of any stream capture.
 int32_t prepare_batch(const SpecRequest& r);

@@ -14,10 +14,25 @@
 #include "mm_common.h"
 #include "mm_kernels.h"
 
+#include <type_traits>
+
 #include <algorithm>
 
 #ifndef MM_TILE_OCC
 #define MM_TILE_OCC
+#endif
+
+// Optional phase timing of the fused kernel (build with -DMM_PHASE_TIMING):
+// lane 0 of every wave accumulates shader-clock cycles spent in each tile
+// phase and stores the sums to TileArgs::phase_cycles at exit.
+#ifdef MM_PHASE_TIMING
+#define MM_PT_DECL long long pt_acc[6] = {0, 0, 0, 0, 0, 0}; long long pt_last = __builtin_amdgcn_s_memtime();
+#define MM_PT(i) do { const long long now_ = __builtin_amdgcn_s_memtime(); pt_acc[i] += now_ - pt_last; pt_last = now_; } while (0)
+#define MM_PT_STORE() do { if (a.phase_cycles && lane0 == 0) for (int i_ = 0; i_ < 6; ++i_) a.phase_cycles[(blockIdx.x * NW + wave) * 6 + i_] += pt_acc[i_]; } while (0)
+#else
+#define MM_PT_DECL
+#define MM_PT(i) do { } while (0)
+#define MM_PT_STORE() do { } while (0)
 #endif
 
 namespace mm {
@@ -85,6 +100,43 @@ __device__ __forceinline__ int opaque(int v) {
   __asm__ __volatile__("" : "+v"(v));
   return v;
 }
+// Same for a wave-uniform value (kept in a scalar register).
+__device__ __forceinline__ int opaque_s(int v) {
+  __asm__ __volatile__("" : "+s"(v));
+  return v;
+}
+
+// Workgroup barrier that waits for this wave's LDS (and scalar) operations
+// only: vector memory loads issued before it stay in flight (the next tile's
+// samples are prefetched across it), unlike __syncthreads, which also drains
+// the vector memory counter.
+__device__ __forceinline__ void lds_barrier() {
+  __asm__ __volatile__("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+// Read-only launch data (tile and clip tables, block lists, prefetch
+// records): read through the constant address space, which promises the
+// compiler that the kernel does not write it, so wave-uniform reads become
+// scalar loads (waited on the scalar counter) instead of vector loads plus a
+// full vector-memory wait.
+template <typename T>
+__device__ __forceinline__ T ldc(const T* p, int64_t i) {
+  return ((const __attribute__((address_space(4))) T*)p)[i];
+}
+__device__ __forceinline__ TilePf ldc_pf(const TilePf* p, int64_t i) {
+  TilePf r;
+  r.base = ldc(&p[i].base, 0);
+  r.mask = ldc(&p[i].mask, 0);
+  r.pad = 0;
+  return r;
+}
+
+// Waits for every vector memory operation of this wave (s_waitcnt vmcnt(0);
+// expcnt and lgkmcnt left at their maxima). Placed where outstanding loads
+// are needed anyway, it also tells the compiler's wait tracking that nothing
+// is pending, so registers that held in-flight loads on some path can be
+// reused later without a conservative full wait.
+__device__ __forceinline__ void wait_vmem_all() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
 // Padded LDS index: one spare complex slot per 16 keeps the strided radix-16
 // accesses of a Stockham pass on distinct banks.
@@ -386,10 +438,10 @@ template <int LOG2M, typename TIn>
 __device__ __forceinline__ void frame_fft(const TileArgs& a, int clip, int fr, float2* scr, int lane) {
   using S = FftShape<LOG2M>;
   constexpr int M = S::M, R0 = S::R0, B0 = S::B0, Q0 = S::Q0;
-  const int64_t len = a.clip_len[clip];
+  const int64_t len = ldc(a.clip_len, clip);
   const int64_t s0 = (int64_t)fr * a.hop - (a.center ? M : 0);
   float2 v[Q0][R0];
-  frame_load<LOG2M>(a, static_cast<const TIn*>(a.signal) + a.clip_off[clip], len, s0, scr, lane, v);
+  frame_load<LOG2M>(a, static_cast<const TIn*>(a.signal) + ldc(a.clip_off, clip), len, s0, scr, lane, v);
   // pass 0 (NS = 1: no twiddles), written in Stockham order
 #pragma unroll
   for (int q = 0; q < Q0; ++q) {
@@ -407,15 +459,30 @@ __device__ __forceinline__ void frame_fft(const TileArgs& a, int clip, int fr, f
 // Same FFT, but the last pass stays in registers: z[q][r] = Z[lane + 64 q + BL r].
 template <int LOG2M, typename TIn>
 __device__ __forceinline__ void frame_fft_regs(const TileArgs& a, int clip, int fr, float2* scr, int lane,
-                                               float2 (&z)[FftShape<LOG2M>::QL][FftShape<LOG2M>::RL]) {
+                                               float2 (&z)[FftShape<LOG2M>::QL][FftShape<LOG2M>::RL],
+                                               const float2 (&pf)[FftShape<LOG2M>::Q0][FftShape<LOG2M>::R0],
+                                               bool use_pf) {
   using S = FftShape<LOG2M>;
   constexpr int M = S::M, R0 = S::R0, B0 = S::B0, Q0 = S::Q0;
-  const int64_t len = a.clip_len[clip];
+  const int64_t len = ldc(a.clip_len, clip);
   const int64_t s0 = (int64_t)fr * a.hop - (a.center ? M : 0);
   {
     float2 v[Q0][R0];
     lane = opaque(lane);
-    frame_load<LOG2M>(a, static_cast<const TIn*>(a.signal) + a.clip_off[clip], len, s0, scr, lane, v);
+    if (use_pf) {   // samples prefetched during the previous tile: apply the window
+#pragma unroll
+      for (int q = 0; q < Q0; ++q) {
+        const int j = lane + 64 * q;
+#pragma unroll
+        for (int r = 0; r < R0; ++r) {
+          const unsigned n = (unsigned)(j + r * B0);
+          const float2 w = *reinterpret_cast<const float2*>(a.window + 2 * n);
+          v[q][r] = make_float2(pf[q][r].x * w.x, pf[q][r].y * w.y);
+        }
+      }
+    } else {
+      frame_load<LOG2M>(a, static_cast<const TIn*>(a.signal) + ldc(a.clip_off, clip), len, s0, scr, lane, v);
+    }
 #pragma unroll
     for (int q = 0; q < Q0; ++q) {
       const int j = lane + 64 * q;
@@ -535,7 +602,7 @@ __global__ __launch_bounds__(NW * 64) MM_TILE_OCC void k_tile(TileArgs a) {
   constexpr bool SPEC = (MODE == OUT_STFT_MAG || MODE == OUT_STFT_POWER || MODE == OUT_STFT_COMPLEX);
   constexpr bool CPLX = (MODE == OUT_STFT_COMPLEX);
   constexpr int F = M + 1;
-  constexpr int MAXB = (32 + NW - 1) / NW;         // mel blocks per wave (n_mels <= 512)
+  constexpr int MAXB = kMaxMelBlocks / NW;         // mel block slots per wave (n_mels <= 256)
   static_assert(TF <= 16, "tile larger than the MFMA N dimension");
 
   const int lane0 = threadIdx.x & 63;
@@ -555,14 +622,58 @@ __global__ __launch_bounds__(NW * 64) MM_TILE_OCC void k_tile(TileArgs a) {
 
   const int t_begin = blockIdx.x * a.tiles_per_wg;
   const int t_end = min(t_begin + a.tiles_per_wg, a.n_tiles);
+  MM_PT_DECL
+  // Samples of this wave's frame in the next tile, loaded while the current
+  // tile's contraction and barriers run (register-split FFTs, one frame per
+  // wave, interior frames with an even start; pf_tile = -1: none).
+  using SHP = FftShape<LOG2M>;
+  constexpr bool PF = SHP::REG_SPLIT && FPW == 1;
+  float2 pf[SHP::Q0][SHP::R0];
+  int pf_tile = -1;
+#pragma unroll
+  for (int q = 0; q < SHP::Q0; ++q)
+#pragma unroll
+    for (int r = 0; r < SHP::R0; ++r) pf[q][r] = make_float2(0.0f, 0.0f);
+  // pf is written on every path, so it is dead between its use at the start
+  // of a tile and the next prefetch (it never occupies registers in the FFT).
+  // `r` is the record of tile_now + 1 (mask 0: nothing to fetch).
+  auto prefetch_next = [&](int tile_now, const TilePf& r) {
+    if constexpr (PF) {
+      const bool ok = (r.mask >> wave) & 1u;
+      pf_tile = ok ? tile_now + 1 : -1;
+      if (ok) {
+        const TIn* xf = static_cast<const TIn*>(a.signal) + r.base + (int64_t)wave * a.hop;
+        const int l = opaque(lane0);
+#pragma unroll
+        for (int q = 0; q < SHP::Q0; ++q)
+#pragma unroll
+          for (int r2 = 0; r2 < SHP::R0; ++r2) {
+            const unsigned n = (unsigned)(l + 64 * q + r2 * SHP::B0);
+            pf[q][r2] = SamplePair<TIn>::load(xf + 2 * n);
+          }
+      } else {
+#pragma unroll
+        for (int q = 0; q < SHP::Q0; ++q)
+#pragma unroll
+          for (int r2 = 0; r2 < SHP::R0; ++r2) pf[q][r2] = make_float2(0.0f, 0.0f);
+      }
+    }
+  };
+  if constexpr (PF) {
+    if (t_begin < t_end && a.phase == 1) prefetch_next(t_begin - 1, ldc_pf(a.tile_pf, t_begin));
+  }
+  const int wv = __builtin_amdgcn_readfirstlane(wave);
   for (int tile = t_begin; tile < t_end; ++tile) {
+    // the prefetched samples are used first thing; on tiles that do not use
+    // them nothing is in flight (see wait_vmem_all)
+    if constexpr (PF) wait_vmem_all();
     // keep lane-derived table addresses inside the loop: hoisted out of it
     // they would stay live as 64-bit register pairs across the whole FFT
     int lane = lane0;
     __asm__ __volatile__("" : "+v"(lane));
-    const int clip = a.tile_clip[tile];
-    const int f0 = a.tile_f0[tile];
-    const int T = a.clip_frames[clip];
+    const int clip = ldc(a.tile_clip, tile);
+    const int f0 = ldc(a.tile_f0, tile);
+    const int T = ldc(a.clip_frames, clip);
     float thr = -INFINITY;
     if constexpr (MODE == OUT_MFCC) {
       if (a.use_top_db && a.phase == 2) {
@@ -590,7 +701,7 @@ __global__ __launch_bounds__(NW * 64) MM_TILE_OCC void k_tile(TileArgs a) {
         float vmax = 0.0f;
         if constexpr (RS) {
           float2 z[QL][RL];
-          frame_fft_regs<LOG2M, TIn>(a, clip, fr, scr, lane, z);
+          frame_fft_regs<LOG2M, TIn>(a, clip, fr, scr, lane, z, pf, PF && pf_tile == tile);
           float2 x[QL][RL];
           float nyq;
           real_split_regs<LOG2M>(z, a.twN, lane, x, nyq);
@@ -668,6 +779,23 @@ __global__ __launch_bounds__(NW * 64) MM_TILE_OCC void k_tile(TileArgs a) {
       }
     }
 
+    MM_PT(0);
+    // Scalar loads for after the barrier, issued now so they have landed by
+    // then (loaded at the top of the tile they would occupy scalar registers
+    // through the FFT): the next tile's prefetch record and this wave's mel
+    // block slots (block | first bin << 6 | chunks << 19; -1: empty).
+    TilePf nrec;
+    nrec.base = 0;
+    nrec.mask = 0u;
+    if constexpr (PF) {
+      if (tile + 1 < t_end && a.phase == 1) nrec = ldc_pf(a.tile_pf, tile + 1);
+    }
+    int slot[MAXB];
+    if constexpr (!SPEC) {
+      const int wq = opaque_s(wv);   // keeps the loop-invariant loads inside the loop
+#pragma unroll
+      for (int bi = 0; bi < MAXB; ++bi) slot[bi] = ldc(a.assign, wq * MAXB + bi);
+    }
     // ---- staging rows (this wave's region; its scratch is no longer used) ----
     auto put = [&](unsigned char* row, int k, const float* v, int e) {
       if constexpr (CPLX) {
@@ -713,12 +841,14 @@ __global__ __launch_bounds__(NW * 64) MM_TILE_OCC void k_tile(TileArgs a) {
         if (lane == 0) sc_exp[fl] = (fr < T) ? fexp[q] : 0;
       }
     }
-    __syncthreads();
+    MM_PT(1);
+    lds_barrier();
+    MM_PT(2);
 
     if constexpr (SPEC) {
       // ---- transposed write-out: [F, T], runs of TF frames per bin ----
       const int nf = min(TF, T - f0);
-      const int64_t base = a.out_off[clip];
+      const int64_t base = ldc(a.out_off, clip);
       for (int idx = threadIdx.x; idx < F * TF; idx += NT) {
         const int f = idx % TF;
         const int k = idx / TF;
@@ -733,61 +863,109 @@ __global__ __launch_bounds__(NW * 64) MM_TILE_OCC void k_tile(TileArgs a) {
           }
         }
       }
-      __syncthreads();
+      prefetch_next(tile, nrec);
+      lds_barrier();
       continue;
     } else {
       // ---- banded mel contraction on MFMA ----
+      // Blocks of 8 mels: MFMA rows 8..15 carry zero weights, which halves
+      // the useful MFMA rate but lets the slowest wave of a tile hold ~40%
+      // fewer bin chunks than blocks of 16 would (MFMA time is not the bound).
       const int ri = lane & 15;          // MFMA row/col index within 16
       const int g = lane >> 4;           // lane group (K slice)
+      const bool arow = ri < kMelBlock;  // lanes that feed filterbank rows
       // columns past an 8-frame tile read row 0 and are never stored
       const unsigned char* prow = row_ptr(ri < TF ? ri : 0);
       const int myexp = (ri < TF) ? sc_exp[ri] : 0;
       const bool frame_ok = (ri < TF) && (f0 + ri) < T;
       f32x4 dbv[MAXB];
       float lmax = -INFINITY, lmin = INFINITY;
-      int nb = 0;
-      // mel^T block b over its bin chunks [c0, c1): operands for up to UC
-      // chunks are loaded before their MFMAs so the filterbank loads overlap
-      auto mel_range = [&](int b, int c0, int c1) -> f32x4 {
-        const int2 bd = a.band[b];
-        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-        if constexpr (PREC == PREC_F32) {
-          constexpr int UC = 2;
-          const float* fbrow = a.fb32 + (int64_t)(16 * b + ri) * a.fbs + bd.x + 4 * g;
-          const float* pr = reinterpret_cast<const float*>(prow) + bd.x + 4 * g;
-          for (int c = c0; c < c1; c += UC) {
-            float4 av[UC], bv[UC];
+      int nb = 0;   // slots in use (filled as a prefix)
 #pragma unroll
-            for (int u = 0; u < UC; ++u) {
-              const int cc = min(c + u, c1 - 1);
-              av[u] = *reinterpret_cast<const float4*>(fbrow + 16 * cc);
-              bv[u] = *reinterpret_cast<const float4*>(pr + 16 * cc);
+      for (int bi = 0; bi < MAXB; ++bi)
+        if (slot[bi] >= 0) nb = bi + 1;
+      // DCT-II operands of (row tile ct, mel block h): A entries for this lane
+      struct DctOp { float f[4]; uint2 hi, lo; };
+      auto load_dct = [&](int ct, int h) -> DctOp {
+        DctOp o;
+        if constexpr (PREC == PREC_F32) {
+          const float* dt = a.dct32 + ((int64_t)(ct * a.nmh + h) * 4) * 64;
+#pragma unroll
+          for (int j = 0; j < 4; ++j) o.f[j] = dt[j * 64 + lane];
+        } else {
+          const int64_t off = ((int64_t)(ct * a.nmh + h) * 64 + lane) * 4;
+          o.hi = *reinterpret_cast<const uint2*>(a.dct_hi + off);
+          o.lo = *reinterpret_cast<const uint2*>(a.dct_lo + off);
+        }
+        return o;
+      };
+      // the first row tile's DCT operands go out ahead of the filterbank loads
+      constexpr int PRE = 0;
+      DctOp dpre[PRE > 0 ? PRE : 1];
+      if constexpr (MODE == OUT_MFCC) {
+#pragma unroll
+        for (int bi = 0; bi < PRE; ++bi)
+          if (bi < nb) dpre[bi] = load_dct(0, slot[bi] & 63);
+      }
+      // mel^T block h over its bin band, in rounds of 8, 4, 2 or 1 chunks
+      // (chosen by the chunks left): a round issues all its filterbank and
+      // spectrum loads before its MFMAs, so their latencies overlap, and loads
+      // at most 3 chunks twice (clamped to the last chunk, MFMA skipped)
+      auto mel_block = [&](int h, int2 bd) -> f32x4 {
+        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        const int n = bd.y;
+        if constexpr (PREC == PREC_F32) {
+          // 32-bit element offsets from the table base (scalar base + vector offset loads)
+          const unsigned fo = (unsigned)((kMelBlock * h + (ri & (kMelBlock - 1))) * a.fbs + bd.x + 4 * g);
+          const float* pr = reinterpret_cast<const float*>(prow) + bd.x + 4 * g;
+          auto round = [&](auto kc, int c) {
+            constexpr int K = decltype(kc)::value;
+            float4 av[K], bv[K];
+#pragma unroll
+            for (int u = 0; u < K; ++u) av[u] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (arow) {
+#pragma unroll
+              for (int u = 0; u < K; ++u)
+                av[u] = *reinterpret_cast<const float4*>(a.fb32 + (fo + 16u * (unsigned)min(c + u, n - 1)));
             }
 #pragma unroll
-            for (int u = 0; u < UC; ++u) {
-              if (c + u < c1) {
+            for (int u = 0; u < K; ++u) bv[u] = *reinterpret_cast<const float4*>(pr + 16 * min(c + u, n - 1));
+#pragma unroll
+            for (int u = 0; u < K; ++u) {
+              if (u == 0 || c + u < n) {
                 acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[u].x, bv[u].x, acc, 0, 0, 0);
                 acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[u].y, bv[u].y, acc, 0, 0, 0);
                 acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[u].z, bv[u].z, acc, 0, 0, 0);
                 acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[u].w, bv[u].w, acc, 0, 0, 0);
               }
             }
+            wait_vmem_all();   // duplicate loads whose MFMA was skipped end here
+          };
+          for (int c = 0; c < n;) {
+            const int rem = n - c;
+            if (rem > 4) { round(std::integral_constant<int, 8>{}, c); c += 8; }
+            else if (rem > 2) { round(std::integral_constant<int, 4>{}, c); c += 4; }
+            else if (rem == 2) { round(std::integral_constant<int, 2>{}, c); c += 2; }
+            else { round(std::integral_constant<int, 1>{}, c); c += 1; }
           }
         } else {
-          constexpr int UC = 4;
-          const uint16_t* fbrow = a.fb16 + (int64_t)(16 * b + ri) * a.fbs + bd.x + 8 * g;
+          const unsigned fo = (unsigned)((kMelBlock * h + (ri & (kMelBlock - 1))) * a.fbs + bd.x + 8 * g);
           const uint16_t* pr = reinterpret_cast<const uint16_t*>(prow) + bd.x + 8 * g;
-          for (int c = c0; c < c1; c += UC) {
-            uint4 av[UC], bv[UC];
+          auto round = [&](auto kc, int c) {
+            constexpr int K = decltype(kc)::value;
+            uint4 av[K], bv[K];
 #pragma unroll
-            for (int u = 0; u < UC; ++u) {
-              const int cc = min(c + u, c1 - 1);
-              av[u] = *reinterpret_cast<const uint4*>(fbrow + 32 * cc);
-              bv[u] = *reinterpret_cast<const uint4*>(pr + 32 * cc);
+            for (int u = 0; u < K; ++u) av[u] = make_uint4(0u, 0u, 0u, 0u);
+            if (arow) {
+#pragma unroll
+              for (int u = 0; u < K; ++u)
+                av[u] = *reinterpret_cast<const uint4*>(a.fb16 + (fo + 32u * (unsigned)min(c + u, n - 1)));
             }
 #pragma unroll
-            for (int u = 0; u < UC; ++u) {
-              if (c + u < c1) {
+            for (int u = 0; u < K; ++u) bv[u] = *reinterpret_cast<const uint4*>(pr + 32 * min(c + u, n - 1));
+#pragma unroll
+            for (int u = 0; u < K; ++u) {
+              if (u == 0 || c + u < n) {
                 if constexpr (PREC == PREC_BF16)
                   acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, av[u]),
                                                                 __builtin_bit_cast(bf16x8, bv[u]), acc, 0, 0, 0);
@@ -796,53 +974,64 @@ __global__ __launch_bounds__(NW * 64) MM_TILE_OCC void k_tile(TileArgs a) {
                                                                __builtin_bit_cast(f16x8, bv[u]), acc, 0, 0, 0);
               }
             }
+            wait_vmem_all();   // duplicate loads whose MFMA was skipped end here
+          };
+          for (int c = 0; c < n;) {
+            const int rem = n - c;
+            if (rem > 4) { round(std::integral_constant<int, 8>{}, c); c += 8; }
+            else if (rem > 2) { round(std::integral_constant<int, 4>{}, c); c += 4; }
+            else if (rem == 2) { round(std::integral_constant<int, 2>{}, c); c += 2; }
+            else { round(std::integral_constant<int, 1>{}, c); c += 1; }
           }
         }
         return acc;
       };
-      // acc (of block b, all chunks) -> mel values, then the MEL store or dB
-      auto finish_block = [&](int b, int bi, f32x4 acc) {
+      // acc (of block h) -> mel values, then the MEL store or dB
+      auto finish_block = [&](int h, int bi, f32x4 acc) {
         if constexpr (PREC != PREC_F32) {
 #pragma unroll
           for (int j = 0; j < 4; ++j) acc[j] = ldexpf(acc[j], myexp);
         }
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[j] *= a.post_scale;   // 2^-power
-        // acc[j] = mel[16b + 4g + j] of tile frame ri
+        // acc[j] = mel[8h + 4g + j] of tile frame ri (g < 2; groups 2, 3 hold zeros)
         if constexpr (MODE == OUT_MEL) {
-          if (frame_ok) {
-            float* o = reinterpret_cast<float*>(a.out) + a.out_off[clip];
+          if (frame_ok && g < 2) {
+            float* o = reinterpret_cast<float*>(a.out) + ldc(a.out_off, clip);
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-              const int m = 16 * b + 4 * g + j;
+              const int m = kMelBlock * h + 4 * g + j;
               if (m < a.n_mels) o[(int64_t)m * T + f0 + ri] = acc[j];
             }
           }
         } else {
 #pragma unroll
           for (int j = 0; j < 4; ++j) {
-            const int m = 16 * b + 4 * g + j;
+            const int m = kMelBlock * h + 4 * g + j;
+            const bool valid = (g < 2) && (m < a.n_mels);
             float d = db10(fmaxf(acc[j], a.amin) / a.ref);
             d = fmaxf(d, thr);
-            if (m < a.n_mels && frame_ok) {
+            if (valid && frame_ok) {
               lmax = fmaxf(lmax, d);
               lmin = fminf(lmin, d);
             }
-            dbv[bi][j] = d;
+            dbv[bi][j] = valid ? d : 0.0f;
           }
         }
       };
 #pragma unroll
       for (int bi = 0; bi < MAXB; ++bi) {
-        const int b = wave + NW * bi;
-        if (b < a.nmb) {
-          nb = bi + 1;
-          finish_block(b, bi, mel_range(b, 0, a.band[b].y));
+        if (bi < nb) {
+          const int v = slot[bi];
+          finish_block(v & 63, bi, mel_block(v & 63, make_int2((v >> 6) & 8191, (v >> 19) & 4095)));
         }
       }
+      // waves without mel blocks reach this point at once: their prefetch
+      // overlaps the other waves' contraction
+      if constexpr (MODE == OUT_MEL) prefetch_next(tile, nrec);
 
       if constexpr (MODE == OUT_MFCC) {
-        const int nact = min(NW, a.nmb);   // waves holding mel blocks
+        const int nact = min(NW, a.nmh);   // waves holding mel blocks: 0 .. nact-1
         lmax = wave_max(lmax);
         lmin = wave_min(lmin);
         if (lane == 0) { red_max[wave] = lmax; red_min[wave] = lmin; }
@@ -851,23 +1040,21 @@ __global__ __launch_bounds__(NW * 64) MM_TILE_OCC void k_tile(TileArgs a) {
         // regions once every wave has read them (4 barriers)
         const bool sep = a.lds_part_off > 0;
         float* red = reinterpret_cast<float*>(lds + (sep ? a.lds_part_off : 0));
-        if (!sep) __syncthreads();   // staging reads done; region memory is reused below
-        if (wave < nact) {
+        if (!sep) lds_barrier();   // staging reads done; region memory is reused below
+        if (nb > 0) {
           for (int ct = 0; ct < a.ndt; ++ct) {
             f32x4 accd = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
             for (int bi = 0; bi < MAXB; ++bi) {
               if (bi < nb) {
-                const int b = wave + NW * bi;
+                DctOp op;
+                if (bi < PRE && ct == 0) op = dpre[bi < PRE ? bi : 0];
+                else op = load_dct(ct, slot[bi] & 63);
                 if constexpr (PREC == PREC_F32) {
-                  const float* dt = a.dct32 + ((int64_t)(ct * a.nmb + b) * 4) * 64;
 #pragma unroll
                   for (int j = 0; j < 4; ++j)
-                    accd = __builtin_amdgcn_mfma_f32_16x16x4f32(dt[j * 64 + lane], dbv[bi][j], accd, 0, 0, 0);
+                    accd = __builtin_amdgcn_mfma_f32_16x16x4f32(op.f[j], dbv[bi][j], accd, 0, 0, 0);
                 } else {
-                  const int64_t off = ((int64_t)(ct * a.nmb + b) * 64 + lane) * 4;
-                  const uint2 hi2 = *reinterpret_cast<const uint2*>(a.dct_hi + off);
-                  const uint2 lo2 = *reinterpret_cast<const uint2*>(a.dct_lo + off);
                   uint16_t bh[4], bl[4];
 #pragma unroll
                   for (int j = 0; j < 4; ++j) {
@@ -883,13 +1070,13 @@ __global__ __launch_bounds__(NW * 64) MM_TILE_OCC void k_tile(TileArgs a) {
                   const uint2 bhi2 = make_uint2(bh[0] | ((unsigned)bh[1] << 16), bh[2] | ((unsigned)bh[3] << 16));
                   const uint2 blo2 = make_uint2(bl[0] | ((unsigned)bl[1] << 16), bl[2] | ((unsigned)bl[3] << 16));
                   if constexpr (PREC == PREC_BF16) {
-                    const s16x4 Ah = __builtin_bit_cast(s16x4, hi2), Al = __builtin_bit_cast(s16x4, lo2);
+                    const s16x4 Ah = __builtin_bit_cast(s16x4, op.hi), Al = __builtin_bit_cast(s16x4, op.lo);
                     const s16x4 Bh = __builtin_bit_cast(s16x4, bhi2), Bl = __builtin_bit_cast(s16x4, blo2);
                     accd = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Ah, Bh, accd, 0, 0, 0);
                     accd = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Al, Bh, accd, 0, 0, 0);
                     accd = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Ah, Bl, accd, 0, 0, 0);
                   } else {
-                    const f16x4 Ah = __builtin_bit_cast(f16x4, hi2), Al = __builtin_bit_cast(f16x4, lo2);
+                    const f16x4 Ah = __builtin_bit_cast(f16x4, op.hi), Al = __builtin_bit_cast(f16x4, op.lo);
                     const f16x4 Bh = __builtin_bit_cast(f16x4, bhi2), Bl = __builtin_bit_cast(f16x4, blo2);
                     accd = __builtin_amdgcn_mfma_f32_16x16x16f16(Ah, Bh, accd, 0, 0, 0);
                     accd = __builtin_amdgcn_mfma_f32_16x16x16f16(Al, Bh, accd, 0, 0, 0);
@@ -902,7 +1089,10 @@ __global__ __launch_bounds__(NW * 64) MM_TILE_OCC void k_tile(TileArgs a) {
             *reinterpret_cast<float4*>(dst) = make_float4(accd[0], accd[1], accd[2], accd[3]);
           }
         }
-        __syncthreads();   // partials and per-wave extrema are in LDS
+        prefetch_next(tile, nrec);
+        MM_PT(3);
+        lds_barrier();   // partials and per-wave extrema are in LDS
+        MM_PT(4);
         if (threadIdx.x == 0 && a.phase == 1 && a.use_top_db) {
           float bm = -INFINITY, bn = INFINITY;
           for (int w = 0; w < NW; ++w) { bm = fmaxf(bm, red_max[w]); bn = fminf(bn, red_min[w]); }
@@ -918,7 +1108,7 @@ __global__ __launch_bounds__(NW * 64) MM_TILE_OCC void k_tile(TileArgs a) {
             sm.x += p.x; sm.y += p.y; sm.z += p.z; sm.w += p.w;
           }
           if (frame_ok) {
-            float* o = reinterpret_cast<float*>(a.out) + a.out_off[clip];
+            float* o = reinterpret_cast<float*>(a.out) + ldc(a.out_off, clip);
             const float sv[4] = {sm.x, sm.y, sm.z, sm.w};
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
@@ -927,11 +1117,13 @@ __global__ __launch_bounds__(NW * 64) MM_TILE_OCC void k_tile(TileArgs a) {
             }
           }
         }
+        MM_PT(5);
         if (sep) continue;
       }
-      __syncthreads();
+      lds_barrier();
     }
   }
+  MM_PT_STORE();
 }
 
 // ---------------------------------------------------------------------------
@@ -1121,13 +1313,13 @@ __global__ void k_tab_mel(float* fb32, uint16_t* fbb, uint16_t* fbh, int fbs, in
   fbh[i] = f2h(wf);
 }
 
-// per block of 16 mel rows: first nonzero bin and chunk counts (16 and 32 bins)
-__global__ void k_tab_bands(const float* fb32, int fbs, int F, int nmb, int2* band32, int2* band16) {
+// per block of 8 mel rows: first nonzero bin and chunk counts (16 and 32 bins)
+__global__ void k_tab_bands(const float* fb32, int fbs, int F, int nmh, int2* band32, int2* band16) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nmb) return;
+  if (b >= nmh) return;
   int lo = F, hi = -1;
-  for (int r = 0; r < 16; ++r) {
-    const float* row = fb32 + (int64_t)(16 * b + r) * fbs;
+  for (int r = 0; r < kMelBlock; ++r) {
+    const float* row = fb32 + (int64_t)(kMelBlock * b + r) * fbs;
     for (int k = 0; k < F; ++k)
       if (row[k] != 0.0f) { lo = min(lo, k); hi = max(hi, k); }
   }
@@ -1137,27 +1329,39 @@ __global__ void k_tab_bands(const float* fb32, int fbs, int F, int nmb, int2* ba
   band16[b] = make_int2(s16, (hi + 1 - s16 + 31) / 32);
 }
 
+// assign[pc][nwi][w * (32 / nw) + i]: block lists of mel_block_assign for
+// pc 0 = 16-bin chunks (FP32), 1 = 32-bin chunks (16-bit) and nwi 0 = 8
+// waves, 1 = 16 waves.
+__global__ void k_tab_assign(const int2* band32, const int2* band16, int nmh, int* assign) {
+  if (blockIdx.x != 0 || threadIdx.x >= 4) return;
+  const int pc = threadIdx.x >> 1, nwi = threadIdx.x & 1;
+  mel_block_assign(pc ? band16 : band32, nmh, nwi ? 16 : 8, assign + (pc * 2 + nwi) * kMaxMelBlocks);
+}
+
 __device__ inline double dct_val(int c, int m, int n_mfcc, int n_mels) {
   if (c >= n_mfcc || m >= n_mels) return 0.0;
   const double norm = (c == 0) ? sqrt(1.0 / n_mels) : sqrt(2.0 / n_mels);
   return cospi((double)c * (2.0 * m + 1.0) / (2.0 * n_mels)) * norm;
 }
 
-// dct32 [ndt][nmb][4][64]: lane l, step j -> D[16ct + l%16][16b + 4(l/16) + j]
-// dct16 [ndt][nmb][64][4]: lane l, elem j -> same entry (hi / lo split)
+// dct32 [ndt][nmh][4][64]: lane l, step j -> D[16ct + l%16][8b + 4(l/16) + j]
+// for l/16 < 2 and zero for l/16 >= 2 (the MFMA K slots that a block of 8
+// mels leaves empty); dct16 [ndt][nmh][64][4]: lane l, elem j -> same entry
+// (hi / lo split)
 __global__ void k_tab_dct(float* dct32, uint16_t* bhi, uint16_t* blo, uint16_t* hhi, uint16_t* hlo, float* dense,
-                          int ndt, int nmb, int n_mfcc, int n_mels) {
+                          int ndt, int nmh, int n_mfcc, int n_mels) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t n = (int64_t)ndt * nmb * 256;
+  const int64_t n = (int64_t)ndt * nmh * 256;
   if (i < n) {
     const int lane = (int)(i % 64);
     const int j = (int)((i / 64) % 4);
-    const int b = (int)((i / 256) % nmb);
-    const int ct = (int)(i / (256 * (int64_t)nmb));
-    const int c = 16 * ct + (lane & 15), m = 16 * b + 4 * (lane >> 4) + j;
-    const double d = dct_val(c, m, n_mfcc, n_mels);
+    const int b = (int)((i / 256) % nmh);
+    const int ct = (int)(i / (256 * (int64_t)nmh));
+    const int g = lane >> 4;
+    const int c = 16 * ct + (lane & 15), m = kMelBlock * b + 4 * g + j;
+    const double d = (g < 2) ? dct_val(c, m, n_mfcc, n_mels) : 0.0;
     dct32[i] = (float)d;
-    const int64_t o = (((int64_t)ct * nmb + b) * 64 + lane) * 4 + j;
+    const int64_t o = (((int64_t)ct * nmh + b) * 64 + lane) * 4 + j;
     const uint16_t h = f2bf((float)d);
     bhi[o] = h;
     blo[o] = f2bf((float)(d - (double)bf2f(h)));
@@ -1199,13 +1403,14 @@ hipError_t gen_tables(const TableKey& key, const DeviceTables& t, hipStream_t st
   if (rows > 0) {
     hipLaunchKernelGGL(k_tab_mel, dim3(nblk((int64_t)rows * t.fbs, 256)), dim3(256), 0, st, t.fb32, t.fb16, t.fbh,
                        t.fbs, rows, key.sr, key.n_fft, key.n_mels, (double)key.fmin, (double)key.fmax);
-    hipLaunchKernelGGL(k_tab_bands, dim3(nblk(t.nmb, 64)), dim3(64), 0, st, t.fb32, t.fbs, F, t.nmb, t.band32,
+    hipLaunchKernelGGL(k_tab_bands, dim3(nblk(t.nmh, 64)), dim3(64), 0, st, t.fb32, t.fbs, F, t.nmh, t.band32,
                        t.band16);
+    hipLaunchKernelGGL(k_tab_assign, dim3(1), dim3(64), 0, st, t.band32, t.band16, t.nmh, t.assign);
   }
-  const int64_t nd = std::max<int64_t>((int64_t)t.ndt * t.nmb * 256, (int64_t)key.n_mfcc * key.n_mels);
+  const int64_t nd = std::max<int64_t>((int64_t)t.ndt * t.nmh * 256, (int64_t)key.n_mfcc * key.n_mels);
   if (nd > 0)
     hipLaunchKernelGGL(k_tab_dct, dim3(nblk(nd, 256)), dim3(256), 0, st, t.dct32, t.dctb_hi, t.dctb_lo, t.dcth_hi,
-                       t.dcth_lo, t.dct_dense, t.ndt, t.nmb, key.n_mfcc, key.n_mels);
+                       t.dcth_lo, t.dct_dense, t.ndt, t.nmh, key.n_mfcc, key.n_mels);
   return hipGetLastError();
 }
 
@@ -1276,7 +1481,7 @@ bool tile_layout(int n_fft, int out_kind, int prec, int n_mels, int n_mfcc, Tile
   const int M = n_fft / 2, F = M + 1;
   const bool spec = out_kind <= OUT_STFT_COMPLEX;
   if (!spec) {
-    if (n_mels < 1 || n_mels > 512) return false;
+    if (n_mels < 1 || n_mels > kMaxMelBlocks * kMelBlock) return false;   // larger: generic path
     if (out_kind == OUT_MFCC && (n_mfcc < 1 || n_mfcc > 128)) return false;
   }
   const int nw = (M > 1024) ? 8 : (g_tile_waves ? g_tile_waves : 16), fpw = 1;
@@ -1303,7 +1508,7 @@ bool tile_layout(int n_fft, int out_kind, int prec, int n_mels, int n_mfcc, Tile
   int part_off = 0;
   if (out_kind == OUT_MFCC) {
     const int ndt = (n_mfcc + 15) / 16;
-    const int nact = std::min(nw, (n_mels + 15) / 16);
+    const int nact = std::min(nw, (n_mels + kMelBlock - 1) / kMelBlock);
     const size_t part = (size_t)ndt * nact * 256 * 4;
     if (total + part + 16 <= lds_limit_bytes()) {
       // DCT partials in their own area: two barriers per tile
