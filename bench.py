@@ -35,6 +35,24 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_tile_mfcc_bf16_uneven_batch_traffic.json")
+
+
+def measured_traffic(bytes_launch):
+    """HBM bytes per launch of the fused kernel from the committed rocprofv3
+    counter summary (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by
+    what the counter reads for a copy of known size), if it was taken on this
+    same workload (same algorithmic bytes); else None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if int(t.get("algorithmic_bytes_per_launch", -1)) != int(bytes_launch):
+        return None
+    return float(t["hbm_bytes_per_launch"])
+
+
 def cpu_baseline(clips, max_frames=60000):
     """Float32 NumPy restatement of the reference CPU path on a bounded sample
     of the same batch (whole clips until max_frames), one core."""
@@ -74,6 +92,8 @@ def main():
     ap.add_argument("--serve-rates", default="1000,5000",
                     help="Poisson request rates (req/s) for the serving leg; empty to skip")
     ap.add_argument("--serve-seconds", type=float, default=2.0)
+    ap.add_argument("--grid", type=int, default=0,
+                    help="persistent workgroups per CU (skips the autotuner; 0 = autotune)")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="profiling aid: also run one device-to-device copy of this many bytes, "
                          "so HBM counters can be scaled by a kernel of known traffic")
@@ -104,8 +124,13 @@ def main():
     stream = torch.cuda.Stream()
     log(f"batch: {len(clips)} clips, {B.total_input} samples, {frames} frames, {B.n_tiles} tiles")
 
-    best_grid, tune_ms = B.autotune(d_in, out, iters=2)
-    log(f"autotuned persistent grid: {best_grid} workgroups/CU ({tune_ms:.3f} ms/step)")
+    if args.grid > 0:
+        best_grid = args.grid
+        B.set_grid(best_grid)
+        log(f"persistent grid: {best_grid} workgroups/CU (fixed)")
+    else:
+        best_grid, tune_ms = B.autotune(d_in, out, iters=2)
+        log(f"autotuned persistent grid: {best_grid} workgroups/CU ({tune_ms:.3f} ms/step)")
 
     steps = 3 if args.profile else args.steps
     warm = 1 if args.profile else args.warmup
@@ -136,6 +161,11 @@ def main():
     bytes_launch = 4 * B.total_input + 4 * n_mfcc * frames
     achieved = bytes_launch / (k_ms * 1e-3) / 1e9
     log(f"fused kernel {k_ms:.3f} ms, step {s_ms:.3f} ms, {achieved:.0f} GB/s algorithmic")
+
+    traffic = measured_traffic(bytes_launch)
+    if traffic is not None:
+        log(f"HBM traffic per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated): {traffic / 1e6:.1f} MB "
+            f"= {traffic / bytes_launch:.2f}x algorithmic")
 
     extra = {}
     if not args.profile:
@@ -221,7 +251,7 @@ def main():
                    "clips": len(clips), "frames": frames, "samples": B.total_input,
                    "persistent_wg_per_cu": best_grid},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel_ms": k_ms, "bytes_per_launch": bytes_launch},
         "cpu_baseline": cpu,
     }
