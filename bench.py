@@ -53,9 +53,23 @@ def measured_traffic(bytes_launch):
     return float(t["hbm_bytes_per_launch"])
 
 
-def cpu_baseline(clips, max_frames=60000):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(clips, max_frames=400000, workers=0):
     """Float32 NumPy restatement of the reference CPU path on a bounded sample
-    of the same batch (whole clips until max_frames), one core."""
+    of the same batch (whole clips until max_frames; the default takes the
+    whole 336,013-frame batch), BLAS limited to one thread. workers = 0: one
+    core, clips in sequence; workers > 0: that many threads over the clips
+    (numpy's FFT, BLAS and elementwise kernels run without the GIL)."""
     from oracle.cpu_baseline import MfccF32
     try:
         from threadpoolctl import threadpool_limits
@@ -71,14 +85,21 @@ def cpu_baseline(clips, max_frames=60000):
         frames += 1 + len(c) // 512
     f(sample[0][:22050])  # warm-up
     t0 = time.perf_counter()
-    for c in sample:
-        f(c)
+    if workers > 0:
+        from concurrent.futures import ThreadPoolExecutor
+        with ThreadPoolExecutor(max_workers=workers) as ex:
+            list(ex.map(f, sample))
+    else:
+        for c in sample:
+            f(c)
     dt = time.perf_counter() - t0
     if limiter is not None:
         limiter.unregister() if hasattr(limiter, "unregister") else None
-    return {"value": frames / dt, "unit": "frames/s", "cores": 1, "kind": "port",
+    return {"value": frames / dt, "unit": "frames/s", "cores": max(workers, 1), "kind": "port",
             "sample": f"{len(sample)} whole clips of the uneven batch, {frames} frames, "
-                      f"float32 numpy rfft + dense FB@P + dB + DCT, {dt:.1f} s"}
+                      f"float32 numpy rfft + dense FB@P + dB + DCT, {dt:.1f} s"
+                      + (f", {workers} threads" if workers else ", 1 thread"),
+            "host_cpus": os.cpu_count(), "cpu_model": cpu_model()}
 
 
 def main():
@@ -187,17 +208,20 @@ def main():
         log(f"fp32 batched: {f32_fps / 1e6:.1f} M frames/s")
 
         if not args.no_plain:
-            # plain path: one clip per launch, FP32, no graph, sequential
+            # plain path: one clip per launch, FP32, no graph, in sequence on
+            # one stream (each plan would otherwise use its own pooled stream
+            # and clips could overlap across hardware queues)
             singles = [SpectralBatch([n], kind="mfcc", n_fft=2048, hop_length=512, n_mels=128,
                                      n_mfcc=n_mfcc, precision="f32") for n in lens]
             ins = [d_in[B.clips[i][0]:B.clips[i][0] + lens[i]] for i in range(len(lens))]
             outs = [s.empty_output() for s in singles]
+            plain_stream = torch.cuda.Stream()
             for s, i_, o_ in zip(singles, ins, outs):
-                s.run(i_, o_)
+                s.run(i_, o_, stream=plain_stream)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             for s, i_, o_ in zip(singles, ins, outs):
-                s.run(i_, o_)
+                s.run(i_, o_, stream=plain_stream)
             torch.cuda.synchronize()
             plain_fps = frames / (time.perf_counter() - t0)
             extra["plain_fp32_frames_per_s"] = plain_fps
@@ -231,6 +255,11 @@ def main():
         log("cpu baseline ...")
         cpu = cpu_baseline(clips)
         log(f"cpu baseline: {cpu['value']:.0f} frames/s ({cpu['sample']})")
+        # all host cores this job may use (at most 16 on a one-GPU share)
+        nw = max(1, min(16, os.cpu_count() or 1))
+        cpu_all = cpu_baseline(clips, workers=nw)
+        extra["cpu_baseline_all_cores"] = cpu_all
+        log(f"cpu baseline, {nw} threads: {cpu_all['value']:.0f} frames/s")
 
     rec = {
         "metric": METRIC,

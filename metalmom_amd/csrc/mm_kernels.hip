@@ -553,7 +553,8 @@ __device__ __forceinline__ void real_split(const float2* scr, const float2* __re
 }
 
 // |X|^power from |X|^2: power 2 and 1 exactly; otherwise 2^(p/2 log2 |X|^2)
-// with the hardware log2/exp2.
+// with the hardware log2/exp2. Callers on the hot path branch on the power
+// once per frame and call this only for powers other than 2.
 __device__ __forceinline__ float pow_from_p2(float p2, float power) {
   if (power == 2.0f) return p2;
   if (power == 1.0f) return sqrtf(p2);
@@ -715,12 +716,10 @@ __global__ __launch_bounds__(NW * 64) MM_TILE_OCC void k_tile(TileArgs a) {
                 hv[q][b][r][EC - 1] = 0.5f * X.y;
               } else {
                 const float p2 = fmaf(X.x, X.x, X.y * X.y);   // 4 |X[k]|^2
-                float v;
+                float v = p2;
                 if constexpr (MODE == OUT_STFT_MAG) v = 0.5f * sqrtf(p2);
                 else if constexpr (MODE == OUT_STFT_POWER) v = 0.25f * p2;
-                else v = pow_from_p2(p2, a.power);   // 2^p |X|^p: removed after the mel product
                 hv[q][b][r][0] = v;
-                vmax = fmaxf(vmax, v);
               }
             }
           if constexpr (CPLX) {
@@ -728,12 +727,35 @@ __global__ __launch_bounds__(NW * 64) MM_TILE_OCC void k_tile(TileArgs a) {
             hn[q][EC - 1] = 0.0f;
           } else {
             const float p2 = nyq * nyq;
-            float v;
+            float v = p2;
             if constexpr (MODE == OUT_STFT_MAG) v = 0.5f * fabsf(nyq);
             else if constexpr (MODE == OUT_STFT_POWER) v = 0.25f * p2;
-            else v = pow_from_p2(p2, a.power);
             hn[q][0] = v;
-            if (lane == 0) vmax = fmaxf(vmax, v);
+          }
+          if constexpr (!SPEC) {
+            // (2|X|)^p from (2|X|)^2 (the 2^p is removed after the mel
+            // product): one wave-uniform branch per frame, so the default
+            // power 2 runs no transcendental instruction at all
+            if (a.power != 2.0f) {
+              if (a.power == 1.0f) {
+#pragma unroll
+                for (int b = 0; b < QL; ++b)
+#pragma unroll
+                  for (int r = 0; r < RL; ++r) hv[q][b][r][0] = sqrtf(hv[q][b][r][0]);
+                hn[q][0] = sqrtf(hn[q][0]);
+              } else {
+#pragma unroll
+                for (int b = 0; b < QL; ++b)
+#pragma unroll
+                  for (int r = 0; r < RL; ++r) hv[q][b][r][0] = pow_from_p2(hv[q][b][r][0], a.power);
+                hn[q][0] = pow_from_p2(hn[q][0], a.power);
+              }
+            }
+#pragma unroll
+            for (int b = 0; b < QL; ++b)
+#pragma unroll
+              for (int r = 0; r < RL; ++r) vmax = fmaxf(vmax, hv[q][b][r][0]);
+            if (lane == 0) vmax = fmaxf(vmax, hn[q][0]);
           }
         } else {
           frame_fft<LOG2M, TIn>(a, clip, fr, scr, lane);
