@@ -123,6 +123,25 @@ template <typename T>
 __device__ __forceinline__ T ldc(const T* p, int64_t i) {
   return ((const __attribute__((address_space(4))) T*)p)[i];
 }
+// Global load at base + byte_off through a 32-bit vector offset. With a
+// wave-uniform base this is the scalar-base addressing form of global loads:
+// one offset register per lane and no 64-bit address arithmetic per load.
+// Constant parts of an address belong in the base (scalar adds only).
+template <typename T>
+__device__ __forceinline__ T ldg(const void* base, unsigned byte_off) {
+  return *reinterpret_cast<const T*>(static_cast<const char*>(base) + byte_off);
+}
+
+// A pointer that is equal in all active lanes of the wave, moved into scalar
+// registers (the compiler cannot see that a per-wave value is uniform).
+template <typename T>
+__device__ __forceinline__ const T* wave_uniform(const T* p) {
+  const unsigned long long v = reinterpret_cast<unsigned long long>(p);
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+  return reinterpret_cast<const T*>(((unsigned long long)hi << 32) | lo);
+}
+
 __device__ __forceinline__ TilePf ldc_pf(const TilePf* p, int64_t i) {
   TilePf r;
   r.base = ldc(&p[i].base, 0);
@@ -296,9 +315,9 @@ __device__ __forceinline__ void stockham_pass(float2* scr, const float2* __restr
     if ((B % 64 == 0) || j < B) {
       const unsigned k = (unsigned)j % NS;
       constexpr int P = __builtin_ctz(NS) / 4;
-      const float2* t = tw + PassTw<M, P>::OFF + k;
 #pragma unroll
-      for (int r = 1; r < R; ++r) v[q][r] = cmul(v[q][r], t[(r - 1) * NS]);
+      for (int r = 1; r < R; ++r)
+        v[q][r] = cmul(v[q][r], ldg<float2>(tw + PassTw<M, P>::OFF + (r - 1) * NS, k * 8u));
       Dft<R>::run(v[q]);
       const int base = (j / NS) * NS * R + k;
 #pragma unroll
@@ -343,9 +362,9 @@ __device__ __forceinline__ void stockham_last_regs(const float2* scr, const floa
   for (int q = 0; q < QB; ++q) {
     const unsigned k = (unsigned)(lane + 64 * q) % NS;
     constexpr int P = __builtin_ctz(NS) / 4;
-    const float2* t = tw + PassTw<M, P>::OFF + k;
 #pragma unroll
-    for (int r = 1; r < R; ++r) z[q][r] = cmul(z[q][r], t[(r - 1) * NS]);
+    for (int r = 1; r < R; ++r)
+      z[q][r] = cmul(z[q][r], ldg<float2>(tw + PassTw<M, P>::OFF + (r - 1) * NS, k * 8u));
     Dft<R>::run(z[q]);
   }
 }
@@ -358,12 +377,15 @@ __device__ __forceinline__ void stockham_last_regs(const float2* scr, const floa
 typedef _Float16 h2v __attribute__((ext_vector_type(2)));
 
 template <typename TIn> struct SamplePair;
+// (x[2n], x[2n+1]) at base + byte offset (see ldg)
 template <> struct SamplePair<float> {
-  __device__ __forceinline__ static float2 load(const float* p) { return *reinterpret_cast<const float2*>(p); }
+  __device__ __forceinline__ static float2 load(const float* base, unsigned byte_off) {
+    return ldg<float2>(base, byte_off);
+  }
 };
 template <> struct SamplePair<_Float16> {
-  __device__ __forceinline__ static float2 load(const _Float16* p) {
-    const h2v h = *reinterpret_cast<const h2v*>(p);
+  __device__ __forceinline__ static float2 load(const _Float16* base, unsigned byte_off) {
+    const h2v h = ldg<h2v>(base, byte_off);
     return make_float2((float)h.x, (float)h.y);
   }
 };
@@ -381,32 +403,34 @@ __device__ __forceinline__ void frame_load(const TileArgs& a, const TIn* __restr
   using S = FftShape<LOG2M>;
   constexpr int M = S::M, R0 = S::R0, B0 = S::B0, Q0 = S::Q0;
   const bool interior = (s0 >= 0) && (s0 + 2 * M <= len);
+  // x + s0 is the same in every lane (one frame per wave): scalar base, and
+  // each load adds only the lane's byte offset
   if (interior && ((s0 & 1) == 0)) {
-    const TIn* xf = x + s0;
+    const TIn* xf = wave_uniform(x + s0);
 #pragma unroll
     for (int q = 0; q < Q0; ++q) {
       const int j = lane + 64 * q;
       if ((B0 % 64 == 0) || j < B0) {
 #pragma unroll
         for (int r = 0; r < R0; ++r) {
-          const unsigned n = (unsigned)(j + r * B0);
-          const float2 w = *reinterpret_cast<const float2*>(a.window + 2 * n);
-          const float2 xs = SamplePair<TIn>::load(xf + 2 * n);
+          const float2 w = ldg<float2>(a.window + 2 * r * B0, (unsigned)j * 8u);
+          const float2 xs = SamplePair<TIn>::load(xf + 2 * r * B0, (unsigned)j * (2u * sizeof(TIn)));
           v[q][r] = make_float2(xs.x * w.x, xs.y * w.y);
         }
       }
     }
   } else if (interior) {
-    const TIn* xf = x + s0;
+    const TIn* xf = wave_uniform(x + s0);
 #pragma unroll
     for (int q = 0; q < Q0; ++q) {
       const int j = lane + 64 * q;
       if ((B0 % 64 == 0) || j < B0) {
 #pragma unroll
         for (int r = 0; r < R0; ++r) {
-          const unsigned n = (unsigned)(j + r * B0);
-          const float2 w = *reinterpret_cast<const float2*>(a.window + 2 * n);
-          v[q][r] = make_float2((float)xf[2 * n] * w.x, (float)xf[2 * n + 1] * w.y);
+          const float2 w = ldg<float2>(a.window + 2 * r * B0, (unsigned)j * 8u);
+          const unsigned o = (unsigned)j * (2u * sizeof(TIn));
+          v[q][r] = make_float2((float)ldg<TIn>(xf + 2 * r * B0, o) * w.x,
+                                (float)ldg<TIn>(xf + 2 * r * B0 + 1, o) * w.y);
         }
       }
     }
@@ -475,8 +499,7 @@ __device__ __forceinline__ void frame_fft_regs(const TileArgs& a, int clip, int 
         const int j = lane + 64 * q;
 #pragma unroll
         for (int r = 0; r < R0; ++r) {
-          const unsigned n = (unsigned)(j + r * B0);
-          const float2 w = *reinterpret_cast<const float2*>(a.window + 2 * n);
+          const float2 w = ldg<float2>(a.window + 2 * r * B0, (unsigned)j * 8u);
           v[q][r] = make_float2(pf[q][r].x * w.x, pf[q][r].y * w.y);
         }
       }
@@ -527,8 +550,8 @@ __device__ __forceinline__ void real_split_regs(const float2 (&z)[FftShape<LOG2M
       const float2 A = z[q][r];
       const float2 E = make_float2(A.x + bz.x, A.y - bz.y);   // 2E
       const float2 O = make_float2(A.x - bz.x, A.y + bz.y);   // 2O
-      const unsigned k = (unsigned)(lane + 64 * q + BL * r);
-      const float2 T = cmul(O, twN[k]);
+      // W_N^k, k = lane + 64 q + BL r
+      const float2 T = cmul(O, ldg<float2>(twN + 64 * q + BL * r, (unsigned)lane * 8u));
       x[q][r] = make_float2(E.x + T.y, E.y - T.x);            // 2 X[k]
     }
   }
@@ -643,15 +666,13 @@ __global__ __launch_bounds__(NW * 64) MM_TILE_OCC void k_tile(TileArgs a) {
       const bool ok = (r.mask >> wave) & 1u;
       pf_tile = ok ? tile_now + 1 : -1;
       if (ok) {
-        const TIn* xf = static_cast<const TIn*>(a.signal) + r.base + (int64_t)wave * a.hop;
-        const int l = opaque(lane0);
+        const TIn* xf = wave_uniform(static_cast<const TIn*>(a.signal) + r.base + (int64_t)wave * a.hop);
+        const unsigned lo = (unsigned)opaque(lane0) * (2u * sizeof(TIn));
 #pragma unroll
         for (int q = 0; q < SHP::Q0; ++q)
 #pragma unroll
-          for (int r2 = 0; r2 < SHP::R0; ++r2) {
-            const unsigned n = (unsigned)(l + 64 * q + r2 * SHP::B0);
-            pf[q][r2] = SamplePair<TIn>::load(xf + 2 * n);
-          }
+          for (int r2 = 0; r2 < SHP::R0; ++r2)   // sample pair n = lane + 64 q + r2 B0
+            pf[q][r2] = SamplePair<TIn>::load(xf + 2 * (64 * q + r2 * SHP::B0), lo);
       } else {
 #pragma unroll
         for (int q = 0; q < SHP::Q0; ++q)
@@ -910,14 +931,13 @@ __global__ __launch_bounds__(NW * 64) MM_TILE_OCC void k_tile(TileArgs a) {
       struct DctOp { float f[4]; uint2 hi, lo; };
       auto load_dct = [&](int ct, int h) -> DctOp {
         DctOp o;
+        const int64_t e0 = (int64_t)(ct * a.nmh + h) * 256;   // entries of (ct, h)
         if constexpr (PREC == PREC_F32) {
-          const float* dt = a.dct32 + ((int64_t)(ct * a.nmh + h) * 4) * 64;
 #pragma unroll
-          for (int j = 0; j < 4; ++j) o.f[j] = dt[j * 64 + lane];
+          for (int j = 0; j < 4; ++j) o.f[j] = ldg<float>(a.dct32 + e0 + j * 64, (unsigned)lane * 4u);
         } else {
-          const int64_t off = ((int64_t)(ct * a.nmh + h) * 64 + lane) * 4;
-          o.hi = *reinterpret_cast<const uint2*>(a.dct_hi + off);
-          o.lo = *reinterpret_cast<const uint2*>(a.dct_lo + off);
+          o.hi = ldg<uint2>(a.dct_hi + e0, (unsigned)lane * 8u);
+          o.lo = ldg<uint2>(a.dct_lo + e0, (unsigned)lane * 8u);
         }
         return o;
       };
@@ -937,8 +957,9 @@ __global__ __launch_bounds__(NW * 64) MM_TILE_OCC void k_tile(TileArgs a) {
         f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
         const int n = bd.y;
         if constexpr (PREC == PREC_F32) {
-          // 32-bit element offsets from the table base (scalar base + vector offset loads)
-          const unsigned fo = (unsigned)((kMelBlock * h + (ri & (kMelBlock - 1))) * a.fbs + bd.x + 4 * g);
+          // chunk base (uniform) + this lane's byte offset within the block
+          const float* fbb = a.fb32 + (int64_t)kMelBlock * h * a.fbs + bd.x;
+          const unsigned fo = (unsigned)(((ri & (kMelBlock - 1)) * a.fbs + 4 * g) * 4);
           const float* pr = reinterpret_cast<const float*>(prow) + bd.x + 4 * g;
           auto round = [&](auto kc, int c) {
             constexpr int K = decltype(kc)::value;
@@ -948,7 +969,7 @@ __global__ __launch_bounds__(NW * 64) MM_TILE_OCC void k_tile(TileArgs a) {
             if (arow) {
 #pragma unroll
               for (int u = 0; u < K; ++u)
-                av[u] = *reinterpret_cast<const float4*>(a.fb32 + (fo + 16u * (unsigned)min(c + u, n - 1)));
+                av[u] = ldg<float4>(fbb + 16 * min(c + u, n - 1), fo);
             }
 #pragma unroll
             for (int u = 0; u < K; ++u) bv[u] = *reinterpret_cast<const float4*>(pr + 16 * min(c + u, n - 1));
@@ -971,7 +992,8 @@ __global__ __launch_bounds__(NW * 64) MM_TILE_OCC void k_tile(TileArgs a) {
             else { round(std::integral_constant<int, 1>{}, c); c += 1; }
           }
         } else {
-          const unsigned fo = (unsigned)((kMelBlock * h + (ri & (kMelBlock - 1))) * a.fbs + bd.x + 8 * g);
+          const uint16_t* fbb = a.fb16 + (int64_t)kMelBlock * h * a.fbs + bd.x;
+          const unsigned fo = (unsigned)(((ri & (kMelBlock - 1)) * a.fbs + 8 * g) * 2);
           const uint16_t* pr = reinterpret_cast<const uint16_t*>(prow) + bd.x + 8 * g;
           auto round = [&](auto kc, int c) {
             constexpr int K = decltype(kc)::value;
@@ -981,7 +1003,7 @@ __global__ __launch_bounds__(NW * 64) MM_TILE_OCC void k_tile(TileArgs a) {
             if (arow) {
 #pragma unroll
               for (int u = 0; u < K; ++u)
-                av[u] = *reinterpret_cast<const uint4*>(a.fb16 + (fo + 32u * (unsigned)min(c + u, n - 1)));
+                av[u] = ldg<uint4>(fbb + 32 * min(c + u, n - 1), fo);
             }
 #pragma unroll
             for (int u = 0; u < K; ++u) bv[u] = *reinterpret_cast<const uint4*>(pr + 32 * min(c + u, n - 1));
