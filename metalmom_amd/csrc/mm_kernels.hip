@@ -160,6 +160,8 @@ __device__ __forceinline__ void wait_vmem_all() { __builtin_amdgcn_s_waitcnt(0x0
 // Padded LDS index: one spare complex slot per 16 keeps the strided radix-16
 // accesses of a Stockham pass on distinct banks.
 __device__ __forceinline__ int pidx(int i) { return i + (i >> 4); }
+// For a stride C that is a multiple of 16, pidx(i + C) = pidx(i) + C + C/16:
+// accesses at such strides use pidx(i) plus a constant (an LDS immediate offset).
 
 // Index of sample `i` of a clip of `len` samples after centre padding.
 // Returns -1 for a zero (constant) pad sample.
@@ -305,7 +307,10 @@ __device__ __forceinline__ void stockham_pass(float2* scr, const float2* __restr
     const int j = lane + 64 * q;
     if ((B % 64 == 0) || j < B) {
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[q][r] = scr[pidx(j + r * B)];
+      for (int r = 0; r < R; ++r) {
+        if constexpr (B % 16 == 0) v[q][r] = scr[pidx(j) + r * (B + B / 16)];
+        else v[q][r] = scr[pidx(j + r * B)];
+      }
     }
   }
   wave_lds_fence();
@@ -321,7 +326,10 @@ __device__ __forceinline__ void stockham_pass(float2* scr, const float2* __restr
       Dft<R>::run(v[q]);
       const int base = (j / NS) * NS * R + k;
 #pragma unroll
-      for (int r = 0; r < R; ++r) scr[pidx(base + r * NS)] = v[q][r];
+      for (int r = 0; r < R; ++r) {
+        if constexpr (NS % 16 == 0) scr[pidx(base) + r * (NS + NS / 16)] = v[q][r];
+        else scr[pidx(base + r * NS)] = v[q][r];
+      }
     }
   }
   wave_lds_fence();
@@ -355,7 +363,10 @@ __device__ __forceinline__ void stockham_last_regs(const float2* scr, const floa
   for (int q = 0; q < QB; ++q) {
     const int j = lane + 64 * q;
 #pragma unroll
-    for (int r = 0; r < R; ++r) z[q][r] = scr[pidx(j + r * B)];
+    for (int r = 0; r < R; ++r) {
+      if constexpr (B % 16 == 0) z[q][r] = scr[pidx(j) + r * (B + B / 16)];
+      else z[q][r] = scr[pidx(j + r * B)];
+    }
   }
   wave_lds_fence();
 #pragma unroll
@@ -449,7 +460,10 @@ __device__ __forceinline__ void frame_load(const TileArgs& a, const TIn* __restr
       const int j = lane + 64 * q;
       if ((B0 % 64 == 0) || j < B0) {
 #pragma unroll
-        for (int r = 0; r < R0; ++r) v[q][r] = scr[pidx(j + r * B0)];
+        for (int r = 0; r < R0; ++r) {
+          if constexpr (B0 % 16 == 0) v[q][r] = scr[pidx(j) + r * (B0 + B0 / 16)];
+          else v[q][r] = scr[pidx(j + r * B0)];
+        }
       }
     }
     wave_lds_fence();
@@ -473,7 +487,10 @@ __device__ __forceinline__ void frame_fft(const TileArgs& a, int clip, int fr, f
     if ((B0 % 64 == 0) || j < B0) {
       Dft<R0>::run(v[q]);
 #pragma unroll
-      for (int r = 0; r < R0; ++r) scr[pidx(j * R0 + r)] = v[q][r];
+      for (int r = 0; r < R0; ++r) {   // R0 = 16: pidx(16 j + r) = 17 j + r
+        if constexpr (R0 == 16) scr[17 * j + r] = v[q][r];
+        else scr[pidx(j * R0 + r)] = v[q][r];
+      }
     }
   }
   wave_lds_fence();
@@ -512,7 +529,10 @@ __device__ __forceinline__ void frame_fft_regs(const TileArgs& a, int clip, int 
       if ((B0 % 64 == 0) || j < B0) {
         Dft<R0>::run(v[q]);
 #pragma unroll
-        for (int r = 0; r < R0; ++r) scr[pidx(j * R0 + r)] = v[q][r];
+        for (int r = 0; r < R0; ++r) {   // R0 = 16: pidx(16 j + r) = 17 j + r
+        if constexpr (R0 == 16) scr[17 * j + r] = v[q][r];
+        else scr[pidx(j * R0 + r)] = v[q][r];
+      }
       }
     }
   }
