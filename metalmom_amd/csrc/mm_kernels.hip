@@ -351,35 +351,6 @@ struct PassChain {
   }
 };
 
-// Last Stockham pass kept in registers: z[q][r] = Z[j + BL*r], j = lane + 64 q
-// (its natural-order outputs are exactly the butterfly outputs).
-template <int M, int R, int NS>
-__device__ __forceinline__ void stockham_last_regs(const float2* scr, const float2* __restrict__ tw, int lane_in,
-                                                   float2 (&z)[M / R / 64][R]) {
-  const int lane = opaque(lane_in);
-  constexpr int B = M / R;
-  constexpr int QB = B / 64;
-#pragma unroll
-  for (int q = 0; q < QB; ++q) {
-    const int j = lane + 64 * q;
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      if constexpr (B % 16 == 0) z[q][r] = scr[pidx(j) + r * (B + B / 16)];
-      else z[q][r] = scr[pidx(j + r * B)];
-    }
-  }
-  wave_lds_fence();
-#pragma unroll
-  for (int q = 0; q < QB; ++q) {
-    const unsigned k = (unsigned)(lane + 64 * q) % NS;
-    constexpr int P = __builtin_ctz(NS) / 4;
-#pragma unroll
-    for (int r = 1; r < R; ++r)
-      z[q][r] = cmul(z[q][r], ldg<float2>(tw + PassTw<M, P>::OFF + (r - 1) * NS, k * 8u));
-    Dft<R>::run(z[q]);
-  }
-}
-
 // Loads frame `fr` of a clip (pass-0 inputs, windowed) into registers, runs
 // pass 0 from registers and the remaining passes in LDS. On return scr holds
 // Z[k] = DFT_M(z)[k] in natural order (padded index), z[n] = x[2n] + i x[2n+1].
@@ -497,50 +468,209 @@ __device__ __forceinline__ void frame_fft(const TileArgs& a, int clip, int fr, f
   if constexpr (S::NP16 > 0) PassChain<M, 1, S::NP16, S::TAIL, S::NPASS>::run(scr, a.twp, lane);
 }
 
-// Same FFT, but the last pass stays in registers: z[q][r] = Z[lane + 64 q + BL r].
-template <int LOG2M, typename TIn>
-__device__ __forceinline__ void frame_fft_regs(const TileArgs& a, int clip, int fr, float2* scr, int lane,
-                                               float2 (&z)[FftShape<LOG2M>::QL][FftShape<LOG2M>::RL],
-                                               const float2 (&pf)[FftShape<LOG2M>::Q0][FftShape<LOG2M>::R0],
-                                               bool use_pf) {
+// ---- register-resident Stockham FFT with half-size LDS exchanges ----
+//
+// Every pass computes on registers; between passes a wave moves its values
+// from the writer's layout to the reader's layout through a float buffer of
+// M + M/16 entries, real parts first and imaginary parts second (half the LDS
+// of a float2 exchange; a wave's LDS operations run in order, so the second
+// half may overwrite the first as soon as it has been read).
+//
+// Layouts of pass p (radix R, NS = product of earlier radices, B = M / R
+// butterflies, lane j = lane + 64 q owns butterfly j when j < B):
+//   read:  element j + r B        (r < R)
+//   write: element (j / NS) NS R + j % NS + r NS
+template <int M, int R, int NS> struct SplitPass {
+  static constexpr int B = M / R;
+  static constexpr int QB = (B + 63) / 64;
+  static constexpr bool ALL = (B % 64 == 0);   // every lane owns butterflies
+  __device__ __forceinline__ static bool ok(int j) { return ALL || j < B; }
+  // padded float index of read element (j, r)
+  __device__ __forceinline__ static int rd(int j, int r) {
+    if constexpr (B % 16 == 0) return pidx(j) + r * (B + B / 16);
+    else return pidx(j + r * B);
+  }
+  // padded float index of written element (j, r)
+  __device__ __forceinline__ static int wr(int j, int r) {
+    if constexpr (NS == 1 && R == 16) return 17 * j + r;   // pidx(16 j + r)
+    else if constexpr (NS % 16 == 0) return pidx((j / NS) * NS * R + j % NS) + r * (NS + NS / 16);
+    else return pidx((j / NS) * NS * R + j % NS + r * NS);
+  }
+};
+
+// Values written in pass W's layout, read back in pass Rd's layout.
+template <class W, class Rd, int RW, int RR>
+__device__ __forceinline__ void exchange_split(float* sf, int lane, const float2 (&in)[W::QB][RW],
+                                               float2 (&out)[Rd::QB][RR]) {
+#pragma unroll
+  for (int half = 0; half < 2; ++half) {
+#pragma unroll
+    for (int q = 0; q < W::QB; ++q) {
+      const int j = lane + 64 * q;
+      if (W::ok(j)) {
+#pragma unroll
+        for (int r = 0; r < RW; ++r) sf[W::wr(j, r)] = half ? in[q][r].y : in[q][r].x;
+      }
+    }
+    wave_lds_fence();
+#pragma unroll
+    for (int q = 0; q < Rd::QB; ++q) {
+      const int j = lane + 64 * q;
+      if (Rd::ok(j)) {
+#pragma unroll
+        for (int r = 0; r < RR; ++r) {
+          if (half) out[q][r].y = sf[Rd::rd(j, r)];
+          else out[q][r].x = sf[Rd::rd(j, r)];
+        }
+      }
+    }
+    wave_lds_fence();
+  }
+}
+
+// Twiddles and DFT of pass P on registers (P >= 1).
+template <int M, int P, int R, int NS>
+__device__ __forceinline__ void split_pass_compute(const float2* __restrict__ tw, int lane,
+                                                   float2 (&v)[SplitPass<M, R, NS>::QB][R]) {
+  using L = SplitPass<M, R, NS>;
+#pragma unroll
+  for (int q = 0; q < L::QB; ++q) {
+    const int j = lane + 64 * q;
+    if (L::ok(j)) {
+      const unsigned k = (unsigned)j % NS;
+#pragma unroll
+      for (int r = 1; r < R; ++r)
+        v[q][r] = cmul(v[q][r], ldg<float2>(tw + PassTw<M, P>::OFF + (r - 1) * NS, k * 8u));
+      Dft<R>::run(v[q]);
+    }
+  }
+}
+
+// Passes P .. NPASS-1 given pass P-1's outputs `in` (in pass P-1's layout
+// WL with radix RW); the last pass's outputs land in z.
+template <int LOG2M, int P, class WL, int RW>
+struct SplitChain {
   using S = FftShape<LOG2M>;
+  static constexpr int M = S::M;
+  static constexpr bool LAST = (P == S::NPASS - 1);
+  static constexpr int R = (P < S::NP16) ? 16 : S::TAIL;
+  static constexpr int NS = 1 << (4 * P);
+  using L = SplitPass<M, R, NS>;
+  __device__ __forceinline__ static void run(float* sf, const float2* tw, int lane, const float2 (&in)[WL::QB][RW],
+                                             float2 (&z)[S::QL][S::RL]) {
+    if constexpr (LAST) {
+      static_assert(R == S::RL && L::QB == S::QL, "last pass shape");
+      exchange_split<WL, L, RW, R>(sf, lane, in, z);
+      split_pass_compute<M, P, R, NS>(tw, lane, z);
+    } else {
+      float2 v[L::QB][R];
+      exchange_split<WL, L, RW, R>(sf, lane, in, v);
+      split_pass_compute<M, P, R, NS>(tw, lane, v);
+      SplitChain<LOG2M, P + 1, L, R>::run(sf, tw, lane, v, z);
+    }
+  }
+};
+
+// Pass-0 inputs of a frame that touches the centre padding: staged through
+// the float buffer in two halves (x[2n] terms, then x[2n+1] terms).
+template <int LOG2M, typename TIn>
+__device__ __forceinline__ void frame_load_padded_split(const TileArgs& a, const TIn* __restrict__ x, int64_t len,
+                                                        int64_t s0, float* sf, int lane,
+                                                        float2 (&v)[FftShape<LOG2M>::Q0][FftShape<LOG2M>::R0]) {
+  using S = FftShape<LOG2M>;
+  using L0 = SplitPass<S::M, S::R0, 1>;
+  constexpr int M = S::M, R0 = S::R0;
+#pragma unroll
+  for (int half = 0; half < 2; ++half) {
+#pragma unroll 1
+    for (int n = lane; n < M; n += 64) {
+      const int64_t i = pad_index(s0 + 2 * n + half, len, a.pad_mode);
+      const float xv = (i >= 0) ? (float)x[i] : 0.0f;
+      sf[pidx(n)] = xv * a.window[2 * n + half];
+    }
+    wave_lds_fence();
+#pragma unroll
+    for (int q = 0; q < S::Q0; ++q) {
+      const int j = lane + 64 * q;
+      if (L0::ok(j)) {
+#pragma unroll
+        for (int r = 0; r < R0; ++r) {
+          if (half) v[q][r].y = sf[L0::rd(j, r)];
+          else v[q][r].x = sf[L0::rd(j, r)];
+        }
+      }
+    }
+    wave_lds_fence();
+  }
+}
+
+// FFT of frame `fr` with the last pass in registers: z[q][r] = Z[lane + 64 q
+// + BL r]; LDS use: the wave's float buffer of M + M/16 entries at sf.
+template <int LOG2M, typename TIn>
+__device__ __forceinline__ void frame_fft_split(const TileArgs& a, int clip, int fr, float* sf, int lane,
+                                                float2 (&z)[FftShape<LOG2M>::QL][FftShape<LOG2M>::RL],
+                                                const float2 (&pf)[FftShape<LOG2M>::Q0][FftShape<LOG2M>::R0],
+                                                bool use_pf) {
+  using S = FftShape<LOG2M>;
+  static_assert(S::REG_SPLIT && S::NP16 > 0, "register-split FFT shapes only");
   constexpr int M = S::M, R0 = S::R0, B0 = S::B0, Q0 = S::Q0;
+  using L0 = SplitPass<M, R0, 1>;
   const int64_t len = ldc(a.clip_len, clip);
   const int64_t s0 = (int64_t)fr * a.hop - (a.center ? M : 0);
-  {
-    float2 v[Q0][R0];
-    lane = opaque(lane);
-    if (use_pf) {   // samples prefetched during the previous tile: apply the window
+  lane = opaque(lane);
+  float2 v[Q0][R0];
+  const TIn* x = static_cast<const TIn*>(a.signal) + ldc(a.clip_off, clip);
+  const bool interior = (s0 >= 0) && (s0 + 2 * M <= len);
+  if (use_pf) {   // samples prefetched during the previous tile: apply the window
 #pragma unroll
-      for (int q = 0; q < Q0; ++q) {
-        const int j = lane + 64 * q;
+    for (int q = 0; q < Q0; ++q) {
+      const int j = lane + 64 * q;
+      if (L0::ok(j)) {
 #pragma unroll
         for (int r = 0; r < R0; ++r) {
           const float2 w = ldg<float2>(a.window + 2 * r * B0, (unsigned)j * 8u);
           v[q][r] = make_float2(pf[q][r].x * w.x, pf[q][r].y * w.y);
         }
       }
-    } else {
-      frame_load<LOG2M>(a, static_cast<const TIn*>(a.signal) + ldc(a.clip_off, clip), len, s0, scr, lane, v);
     }
+  } else if (interior && (s0 & 1) == 0) {   // one pair load per complex value
+    const TIn* xf = wave_uniform(x + s0);
 #pragma unroll
     for (int q = 0; q < Q0; ++q) {
       const int j = lane + 64 * q;
-      if ((B0 % 64 == 0) || j < B0) {
-        Dft<R0>::run(v[q]);
+      if (L0::ok(j)) {
 #pragma unroll
-        for (int r = 0; r < R0; ++r) {   // R0 = 16: pidx(16 j + r) = 17 j + r
-        if constexpr (R0 == 16) scr[17 * j + r] = v[q][r];
-        else scr[pidx(j * R0 + r)] = v[q][r];
-      }
+        for (int r = 0; r < R0; ++r) {
+          const float2 w = ldg<float2>(a.window + 2 * r * B0, (unsigned)j * 8u);
+          const float2 xs = SamplePair<TIn>::load(xf + 2 * r * B0, (unsigned)j * (2u * sizeof(TIn)));
+          v[q][r] = make_float2(xs.x * w.x, xs.y * w.y);
+        }
       }
     }
+  } else if (interior) {   // odd start: element loads
+    const TIn* xf = wave_uniform(x + s0);
+#pragma unroll
+    for (int q = 0; q < Q0; ++q) {
+      const int j = lane + 64 * q;
+      if (L0::ok(j)) {
+#pragma unroll
+        for (int r = 0; r < R0; ++r) {
+          const float2 w = ldg<float2>(a.window + 2 * r * B0, (unsigned)j * 8u);
+          const unsigned o = (unsigned)j * (2u * sizeof(TIn));
+          v[q][r] = make_float2((float)ldg<TIn>(xf + 2 * r * B0, o) * w.x,
+                                (float)ldg<TIn>(xf + 2 * r * B0 + 1, o) * w.y);
+        }
+      }
+    }
+  } else {
+    frame_load_padded_split<LOG2M, TIn>(a, x, len, s0, sf, lane, v);
   }
-  wave_lds_fence();
-  PassChain<M, 1, S::NP16, S::TAIL, S::NPASS - 1>::run(scr, a.twp, lane);
-  constexpr int NSL = M / S::RL;
-  stockham_last_regs<M, S::RL, NSL>(scr, a.twp, lane, z);
-  wave_lds_fence();
+#pragma unroll
+  for (int q = 0; q < Q0; ++q) {
+    const int j = lane + 64 * q;
+    if (L0::ok(j)) Dft<R0>::run(v[q]);
+  }
+  SplitChain<LOG2M, 1, L0, R0>::run(sf, a.twp, lane, v, z);
 }
 
 // Real-FFT split in registers. Lane l owns Z[k], k = l + 64 q + BL r; its
@@ -687,12 +817,21 @@ __global__ __launch_bounds__(NW * 64) MM_TILE_OCC void k_tile(TileArgs a) {
       pf_tile = ok ? tile_now + 1 : -1;
       if (ok) {
         const TIn* xf = wave_uniform(static_cast<const TIn*>(a.signal) + r.base + (int64_t)wave * a.hop);
-        const unsigned lo = (unsigned)opaque(lane0) * (2u * sizeof(TIn));
+        const int l = opaque(lane0);
+        const unsigned lo = (unsigned)l * (2u * sizeof(TIn));
 #pragma unroll
-        for (int q = 0; q < SHP::Q0; ++q)
+        for (int q = 0; q < SHP::Q0; ++q) {
+          // lanes past the pass-0 butterflies (B0 < 64) hold no sample pairs:
+          // their indices would run past the frame
+          if ((SHP::B0 % 64 == 0) || l + 64 * q < SHP::B0) {
 #pragma unroll
-          for (int r2 = 0; r2 < SHP::R0; ++r2)   // sample pair n = lane + 64 q + r2 B0
-            pf[q][r2] = SamplePair<TIn>::load(xf + 2 * (64 * q + r2 * SHP::B0), lo);
+            for (int r2 = 0; r2 < SHP::R0; ++r2)   // sample pair n = lane + 64 q + r2 B0
+              pf[q][r2] = SamplePair<TIn>::load(xf + 2 * (64 * q + r2 * SHP::B0), lo);
+          } else {
+#pragma unroll
+            for (int r2 = 0; r2 < SHP::R0; ++r2) pf[q][r2] = make_float2(0.0f, 0.0f);
+          }
+        }
       } else {
 #pragma unroll
         for (int q = 0; q < SHP::Q0; ++q)
@@ -743,7 +882,7 @@ __global__ __launch_bounds__(NW * 64) MM_TILE_OCC void k_tile(TileArgs a) {
         float vmax = 0.0f;
         if constexpr (RS) {
           float2 z[QL][RL];
-          frame_fft_regs<LOG2M, TIn>(a, clip, fr, scr, lane, z, pf, PF && pf_tile == tile);
+          frame_fft_split<LOG2M, TIn>(a, clip, fr, reinterpret_cast<float*>(scr), lane, z, pf, PF && pf_tile == tile);
           float2 x[QL][RL];
           float nyq;
           real_split_regs<LOG2M>(z, a.twN, lane, x, nyq);
@@ -1549,7 +1688,9 @@ bool tile_layout(int n_fft, int out_kind, int prec, int n_mels, int n_mfcc, Tile
     if (out_kind == OUT_MFCC && (n_mfcc < 1 || n_mfcc > 128)) return false;
   }
   const int nw = (M > 1024) ? 8 : (g_tile_waves ? g_tile_waves : 16), fpw = 1;
-  const int scr = (M + M / 16) * 8;
+  // FFT scratch: float2 exchanges for the small sizes, half-size float
+  // exchanges for the register-split sizes (n_fft >= 1024)
+  const int scr = (M + M / 16) * ((M >= 512) ? 4 : 8);
   int esz, S;
   if (spec) {
     esz = (out_kind == OUT_STFT_COMPLEX) ? 8 : 4;
