@@ -45,7 +45,8 @@ struct DeviceTables {
   uint16_t* fbh = nullptr;      // f16 copy
   int2* band32 = nullptr;       // [nmh] (k_start, n_chunks of 16 bins) per block of 8 mels
   int2* band16 = nullptr;       // [nmh] (k_start, n_chunks of 32 bins)
-  int* assign = nullptr;        // [2 chunk widths][2 wave counts (8, 16)][64]: blocks per wave
+  int* assign = nullptr;        // [2 chunk widths][2 wave counts (8, 16)][32]: blocks per wave
+  int* cache_src = nullptr;     // [2 chunk widths][kMaxCacheChunks]: filterbank cache sources
   float* dct32 = nullptr;       // [ndt][nmh][4][64] permuted DCT-II rows (A operand)
   uint16_t* dctb_hi = nullptr;  // [ndt][nmh][64][4] bf16 hi part
   uint16_t* dctb_lo = nullptr;  // bf16 lo part
@@ -100,21 +101,46 @@ struct TileArgs {
   int lds_row_stride;          // staging row stride in elements
   int lds_region_bytes;        // per-wave LDS region
   int lds_part_off;            // MFCC: LDS offset of the DCT partials (0: over the regions)
+  int fb_lds_off;              // LDS copy of the banded filterbank (0: read it from global memory)
+  int dct_lds_off;             // LDS copy of the DCT operand tables (0: global memory)
+  const int* cache_src;        // [cache_chunks] sources of the filterbank cache (cache_sources)
+  int cache_chunks;
   long long* phase_cycles;     // instrumented builds only: [grid][waves][6] cycle sums
 };
+
+// A wave's mel block slot, packed in 32 bits: bit 31 set for a used slot,
+// bits 0-4 the block, 5-14 its first bin / 4, 15-22 its chunks, 23-30 the
+// index of its first chunk in the LDS filterbank cache (blocks in order,
+// chunks back to back; meaningful when the total is at most kMaxCacheChunks).
+constexpr int kMaxCacheChunks = 255;
+__host__ __device__ inline int slot_pack(int block, int first_bin, int chunks, int cache_first) {
+  return (int)(0x80000000u | (unsigned)block | ((unsigned)(first_bin >> 2) << 5) | ((unsigned)chunks << 15) |
+               ((unsigned)(cache_first & 255) << 23));
+}
+__host__ __device__ inline bool slot_used(int v) { return v < 0; }
+__host__ __device__ inline int slot_block(int v) { return v & 31; }
+__host__ __device__ inline int slot_first_bin(int v) { return ((v >> 5) & 1023) << 2; }
+__host__ __device__ inline int slot_chunks(int v) { return (v >> 15) & 255; }
+__host__ __device__ inline int slot_cache_first(int v) { return (int)(((unsigned)v >> 23) & 255u); }
 
 // Mel blocks per wavefront of the fused contraction, longest processing time
 // first: blocks in order of decreasing cost (chunks + 1; highest index first on
 // ties) each go to the least loaded wave that still has a free slot (lowest
 // index on ties). The first min(nw, nmh) blocks land on waves 0 .. nw-1, so
 // the waves holding blocks are always 0 .. min(nw, nmh)-1. Writes
-// out[w * (kMaxMelBlocks / nw) + i] = block | first bin << 6 | chunks << 19,
-// -1 for an empty slot (all slots when nmh is 0 or above kMaxMelBlocks).
-// band[b] = (first bin, chunks) of block b.
+// out[w * (kMaxMelBlocks / nw) + i] = slot_pack(...) of the block, 0 for an
+// empty slot (all slots when nmh is 0 or above kMaxMelBlocks). band[b] =
+// (first bin, chunks) of block b; first bins are multiples of 4.
 __host__ __device__ inline void mel_block_assign(const int2* band, int nmh, int nw, int* out) {
   const int slots = kMaxMelBlocks / nw;
-  for (int i = 0; i < kMaxMelBlocks; ++i) out[i] = -1;
+  for (int i = 0; i < kMaxMelBlocks; ++i) out[i] = 0;
   if (nmh < 1 || nmh > kMaxMelBlocks || nw < 1 || nw > 16) return;
+  int cfirst[kMaxMelBlocks];   // first cache chunk of each block
+  int acc = 0;
+  for (int b = 0; b < nmh; ++b) {
+    cfirst[b] = acc;
+    acc += band[b].y;
+  }
   int load[16], cnt[16];
   for (int w = 0; w < 16; ++w) { load[w] = 0; cnt[w] = 0; }
   bool taken[kMaxMelBlocks];
@@ -127,10 +153,23 @@ __host__ __device__ inline void mel_block_assign(const int2* band, int nmh, int 
     int wb = -1;   // nw * slots >= nmh, so a wave with a free slot exists
     for (int w = 0; w < nw; ++w)
       if (cnt[w] < slots && (wb < 0 || load[w] < load[wb])) wb = w;
-    out[wb * slots + cnt[wb]] = best | (band[best].x << 6) | (band[best].y << 19);
+    out[wb * slots + cnt[wb]] = slot_pack(best, band[best].x, band[best].y, cfirst[best]);
     cnt[wb] += 1;
     load[wb] += band[best].y + 1;
   }
+}
+
+// Source of each LDS filterbank cache chunk: the element offset of its row 0
+// (mel 8 b of block b) at its first bin, chunks of `ch` bins, blocks in
+// order. Returns the number of chunks (0 when more than kMaxCacheChunks).
+__host__ __device__ inline int cache_sources(const int2* band, int nmh, int fbs, int ch, int* src) {
+  int n = 0;
+  for (int b = 0; b < nmh; ++b) n += band[b].y;
+  if (n > kMaxCacheChunks) return 0;
+  int i = 0;
+  for (int b = 0; b < nmh; ++b)
+    for (int c = 0; c < band[b].y; ++c) src[i++] = kMelBlock * b * fbs + band[b].x + ch * c;
+  return n;
 }
 
 // Ordered encoding of a float so that unsigned comparison matches float order.

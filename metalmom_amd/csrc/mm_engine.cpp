@@ -110,6 +110,7 @@ Lane::~Lane() {
 struct Tables {
   TableKey key;
   DeviceTables t;
+  int cache_chunks[2] = {0, 0};   // filterbank chunks of the LDS cache per chunk width (0: no cache)
   void* mem = nullptr;
   ~Tables() {
     if (mem) (void)hipFree(mem);
@@ -156,6 +157,7 @@ static int32_t make_tables(const TableKey& key, hipStream_t st, std::unique_ptr<
   const size_t o_b32 = take((size_t)t.nmh * 8);
   const size_t o_b16 = take((size_t)t.nmh * 8);
   const size_t o_asg = take((size_t)4 * kMaxMelBlocks * 4);
+  const size_t o_csrc = take((size_t)2 * kMaxCacheChunks * 4);
   const size_t nd = (size_t)t.ndt * t.nmh * 256;
   const size_t o_d32 = take(nd * 4);
   const size_t o_dbh = take(nd * 2);
@@ -176,6 +178,7 @@ static int32_t make_tables(const TableKey& key, hipStream_t st, std::unique_ptr<
   t.band32 = reinterpret_cast<int2*>(b + o_b32);
   t.band16 = reinterpret_cast<int2*>(b + o_b16);
   t.assign = reinterpret_cast<int*>(b + o_asg);
+  t.cache_src = reinterpret_cast<int*>(b + o_csrc);
   t.dct32 = reinterpret_cast<float*>(b + o_d32);
   t.dctb_hi = reinterpret_cast<uint16_t*>(b + o_dbh);
   t.dctb_lo = reinterpret_cast<uint16_t*>(b + o_dbl);
@@ -184,6 +187,17 @@ static int32_t make_tables(const TableKey& key, hipStream_t st, std::unique_ptr<
   t.dct_dense = reinterpret_cast<float*>(b + o_dd);
   MM_HIP(gen_tables(key, t, st));
   MM_HIP(hipStreamSynchronize(st));
+  if (t.nmh >= 1 && t.nmh <= kMaxMelBlocks) {
+    // chunk totals of the two chunk widths (they size the LDS filterbank cache)
+    std::vector<int2> bands(2 * (size_t)t.nmh);
+    MM_HIP(hipMemcpy(bands.data(), t.band32, (size_t)t.nmh * sizeof(int2), hipMemcpyDeviceToHost));
+    MM_HIP(hipMemcpy(bands.data() + t.nmh, t.band16, (size_t)t.nmh * sizeof(int2), hipMemcpyDeviceToHost));
+    for (int pc = 0; pc < 2; ++pc) {
+      int n = 0;
+      for (int b = 0; b < t.nmh; ++b) n += bands[(size_t)pc * t.nmh + b].y;
+      T->cache_chunks[pc] = (n <= kMaxCacheChunks) ? n : 0;
+    }
+  }
   *out = std::move(T);
   return OK;
 }
@@ -579,13 +593,30 @@ int32_t enqueue_batch(Engine* eng, const SpecRequest& r, const Tables* tb, const
   const int grid = grid_override > 0 ? std::min(grid_override, m.n_tiles) : eng->grid_for(r, m.n_tiles, L);
   TileArgs a = make_args(r, tb, m, mp, d_signal, d_out, L, grid);
   const int g = (m.n_tiles + a.tiles_per_wg - 1) / a.tiles_per_wg;
+  // LDS copies of the filterbank band and the DCT operands, when they fit
+  // beside the kernel's own layout
+  TileLayout LL = L;
+  if (r.out_kind == OUT_MEL || r.out_kind == OUT_MFCC) {
+    const int pc = (r.precision == PREC_F32) ? 0 : 1;
+    const int nch = tb->cache_chunks[pc];
+    const size_t off = (L.lds_bytes + 255) / 256 * 256;
+    const size_t fbb = (size_t)nch * 512;
+    const size_t dcb = (r.out_kind == OUT_MFCC) ? (size_t)tb->t.ndt * tb->t.nmh * 1024 : 0;
+    if (nch > 0 && off + fbb + dcb <= lds_limit_bytes()) {
+      a.fb_lds_off = (int)off;
+      a.dct_lds_off = dcb ? (int)(off + fbb) : 0;
+      a.cache_src = tb->t.cache_src + pc * kMaxCacheChunks;
+      a.cache_chunks = nch;
+      LL.lds_bytes = off + fbb + dcb;
+    }
+  }
   if (a.use_top_db) MM_HIP(launch_fill_u32(mp.clip_max_key, m.n_clips, float_to_key(-INFINITY), st));
   if (ev_p1_begin) MM_HIP(hipEventRecord(ev_p1_begin, st));
-  MM_HIP(launch_tile(a, r.precision, L, g, st));
+  MM_HIP(launch_tile(a, r.precision, LL, g, st));
   if (ev_p1_end) MM_HIP(hipEventRecord(ev_p1_end, st));
   if (a.use_top_db) {
     a.phase = 2;
-    MM_HIP(launch_tile(a, r.precision, L, g, st));
+    MM_HIP(launch_tile(a, r.precision, LL, g, st));
   }
   return OK;
 }

@@ -737,16 +737,28 @@ __device__ __forceinline__ float pow_from_p2(float p2, float power) {
 // 10 log10(v) = 10 log10(2) * log2(v), hardware log2 (v >= amin > 0 here)
 __device__ __forceinline__ float db10(float v) { return 3.0102999566398120f * __builtin_amdgcn_logf(v); }
 
-__device__ __forceinline__ float wave_max(float v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
-  return v;
+// Maximum / minimum over the 64 lanes of a wave (all lanes active), returned
+// in every lane: DPP steps inside each row of 16 lanes (VALU only, no LDS
+// round trips), then the four row results through readlane.
+template <bool MAX, int CTRL>
+__device__ __forceinline__ float dpp_step(float v) {
+  const float o = __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), CTRL, 0xF, 0xF, false));
+  return MAX ? fmaxf(v, o) : fminf(v, o);
 }
-__device__ __forceinline__ float wave_min(float v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
-  return v;
+template <bool MAX>
+__device__ __forceinline__ float wave_reduce(float v) {
+  v = dpp_step<MAX, 0xB1>(v);    // quad_perm [1, 0, 3, 2]
+  v = dpp_step<MAX, 0x4E>(v);    // quad_perm [2, 3, 0, 1]
+  v = dpp_step<MAX, 0x141>(v);   // row_half_mirror
+  v = dpp_step<MAX, 0x140>(v);   // row_mirror: every lane holds its row's result
+  const float r0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0));
+  const float r1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 16));
+  const float r2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 32));
+  const float r3 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 48));
+  return MAX ? fmaxf(fmaxf(r0, r1), fmaxf(r2, r3)) : fminf(fminf(r0, r1), fminf(r2, r3));
 }
+__device__ __forceinline__ float wave_max(float v) { return wave_reduce<true>(v); }
+__device__ __forceinline__ float wave_min(float v) { return wave_reduce<false>(v); }
 
 // ---------------------------------------------------------------------------
 // the fused tile kernel
@@ -842,6 +854,35 @@ __global__ __launch_bounds__(NW * 64) MM_TILE_OCC void k_tile(TileArgs a) {
   };
   if constexpr (PF) {
     if (t_begin < t_end && a.phase == 1) prefetch_next(t_begin - 1, ldc_pf(a.tile_pf, t_begin));
+  }
+  // LDS copies of the banded filterbank ([chunk][8 rows][64 B]) and of the
+  // DCT operand tables; the first tile's barrier publishes them
+  if constexpr (!SPEC) {
+    if (a.fb_lds_off && t_begin < t_end) {
+      using FE = std::conditional_t<PREC == PREC_F32, float, uint16_t>;
+      const FE* tab;
+      if constexpr (PREC == PREC_F32) tab = a.fb32;
+      else tab = a.fb16;
+      constexpr int EPP = 16 / (int)sizeof(FE);   // elements per 16-byte piece
+      for (int p = threadIdx.x; p < a.cache_chunks * 32; p += NT) {
+        const int c = p >> 5, r = (p >> 2) & 7, q = p & 3;
+        const FE* src = tab + ldc(a.cache_src, c) + r * a.fbs + q * EPP;
+        *reinterpret_cast<uint4*>(lds + a.fb_lds_off + c * 512 + r * 64 + q * 16) =
+            *reinterpret_cast<const uint4*>(src);
+      }
+    }
+    if (MODE == OUT_MFCC && a.dct_lds_off && t_begin < t_end) {
+      const int n16 = a.ndt * a.nmh * 64;   // 16-byte pieces per table (1 KB per (ct, block))
+      if constexpr (PREC == PREC_F32) {
+        for (int p = threadIdx.x; p < n16; p += NT)
+          *reinterpret_cast<uint4*>(lds + a.dct_lds_off + p * 16) = reinterpret_cast<const uint4*>(a.dct32)[p];
+      } else {
+        const int nh = n16 / 2;   // hi and lo tables are 512 B per (ct, block)
+        for (int p = threadIdx.x; p < n16; p += NT)
+          *reinterpret_cast<uint4*>(lds + a.dct_lds_off + p * 16) =
+              (p < nh) ? reinterpret_cast<const uint4*>(a.dct_hi)[p] : reinterpret_cast<const uint4*>(a.dct_lo)[p - nh];
+      }
+    }
   }
   const int wv = __builtin_amdgcn_readfirstlane(wave);
   for (int tile = t_begin; tile < t_end; ++tile) {
@@ -1070,12 +1111,14 @@ __global__ __launch_bounds__(NW * 64) MM_TILE_OCC void k_tile(TileArgs a) {
       continue;
     } else {
       // ---- banded mel contraction on MFMA ----
-      // Blocks of 8 mels: MFMA rows 8..15 carry zero weights, which halves
-      // the useful MFMA rate but lets the slowest wave of a tile hold ~40%
-      // fewer bin chunks than blocks of 16 would (MFMA time is not the bound).
+      // Blocks of 8 mels: MFMA rows 8..15 repeat rows 0..7 (each output row
+      // depends on its own A row only, and rows 8..15 are discarded), which
+      // halves the useful MFMA rate but lets the slowest wave of a tile hold
+      // ~40% fewer bin chunks than blocks of 16 would (MFMA time is not the
+      // bound). Filterbank and DCT operands come from their LDS copies when
+      // the launch has them (fb_lds_off / dct_lds_off), else from L2.
       const int ri = lane & 15;          // MFMA row/col index within 16
       const int g = lane >> 4;           // lane group (K slice)
-      const bool arow = ri < kMelBlock;  // lanes that feed filterbank rows
       // columns past an 8-frame tile read row 0 and are never stored
       const unsigned char* prow = row_ptr(ri < TF ? ri : 0);
       const int myexp = (ri < TF) ? sc_exp[ri] : 0;
@@ -1085,13 +1128,23 @@ __global__ __launch_bounds__(NW * 64) MM_TILE_OCC void k_tile(TileArgs a) {
       int nb = 0;   // slots in use (filled as a prefix)
 #pragma unroll
       for (int bi = 0; bi < MAXB; ++bi)
-        if (slot[bi] >= 0) nb = bi + 1;
+        if (slot_used(slot[bi])) nb = bi + 1;
       // DCT-II operands of (row tile ct, mel block h): A entries for this lane
       struct DctOp { float f[4]; uint2 hi, lo; };
       auto load_dct = [&](int ct, int h) -> DctOp {
         DctOp o;
-        const int64_t e0 = (int64_t)(ct * a.nmh + h) * 256;   // entries of (ct, h)
-        if constexpr (PREC == PREC_F32) {
+        const int e0 = (ct * a.nmh + h) * 256;   // entries of (ct, h)
+        if (a.dct_lds_off) {   // f32 [..][4][64]; 16-bit hi block then lo block, [..][64][4]
+          if constexpr (PREC == PREC_F32) {
+            const float* d = reinterpret_cast<const float*>(lds + a.dct_lds_off) + e0;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) o.f[j] = d[j * 64 + lane];
+          } else {
+            const uint16_t* d = reinterpret_cast<const uint16_t*>(lds + a.dct_lds_off);
+            o.hi = *reinterpret_cast<const uint2*>(d + e0 + lane * 4);
+            o.lo = *reinterpret_cast<const uint2*>(d + a.ndt * a.nmh * 256 + e0 + lane * 4);
+          }
+        } else if constexpr (PREC == PREC_F32) {
 #pragma unroll
           for (int j = 0; j < 4; ++j) o.f[j] = ldg<float>(a.dct32 + e0 + j * 64, (unsigned)lane * 4u);
         } else {
@@ -1100,92 +1153,63 @@ __global__ __launch_bounds__(NW * 64) MM_TILE_OCC void k_tile(TileArgs a) {
         }
         return o;
       };
-      // the first row tile's DCT operands go out ahead of the filterbank loads
-      constexpr int PRE = 0;
-      DctOp dpre[PRE > 0 ? PRE : 1];
-      if constexpr (MODE == OUT_MFCC) {
-#pragma unroll
-        for (int bi = 0; bi < PRE; ++bi)
-          if (bi < nb) dpre[bi] = load_dct(0, slot[bi] & 63);
-      }
-      // mel^T block h over its bin band, in rounds of 8, 4, 2 or 1 chunks
-      // (chosen by the chunks left): a round issues all its filterbank and
-      // spectrum loads before its MFMAs, so their latencies overlap, and loads
-      // at most 3 chunks twice (clamped to the last chunk, MFMA skipped)
-      auto mel_block = [&](int h, int2 bd) -> f32x4 {
-        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-        const int n = bd.y;
+      using AV = std::conditional_t<PREC == PREC_F32, float4, uint4>;   // one chunk, one lane
+      using FE = std::conditional_t<PREC == PREC_F32, float, uint16_t>;
+      constexpr int CH = (PREC == PREC_F32) ? 16 : 32;                  // bins per chunk
+      auto mfma_chunk = [&](const AV& av, const AV& bv, f32x4 acc) -> f32x4 {
         if constexpr (PREC == PREC_F32) {
-          // chunk base (uniform) + this lane's byte offset within the block
-          const float* fbb = a.fb32 + (int64_t)kMelBlock * h * a.fbs + bd.x;
-          const unsigned fo = (unsigned)(((ri & (kMelBlock - 1)) * a.fbs + 4 * g) * 4);
-          const float* pr = reinterpret_cast<const float*>(prow) + bd.x + 4 * g;
-          auto round = [&](auto kc, int c) {
-            constexpr int K = decltype(kc)::value;
-            float4 av[K], bv[K];
-#pragma unroll
-            for (int u = 0; u < K; ++u) av[u] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            if (arow) {
-#pragma unroll
-              for (int u = 0; u < K; ++u)
-                av[u] = ldg<float4>(fbb + 16 * min(c + u, n - 1), fo);
-            }
-#pragma unroll
-            for (int u = 0; u < K; ++u) bv[u] = *reinterpret_cast<const float4*>(pr + 16 * min(c + u, n - 1));
-#pragma unroll
-            for (int u = 0; u < K; ++u) {
-              if (u == 0 || c + u < n) {
-                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[u].x, bv[u].x, acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[u].y, bv[u].y, acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[u].z, bv[u].z, acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[u].w, bv[u].w, acc, 0, 0, 0);
-              }
-            }
-            wait_vmem_all();   // duplicate loads whose MFMA was skipped end here
-          };
-          for (int c = 0; c < n;) {
-            const int rem = n - c;
-            if (rem > 4) { round(std::integral_constant<int, 8>{}, c); c += 8; }
-            else if (rem > 2) { round(std::integral_constant<int, 4>{}, c); c += 4; }
-            else if (rem == 2) { round(std::integral_constant<int, 2>{}, c); c += 2; }
-            else { round(std::integral_constant<int, 1>{}, c); c += 1; }
-          }
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av.x, bv.x, acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av.y, bv.y, acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av.z, bv.z, acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av.w, bv.w, acc, 0, 0, 0);
+        } else if constexpr (PREC == PREC_BF16) {
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, av), __builtin_bit_cast(bf16x8, bv),
+                                                        acc, 0, 0, 0);
         } else {
-          const uint16_t* fbb = a.fb16 + (int64_t)kMelBlock * h * a.fbs + bd.x;
-          const unsigned fo = (unsigned)(((ri & (kMelBlock - 1)) * a.fbs + 8 * g) * 2);
-          const uint16_t* pr = reinterpret_cast<const uint16_t*>(prow) + bd.x + 8 * g;
-          auto round = [&](auto kc, int c) {
-            constexpr int K = decltype(kc)::value;
-            uint4 av[K], bv[K];
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8, av), __builtin_bit_cast(f16x8, bv),
+                                                       acc, 0, 0, 0);
+        }
+        return acc;
+      };
+      // mel^T of the block in slot v over its bin band, in rounds of 8, 4, 2
+      // or 1 chunks (chosen by the chunks left): a round issues all its
+      // operand loads before its MFMAs, so their latencies overlap, and loads
+      // at most 3 chunks twice (clamped to the last chunk, MFMA skipped)
+      auto mel_block = [&](int v) -> f32x4 {
+        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        const int h = slot_block(v), x0 = slot_first_bin(v), n = slot_chunks(v);
+        const FE* pr = reinterpret_cast<const FE*>(prow) + x0 + (CH / 4) * g;
+        // this lane's filterbank operand: row ri & 7, bins (CH/4) g .. of a chunk
+        const bool cached = a.fb_lds_off != 0;
+        const unsigned char* fbl = lds + a.fb_lds_off + slot_cache_first(v) * 512 + (ri & 7) * 64 + g * 16;
+        const FE* fbtab;
+        if constexpr (PREC == PREC_F32) fbtab = a.fb32;
+        else fbtab = a.fb16;
+        const FE* fbb = fbtab + (int64_t)kMelBlock * h * a.fbs + x0;
+        const unsigned fo = (unsigned)(((ri & (kMelBlock - 1)) * a.fbs + (CH / 4) * g) * (int)sizeof(FE));
+        auto round = [&](auto kc, int c) {
+          constexpr int K = decltype(kc)::value;
+          AV av[K], bv[K];
+          if (cached) {   // LDS copy: [chunk][8 rows][64 B]
 #pragma unroll
-            for (int u = 0; u < K; ++u) av[u] = make_uint4(0u, 0u, 0u, 0u);
-            if (arow) {
+            for (int u = 0; u < K; ++u) av[u] = *reinterpret_cast<const AV*>(fbl + 512 * min(c + u, n - 1));
+          } else {
 #pragma unroll
-              for (int u = 0; u < K; ++u)
-                av[u] = ldg<uint4>(fbb + 32 * min(c + u, n - 1), fo);
-            }
-#pragma unroll
-            for (int u = 0; u < K; ++u) bv[u] = *reinterpret_cast<const uint4*>(pr + 32 * min(c + u, n - 1));
-#pragma unroll
-            for (int u = 0; u < K; ++u) {
-              if (u == 0 || c + u < n) {
-                if constexpr (PREC == PREC_BF16)
-                  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, av[u]),
-                                                                __builtin_bit_cast(bf16x8, bv[u]), acc, 0, 0, 0);
-                else
-                  acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8, av[u]),
-                                                               __builtin_bit_cast(f16x8, bv[u]), acc, 0, 0, 0);
-              }
-            }
-            wait_vmem_all();   // duplicate loads whose MFMA was skipped end here
-          };
-          for (int c = 0; c < n;) {
-            const int rem = n - c;
-            if (rem > 4) { round(std::integral_constant<int, 8>{}, c); c += 8; }
-            else if (rem > 2) { round(std::integral_constant<int, 4>{}, c); c += 4; }
-            else if (rem == 2) { round(std::integral_constant<int, 2>{}, c); c += 2; }
-            else { round(std::integral_constant<int, 1>{}, c); c += 1; }
+            for (int u = 0; u < K; ++u) av[u] = ldg<AV>(fbb + CH * min(c + u, n - 1), fo);
           }
+#pragma unroll
+          for (int u = 0; u < K; ++u) bv[u] = *reinterpret_cast<const AV*>(pr + CH * min(c + u, n - 1));
+#pragma unroll
+          for (int u = 0; u < K; ++u)
+            if (u == 0 || c + u < n) acc = mfma_chunk(av[u], bv[u], acc);
+          wait_vmem_all();   // duplicate loads whose MFMA was skipped end here
+        };
+        for (int c = 0; c < n;) {
+          const int rem = n - c;
+          if (rem > 4) { round(std::integral_constant<int, 8>{}, c); c += 8; }
+          else if (rem > 2) { round(std::integral_constant<int, 4>{}, c); c += 4; }
+          else if (rem == 2) { round(std::integral_constant<int, 2>{}, c); c += 2; }
+          else { round(std::integral_constant<int, 1>{}, c); c += 1; }
         }
         return acc;
       };
@@ -1226,7 +1250,7 @@ __global__ __launch_bounds__(NW * 64) MM_TILE_OCC void k_tile(TileArgs a) {
       for (int bi = 0; bi < MAXB; ++bi) {
         if (bi < nb) {
           const int v = slot[bi];
-          finish_block(v & 63, bi, mel_block(v & 63, make_int2((v >> 6) & 8191, (v >> 19) & 4095)));
+          finish_block(slot_block(v), bi, mel_block(v));
         }
       }
       // waves without mel blocks reach this point at once: their prefetch
@@ -1250,9 +1274,7 @@ __global__ __launch_bounds__(NW * 64) MM_TILE_OCC void k_tile(TileArgs a) {
 #pragma unroll
             for (int bi = 0; bi < MAXB; ++bi) {
               if (bi < nb) {
-                DctOp op;
-                if (bi < PRE && ct == 0) op = dpre[bi < PRE ? bi : 0];
-                else op = load_dct(ct, slot[bi] & 63);
+                const DctOp op = load_dct(ct, slot_block(slot[bi]));
                 if constexpr (PREC == PREC_F32) {
 #pragma unroll
                   for (int j = 0; j < 4; ++j)
@@ -1535,10 +1557,13 @@ __global__ void k_tab_bands(const float* fb32, int fbs, int F, int nmh, int2* ba
 // assign[pc][nwi][w * (32 / nw) + i]: block lists of mel_block_assign for
 // pc 0 = 16-bin chunks (FP32), 1 = 32-bin chunks (16-bit) and nwi 0 = 8
 // waves, 1 = 16 waves.
-__global__ void k_tab_assign(const int2* band32, const int2* band16, int nmh, int* assign) {
+// cache_src[pc][..]: sources of the LDS filterbank cache (cache_sources).
+__global__ void k_tab_assign(const int2* band32, const int2* band16, int nmh, int fbs, int* assign, int* cache_src) {
   if (blockIdx.x != 0 || threadIdx.x >= 4) return;
   const int pc = threadIdx.x >> 1, nwi = threadIdx.x & 1;
   mel_block_assign(pc ? band16 : band32, nmh, nwi ? 16 : 8, assign + (pc * 2 + nwi) * kMaxMelBlocks);
+  if (nwi == 0 && nmh <= kMaxMelBlocks)
+    (void)cache_sources(pc ? band16 : band32, nmh, fbs, pc ? 32 : 16, cache_src + pc * kMaxCacheChunks);
 }
 
 __device__ inline double dct_val(int c, int m, int n_mfcc, int n_mels) {
@@ -1608,7 +1633,8 @@ hipError_t gen_tables(const TableKey& key, const DeviceTables& t, hipStream_t st
                        t.fbs, rows, key.sr, key.n_fft, key.n_mels, (double)key.fmin, (double)key.fmax);
     hipLaunchKernelGGL(k_tab_bands, dim3(nblk(t.nmh, 64)), dim3(64), 0, st, t.fb32, t.fbs, F, t.nmh, t.band32,
                        t.band16);
-    hipLaunchKernelGGL(k_tab_assign, dim3(1), dim3(64), 0, st, t.band32, t.band16, t.nmh, t.assign);
+    hipLaunchKernelGGL(k_tab_assign, dim3(1), dim3(64), 0, st, t.band32, t.band16, t.nmh, t.fbs, t.assign,
+                       t.cache_src);
   }
   const int64_t nd = std::max<int64_t>((int64_t)t.ndt * t.nmh * 256, (int64_t)key.n_mfcc * key.n_mels);
   if (nd > 0)

@@ -5,6 +5,7 @@
 //  * the mel-block-to-wave assignment: every block exactly once, no wave over
 //    its slot count, waves holding blocks form a prefix, fields decode.
 // Built against libmetalmom.so by tests/test_host_logic.py; prints "ok".
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <random>
@@ -66,24 +67,27 @@ static void check_assign(const std::vector<int2>& band, int nw) {
   const int nmh = (int)band.size();
   mel_block_assign(band.data(), nmh, nw, out.data());
   const int slots = kMaxMelBlocks / nw;
-  std::vector<int> seen(nmh, 0);
+  std::vector<int> seen(nmh, 0), first(nmh, 0);
+  for (int b = 1; b < nmh; ++b) first[b] = first[b - 1] + band[b - 1].y;
   int last_busy = -1;
   for (int w = 0; w < nw; ++w) {
     bool prefix = true;
     int nb = 0;
     for (int i = 0; i < slots; ++i) {
       const int v = out[w * slots + i];
-      if (v < 0) {
+      if (!slot_used(v)) {
+        CHECK(v == 0, "empty slot %d of wave %d is not 0", i, w);
         prefix = false;
         continue;
       }
       CHECK(prefix, "slots of wave %d are not a prefix", w);
       ++nb;
-      const int b = v & 63, x = (v >> 6) & 8191, n = (v >> 19) & 4095;
+      const int b = slot_block(v);
       CHECK(b < nmh, "block id %d", b);
       if (b < nmh) {
         seen[b] += 1;
-        CHECK(x == band[b].x && n == band[b].y, "decode of block %d", b);
+        CHECK(slot_first_bin(v) == band[b].x && slot_chunks(v) == band[b].y, "decode of block %d", b);
+        CHECK(slot_cache_first(v) == (first[b] & 255), "cache offset of block %d", b);
       }
     }
     if (nb > 0) {
@@ -97,6 +101,30 @@ static void check_assign(const std::vector<int2>& band, int nw) {
   } else {
     CHECK(last_busy == -1, "oversized configuration must leave every slot empty");
   }
+}
+
+// LDS filterbank cache sources: chunk order, offsets, and every 8-row source
+// chunk inside the table [rows][fbs]
+static void check_cache(const std::vector<int2>& band, int fbs, int ch) {
+  const int nmh = (int)band.size();
+  std::vector<int> src(kMaxCacheChunks + 1, -7);
+  const int n = cache_sources(band.data(), nmh, fbs, ch, src.data());
+  int total = 0;
+  for (int b = 0; b < nmh; ++b) total += band[b].y;
+  if (total > kMaxCacheChunks) {
+    CHECK(n == 0, "oversized cache must be refused (%d chunks)", total);
+    return;
+  }
+  CHECK(n == total, "cache chunks %d vs %d", n, total);
+  const int rows = 16 * ((nmh * kMelBlock + 15) / 16);
+  int i = 0;
+  for (int b = 0; b < nmh; ++b)
+    for (int c = 0; c < band[b].y; ++c, ++i) {
+      CHECK(src[i] == kMelBlock * b * fbs + band[b].x + ch * c, "source of chunk %d", i);
+      CHECK(band[b].x + ch * (c + 1) <= fbs, "chunk %d of block %d runs past the row", c, b);
+      CHECK(kMelBlock * b + kMelBlock <= rows, "block %d rows", b);
+    }
+  CHECK(src[n] == -7, "write past the chunk list");
 }
 
 int main() {
@@ -134,6 +162,18 @@ int main() {
           band[b] = make_int2((int)(rng() % 2000) & ~7, n);
         }
         check_assign(band, nw);
+        if (nw == 16) {
+          // filterbank rows are fbs wide: a band of n chunks of ch bins fits
+          const int fbs = 2048 + 64;
+          for (int ch : {16, 32}) {
+            std::vector<int2> bb = band;
+            for (auto& x : bb) {   // bands as the table generator makes them
+              x.x = (x.x & ~7) % 2048;
+              x.y = std::min(x.y, (fbs - x.x) / ch);
+            }
+            check_cache(bb, fbs, ch);
+          }
+        }
       }
   if (fails) {
     std::fprintf(stderr, "%d failures\n", fails);
