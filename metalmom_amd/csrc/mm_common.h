@@ -101,6 +101,7 @@ struct TileArgs {
   int lds_row_stride;          // staging row stride in elements
   int lds_region_bytes;        // per-wave LDS region
   int lds_part_off;            // MFCC: LDS offset of the DCT partials (0: over the regions)
+  int tab_lds_off;             // LDS copies of window and twiddles (16-bit mel / MFCC kernels)
   int fb_lds_off;              // LDS copy of the banded filterbank (0: read it from global memory)
   int dct_lds_off;             // LDS copy of the DCT operand tables (0: global memory)
   const int* cache_src;        // [cache_chunks] sources of the filterbank cache (cache_sources)

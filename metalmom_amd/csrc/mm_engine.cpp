@@ -528,6 +528,7 @@ static TileArgs make_args(const SpecRequest& r, const Tables* tb, const BatchMet
   a.lds_row_stride = L.row_stride;
   a.lds_region_bytes = L.region_bytes;
   a.lds_part_off = L.part_off;
+  a.tab_lds_off = L.tab_off;
   a.phase_cycles = phase_cycles_buffer();
   return a;
 }

@@ -23,6 +23,7 @@ struct TileLayout {
   int region_bytes;  // LDS bytes per wavefront region
   size_t lds_bytes;  // dynamic LDS per workgroup
   int part_off;      // LDS offset of the DCT partial area (0: reuse the staging regions)
+  int tab_off;       // LDS offset of the FFT table copies (16-bit mel / MFCC, n_fft 1024, 2048)
 };
 
 void set_lds_limit_bytes(size_t n);

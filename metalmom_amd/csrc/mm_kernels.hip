@@ -575,7 +575,7 @@ struct SplitChain {
 // the float buffer in two halves (x[2n] terms, then x[2n+1] terms).
 template <int LOG2M, typename TIn>
 __device__ __forceinline__ void frame_load_padded_split(const TileArgs& a, const TIn* __restrict__ x, int64_t len,
-                                                        int64_t s0, float* sf, int lane,
+                                                        int64_t s0, float* sf, int lane, const float* win,
                                                         float2 (&v)[FftShape<LOG2M>::Q0][FftShape<LOG2M>::R0]) {
   using S = FftShape<LOG2M>;
   using L0 = SplitPass<S::M, S::R0, 1>;
@@ -586,7 +586,7 @@ __device__ __forceinline__ void frame_load_padded_split(const TileArgs& a, const
     for (int n = lane; n < M; n += 64) {
       const int64_t i = pad_index(s0 + 2 * n + half, len, a.pad_mode);
       const float xv = (i >= 0) ? (float)x[i] : 0.0f;
-      sf[pidx(n)] = xv * a.window[2 * n + half];
+      sf[pidx(n)] = xv * win[2 * n + half];
     }
     wave_lds_fence();
 #pragma unroll
@@ -605,9 +605,11 @@ __device__ __forceinline__ void frame_load_padded_split(const TileArgs& a, const
 }
 
 // FFT of frame `fr` with the last pass in registers: z[q][r] = Z[lane + 64 q
-// + BL r]; LDS use: the wave's float buffer of M + M/16 entries at sf.
+// + BL r]; LDS use: the wave's float buffer of M + M/16 entries at sf. win and
+// twp: the window and the per-pass twiddles (global memory or LDS copies).
 template <int LOG2M, typename TIn>
 __device__ __forceinline__ void frame_fft_split(const TileArgs& a, int clip, int fr, float* sf, int lane,
+                                                const float* win, const float2* twp,
                                                 float2 (&z)[FftShape<LOG2M>::QL][FftShape<LOG2M>::RL],
                                                 const float2 (&pf)[FftShape<LOG2M>::Q0][FftShape<LOG2M>::R0],
                                                 bool use_pf) {
@@ -628,7 +630,7 @@ __device__ __forceinline__ void frame_fft_split(const TileArgs& a, int clip, int
       if (L0::ok(j)) {
 #pragma unroll
         for (int r = 0; r < R0; ++r) {
-          const float2 w = ldg<float2>(a.window + 2 * r * B0, (unsigned)j * 8u);
+          const float2 w = ldg<float2>(win + 2 * r * B0, (unsigned)j * 8u);
           v[q][r] = make_float2(pf[q][r].x * w.x, pf[q][r].y * w.y);
         }
       }
@@ -641,7 +643,7 @@ __device__ __forceinline__ void frame_fft_split(const TileArgs& a, int clip, int
       if (L0::ok(j)) {
 #pragma unroll
         for (int r = 0; r < R0; ++r) {
-          const float2 w = ldg<float2>(a.window + 2 * r * B0, (unsigned)j * 8u);
+          const float2 w = ldg<float2>(win + 2 * r * B0, (unsigned)j * 8u);
           const float2 xs = SamplePair<TIn>::load(xf + 2 * r * B0, (unsigned)j * (2u * sizeof(TIn)));
           v[q][r] = make_float2(xs.x * w.x, xs.y * w.y);
         }
@@ -655,7 +657,7 @@ __device__ __forceinline__ void frame_fft_split(const TileArgs& a, int clip, int
       if (L0::ok(j)) {
 #pragma unroll
         for (int r = 0; r < R0; ++r) {
-          const float2 w = ldg<float2>(a.window + 2 * r * B0, (unsigned)j * 8u);
+          const float2 w = ldg<float2>(win + 2 * r * B0, (unsigned)j * 8u);
           const unsigned o = (unsigned)j * (2u * sizeof(TIn));
           v[q][r] = make_float2((float)ldg<TIn>(xf + 2 * r * B0, o) * w.x,
                                 (float)ldg<TIn>(xf + 2 * r * B0 + 1, o) * w.y);
@@ -663,14 +665,14 @@ __device__ __forceinline__ void frame_fft_split(const TileArgs& a, int clip, int
       }
     }
   } else {
-    frame_load_padded_split<LOG2M, TIn>(a, x, len, s0, sf, lane, v);
+    frame_load_padded_split<LOG2M, TIn>(a, x, len, s0, sf, lane, win, v);
   }
 #pragma unroll
   for (int q = 0; q < Q0; ++q) {
     const int j = lane + 64 * q;
     if (L0::ok(j)) Dft<R0>::run(v[q]);
   }
-  SplitChain<LOG2M, 1, L0, R0>::run(sf, a.twp, lane, v, z);
+  SplitChain<LOG2M, 1, L0, R0>::run(sf, twp, lane, v, z);
 }
 
 // Real-FFT split in registers. Lane l owns Z[k], k = l + 64 q + BL r; its
@@ -855,6 +857,30 @@ __global__ __launch_bounds__(NW * 64) MM_TILE_OCC void k_tile(TileArgs a) {
   if constexpr (PF) {
     if (t_begin < t_end && a.phase == 1) prefetch_next(t_begin - 1, ldc_pf(a.tile_pf, t_begin));
   }
+  // LDS copies of the FFT tables (16-bit mel / MFCC, n_fft 1024 and 2048):
+  // [window: n_fft floats][per-pass twiddles: M float2][W_N^k: M float2],
+  // published by a barrier before the first FFT
+  constexpr bool TABL = !SPEC && PREC != PREC_F32 && NW == 16 && SHP::REG_SPLIT && LOG2M <= 10;
+  const float* win = a.window;
+  const float2* twp = a.twp;
+  const float2* twn = a.twN;
+  if constexpr (TABL) {
+    if (t_begin < t_end) {
+      unsigned char* tb = lds + a.tab_lds_off;
+      const int nw4 = a.n_fft / 4, nt = M / 2;   // 16-byte pieces of the window / of each twiddle table
+      for (int p = threadIdx.x; p < nw4 + 2 * nt; p += NT) {
+        uint4 v;
+        if (p < nw4) v = reinterpret_cast<const uint4*>(a.window)[p];
+        else if (p < nw4 + nt) v = reinterpret_cast<const uint4*>(a.twp)[p - nw4];
+        else v = reinterpret_cast<const uint4*>(a.twN)[p - nw4 - nt];
+        reinterpret_cast<uint4*>(tb)[p] = v;
+      }
+      lds_barrier();
+      win = reinterpret_cast<const float*>(tb);
+      twp = reinterpret_cast<const float2*>(tb + a.n_fft * 4);
+      twn = twp + M;
+    }
+  }
   // LDS copies of the banded filterbank ([chunk][8 rows][64 B]) and of the
   // DCT operand tables; the first tile's barrier publishes them
   if constexpr (!SPEC) {
@@ -923,10 +949,11 @@ __global__ __launch_bounds__(NW * 64) MM_TILE_OCC void k_tile(TileArgs a) {
         float vmax = 0.0f;
         if constexpr (RS) {
           float2 z[QL][RL];
-          frame_fft_split<LOG2M, TIn>(a, clip, fr, reinterpret_cast<float*>(scr), lane, z, pf, PF && pf_tile == tile);
+          frame_fft_split<LOG2M, TIn>(a, clip, fr, reinterpret_cast<float*>(scr), lane, win, twp, z, pf,
+                                      PF && pf_tile == tile);
           float2 x[QL][RL];
           float nyq;
-          real_split_regs<LOG2M>(z, a.twN, lane, x, nyq);
+          real_split_regs<LOG2M>(z, twn, lane, x, nyq);
 #pragma unroll
           for (int b = 0; b < QL; ++b)
 #pragma unroll
@@ -1736,6 +1763,13 @@ bool tile_layout(int n_fft, int out_kind, int prec, int n_mels, int n_mfcc, Tile
   const size_t extras = 16 * 4 + 2 * nw * 4;
   size_t total = (size_t)nw * rb + extras;
   if (total > lds_limit_bytes()) return false;
+  int tab_off = 0;
+  if (!spec && prec != PREC_F32 && nw == 16 && (M == 512 || M == 1024)) {
+    // FFT tables in LDS (the kernel's TABL case): window, per-pass twiddles, W_N^k
+    tab_off = (int)((total + 15) / 16 * 16);
+    total = (size_t)tab_off + (size_t)n_fft * 4 + (size_t)M * 8 * 2;
+    if (total > lds_limit_bytes()) return false;
+  }
   int part_off = 0;
   if (out_kind == OUT_MFCC) {
     const int ndt = (n_mfcc + 15) / 16;
@@ -1755,6 +1789,7 @@ bool tile_layout(int n_fft, int out_kind, int prec, int n_mels, int n_mfcc, Tile
   L->region_bytes = rb;
   L->lds_bytes = total;
   L->part_off = part_off;
+  L->tab_off = tab_off;
   L->tile_frames = nw * fpw;
   return true;
 }
