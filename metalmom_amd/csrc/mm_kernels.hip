@@ -132,6 +132,17 @@ __device__ __forceinline__ T ldg(const void* base, unsigned byte_off) {
   return *reinterpret_cast<const T*>(static_cast<const char*>(base) + byte_off);
 }
 
+// ldg for memory known to be global (HBM): the explicit global address space
+// keeps global loads for pointers whose address space the compiler cannot
+// infer (e.g. after wave_uniform), which would otherwise become flat loads.
+// T: a builtin or ext_vector type.
+template <typename T>
+__device__ __forceinline__ T ldg_global(const void* base, unsigned byte_off) {
+  using GC = const __attribute__((address_space(1))) char*;
+  using GT = const __attribute__((address_space(1))) T*;
+  return *(GT)((GC)base + byte_off);
+}
+
 // A pointer that is equal in all active lanes of the wave, moved into scalar
 // registers (the compiler cannot see that a per-wave value is uniform).
 template <typename T>
@@ -156,6 +167,16 @@ __device__ __forceinline__ TilePf ldc_pf(const TilePf* p, int64_t i) {
 // is pending, so registers that held in-flight loads on some path can be
 // reused later without a conservative full wait.
 __device__ __forceinline__ void wait_vmem_all() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+// 16 wait states after the last MFMA that wrote acc, before any other
+// instruction can read it. The compiler puts only 6 between a
+// v_mfma_f32_16x16x32_{bf16,f16} and a VALU copy of its result (the
+// loop-carried accumulator of a multi-round contraction), and such a copy
+// has been measured to read the accumulator as it was before that MFMA now
+// and then (one chunk of a mel band lost, differing from run to run). The
+// asm operand ties the wait to acc: it comes after acc's producers and
+// before its readers.
+__device__ __forceinline__ void mfma_settle(f32x4& acc) { asm volatile("s_nop 7\n\ts_nop 7" : "+v"(acc)); }
 
 // Padded LDS index: one spare complex slot per 16 keeps the strided radix-16
 // accesses of a Stockham pass on distinct banks.
@@ -359,16 +380,24 @@ struct PassChain {
 typedef _Float16 h2v __attribute__((ext_vector_type(2)));
 
 template <typename TIn> struct SamplePair;
-// (x[2n], x[2n+1]) at base + byte offset (see ldg)
+// (x[2n], x[2n+1]) at base + byte offset of a signal in global memory (see ldg)
+typedef float f2v __attribute__((ext_vector_type(2)));
 template <> struct SamplePair<float> {
   __device__ __forceinline__ static float2 load(const float* base, unsigned byte_off) {
-    return ldg<float2>(base, byte_off);
+    const f2v v = ldg_global<f2v>(base, byte_off);
+    return make_float2(v.x, v.y);
+  }
+  __device__ __forceinline__ static float one(const float* base, unsigned byte_off) {
+    return ldg_global<float>(base, byte_off);
   }
 };
 template <> struct SamplePair<_Float16> {
   __device__ __forceinline__ static float2 load(const _Float16* base, unsigned byte_off) {
-    const h2v h = ldg<h2v>(base, byte_off);
+    const h2v h = ldg_global<h2v>(base, byte_off);
     return make_float2((float)h.x, (float)h.y);
+  }
+  __device__ __forceinline__ static float one(const _Float16* base, unsigned byte_off) {
+    return (float)ldg_global<_Float16>(base, byte_off);
   }
 };
 
@@ -411,8 +440,8 @@ __device__ __forceinline__ void frame_load(const TileArgs& a, const TIn* __restr
         for (int r = 0; r < R0; ++r) {
           const float2 w = ldg<float2>(a.window + 2 * r * B0, (unsigned)j * 8u);
           const unsigned o = (unsigned)j * (2u * sizeof(TIn));
-          v[q][r] = make_float2((float)ldg<TIn>(xf + 2 * r * B0, o) * w.x,
-                                (float)ldg<TIn>(xf + 2 * r * B0 + 1, o) * w.y);
+          v[q][r] = make_float2(SamplePair<TIn>::one(xf + 2 * r * B0, o) * w.x,
+                                SamplePair<TIn>::one(xf + 2 * r * B0 + 1, o) * w.y);
         }
       }
     }
@@ -659,8 +688,8 @@ __device__ __forceinline__ void frame_fft_split(const TileArgs& a, int clip, int
         for (int r = 0; r < R0; ++r) {
           const float2 w = ldg<float2>(win + 2 * r * B0, (unsigned)j * 8u);
           const unsigned o = (unsigned)j * (2u * sizeof(TIn));
-          v[q][r] = make_float2((float)ldg<TIn>(xf + 2 * r * B0, o) * w.x,
-                                (float)ldg<TIn>(xf + 2 * r * B0 + 1, o) * w.y);
+          v[q][r] = make_float2(SamplePair<TIn>::one(xf + 2 * r * B0, o) * w.x,
+                                SamplePair<TIn>::one(xf + 2 * r * B0 + 1, o) * w.y);
         }
       }
     }
@@ -861,12 +890,14 @@ __global__ __launch_bounds__(NW * 64) MM_TILE_OCC void k_tile(TileArgs a) {
   // [window: n_fft floats][per-pass twiddles: M float2][W_N^k: M float2],
   // published by a barrier before the first FFT
   constexpr bool TABL = !SPEC && PREC != PREC_F32 && NW == 16 && SHP::REG_SPLIT && LOG2M <= 10;
-  const float* win = a.window;
-  const float2* twp = a.twp;
-  const float2* twn = a.twN;
+  // (the pointers are set in a compile-time branch: a pointer that may be
+  // either global or LDS would compile to flat loads)
+  const float* win;
+  const float2* twp;
+  const float2* twn;
   if constexpr (TABL) {
+    unsigned char* tb = lds + a.tab_lds_off;
     if (t_begin < t_end) {
-      unsigned char* tb = lds + a.tab_lds_off;
       const int nw4 = a.n_fft / 4, nt = M / 2;   // 16-byte pieces of the window / of each twiddle table
       for (int p = threadIdx.x; p < nw4 + 2 * nt; p += NT) {
         uint4 v;
@@ -876,10 +907,14 @@ __global__ __launch_bounds__(NW * 64) MM_TILE_OCC void k_tile(TileArgs a) {
         reinterpret_cast<uint4*>(tb)[p] = v;
       }
       lds_barrier();
-      win = reinterpret_cast<const float*>(tb);
-      twp = reinterpret_cast<const float2*>(tb + a.n_fft * 4);
-      twn = twp + M;
     }
+    win = reinterpret_cast<const float*>(tb);
+    twp = reinterpret_cast<const float2*>(tb + a.n_fft * 4);
+    twn = twp + M;
+  } else {
+    win = a.window;
+    twp = a.twp;
+    twn = a.twN;
   }
   // LDS copies of the banded filterbank ([chunk][8 rows][64 B]) and of the
   // DCT operand tables; the first tile's barrier publishes them
@@ -1229,6 +1264,7 @@ __global__ __launch_bounds__(NW * 64) MM_TILE_OCC void k_tile(TileArgs a) {
 #pragma unroll
           for (int u = 0; u < K; ++u)
             if (u == 0 || c + u < n) acc = mfma_chunk(av[u], bv[u], acc);
+          mfma_settle(acc);
           wait_vmem_all();   // duplicate loads whose MFMA was skipped end here
         };
         for (int c = 0; c < n;) {
@@ -1337,6 +1373,7 @@ __global__ __launch_bounds__(NW * 64) MM_TILE_OCC void k_tile(TileArgs a) {
                 }
               }
             }
+            mfma_settle(accd);
             float* dst = red + ((int64_t)(ct * nact + wave) * 64 + lane) * 4;
             *reinterpret_cast<float4*>(dst) = make_float4(accd[0], accd[1], accd[2], accd[3]);
           }

@@ -301,3 +301,45 @@ def test_f16_input_large_fft():
         O = o.mfcc(c.astype(np.float64), sr, 20, 4096, 1024, n_mels=256)
         assert r.shape == O.shape == (20, 1 + n // 1024)
         assert rel_err(r, O) <= 2e-3
+
+
+@pytest.mark.parametrize("prec,idt", [("bf16", "f32"), ("bf16", "f16"), ("f16", "f32"), ("f16", "f16")])
+def test_multi_round_mel_blocks(prec, idt):
+    """n_fft 4096 with 256 mels at 44.1 kHz: the top block of 8 mels spans 9
+    chunks of 32 bins, so its contraction runs in two rounds with the
+    accumulator carried between them. A round whose MFMA result is read too
+    early loses one chunk: the top bands come out several dB low in whole
+    tiles, depending on timing, so the launches are repeated. Every band is
+    checked in dB (16-bit tier, 0.05 dB) and the MFCC at the 16-bit tier."""
+    import torch
+    from metalmom_amd.batch import SpectralBatch
+    sr, n = 44100, 3 * 44100
+    fb = o.mel_filterbank(sr, 4096, 256)
+    nz = np.nonzero(fb[248:256].any(axis=0))[0]
+    s16 = int(nz[0]) & ~7
+    assert (int(nz[-1]) + 1 - s16 + 31) // 32 >= 9   # the case this test is for
+    rng = np.random.default_rng(11)
+    t = np.arange(n) / sr
+    clips = []
+    for i in range(4):
+        x = 0.2 * rng.uniform(-1, 1, n)
+        for f in rng.uniform(50, 20000, 4):
+            x += 0.2 * np.sin(2 * np.pi * f * t)
+        clips.append(np.clip(x, -1, 1).astype(np.float16 if idt == "f16" else np.float32))
+    ref = [c.astype(np.float64) for c in clips]
+    Om = [o.power_to_db(o.melspectrogram(c, sr, 4096, 1024, n_mels=256)) for c in ref]
+    Of = [o.mfcc(c, sr, 20, 4096, 1024, n_mels=256) for c in ref]
+    Bm = SpectralBatch([n] * 4, kind="mel", sr=sr, n_fft=4096, hop_length=1024, n_mels=256,
+                       precision=prec, input_dtype=idt)
+    Bf = SpectralBatch([n] * 4, kind="mfcc", sr=sr, n_fft=4096, hop_length=1024, n_mels=256, n_mfcc=20,
+                       precision=prec, input_dtype=idt)
+    dm, df = Bm.pack(clips), Bf.pack(clips)
+    for rep in range(3):
+        om, of = Bm.empty_output(), Bf.empty_output()
+        Bm.run(dm, om)
+        Bf.run(df, of)
+        torch.cuda.synchronize()
+        for i, (m, f) in enumerate(zip(Bm.unpack(om), Bf.unpack(of))):
+            d = np.abs(o.power_to_db(m.astype(np.float64)) - Om[i])
+            assert d.max() <= 0.05, (rep, i, int(np.argmax(d.max(axis=1))), float(d.max()))
+            assert rel_err(f, Of[i]) <= 2e-3, (rep, i)
