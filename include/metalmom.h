@@ -63,6 +63,21 @@ void mm_destroy(mm_context ctx);
 int32_t mm_stft(mm_context ctx, const float* signal_data, int64_t signal_length,
                 int32_t sample_rate, const MMSTFTParams* params, MMBuffer* out);
 
+/* Replaces Bridge.swift:144-221 (mm_istft), reference metalmom.h:41-45:
+ * inverse STFT of a complex spectrogram given as interleaved (re, im) floats,
+ * row-major [n_freqs, n_frames] (stft_count = 2 n_freqs n_frames). n_fft =
+ * 2 (n_freqs - 1); windowed overlap-add divided by the summed squared window
+ * where it exceeds 1e-10 (STFT.swift:686-861). center trims n_fft/2 at both
+ * ends; output_length > 0 cuts or zero-pads the result to that length; a
+ * non-power-of-two n_fft gives n_fft + (n_frames-1) hop zeros. 1-D output.
+ * Rejected (-1) where the reference would trap or this library has no kernel:
+ * a stft_count that does not match the shape, hop < 0, a power-of-two n_fft
+ * above 16384, or 2^31 output samples or more. */
+int32_t mm_istft(mm_context ctx, const float* stft_data, int64_t stft_count,
+                 int32_t n_freqs, int32_t n_frames, int32_t sample_rate,
+                 int32_t hop_length, int32_t win_length, int32_t center,
+                 int64_t output_length, MMBuffer* out);
+
 /* Replaces Bridge.swift:225-263 (mm_amplitude_to_db): 20 log10(max(x, amin)/ref),
  * clipped to max - top_db when top_db > 0. 1-D output of `count` values. */
 int32_t mm_amplitude_to_db(mm_context ctx, const float* data, int64_t count,
@@ -138,6 +153,19 @@ typedef struct {
  * the reference). */
 int32_t mm_stft_complex(mm_context ctx, const float* signal_data, int64_t signal_length,
                         int32_t sample_rate, const MMSTFTParams* params, MMBuffer* out);
+
+/* Number of samples mm_istft returns for this shape, or -1 if it would
+ * reject it. */
+int64_t mm_istft_length(int32_t n_freqs, int32_t n_frames, int32_t hop_length, int32_t center,
+                        int64_t output_length);
+/* mm_istft on device memory: d_stft holds the interleaved [n_freqs, n_frames]
+ * spectrogram, d_out mm_istft_length(...) floats; enqueued on `stream`
+ * (a hipStream_t; NULL = the null stream) without a host synchronisation.
+ * The context provides the scratch of the generic path; calls on one
+ * context must not overlap. */
+int32_t mm_istft_device(mm_context ctx, const float* d_stft, int32_t n_freqs, int32_t n_frames,
+                        int32_t sample_rate, int32_t hop_length, int32_t win_length, int32_t center,
+                        int64_t output_length, float* d_out, void* stream);
 
 /* One clip with every option of MMSpecParams (input must be float32 here). */
 int32_t mm_spectral(mm_context ctx, const MMSpecParams* params, const float* signal_data,

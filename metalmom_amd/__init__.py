@@ -1,7 +1,8 @@
 """metalmom_amd: MetalMom's spectral front end on AMD Instinct MI355X (gfx950).
 
-Drop-in for the reference hot path: ``stft``, ``feature.melspectrogram``,
-``feature.mfcc``, ``feature.power_to_db`` / ``amplitude_to_db`` keep the
+Drop-in for the reference hot path: ``stft``, ``istft``,
+``feature.melspectrogram``, ``feature.mfcc``, ``feature.power_to_db`` /
+``amplitude_to_db`` keep the
 signatures of ``metalmom`` (zakkeown/MetalMom ``python/metalmom``) and call
 the same C ABI (``mm_*``) implemented by ``libmetalmom.so`` with HIP kernels.
 Additive: ``stft_complex``, ``spectral`` (all options), device-resident
@@ -9,6 +10,6 @@ batches (``batch.SpectralBatch``) and a request server (``serve``).
 """
 __version__ = "0.1.0"
 
-from .core import stft, stft_complex, spectral  # noqa: F401
+from .core import stft, stft_complex, istft, spectral  # noqa: F401
 from .feature import (amplitude_to_db, power_to_db, melspectrogram, mfcc)  # noqa: F401
 from . import feature  # noqa: F401

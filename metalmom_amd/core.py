@@ -1,4 +1,4 @@
-"""STFT entry points, mirroring the reference ``python/metalmom/core.py:198-250``."""
+"""STFT and iSTFT entry points, mirroring the reference ``python/metalmom/core.py:198-318``."""
 from __future__ import annotations
 
 import ctypes as C
@@ -42,6 +42,35 @@ def stft_complex(y, n_fft=2048, hop_length=None, win_length=None, center=True):
     """Complex64 STFT with the same framing as :func:`stft` (the reference's
     ``STFT.computeComplex``, ``STFT.swift:512-668``; not exported by its ABI)."""
     return _call_stft("mm_stft_complex", y, n_fft, hop_length, win_length, center)
+
+
+def istft(stft_matrix, hop_length=None, win_length=None, center=True, length=None):
+    """Inverse STFT of a complex spectrogram ``(n_fft // 2 + 1, n_frames)``
+    (the reference's ``core.py:253-318``): windowed overlap-add divided by the
+    summed squared window. Defaults ``hop_length = n_fft // 4`` and
+    ``win_length = n_fft``; ``center`` trims ``n_fft // 2`` samples at both
+    ends; ``length`` cuts or zero-pads the result. Real input is taken as
+    ``x + 0j``. Returns a 1-D float32 array.
+    """
+    S = np.asarray(stft_matrix)
+    S = S.astype(np.complex64) if not np.iscomplexobj(S) else np.asarray(S, dtype=np.complex64)
+    n_freqs, n_frames = S.shape
+    n_fft = (n_freqs - 1) * 2
+    hop = hop_length if hop_length is not None else n_fft // 4
+    win = win_length if win_length is not None else n_fft
+    flat = np.ascontiguousarray(S).view(np.float32)
+    ctx = lib.mm_init()
+    if not ctx:
+        raise RuntimeError("Failed to initialize MetalMom context")
+    try:
+        out = MMBuffer()
+        status = lib.mm_istft(ctx, float_ptr(flat), flat.size, n_freqs, n_frames, 22050, hop, win,
+                              1 if center else 0, length if length is not None else 0, C.byref(out))
+        if status != 0:
+            raise RuntimeError(f"mm_istft failed with status {status}")
+        return buffer_to_numpy(out)
+    finally:
+        lib.mm_destroy(ctx)
 
 
 def spectral(y, kind="mfcc", sr=22050, n_fft=2048, hop_length=512, win_length=None, center=True,

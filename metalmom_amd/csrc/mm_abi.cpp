@@ -4,6 +4,7 @@
 // reference bridge, Sources/MetalMomBridge/Bridge.swift:
 //   mm_init/mm_destroy registry      :16-40
 //   mm_stft                          :44-140
+//   mm_istft                         :144-221
 //   mm_amplitude_to_db               :225-263
 //   mm_power_to_db                   :267-306
 //   mm_mel_spectrogram               :310-400
@@ -168,6 +169,25 @@ int32_t mm_stft(mm_context ctx, const float* signal_data, int64_t signal_length,
   return run_to_buffer(ctx, r, signal_data, signal_length, out);
 }
 
+int64_t mm_istft_length(int32_t n_freqs, int32_t n_frames, int32_t hop_length, int32_t center,
+                        int64_t output_length) {
+  IstftShape sh;
+  if (istft_shape(n_freqs, n_frames, hop_length, center, output_length, &sh) != OK) return -1;
+  return sh.out_len;
+}
+
+int32_t mm_istft_device(mm_context ctx, const float* d_stft, int32_t n_freqs, int32_t n_frames, int32_t sample_rate,
+                        int32_t hop_length, int32_t win_length, int32_t center, int64_t output_length, float* d_out,
+                        void* stream) {
+  if (!ctx || !is_live(ctx) || !d_stft || !d_out) return MM_ERR_INVALID_INPUT;
+  IstftShape sh;
+  if (istft_shape(n_freqs, n_frames, hop_length, center, output_length, &sh) != OK) return MM_ERR_INVALID_INPUT;
+  Lane* lane = static_cast<Ctx*>(ctx)->get();
+  if (!lane) return ERR_NO_GPU;
+  return enqueue_istft(sh, sample_rate, win_length, d_stft, d_out, &lane->d_scratch,
+                       static_cast<hipStream_t>(stream));
+}
+
 int32_t mm_stft_complex(mm_context ctx, const float* signal_data, int64_t signal_length, int32_t sample_rate,
                         const MMSTFTParams* params, MMBuffer* out) {
   if (!ctx || !signal_data || signal_length <= 0 || !params || !out) return MM_ERR_INVALID_INPUT;
@@ -180,6 +200,32 @@ int32_t mm_stft_complex(mm_context ctx, const float* signal_data, int64_t signal
   r.win = params->win_length;
   r.center = params->center != 0;
   return run_to_buffer(ctx, r, signal_data, signal_length, out);
+}
+
+int32_t mm_istft(mm_context ctx, const float* stft_data, int64_t stft_count, int32_t n_freqs, int32_t n_frames,
+                 int32_t sample_rate, int32_t hop_length, int32_t win_length, int32_t center, int64_t output_length,
+                 MMBuffer* out) {
+  if (!ctx || !stft_data || stft_count <= 0 || !out) return MM_ERR_INVALID_INPUT;
+  if (n_freqs < 1 || n_frames < 1 || stft_count != 2 * (int64_t)n_freqs * n_frames) return MM_ERR_INVALID_INPUT;
+  IstftShape sh;
+  if (istft_shape(n_freqs, n_frames, hop_length, center, output_length, &sh) != OK) return MM_ERR_INVALID_INPUT;
+  if (sh.out_len == 0) {   // Bridge.swift:190-201: data NULL, shape all zero
+    fill_shape(out, {0}, 0, 0, nullptr);
+    return OK;
+  }
+  if (!sh.pow2) {   // STFT.swift:720-724: the zero buffer, nothing computed
+    float* z = static_cast<float*>(std::calloc((size_t)sh.out_len, sizeof(float)));
+    if (!z) return MM_ERR_ALLOC_FAILED;
+    fill_shape(out, {sh.out_len}, 0, sh.out_len, z);
+    return OK;
+  }
+  Lane* lane = static_cast<Ctx*>(ctx)->get();
+  if (!lane) return ERR_NO_GPU;
+  float* data = nullptr;
+  const int32_t s = run_istft(lane, sh, sample_rate, win_length, stft_data, &data);
+  if (s != OK) return s;
+  fill_shape(out, {sh.out_len}, 0, sh.out_len, data);
+  return OK;
 }
 
 int32_t mm_amplitude_to_db(mm_context ctx, const float* data, int64_t count, float ref, float amin, float top_db,

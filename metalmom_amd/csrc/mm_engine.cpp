@@ -692,4 +692,113 @@ int32_t run_db(Lane* lane, const float* data, int64_t n, float mult, float ref, 
   return OK;
 }
 
+// ---------------------------------------------------------------------------
+// inverse STFT
+// ---------------------------------------------------------------------------
+
+int32_t istft_shape(int n_freqs, int n_frames, int hop, int center, int64_t length, IstftShape* sh) {
+  if (n_freqs < 1 || n_frames < 1 || hop < 0) return ERR_INVALID;
+  IstftShape s;
+  s.n_fft = (n_freqs - 1) * 2;
+  s.hop = hop;
+  s.T = n_frames;
+  s.total = (int64_t)s.n_fft + (int64_t)(n_frames - 1) * hop;
+  s.pow2 = s.n_fft > 0 && (s.n_fft & (s.n_fft - 1)) == 0;
+  if (!s.pow2) {   // STFT.swift:720-724: the zero buffer itself
+    s.start = 0;
+    s.out_len = s.total;
+  } else {
+    if (s.n_fft > 16384) return ERR_INVALID;
+    s.start = center ? s.n_fft / 2 : 0;
+    if (length > 0) s.out_len = length;
+    else if (center) s.out_len = (s.total - s.n_fft / 2 > s.n_fft / 2) ? s.total - s.n_fft : 0;
+    else s.out_len = s.total;
+  }
+  if (s.total >= ((int64_t)1 << 31) || s.out_len >= ((int64_t)1 << 31)) return ERR_INVALID;
+  *sh = s;
+  return OK;
+}
+
+int32_t enqueue_istft(const IstftShape& sh, int sr, int win, const float* d_X, float* d_out, DevBuf* scratch,
+                      hipStream_t st) {
+  Engine* eng = Engine::get();
+  if (!eng) return ERR_NO_GPU;
+  if (sh.out_len == 0) return OK;
+  if (!sh.pow2) {
+    MM_HIP(launch_fill_u32(reinterpret_cast<unsigned*>(d_out), sh.out_len, 0u, st));
+    return OK;
+  }
+  SpecRequest r;
+  r.out_kind = OUT_STFT_COMPLEX;
+  r.sr = sr;
+  r.n_fft = sh.n_fft;
+  r.hop = sh.hop > 0 ? sh.hop : 1;
+  r.win = win;
+  const Tables* tb = nullptr;
+  int32_t s = eng->tables_for(r, st, &tb);
+  if (s != OK) return s;
+  IstftArgs a{};
+  a.X = reinterpret_cast<const float2*>(d_X);
+  a.out = d_out;
+  a.window = tb->t.window;
+  a.tw = tb->t.tw;
+  a.twN = tb->t.twN;
+  a.twp = tb->t.twp;
+  a.n_fft = sh.n_fft;
+  a.hop = sh.hop;
+  a.T = sh.T;
+  a.total = sh.total;
+  a.start = sh.start;
+  a.out_len = sh.out_len;
+  int nw = 0, nf = 0, rb = 0;
+  size_t lds = 0;
+  if (istft_fused_plan(sh.n_fft, sh.hop, &nw, &nf, &rb, &lds)) {
+    const int64_t span = (int64_t)nf * sh.hop;
+    const int64_t t0 = sh.start / span;
+    const int64_t t1 = (sh.start + sh.out_len + span - 1) / span;
+    const int n_tiles = (int)(t1 - t0);
+    const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(4, lds_limit_bytes() / std::max<size_t>(lds, 1)));
+    int grid = std::min(n_tiles, std::max(1, eng->num_cus() * per_cu));
+    const int per_wg = (n_tiles + grid - 1) / grid;
+    grid = (n_tiles + per_wg - 1) / per_wg;
+    a.nf = nf;
+    a.tile0 = (int)t0;
+    a.n_tiles = n_tiles;
+    a.tiles_per_wg = per_wg;
+    a.region_bytes = rb;
+    MM_HIP(launch_istft_tile(a, nw, grid, lds, st));
+    return OK;
+  }
+  MM_HIP(scratch->ensure((size_t)sh.T * sh.n_fft * sizeof(float)));
+  MM_HIP(launch_istft_generic(a, static_cast<float*>(scratch->p), st));
+  return OK;
+}
+
+int32_t run_istft(Lane* lane, const IstftShape& sh, int sr, int win, const float* X, float** out) {
+  *out = nullptr;
+  if (sh.out_len == 0) return OK;
+  hipStream_t st = lane->stream;
+  const size_t in_bytes = (size_t)(sh.n_fft / 2 + 1) * sh.T * 2 * sizeof(float);
+  const size_t out_bytes = (size_t)sh.out_len * sizeof(float);
+  MM_HIP(lane->h_in.ensure(in_bytes));
+  MM_HIP(lane->d_in.ensure(in_bytes));
+  MM_HIP(lane->d_out.ensure(out_bytes));
+  MM_HIP(lane->h_out.ensure(out_bytes));
+  std::memcpy(lane->h_in.p, X, in_bytes);
+  MM_HIP(hipMemcpyAsync(lane->d_in.p, lane->h_in.p, in_bytes, hipMemcpyHostToDevice, st));
+  const int32_t s = enqueue_istft(sh, sr, win, static_cast<const float*>(lane->d_in.p),
+                                  static_cast<float*>(lane->d_out.p), &lane->d_scratch, st);
+  if (s != OK) {
+    (void)hipStreamSynchronize(st);
+    return s;
+  }
+  MM_HIP(hipMemcpyAsync(lane->h_out.p, lane->d_out.p, out_bytes, hipMemcpyDeviceToHost, st));
+  MM_HIP(hipStreamSynchronize(st));
+  float* res = static_cast<float*>(std::malloc(out_bytes));
+  if (!res) return ERR_ALLOC;
+  std::memcpy(res, lane->h_out.p, out_bytes);
+  *out = res;
+  return OK;
+}
+
 }  // namespace mm

@@ -156,4 +156,24 @@ int32_t run_single(Lane* lane, const SpecRequest& r, const float* signal, int64_
 int32_t run_db(Lane* lane, const float* data, int64_t n, float mult, float ref, float amin, float top_db,
                float** out);
 
+// ---- inverse STFT ----
+// Output geometry of mm_istft (STFT.swift:703-704, 720-724, 832-858).
+struct IstftShape {
+  int n_fft = 0, hop = 0, T = 0;
+  bool pow2 = false;     // false: the result is total zeros, untrimmed
+  int64_t total = 0;     // overlap-add buffer, n_fft + (T - 1) hop
+  int64_t start = 0;     // buffer sample of output sample 0
+  int64_t out_len = 0;   // samples of the result
+};
+// ERR_INVALID for shapes the library does not take (n_freqs or n_frames < 1,
+// hop < 0, a power-of-two n_fft above 16384, a buffer of 2^31 samples or more).
+int32_t istft_shape(int n_freqs, int n_frames, int hop, int center, int64_t length, IstftShape* sh);
+// d_X: [n_freqs][n_frames] complex on the device; d_out: sh.out_len floats.
+// Scratch (generic path) comes from `scratch`. Enqueued on st.
+int32_t enqueue_istft(const IstftShape& sh, int sr, int win, const float* d_X, float* d_out, DevBuf* scratch,
+                      hipStream_t st);
+// Host in, host out through the lane (synchronous); *out is malloc'd
+// (nullptr when the result is empty).
+int32_t run_istft(Lane* lane, const IstftShape& sh, int sr, int win, const float* X, float** out);
+
 }  // namespace mm

@@ -54,4 +54,29 @@ hipError_t launch_db_clamp(float* x, int64_t n, const unsigned* max_key, float t
 hipError_t launch_generic_dct(const float* logmel, int n_mels, int T, const float* dct, int n_mfcc, float* out,
                               hipStream_t st);
 
+// ---- inverse STFT ----
+struct IstftArgs {
+  const float2* X;       // [n_fft/2 + 1][T] complex, row-major (the reference's interleaved floats)
+  float* out;            // [out_len]
+  const float* window;   // [n_fft] synthesis window
+  const float2* tw;      // W_M^k, k < M (generic path)
+  const float2* twN;     // W_N^k, k < M
+  const float2* twp;     // per-pass twiddles of the register-split FFT
+  int n_fft, hop, T;
+  int nf;                // fused: new frames per tile
+  int tile0;             // fused: index of the first tile
+  int n_tiles;           // fused: tiles of this launch
+  int tiles_per_wg;      // fused: tiles per workgroup (persistent grid)
+  int region_bytes;      // fused: LDS bytes per wave
+  int64_t total;         // overlap-add buffer length, n_fft + (T - 1) hop
+  int64_t start;         // first buffer sample of the output (n_fft/2 with centre, else 0)
+  int64_t out_len;       // output samples (zeros past the buffer)
+};
+// Fused tile kernel plan: false when n_fft is not 1024 / 2048 / 4096 or the
+// halo (ceil(n_fft / hop) - 1 frames) exceeds half a tile.
+bool istft_fused_plan(int n_fft, int hop, int* nw, int* nf, int* region_bytes, size_t* lds_bytes);
+hipError_t launch_istft_tile(const IstftArgs& a, int nw, int grid, size_t lds, hipStream_t st);
+// Per-frame inverse FFTs into frames[T][n_fft], then the overlap-add.
+hipError_t launch_istft_generic(const IstftArgs& a, float* frames, hipStream_t st);
+
 }  // namespace mm

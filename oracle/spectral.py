@@ -15,7 +15,7 @@ import numpy as np
 
 __all__ = [
     "hann_periodic", "prepare_window", "frame_count", "pad_signal",
-    "stft_complex", "stft_magnitude", "power_spectrogram",
+    "stft_complex", "stft_magnitude", "power_spectrogram", "istft_length", "istft",
     "hz_to_mel", "mel_to_hz", "mel_filterbank", "melspectrogram",
     "linear_to_db", "power_to_db", "amplitude_to_db",
     "dct_ortho_basis", "mfcc", "mfcc_from_logmel",
@@ -119,6 +119,69 @@ def stft_complex(y, n_fft=2048, hop=None, win_length=None, center=True,
     idx = np.arange(n_frames)[:, None] * hop + np.arange(n_fft)[None, :]
     frames = x[idx] * w[None, :]
     return np.fft.rfft(frames, n=n_fft, axis=1).T.copy()
+
+
+def istft_length(n_freqs: int, n_frames: int, hop: int, center: bool, length=None) -> int:
+    """Sample count of ``istft``'s result (``STFT.swift:703-704, 832-858``).
+
+    The overlap-add buffer holds ``n_fft + (n_frames - 1) * hop`` samples
+    (``n_fft = 2 (n_freqs - 1)``). ``length`` (if given) is the result's
+    size; otherwise ``center`` trims ``n_fft//2`` from both ends (nothing
+    left when the buffer is not longer than ``n_fft``). A non-power-of-two
+    ``n_fft`` returns the whole zero buffer untrimmed (``:720-724``).
+    """
+    n_fft = (n_freqs - 1) * 2
+    total = n_fft + (n_frames - 1) * hop
+    if not (n_fft > 0 and (n_fft & (n_fft - 1)) == 0):
+        return max(total, 0)
+    if length is not None:
+        return int(length)
+    if center:
+        return max(total - n_fft, 0) if total - n_fft // 2 > n_fft // 2 else 0
+    return total
+
+
+def istft(stft_matrix, hop=None, win_length=None, center=True, length=None) -> np.ndarray:
+    """Inverse STFT by windowed overlap-add, the output of ``mm_istft``.
+
+    Restates ``STFT.swift:686-861`` (``STFT.inverse``): ``n_fft = 2 (F - 1)``;
+    every frame is the inverse real DFT of its column with the imaginary parts
+    of the DC and Nyquist bins ignored (vDSP's packed format, ``:760-768``),
+    scaled to the true inverse (``1/n_fft``); it is multiplied by the
+    synthesis window (the analysis window of ``prepare_window``) and added
+    at ``f * hop``. Samples whose sum of squared windows exceeds ``1e-10`` are
+    divided by it (``:820-826``). Then the centre trim / ``length`` rule of
+    ``:832-858``: with ``center`` the result starts at ``n_fft//2`` and is cut
+    or zero-padded to ``length``; without it the buffer itself is cut or
+    padded. A non-power-of-two ``n_fft`` gives the all-zero buffer, untrimmed.
+    Computed here with ``numpy.fft.irfft`` in float64.
+    """
+    S = np.asarray(stft_matrix)
+    n_freqs, n_frames = S.shape
+    n_fft = (n_freqs - 1) * 2
+    hop = n_fft // 4 if hop is None else hop
+    win_length = n_fft if win_length is None else win_length
+    total = n_fft + (n_frames - 1) * hop
+    if not (n_fft > 0 and (n_fft & (n_fft - 1)) == 0):
+        return np.zeros(max(total, 0), dtype=np.float64)
+    w = prepare_window(win_length, n_fft)
+    out = np.zeros(total, dtype=np.float64)
+    wsum = np.zeros(total, dtype=np.float64)
+    X = S.astype(np.complex128).copy()
+    X[0, :] = X[0, :].real
+    X[-1, :] = X[-1, :].real
+    frames = np.fft.irfft(X.T, n=n_fft, axis=1) * w[None, :]
+    for f in range(n_frames):
+        out[f * hop:f * hop + n_fft] += frames[f]
+        wsum[f * hop:f * hop + n_fft] += w * w
+    nz = wsum > 1e-10
+    out[nz] /= wsum[nz]
+    n_out = istft_length(n_freqs, n_frames, hop, center, length)
+    start = n_fft // 2 if center else 0
+    res = np.zeros(n_out, dtype=np.float64)
+    avail = max(min(n_out, total - start), 0)
+    res[:avail] = out[start:start + avail]
+    return res
 
 
 def stft_magnitude(y, n_fft=2048, hop=None, win_length=None, center=True,
@@ -286,6 +349,9 @@ PINS = {
     "prepare_window": "centre zero-pad / centre truncation indices checked by hand on small cases",
     "frame_count": "22050 -> 44 (STFTTests.swift:8-24), 661500 -> 1292, empty -> 1/0 (EdgeCaseTests.swift:69-81)",
     "stft_complex": "golden stft_440hz_default_magnitude.npy; O(N^2) brute-force DFT for n_fft <= 256; Parseval",
+    "istft": "round trip istft(stft_complex(y)) = y (centre, no centre, several hops and window lengths); "
+             "O(N^2) brute-force inverse DFT + overlap-add on small cases; length / trim rules on hand cases "
+             "(ISTFTTests.swift:66-89); linearity",
     "mel_filterbank": "goldens mel_filterbank_{128_2048,40_1024,64_2048_300_8000}.npy",
     "melspectrogram": "goldens mel_spectrogram_440hz_*.npy",
     "power_to_db": "golden stft_440hz_power_db.npy; 0.5 -> -3.0103 dB, 100 -> 20 dB (ScalingTests.swift:39-56)",

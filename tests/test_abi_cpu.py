@@ -102,3 +102,50 @@ def test_compute_without_gpu_reports_unavailable():
                     C.byref(MMSTFTParams(2048, 512, 2048, 1)), C.byref(out))
     assert s == -2
     lib.mm_destroy(ctx)
+
+
+def test_istft_shapes_match_the_oracle():
+    """mm_istft_length follows STFT.swift:703-704, 720-724, 832-858 (centre trim,
+    length, untrimmed zeros for a non-power-of-two n_fft) and rejects what
+    mm_istft rejects."""
+    from oracle import spectral as o
+    lib = _lib()
+    for F, T, hop, center, length in [(1025, 45, 512, 1, 0), (1025, 45, 512, 1, 22050), (1025, 45, 512, 0, 0),
+                                      (1025, 1, 512, 1, 0), (513, 10, 100, 0, 7), (4, 3, 2, 1, 0),
+                                      (2, 3, 0, 1, 0), (2, 3, 0, 0, 0), (8193, 2, 4096, 1, 0), (3, 5, 1, 1, 3)]:
+        want = o.istft_length(F, T, hop, bool(center), length if length > 0 else None)
+        assert lib.mm_istft_length(F, T, hop, center, length) == want, (F, T, hop, center, length)
+    assert lib.mm_istft_length(0, 3, 1, 1, 0) == -1          # no bins
+    assert lib.mm_istft_length(5, 0, 1, 1, 0) == -1          # no frames
+    assert lib.mm_istft_length(5, 3, -1, 1, 0) == -1         # negative hop
+    assert lib.mm_istft_length(16385, 2, 4096, 1, 0) == -1   # n_fft 32768: beyond the kernels
+
+
+def test_istft_validation_and_host_only_results():
+    """Bridge.swift:157-162 argument checks, the stft_count / shape check, the
+    empty result (data NULL, shape all zero, Bridge.swift:190-201) and the
+    zero buffer of a non-power-of-two n_fft, none of which needs the GPU."""
+    lib = _lib()
+    from metalmom_amd._native import MMBuffer
+    ctx = lib.mm_init()
+    fp = C.POINTER(C.c_float)
+    S = np.zeros((4, 3), np.complex64)
+    flat = np.ascontiguousarray(S).view(np.float32)
+    out = MMBuffer()
+    assert lib.mm_istft(None, flat.ctypes.data_as(fp), flat.size, 4, 3, 22050, 2, 6, 1, 0, C.byref(out)) == -1
+    assert lib.mm_istft(ctx, None, flat.size, 4, 3, 22050, 2, 6, 1, 0, C.byref(out)) == -1
+    assert lib.mm_istft(ctx, flat.ctypes.data_as(fp), 0, 4, 3, 22050, 2, 6, 1, 0, C.byref(out)) == -1
+    assert lib.mm_istft(ctx, flat.ctypes.data_as(fp), flat.size - 2, 4, 3, 22050, 2, 6, 1, 0, C.byref(out)) == -1
+    assert lib.mm_istft(ctx, flat.ctypes.data_as(fp), flat.size, 4, 3, 22050, -1, 6, 1, 0, C.byref(out)) == -1
+    assert lib.mm_istft(ctx, flat.ctypes.data_as(fp), flat.size, 4, 3, 22050, 2, 6, 1, 0, None) == -1
+    # n_fft 6 (not a power of two): 6 + 2 * 2 zeros, untrimmed
+    s = lib.mm_istft(ctx, flat.ctypes.data_as(fp), flat.size, 4, 3, 22050, 2, 6, 1, 0, C.byref(out))
+    assert s == 0 and out.ndim == 1 and out.shape[0] == 10 and out.count == 10
+    assert not np.ctypeslib.as_array(out.data, shape=(10,)).any()
+    lib.mm_buffer_free(C.byref(out))
+    # one 2048-point frame with centre trim: nothing left
+    S1 = np.ones((1025, 1), np.complex64)
+    f1 = np.ascontiguousarray(S1).view(np.float32)
+    s = lib.mm_istft(ctx, f1.ctypes.data_as(fp), f1.size, 1025, 1, 22050, 512, 2048, 1, 0, C.byref(out))
+    assert s == 0 and out.ndim == 1 and out.count == 0 and not out.data and out.shape[0] == 0
+    lib.mm_destroy(ctx)

@@ -187,3 +187,110 @@ def test_planted_mistakes_are_caught(signal_440):
     assert np.abs(bad_sign - G).max() > 1.0 and np.abs(bad_T - G).max() > 1.0
     assert np.abs(fb - load_golden("mel_filterbank_128_2048")).max() < 1e-8
     assert lo[0] >= 1  # first triangle starts above DC
+
+
+# -- iSTFT (STFT.swift:686-861) ----------------------------------------------
+
+def _istft_bruteforce(S, hop, win_length, center, length):
+    """Direct inverse DFT of each column (O(N^2), DC and Nyquist imaginary parts
+    left out as in vDSP's packed format), then overlap-add, window-sum division
+    and the trim / length rule, written out with loops. Small sizes only."""
+    F, T = S.shape
+    N = 2 * (F - 1)
+    w = o.prepare_window(win_length, N)
+    total = N + (T - 1) * hop
+    out = np.zeros(total)
+    ws = np.zeros(total)
+    n = np.arange(N)
+    for f in range(T):
+        X = S[:, f]
+        x = X[0].real + X[F - 1].real * np.cos(np.pi * n)
+        for k in range(1, F - 1):
+            x = x + 2.0 * (X[k].real * np.cos(2 * np.pi * k * n / N) - X[k].imag * np.sin(2 * np.pi * k * n / N))
+        x = x / N
+        for i in range(N):
+            out[f * hop + i] += x[i] * w[i]
+            ws[f * hop + i] += w[i] * w[i]
+    for i in range(total):
+        if ws[i] > 1e-10:
+            out[i] /= ws[i]
+    start = N // 2 if center else 0
+    if length is None:
+        end = total - N // 2 if center else total
+        return out[start:end] if end > start else np.zeros(0)
+    res = np.zeros(length)
+    m = max(min(length, total - start), 0)
+    res[:m] = out[start:start + m]
+    return res
+
+
+@pytest.mark.parametrize("F,T,hop,win,center,length", [
+    (5, 6, 3, 8, True, None), (5, 6, 2, 6, False, None), (9, 4, 5, 12, True, 11),
+    (9, 7, 4, 16, False, 40), (17, 3, 8, 32, True, 30), (2, 3, 1, 2, False, None)])
+def test_istft_bruteforce(F, T, hop, win, center, length):
+    rng = np.random.default_rng(F * 100 + T)
+    S = rng.standard_normal((F, T)) + 1j * rng.standard_normal((F, T))   # DC/Nyquist imag set on purpose
+    got = o.istft(S, hop, win, center, length)
+    ref = _istft_bruteforce(S, hop, win, center, length)
+    assert got.shape == ref.shape
+    assert np.abs(got - ref).max() <= 1e-12 * max(np.abs(ref).max(), 1.0)
+
+
+@pytest.mark.parametrize("n_fft,hop,win,center", [
+    (2048, 512, 2048, True), (1024, 256, 1024, False), (256, 64, 200, True), (512, 100, 512, True),
+    (4096, 1024, 4096, False), (64, 16, 64, True), (2048, 128, 2048, True)])
+def test_istft_inverts_the_stft(n_fft, hop, win, center):
+    """istft(stft_complex(y)) = y wherever the summed squared window is well
+    away from zero (everywhere with centre padding; the interior without)."""
+    y = np.random.default_rng(n_fft + hop).standard_normal(12000)
+    S = o.stft_complex(y, n_fft, hop, win, center)
+    r = o.istft(S, hop, win, center, length=len(y))
+    assert r.shape == y.shape
+    sl = slice(0, len(y)) if center and win == n_fft else slice(n_fft, len(y) - n_fft)
+    assert np.abs(r[sl] - y[sl]).max() < 1e-9
+
+
+def test_istft_length_known_answers():
+    # ISTFTTests.swift:8-37: 44 hops of a 2048-point STFT (45 frames) come back whole
+    assert o.istft_length(1025, 45, 512, True) == 44 * 512
+    assert o.istft_length(1025, 45, 512, True, 22050) == 22050
+    assert o.istft_length(1025, 45, 512, False) == 2048 + 44 * 512
+    assert o.istft_length(1025, 1, 512, True) == 0            # nothing left after the trim
+    assert o.istft_length(4, 3, 2, True) == 6 + 2 * 2         # n_fft 6: zeros, untrimmed
+    assert o.istft_length(2, 3, 0, True) == 0                 # n_fft 2 at hop 0: buffer of 2
+    z = o.istft(np.ones((4, 3), complex), 2, 6, True, None)
+    assert z.shape == (10,) and not z.any()
+    S = o.stft_complex(np.random.default_rng(5).standard_normal(22528), 2048, 512)
+    assert S.shape == (1025, 45)
+    assert o.istft(S, 512, 2048, True).shape == (22528,)
+    short = o.istft(S, 512, 2048, True, 1000)                  # ISTFTTests.swift:66-89
+    assert short.shape == (1000,)
+    longer = o.istft(S, 512, 2048, True, 30000)
+    assert longer.shape == (30000,) and not longer[24576 - 1024:].any()
+
+
+def test_istft_planted_mistakes_are_caught():
+    """Reading the Nyquist bin from the DC imaginary slot (vDSP's packed
+    layout misread), skipping the window-sum division or a missing 1/N fails
+    the brute-force comparison."""
+    rng = np.random.default_rng(9)
+    S = rng.standard_normal((9, 5)) + 1j * rng.standard_normal((9, 5))
+    ref = _istft_bruteforce(S, 4, 16, True, None)
+    assert np.abs(o.istft(S, 4, 16, True) - ref).max() < 1e-12
+    N, w = 16, o.prepare_window(16, 16)
+    Sb = S.copy()
+    Sb[-1] = S[0].imag
+    frames = [np.fft.irfft(S.T, n=N, axis=1) * w, np.fft.irfft(Sb.T, n=N, axis=1) * w,
+              np.fft.irfft(S.T, n=N, axis=1) * w * N]
+    outs = []
+    for k, fr in enumerate(frames):
+        out, ws = np.zeros(N + 4 * 4), np.zeros(N + 4 * 4)
+        for f in range(5):
+            out[4 * f:4 * f + N] += fr[f]
+            ws[4 * f:4 * f + N] += w * w
+        if k != 2:
+            nz = ws > 1e-10
+            out[nz] /= ws[nz]
+        outs.append(out[N // 2:len(out) - N // 2])
+    assert np.abs(outs[0] - ref).max() < 1e-12
+    assert np.abs(outs[1] - ref).max() > 1e-3 and np.abs(outs[2] - ref).max() > 1e-3
