@@ -10,7 +10,8 @@ HBM before the timed region starts.
 Also reported on the same batch: the FP32 batched path and the plain path (one
 clip per launch, FP32, no graph, clips in sequence), the fused kernel's
 roofline position, and the float32 NumPy restatement of the reference's CPU
-path timed on this host (bounded sample).
+path timed on this host (bounded sample). The ``istft`` entry measures the
+inverse STFT of one clip of the same total length on device-resident data.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -102,6 +103,73 @@ def cpu_baseline(clips, max_frames=400000, workers=0):
             "host_cpus": os.cpu_count(), "cpu_model": cpu_model()}
 
 
+def istft_leg(torch, n_samples, iters, with_cpu):
+    """Inverse STFT (mm_istft_device) of one long clip on device-resident data:
+    n_fft 2048, hop 512, centred, the spectrogram of n_samples samples
+    (synthetic complex64 values, [1025, T] row-major), n_samples samples out.
+    Kernel time from HIP events on the launch stream; algorithmic bytes are
+    the spectrogram read once plus the samples written."""
+    import ctypes as C
+    from metalmom_amd._native import lib
+    n_fft, hop = 2048, 512
+    F = n_fft // 2 + 1
+    T = 1 + n_samples // hop
+    g = torch.Generator(device="cuda")
+    g.manual_seed(5)
+    X = torch.randn(F * T * 2, device="cuda", generator=g)
+    n = int(lib.mm_istft_length(F, T, hop, 1, n_samples))
+    Y = torch.empty(n, device="cuda")
+    st = torch.cuda.Stream()
+    ctx = lib.mm_init()
+
+    def run():
+        s = lib.mm_istft_device(ctx, C.c_void_p(X.data_ptr()), F, T, 22050, hop, n_fft, 1, n_samples,
+                                C.c_void_p(Y.data_ptr()), C.c_void_p(st.cuda_stream))
+        if s != 0:
+            raise RuntimeError(f"mm_istft_device failed with status {s}")
+
+    run()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(iters):
+        run()
+    e1.record(st)
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / iters
+    lib.mm_destroy(ctx)
+    del X, Y
+    bytes_launch = F * T * 8 + n * 4
+    gbs = bytes_launch / (ms * 1e-3) / 1e9
+    res = {"workload": f"one clip, spectrogram [1025, {T}] complex64 -> {n} samples, n_fft 2048, hop 512, "
+                       f"centred, Hann synthesis window",
+           "frames": T, "samples": n, "kernel_ms": ms, "frames_per_s": T / (ms * 1e-3),
+           "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": gbs / HBM_PEAK_GBS, "bytes_per_launch": bytes_launch}}
+    if with_cpu:
+        from oracle.cpu_baseline import IstftF32
+        try:
+            from threadpoolctl import threadpool_limits
+            limiter = threadpool_limits(limits=1)
+        except Exception:
+            limiter = None
+        Tc = 40000
+        rng = np.random.default_rng(5)
+        S = (rng.standard_normal((F, Tc), dtype=np.float32)
+             + 1j * rng.standard_normal((F, Tc), dtype=np.float32)).astype(np.complex64)
+        f = IstftF32(n_fft, hop)
+        f(S[:, :64])
+        t0 = time.perf_counter()
+        f(S, (Tc - 1) * hop)
+        dt = time.perf_counter() - t0
+        if limiter is not None and hasattr(limiter, "unregister"):
+            limiter.unregister()
+        res["cpu_baseline"] = {"value": Tc / dt, "unit": "frames/s", "cores": 1, "kind": "port",
+                               "sample": f"spectrogram [1025, {Tc}], float32 numpy irfft + overlap-add, "
+                                         f"{dt:.1f} s, 1 thread", "host_cpus": os.cpu_count()}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -115,6 +183,7 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--grid", type=int, default=0,
                     help="persistent workgroups per CU (skips the autotuner; 0 = autotune)")
+    ap.add_argument("--no-istft", action="store_true", help="skip the inverse-STFT leg")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="profiling aid: also run one device-to-device copy of this many bytes, "
                          "so HBM counters can be scaled by a kernel of known traffic")
@@ -249,6 +318,13 @@ def main():
                             "frames_per_s": r["frames_per_s"], "device_steps": st["steps"]})
             log(f"serving @{rate:.0f} req/s: p50 {r['p50'] * 1e3:.2f} ms, p99 {r['p99'] * 1e3:.2f} ms, "
                 f"{r['frames_per_s'] / 1e6:.2f} M frames/s, {st['steps']} steps for {r['n_requests']} requests")
+
+    if not args.no_istft:
+        ist = istft_leg(torch, B.total_input, 3 if args.profile else 20, not args.no_cpu and not args.profile)
+        extra["istft"] = ist
+        log(f"istft: {ist['frames_per_s'] / 1e6:.1f} M frames/s, kernel {ist['kernel_ms']:.3f} ms, "
+            f"{ist['roofline']['achieved']:.0f} GB/s algorithmic ({100 * ist['roofline']['frac']:.1f} % of HBM)"
+            + (f"; cpu {ist['cpu_baseline']['value']:.0f} frames/s" if "cpu_baseline" in ist else ""))
 
     cpu = None
     if not args.no_cpu and not args.profile:

@@ -12,7 +12,11 @@ port of its semantics in float32:
 * dense ``FB @ P`` as ``vDSP_mmul`` does (``Spectral/MelSpectrogram.swift:129-136``);
 * ``10 log10(max(x, 1e-10))`` and the whole-clip ``top_db`` floor
   (``Spectral/Scaling.swift:73-114``);
-* DCT-II ortho basis product (``Features/MFCC.swift:79-106``).
+* DCT-II ortho basis product (``Features/MFCC.swift:79-106``);
+* inverse STFT (``STFT.swift:686-861``): per-frame inverse real FFT
+  (``numpy.fft.irfft`` on complex64 columns), synthesis window, overlap-add
+  and division by the summed squared window where it exceeds 1e-10, centre
+  trim and length.
 """
 from __future__ import annotations
 
@@ -42,3 +46,36 @@ class MfccF32:
         if self.top_db is not None:
             db = np.maximum(db, db.max() - np.float32(self.top_db))
         return self.dct @ db
+
+
+class IstftF32:
+    """Float32 inverse STFT with a hop that divides n_fft (the overlap-add runs
+    as n_fft/hop shifted block sums)."""
+
+    def __init__(self, n_fft=2048, hop=512):
+        assert n_fft % hop == 0
+        self.n_fft, self.hop = n_fft, hop
+        self.win = o.hann_periodic(n_fft).astype(np.float32)
+
+    def __call__(self, S: np.ndarray, length=None) -> np.ndarray:
+        N, hop = self.n_fft, self.hop
+        T = S.shape[1]
+        R = N // hop
+        frames = np.fft.irfft(np.ascontiguousarray(S.T), n=N, axis=1).astype(np.float32) * self.win
+        total = N + (T - 1) * hop
+        out = np.zeros((total // hop, hop), dtype=np.float32)
+        ws = np.zeros_like(out)
+        w2 = (self.win * self.win).reshape(R, hop)
+        fb = frames.reshape(T, R, hop)
+        for b in range(R):
+            out[b:b + T] += fb[:, b, :]
+            ws[b:b + T] += w2[b]
+        out, ws = out.reshape(-1), ws.reshape(-1)
+        nz = ws > np.float32(1e-10)
+        out[nz] /= ws[nz]
+        start = N // 2
+        n = length if length is not None else total - N
+        res = np.zeros(n, dtype=np.float32)
+        m = max(min(n, total - start), 0)
+        res[:m] = out[start:start + m]
+        return res
