@@ -37,15 +37,16 @@ def log(*a):
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_tile_mfcc_bf16_uneven_batch_traffic.json")
+ISTFT_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_istft_tile_2048_long_clip_traffic.json")
 
 
-def measured_traffic(bytes_launch):
-    """HBM bytes per launch of the fused kernel from the committed rocprofv3
-    counter summary (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by
-    what the counter reads for a copy of known size), if it was taken on this
-    same workload (same algorithmic bytes); else None."""
+def measured_traffic(bytes_launch, path=TRAFFIC_FILE):
+    """HBM bytes per launch of a kernel from its committed rocprofv3 counter
+    summary (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by what the
+    counter reads for a copy of known size), if it was taken on this same
+    workload (same algorithmic bytes); else None."""
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(path) as f:
             t = json.load(f)
     except (OSError, ValueError):
         return None
@@ -114,16 +115,17 @@ def istft_leg(torch, n_samples, iters, with_cpu):
     n_fft, hop = 2048, 512
     F = n_fft // 2 + 1
     T = 1 + n_samples // hop
+    ld = (T + 15) // 16 * 16   # rows padded to whole 128-byte lines of 16 frames
     g = torch.Generator(device="cuda")
     g.manual_seed(5)
-    X = torch.randn(F * T * 2, device="cuda", generator=g)
+    X = torch.randn(F * ld * 2, device="cuda", generator=g)
     n = int(lib.mm_istft_length(F, T, hop, 1, n_samples))
     Y = torch.empty(n, device="cuda")
     st = torch.cuda.Stream()
     ctx = lib.mm_init()
 
     def run():
-        s = lib.mm_istft_device(ctx, C.c_void_p(X.data_ptr()), F, T, 22050, hop, n_fft, 1, n_samples,
+        s = lib.mm_istft_device(ctx, C.c_void_p(X.data_ptr()), ld, F, T, 22050, hop, n_fft, 1, n_samples,
                                 C.c_void_p(Y.data_ptr()), C.c_void_p(st.cuda_stream))
         if s != 0:
             raise RuntimeError(f"mm_istft_device failed with status {s}")
@@ -141,11 +143,12 @@ def istft_leg(torch, n_samples, iters, with_cpu):
     del X, Y
     bytes_launch = F * T * 8 + n * 4
     gbs = bytes_launch / (ms * 1e-3) / 1e9
-    res = {"workload": f"one clip, spectrogram [1025, {T}] complex64 -> {n} samples, n_fft 2048, hop 512, "
-                       f"centred, Hann synthesis window",
+    traffic = measured_traffic(bytes_launch, ISTFT_TRAFFIC_FILE)
+    res = {"workload": f"one clip, spectrogram [1025, {T}] complex64 (rows padded to {ld} frames) -> {n} "
+                       f"samples, n_fft 2048, hop 512, centred, Hann synthesis window",
            "frames": T, "samples": n, "kernel_ms": ms, "frames_per_s": T / (ms * 1e-3),
            "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": gbs / HBM_PEAK_GBS, "bytes_per_launch": bytes_launch}}
+                        "frac": gbs / HBM_PEAK_GBS, "traffic": traffic, "bytes_per_launch": bytes_launch}}
     if with_cpu:
         from oracle.cpu_baseline import IstftF32
         try:

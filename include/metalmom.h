@@ -158,14 +158,15 @@ int32_t mm_stft_complex(mm_context ctx, const float* signal_data, int64_t signal
  * reject it. */
 int64_t mm_istft_length(int32_t n_freqs, int32_t n_frames, int32_t hop_length, int32_t center,
                         int64_t output_length);
-/* mm_istft on device memory: d_stft holds the interleaved [n_freqs, n_frames]
- * spectrogram, d_out mm_istft_length(...) floats; enqueued on `stream`
- * (a hipStream_t; NULL = the null stream) without a host synchronisation.
- * The context provides the scratch of the generic path; calls on one
- * context must not overlap. */
-int32_t mm_istft_device(mm_context ctx, const float* d_stft, int32_t n_freqs, int32_t n_frames,
-                        int32_t sample_rate, int32_t hop_length, int32_t win_length, int32_t center,
-                        int64_t output_length, float* d_out, void* stream);
+/* mm_istft on device memory: d_stft holds the interleaved spectrogram with
+ * rows of ld_frames complex values (ld_frames >= n_frames; 0 = n_frames; a
+ * multiple of 16 lets the kernel read whole cache lines), d_out
+ * mm_istft_length(...) floats; enqueued on `stream` (a hipStream_t; NULL =
+ * the null stream) without a host synchronisation. The context provides the
+ * scratch of the generic path; calls on one context must not overlap. */
+int32_t mm_istft_device(mm_context ctx, const float* d_stft, int32_t ld_frames, int32_t n_freqs,
+                        int32_t n_frames, int32_t sample_rate, int32_t hop_length, int32_t win_length,
+                        int32_t center, int64_t output_length, float* d_out, void* stream);
 
 /* One clip with every option of MMSpecParams (input must be float32 here). */
 int32_t mm_spectral(mm_context ctx, const MMSpecParams* params, const float* signal_data,

@@ -176,15 +176,17 @@ int64_t mm_istft_length(int32_t n_freqs, int32_t n_frames, int32_t hop_length, i
   return sh.out_len;
 }
 
-int32_t mm_istft_device(mm_context ctx, const float* d_stft, int32_t n_freqs, int32_t n_frames, int32_t sample_rate,
-                        int32_t hop_length, int32_t win_length, int32_t center, int64_t output_length, float* d_out,
-                        void* stream) {
+int32_t mm_istft_device(mm_context ctx, const float* d_stft, int32_t ld_frames, int32_t n_freqs, int32_t n_frames,
+                        int32_t sample_rate, int32_t hop_length, int32_t win_length, int32_t center,
+                        int64_t output_length, float* d_out, void* stream) {
   if (!ctx || !is_live(ctx) || !d_stft || !d_out) return MM_ERR_INVALID_INPUT;
   IstftShape sh;
   if (istft_shape(n_freqs, n_frames, hop_length, center, output_length, &sh) != OK) return MM_ERR_INVALID_INPUT;
+  const int ld = (ld_frames > 0) ? ld_frames : n_frames;
+  if (ld < n_frames) return MM_ERR_INVALID_INPUT;
   Lane* lane = static_cast<Ctx*>(ctx)->get();
   if (!lane) return ERR_NO_GPU;
-  return enqueue_istft(sh, sample_rate, win_length, d_stft, d_out, &lane->d_scratch,
+  return enqueue_istft(sh, sample_rate, win_length, d_stft, ld, d_out, &lane->d_scratch,
                        static_cast<hipStream_t>(stream));
 }
 

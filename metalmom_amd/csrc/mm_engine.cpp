@@ -719,8 +719,8 @@ int32_t istft_shape(int n_freqs, int n_frames, int hop, int center, int64_t leng
   return OK;
 }
 
-int32_t enqueue_istft(const IstftShape& sh, int sr, int win, const float* d_X, float* d_out, DevBuf* scratch,
-                      hipStream_t st) {
+int32_t enqueue_istft(const IstftShape& sh, int sr, int win, const float* d_X, int ld, float* d_out,
+                      DevBuf* scratch, hipStream_t st) {
   Engine* eng = Engine::get();
   if (!eng) return ERR_NO_GPU;
   if (sh.out_len == 0) return OK;
@@ -747,26 +747,29 @@ int32_t enqueue_istft(const IstftShape& sh, int sr, int win, const float* d_X, f
   a.n_fft = sh.n_fft;
   a.hop = sh.hop;
   a.T = sh.T;
+  a.ld = ld;
   a.total = sh.total;
   a.start = sh.start;
   a.out_len = sh.out_len;
-  int nw = 0, nf = 0, rb = 0;
-  size_t lds = 0;
-  if (istft_fused_plan(sh.n_fft, sh.hop, &nw, &nf, &rb, &lds)) {
-    const int64_t span = (int64_t)nf * sh.hop;
+  IstftPlan pl;
+  if (istft_fused_plan(sh.n_fft, sh.hop, &pl)) {
+    const int64_t span = (int64_t)pl.nw * sh.hop;   // output samples per tile
     const int64_t t0 = sh.start / span;
     const int64_t t1 = (sh.start + sh.out_len + span - 1) / span;
     const int n_tiles = (int)(t1 - t0);
-    const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(4, lds_limit_bytes() / std::max<size_t>(lds, 1)));
+    const int per_cu =
+        (int)std::max<size_t>(1, std::min<size_t>(4, lds_limit_bytes() / std::max<size_t>(pl.lds_bytes, 1)));
     int grid = std::min(n_tiles, std::max(1, eng->num_cus() * per_cu));
     const int per_wg = (n_tiles + grid - 1) / grid;
     grid = (n_tiles + per_wg - 1) / per_wg;
-    a.nf = nf;
+    a.halo = pl.halo;
+    a.carry = pl.carry;
+    a.carry_stride = pl.carry_stride;
     a.tile0 = (int)t0;
     a.n_tiles = n_tiles;
     a.tiles_per_wg = per_wg;
-    a.region_bytes = rb;
-    MM_HIP(launch_istft_tile(a, nw, grid, lds, st));
+    a.region_bytes = pl.region_bytes;
+    MM_HIP(launch_istft_tile(a, pl.nw, grid, pl.lds_bytes, st));
     return OK;
   }
   MM_HIP(scratch->ensure((size_t)sh.T * sh.n_fft * sizeof(float)));
@@ -778,15 +781,20 @@ int32_t run_istft(Lane* lane, const IstftShape& sh, int sr, int win, const float
   *out = nullptr;
   if (sh.out_len == 0) return OK;
   hipStream_t st = lane->stream;
-  const size_t in_bytes = (size_t)(sh.n_fft / 2 + 1) * sh.T * 2 * sizeof(float);
+  // device rows padded to a multiple of 16 frames: the fused kernel then
+  // reads whole 128-byte lines
+  const int F = sh.n_fft / 2 + 1;
+  const int ld = (sh.T + 15) / 16 * 16;
+  const size_t row = (size_t)sh.T * 8, drow = (size_t)ld * 8;
+  const size_t in_bytes = row * F;
   const size_t out_bytes = (size_t)sh.out_len * sizeof(float);
   MM_HIP(lane->h_in.ensure(in_bytes));
-  MM_HIP(lane->d_in.ensure(in_bytes));
+  MM_HIP(lane->d_in.ensure(drow * F));
   MM_HIP(lane->d_out.ensure(out_bytes));
   MM_HIP(lane->h_out.ensure(out_bytes));
   std::memcpy(lane->h_in.p, X, in_bytes);
-  MM_HIP(hipMemcpyAsync(lane->d_in.p, lane->h_in.p, in_bytes, hipMemcpyHostToDevice, st));
-  const int32_t s = enqueue_istft(sh, sr, win, static_cast<const float*>(lane->d_in.p),
+  MM_HIP(hipMemcpy2DAsync(lane->d_in.p, drow, lane->h_in.p, row, row, F, hipMemcpyHostToDevice, st));
+  const int32_t s = enqueue_istft(sh, sr, win, static_cast<const float*>(lane->d_in.p), ld,
                                   static_cast<float*>(lane->d_out.p), &lane->d_scratch, st);
   if (s != OK) {
     (void)hipStreamSynchronize(st);

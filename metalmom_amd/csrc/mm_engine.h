@@ -168,10 +168,11 @@ struct IstftShape {
 // ERR_INVALID for shapes the library does not take (n_freqs or n_frames < 1,
 // hop < 0, a power-of-two n_fft above 16384, a buffer of 2^31 samples or more).
 int32_t istft_shape(int n_freqs, int n_frames, int hop, int center, int64_t length, IstftShape* sh);
-// d_X: [n_freqs][n_frames] complex on the device; d_out: sh.out_len floats.
+// d_X: [n_freqs][ld] complex on the device (ld >= n_frames; a multiple of 16
+// lets the fused kernel read whole cache lines); d_out: sh.out_len floats.
 // Scratch (generic path) comes from `scratch`. Enqueued on st.
-int32_t enqueue_istft(const IstftShape& sh, int sr, int win, const float* d_X, float* d_out, DevBuf* scratch,
-                      hipStream_t st);
+int32_t enqueue_istft(const IstftShape& sh, int sr, int win, const float* d_X, int ld, float* d_out,
+                      DevBuf* scratch, hipStream_t st);
 // Host in, host out through the lane (synchronous); *out is malloc'd
 // (nullptr when the result is empty).
 int32_t run_istft(Lane* lane, const IstftShape& sh, int sr, int win, const float* X, float** out);

@@ -56,25 +56,32 @@ hipError_t launch_generic_dct(const float* logmel, int n_mels, int T, const floa
 
 // ---- inverse STFT ----
 struct IstftArgs {
-  const float2* X;       // [n_fft/2 + 1][T] complex, row-major (the reference's interleaved floats)
+  const float2* X;       // [n_fft/2 + 1][ld] complex, row-major (the reference's interleaved floats at ld = T)
   float* out;            // [out_len]
   const float* window;   // [n_fft] synthesis window
   const float2* tw;      // W_M^k, k < M (generic path)
   const float2* twN;     // W_N^k, k < M
   const float2* twp;     // per-pass twiddles of the register-split FFT
   int n_fft, hop, T;
-  int nf;                // fused: new frames per tile
-  int tile0;             // fused: index of the first tile
+  int ld;                // row stride of X in complex elements (>= T)
+  int halo;              // fused: frames before a tile that touch its samples, ceil(n_fft / hop) - 1
+  int carry;             // fused: carried partial sums per tile, n_fft - hop
+  int carry_stride;      // fused: floats per carry buffer (carry rounded up to 4)
+  int tile0;             // fused: index of the first tile (tile t = frames t*nw .. t*nw + nw - 1)
   int n_tiles;           // fused: tiles of this launch
-  int tiles_per_wg;      // fused: tiles per workgroup (persistent grid)
+  int tiles_per_wg;      // fused: consecutive tiles per workgroup (persistent grid)
   int region_bytes;      // fused: LDS bytes per wave
   int64_t total;         // overlap-add buffer length, n_fft + (T - 1) hop
   int64_t start;         // first buffer sample of the output (n_fft/2 with centre, else 0)
   int64_t out_len;       // output samples (zeros past the buffer)
 };
+struct IstftPlan {
+  int nw, halo, carry, carry_stride, region_bytes;
+  size_t lds_bytes;
+};
 // Fused tile kernel plan: false when n_fft is not 1024 / 2048 / 4096 or the
 // halo (ceil(n_fft / hop) - 1 frames) exceeds half a tile.
-bool istft_fused_plan(int n_fft, int hop, int* nw, int* nf, int* region_bytes, size_t* lds_bytes);
+bool istft_fused_plan(int n_fft, int hop, IstftPlan* p);
 hipError_t launch_istft_tile(const IstftArgs& a, int nw, int grid, size_t lds, hipStream_t st);
 // Per-frame inverse FFTs into frames[T][n_fft], then the overlap-add.
 hipError_t launch_istft_generic(const IstftArgs& a, float* frames, hipStream_t st);

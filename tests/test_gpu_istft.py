@@ -150,10 +150,27 @@ def test_istft_device_entry_matches_host_entry():
         dY = torch.full((n,), float("nan"), device="cuda")
         ctx = lib.mm_init()
         try:
-            s = lib.mm_istft_device(ctx, C.c_void_p(dX.data_ptr()), F, T, 22050, hop, n_fft, 1, 0,
+            s = lib.mm_istft_device(ctx, C.c_void_p(dX.data_ptr()), 0, F, T, 22050, hop, n_fft, 1, 0,
                                     C.c_void_p(dY.data_ptr()), C.c_void_p(torch.cuda.current_stream().cuda_stream))
             assert s == 0
             torch.cuda.synchronize()
         finally:
             lib.mm_destroy(ctx)
         assert np.array_equal(dY.cpu().numpy(), host)
+        # rows padded to a multiple of 16 frames (ld_frames), as the host entry lays them out
+        ld = (T + 15) // 16 * 16
+        padded = np.zeros((F, ld), np.complex64)
+        padded[:, :T] = S
+        dP = torch.from_numpy(np.ascontiguousarray(padded).view(np.float32).copy()).cuda()
+        dZ = torch.full((n,), float("nan"), device="cuda")
+        ctx = lib.mm_init()
+        try:
+            assert lib.mm_istft_device(ctx, C.c_void_p(dP.data_ptr()), T - 1, F, T, 22050, hop, n_fft, 1, 0,
+                                       C.c_void_p(dZ.data_ptr()), None) == -1   # ld below n_frames
+            s = lib.mm_istft_device(ctx, C.c_void_p(dP.data_ptr()), ld, F, T, 22050, hop, n_fft, 1, 0,
+                                    C.c_void_p(dZ.data_ptr()), C.c_void_p(torch.cuda.current_stream().cuda_stream))
+            assert s == 0
+            torch.cuda.synchronize()
+        finally:
+            lib.mm_destroy(ctx)
+        assert np.array_equal(dZ.cpu().numpy(), host)
