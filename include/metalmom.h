@@ -78,6 +78,22 @@ int32_t mm_istft(mm_context ctx, const float* stft_data, int64_t stft_count,
                  int32_t hop_length, int32_t win_length, int32_t center,
                  int64_t output_length, MMBuffer* out);
 
+/* Replaces Bridge.swift:1319-1358 (mm_onset_strength), reference
+ * metalmom.h:170-173: onset strength envelope (OnsetDetection.swift:23-128):
+ * power mel spectrogram (win = n_fft), power_to_db(ref 1, amin 1e-10,
+ * top_db 80) over the whole matrix, lag-1 half-wave rectified difference,
+ * mean over the bands (aggregate != 0: [1, frames]) or per band
+ * ([n_mels, frames]), 1 + n_fft/(2 hop) leading zeros with center (1
+ * without), cut to the frame count with center. hop <= 0 -> 512, f_max <= 0
+ * -> sr/2. A clip of at most one frame (or a non-power-of-two n_fft) gives
+ * one zero column. ctx may be NULL (the reference does not check it).
+ * Rejected (-1) where the reference would divide by zero or produce NaN:
+ * sample_rate <= 0, n_mels < 1. */
+int32_t mm_onset_strength(mm_context ctx, const float* signal_data, int64_t signal_length,
+                          int32_t sample_rate, int32_t n_fft, int32_t hop_length,
+                          int32_t n_mels, float f_min, float f_max,
+                          int32_t center, int32_t aggregate, MMBuffer* out);
+
 /* Replaces Bridge.swift:225-263 (mm_amplitude_to_db): 20 log10(max(x, amin)/ref),
  * clipped to max - top_db when top_db > 0. 1-D output of `count` values. */
 int32_t mm_amplitude_to_db(mm_context ctx, const float* data, int64_t count,

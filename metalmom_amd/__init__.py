@@ -1,6 +1,6 @@
 """metalmom_amd: MetalMom's spectral front end on AMD Instinct MI355X (gfx950).
 
-Drop-in for the reference hot path: ``stft``, ``istft``,
+Drop-in for the reference hot path: ``stft``, ``istft``, ``onset_strength``,
 ``feature.melspectrogram``, ``feature.mfcc``, ``feature.power_to_db`` /
 ``amplitude_to_db`` keep the
 signatures of ``metalmom`` (zakkeown/MetalMom ``python/metalmom``) and call
@@ -13,3 +13,4 @@ __version__ = "0.1.0"
 from .core import stft, stft_complex, istft, spectral  # noqa: F401
 from .feature import (amplitude_to_db, power_to_db, melspectrogram, mfcc)  # noqa: F401
 from . import feature  # noqa: F401
+from .onset import onset_strength  # noqa: F401

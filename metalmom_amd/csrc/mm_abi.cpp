@@ -5,6 +5,7 @@
 //   mm_init/mm_destroy registry      :16-40
 //   mm_stft                          :44-140
 //   mm_istft                         :144-221
+//   mm_onset_strength                :1319-1358
 //   mm_amplitude_to_db               :225-263
 //   mm_power_to_db                   :267-306
 //   mm_mel_spectrogram               :310-400
@@ -227,6 +228,52 @@ int32_t mm_istft(mm_context ctx, const float* stft_data, int64_t stft_count, int
   const int32_t s = run_istft(lane, sh, sample_rate, win_length, stft_data, &data);
   if (s != OK) return s;
   fill_shape(out, {sh.out_len}, 0, sh.out_len, data);
+  return OK;
+}
+
+int32_t mm_onset_strength(mm_context ctx, const float* signal_data, int64_t signal_length, int32_t sample_rate,
+                          int32_t n_fft, int32_t hop_length, int32_t n_mels, float f_min, float f_max, int32_t center,
+                          int32_t aggregate, MMBuffer* out) {
+  // Bridge.swift:1334-1336 checks the signal, its length and out, not the context
+  if (!signal_data || signal_length <= 0 || !out) return MM_ERR_INVALID_INPUT;
+  if (sample_rate <= 0 || n_mels < 1) return MM_ERR_INVALID_INPUT;
+  SpecRequest r;
+  r.out_kind = OUT_MEL;
+  r.sr = sample_rate;
+  r.n_fft = n_fft;
+  r.hop = hop_length > 0 ? hop_length : 512;   // OnsetDetection.swift:38 (not n_fft/4)
+  r.win = n_fft;
+  r.center = center != 0;
+  r.power = 2.0f;
+  r.n_mels = n_mels;
+  r.fmin = f_min;
+  r.fmax = f_max > 0.0f ? f_max : (float)sample_rate / 2.0f;
+  const int agg = aggregate != 0;
+  // no STFT frames (a non-power-of-two n_fft) or a single frame: one zero
+  // column (OnsetDetection.swift:63-71), nothing computed
+  if (!pow2(n_fft) || frame_count(signal_length, n_fft, r.hop, r.center) <= 1) {
+    const int64_t rows = agg ? 1 : n_mels;
+    float* z = static_cast<float*>(std::calloc((size_t)rows, sizeof(float)));
+    if (!z) return MM_ERR_ALLOC_FAILED;
+    fill_shape(out, {rows, 1}, 0, rows, z);
+    return OK;
+  }
+  Lane* lane = nullptr;
+  if (ctx && is_live(ctx)) lane = static_cast<Ctx*>(ctx)->get();
+  Engine* e = Engine::get();
+  if (!e) return ERR_NO_GPU;
+  bool borrowed = false;
+  if (!lane) {
+    lane = e->acquire_lane();
+    borrowed = true;
+    if (!lane) return ERR_INTERNAL;
+  }
+  float* data = nullptr;
+  int64_t rows = 0, cols = 0;
+  const int32_t s = run_onset(lane, r, signal_data, signal_length, agg, 1, &data, &rows, &cols);
+  if (borrowed) e->release_lane(lane);
+  if (s != OK) return s;
+  fill_shape(out, {rows, cols}, 0, rows * cols, data);
   return OK;
 }
 

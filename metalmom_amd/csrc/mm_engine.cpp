@@ -665,6 +665,68 @@ int32_t run_single(Lane* lane, const SpecRequest& r, const float* signal, int64_
   return OK;
 }
 
+int32_t run_onset(Lane* lane, const SpecRequest& r, const float* signal, int64_t len, int aggregate, int lag,
+                  float** out, int64_t* rows, int64_t* cols) {
+  *out = nullptr;
+  const int T = (int)frame_count(len, r.n_fft, r.hop, r.center);
+  const int bands = r.n_mels;
+  *rows = aggregate ? 1 : bands;
+  if (T <= lag) {   // OnsetDetection.swift:63-71: one zero column
+    *cols = 1;
+    float* z = static_cast<float*>(std::calloc((size_t)*rows, sizeof(float)));
+    if (!z) return ERR_ALLOC;
+    *out = z;
+    return OK;
+  }
+  const int pad = lag + (r.center ? r.n_fft / (2 * r.hop) : 0);
+  const int n_out = r.center ? std::min(T - lag + pad, T) : T - lag + pad;
+  *cols = n_out;
+  Engine* eng = Engine::get();
+  if (!eng) return ERR_NO_GPU;
+  hipStream_t st = lane->stream;
+  const Tables* tb = nullptr;
+  int32_t s = eng->tables_for(r, st, &tb);
+  if (s != OK) return s;
+  TileLayout L;
+  const bool fused = tile_layout(r.n_fft, r.out_kind, r.precision, r.n_mels, 0, &L);
+  BatchMeta m;
+  build_meta(r, &len, 1, fused ? L.tile_frames : kTileFrames, &m);
+  // every buffer is sized before the first launch: d_in = signal | max key,
+  // d_out = mel (then its dB, in place) | result
+  const size_t in_bytes = (size_t)len * 4, key_off = (in_bytes + 255) & ~(size_t)255;
+  const size_t mel_bytes = (size_t)m.total_out * 4, res_off = (mel_bytes + 255) & ~(size_t)255;
+  const size_t res_bytes = (size_t)*rows * n_out * 4;
+  const size_t mb = meta_bytes(m);
+  MM_HIP(lane->h_in.ensure(in_bytes + mb));
+  MM_HIP(lane->d_in.ensure(key_off + 256));
+  MM_HIP(lane->d_meta.ensure(mb));
+  MM_HIP(lane->d_out.ensure(res_off + res_bytes));
+  MM_HIP(lane->h_out.ensure(res_bytes));
+  std::memcpy(lane->h_in.p, signal, in_bytes);
+  pack_meta(m, static_cast<char*>(lane->h_in.p) + in_bytes);
+  MM_HIP(hipMemcpyAsync(lane->d_in.p, lane->h_in.p, in_bytes, hipMemcpyHostToDevice, st));
+  MM_HIP(hipMemcpyAsync(lane->d_meta.p, static_cast<char*>(lane->h_in.p) + in_bytes, mb, hipMemcpyHostToDevice, st));
+  const MetaPtrs mp = meta_ptrs(m, lane->d_meta.p);
+  s = enqueue_batch(eng, r, tb, m, mp, lane->d_in.p, lane->d_out.p, &lane->d_scratch, &lane->d_scratch2, 0, st);
+  if (s != OK) {
+    (void)hipStreamSynchronize(st);
+    return s;
+  }
+  float* mel = static_cast<float*>(lane->d_out.p);
+  float* res = reinterpret_cast<float*>(static_cast<char*>(lane->d_out.p) + res_off);
+  unsigned* key = reinterpret_cast<unsigned*>(static_cast<char*>(lane->d_in.p) + key_off);
+  MM_HIP(launch_fill_u32(key, 1, float_to_key(-INFINITY), st));
+  MM_HIP(launch_db(mel, (int64_t)bands * T, 10.0f, 1.0f, 1e-10f, mel, key, st));   // Scaling.swift:73-114
+  MM_HIP(launch_onset(mel, bands, T, key, 80.0f, lag, pad, n_out, aggregate, res, st));
+  MM_HIP(hipMemcpyAsync(lane->h_out.p, res, res_bytes, hipMemcpyDeviceToHost, st));
+  MM_HIP(hipStreamSynchronize(st));
+  float* o = static_cast<float*>(std::malloc(res_bytes));
+  if (!o) return ERR_ALLOC;
+  std::memcpy(o, lane->h_out.p, res_bytes);
+  *out = o;
+  return OK;
+}
+
 int32_t run_db(Lane* lane, const float* data, int64_t n, float mult, float ref, float amin, float top_db,
                float** out) {
   Engine* eng = Engine::get();

@@ -156,6 +156,13 @@ int32_t run_single(Lane* lane, const SpecRequest& r, const float* signal, int64_
 int32_t run_db(Lane* lane, const float* data, int64_t n, float mult, float ref, float amin, float top_db,
                float** out);
 
+// Onset strength of one clip (OnsetDetection.swift:23-128): r is the power-2
+// mel request (win = n_fft); aggregate -> [1, frames], else [n_mels, frames].
+// One zero column when the clip has at most `lag` frames. Synchronous; *out
+// is malloc'd.
+int32_t run_onset(Lane* lane, const SpecRequest& r, const float* signal, int64_t len, int aggregate, int lag,
+                  float** out, int64_t* rows, int64_t* cols);
+
 // ---- inverse STFT ----
 // Output geometry of mm_istft (STFT.swift:703-704, 720-724, 832-858).
 struct IstftShape {

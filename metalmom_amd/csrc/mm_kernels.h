@@ -54,6 +54,12 @@ hipError_t launch_db_clamp(float* x, int64_t n, const unsigned* max_key, float t
 hipError_t launch_generic_dct(const float* logmel, int n_mels, int T, const float* dct, int n_mfcc, float* out,
                               hipStream_t st);
 
+// Onset strength from the dB mel spectrogram db [n_bands, T] (before the
+// top_db floor) and its maximum key: out [n_out] (aggregate) or
+// [n_bands, n_out]; output frame o holds the flux at i = o - pad.
+hipError_t launch_onset(const float* db, int n_bands, int T, const unsigned* max_key, float top_db, int lag, int pad,
+                        int n_out, int aggregate, float* out, hipStream_t st);
+
 // ---- inverse STFT ----
 struct IstftArgs {
   const float2* X;       // [n_fft/2 + 1][ld] complex, row-major (the reference's interleaved floats at ld = T)

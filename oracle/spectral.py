@@ -18,7 +18,7 @@ __all__ = [
     "stft_complex", "stft_magnitude", "power_spectrogram", "istft_length", "istft",
     "hz_to_mel", "mel_to_hz", "mel_filterbank", "melspectrogram",
     "linear_to_db", "power_to_db", "amplitude_to_db",
-    "dct_ortho_basis", "mfcc", "mfcc_from_logmel",
+    "dct_ortho_basis", "mfcc", "mfcc_from_logmel", "onset_strength",
 ]
 
 
@@ -344,6 +344,37 @@ def mfcc(y, sr=22050, n_mfcc=20, n_fft=2048, hop=512, win_length=None, n_mels=12
 
 
 # How each oracle function is pinned (checked by tests/test_oracle.py).
+# ---------------------------------------------------------------------------
+# Onset strength (spectral flux of the mel dB spectrogram)
+# ---------------------------------------------------------------------------
+
+def onset_strength(y, sr=22050, n_fft=2048, hop=512, n_mels=128, fmin=0.0, fmax=None, center=True,
+                   aggregate=True, lag=1) -> np.ndarray:
+    """Onset strength envelope, the output of ``mm_onset_strength``.
+
+    Restates ``Rhythm/OnsetDetection.swift:23-128``: power mel spectrogram
+    (``win = n_fft``), ``power_to_db(ref 1, amin 1e-10, top_db 80)`` over the
+    whole matrix, ``max(0, S[:, t] - S[:, t - lag])``, averaged over the bands
+    (``aggregate``) or kept per band, then ``lag + n_fft // (2 hop)`` (centre)
+    or ``lag`` zeros in front and, with ``center``, cut to the frame count.
+    With ``n_frames <= lag`` the result is a single zero column. Shape
+    ``[1, frames]`` or ``[n_mels, frames]``.
+    """
+    mel = melspectrogram(y, sr, n_fft, hop, n_fft, center, 2.0, n_mels, fmin, fmax)
+    n_bands, T = mel.shape
+    if T <= lag:
+        return np.zeros((1 if aggregate else n_bands, 1))
+    db = power_to_db(mel, 1.0, 1e-10, 80.0)
+    flux = np.maximum(0.0, db[:, lag:] - db[:, :-lag])
+    pad = lag + (n_fft // (2 * hop) if center else 0)
+    n_out = min(T - lag + pad, T) if center else T - lag + pad
+    env = flux.mean(axis=0, keepdims=True) if aggregate else flux
+    out = np.zeros((env.shape[0], n_out))
+    m = min(env.shape[1], n_out - pad)
+    out[:, pad:pad + m] = env[:, :m]
+    return out
+
+
 PINS = {
     "hann_periodic": "known answer hann(4) = [0, .5, 1, .5] (WindowTests.swift:60-66); closed form",
     "prepare_window": "centre zero-pad / centre truncation indices checked by hand on small cases",
@@ -358,4 +389,6 @@ PINS = {
     "amplitude_to_db": "goldens stft_440hz_amplitude_db{,_ref1}.npy",
     "dct_ortho_basis": "scipy.fft.dct(type=2, norm='ortho'); orthonormality",
     "mfcc": "goldens mfcc_440hz_*.npy",
+    "onset_strength": "golden onset_strength_440hz_default.npy (librosa.onset.onset_strength); silence -> 0; "
+                      "non-negative; shape / padding rules; planted mistakes (no rectification, pad width) fail",
 }

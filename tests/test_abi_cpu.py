@@ -149,3 +149,29 @@ def test_istft_validation_and_host_only_results():
     s = lib.mm_istft(ctx, f1.ctypes.data_as(fp), f1.size, 1025, 1, 22050, 512, 2048, 1, 0, C.byref(out))
     assert s == 0 and out.ndim == 1 and out.count == 0 and not out.data and out.shape[0] == 0
     lib.mm_destroy(ctx)
+
+
+def test_onset_strength_validation_and_zero_columns():
+    """Bridge.swift:1334-1336 checks the signal and out only (ctx may be NULL);
+    sr <= 0 and n_mels < 1 are rejected; no STFT frame or a single frame gives
+    one zero column (OnsetDetection.swift:63-71) without any GPU work."""
+    lib = _lib()
+    from metalmom_amd._native import MMBuffer
+    fp = C.POINTER(C.c_float)
+    y = np.ones(4096, np.float32)
+    out = MMBuffer()
+    assert lib.mm_onset_strength(None, None, 4096, 22050, 2048, 512, 128, 0.0, 0.0, 1, 1, C.byref(out)) == -1
+    assert lib.mm_onset_strength(None, y.ctypes.data_as(fp), 0, 22050, 2048, 512, 128, 0.0, 0.0, 1, 1,
+                                 C.byref(out)) == -1
+    assert lib.mm_onset_strength(None, y.ctypes.data_as(fp), 4096, 22050, 2048, 512, 128, 0.0, 0.0, 1, 1, None) == -1
+    assert lib.mm_onset_strength(None, y.ctypes.data_as(fp), 4096, 0, 2048, 512, 128, 0.0, 0.0, 1, 1,
+                                 C.byref(out)) == -1
+    assert lib.mm_onset_strength(None, y.ctypes.data_as(fp), 4096, 22050, 2048, 512, 0, 0.0, 0.0, 1, 1,
+                                 C.byref(out)) == -1
+    for n_fft, n, center, agg, rows in [(1000, 4096, 1, 1, 1), (1000, 4096, 1, 0, 128), (2048, 2048, 0, 1, 1),
+                                        (2048, 100, 0, 0, 128)]:
+        s = lib.mm_onset_strength(None, y.ctypes.data_as(fp), n, 22050, n_fft, 512, 128, 0.0, 0.0, center, agg,
+                                  C.byref(out))
+        assert s == 0 and out.ndim == 2 and out.shape[0] == rows and out.shape[1] == 1 and out.count == rows
+        assert not np.ctypeslib.as_array(out.data, shape=(rows,)).any()
+        lib.mm_buffer_free(C.byref(out))

@@ -294,3 +294,40 @@ def test_istft_planted_mistakes_are_caught():
         outs.append(out[N // 2:len(out) - N // 2])
     assert np.abs(outs[0] - ref).max() < 1e-12
     assert np.abs(outs[1] - ref).max() > 1e-3 and np.abs(outs[2] - ref).max() > 1e-3
+
+
+# -- onset strength (OnsetDetection.swift:23-128) ------------------------------
+
+def test_onset_strength_golden(signal_440):
+    G = load_golden("onset_strength_440hz_default")          # librosa.onset.onset_strength
+    env = o.onset_strength(signal_440, 22050)
+    assert env.shape == (1, 44)
+    assert np.abs(env[0] - G).max() <= 1e-6 * max(np.abs(G).max(), 1.0)
+
+
+def test_onset_strength_known_answers():
+    rng = np.random.default_rng(2)
+    y = rng.standard_normal(30000)
+    env = o.onset_strength(y)
+    T = o.frame_count(len(y), 2048, 512, True)
+    assert env.shape == (1, T) and (env >= 0).all() and not env[0, :3].any()   # 1 + 2048 // 1024 zeros
+    per = o.onset_strength(y, aggregate=False)
+    assert per.shape == (128, T) and np.allclose(per.mean(axis=0), env[0])
+    assert not o.onset_strength(np.zeros(22050)).any()                  # silence (test_onset_strength.py:23-26)
+    un = o.onset_strength(y, center=False)
+    assert un.shape == (1, o.frame_count(len(y), 2048, 512, False)) and not un[0, 0]
+    assert o.onset_strength(np.ones(100), center=False).shape == (1, 1)   # one frame or none: a zero column
+    assert o.onset_strength(y, hop=300).shape == (1, o.frame_count(len(y), 2048, 300, True))
+
+
+def test_onset_strength_planted_mistakes_are_caught(signal_440):
+    """No half-wave rectification, or a pad without the centre term, fails
+    the golden."""
+    G = load_golden("onset_strength_440hz_default")
+    db = o.power_to_db(o.melspectrogram(signal_440, 22050))
+    raw = (db[:, 1:] - db[:, :-1]).mean(axis=0)
+    no_rect = np.concatenate([np.zeros(3), raw])[:44]
+    rect = np.maximum(0, db[:, 1:] - db[:, :-1]).mean(axis=0)
+    short_pad = np.concatenate([np.zeros(1), rect])[:44]
+    assert np.abs(np.concatenate([np.zeros(3), rect])[:44] - G).max() < 1e-6
+    assert np.abs(no_rect - G).max() > 1e-3 and np.abs(short_pad - G).max() > 1e-3
