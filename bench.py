@@ -10,8 +10,9 @@ HBM before the timed region starts.
 Also reported on the same batch: the FP32 batched path and the plain path (one
 clip per launch, FP32, no graph, clips in sequence), the fused kernel's
 roofline position, and the float32 NumPy restatement of the reference's CPU
-path timed on this host (bounded sample). The ``istft`` entry measures the
-inverse STFT of one clip of the same total length on device-resident data.
+path timed on this host (bounded sample). The ``istft`` and
+``onset_strength`` entries measure the inverse STFT and the onset strength of
+one clip of the same total length on device-resident data.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -173,6 +174,73 @@ def istft_leg(torch, n_samples, iters, with_cpu):
     return res
 
 
+def onset_leg(torch, n_samples, iters, with_cpu=False):
+    """Onset strength (mm_onset_strength_device) of one long clip on
+    device-resident data: n_fft 2048, hop 512, 128 mels, centred, aggregated;
+    the signal is the uneven batch's clips joined end to end. Time from HIP
+    events on the launch stream; algorithmic bytes are the samples read once
+    plus the envelope written."""
+    import ctypes as C
+    from metalmom_amd._native import lib
+    from metalmom_amd.workloads import uneven_batch
+    y = np.concatenate(uneven_batch())[:n_samples]
+    X = torch.from_numpy(y).cuda()
+    T = 1 + len(y) // 512
+    Y = torch.empty(T, device="cuda")
+    rows, cols = C.c_int64(0), C.c_int64(0)
+    st = torch.cuda.Stream()
+    ctx = lib.mm_init()
+
+    def run():
+        s = lib.mm_onset_strength_device(ctx, C.c_void_p(X.data_ptr()), len(y), 22050, 2048, 512, 128, 0.0, 0.0,
+                                         1, 1, C.c_void_p(Y.data_ptr()), C.byref(rows), C.byref(cols),
+                                         C.c_void_p(st.cuda_stream))
+        if s != 0:
+            raise RuntimeError(f"mm_onset_strength_device failed with status {s}")
+
+    run()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(iters):
+        run()
+    e1.record(st)
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / iters
+    lib.mm_destroy(ctx)
+    del X, Y
+    bytes_launch = len(y) * 4 + cols.value * 4
+    gbs = bytes_launch / (ms * 1e-3) / 1e9
+    cpu = None
+    if with_cpu:
+        from oracle.cpu_baseline import OnsetF32
+        try:
+            from threadpoolctl import threadpool_limits
+            limiter = threadpool_limits(limits=1)
+        except Exception:
+            limiter = None
+        f = OnsetF32()
+        f(y[:22050])
+        sample = y[:20_000_000]
+        t0 = time.perf_counter()
+        f(sample)
+        dt = time.perf_counter() - t0
+        if limiter is not None and hasattr(limiter, "unregister"):
+            limiter.unregister()
+        fr = 1 + len(sample) // 512
+        cpu = {"value": fr / dt, "unit": "frames/s", "cores": 1, "kind": "port",
+               "sample": f"first {len(sample)} samples of the clip ({fr} frames), float32 numpy rfft + dense "
+                         f"FB@P + dB + flux, {dt:.1f} s, 1 thread", "host_cpus": os.cpu_count()}
+    res = {"workload": f"one clip of {len(y)} samples (the uneven batch joined), n_fft 2048, hop 512, 128 mels, "
+                        f"centred, aggregated: fused mel kernel + dB with the clip maximum + flux kernel",
+            "frames": T, "step_ms": ms, "frames_per_s": T / (ms * 1e-3),
+            "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": gbs / HBM_PEAK_GBS, "bytes_per_launch": bytes_launch}}
+    if cpu is not None:
+        res["cpu_baseline"] = cpu
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -187,6 +255,7 @@ def main():
     ap.add_argument("--grid", type=int, default=0,
                     help="persistent workgroups per CU (skips the autotuner; 0 = autotune)")
     ap.add_argument("--no-istft", action="store_true", help="skip the inverse-STFT leg")
+    ap.add_argument("--no-onset", action="store_true", help="skip the onset-strength leg")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="profiling aid: also run one device-to-device copy of this many bytes, "
                          "so HBM counters can be scaled by a kernel of known traffic")
@@ -328,6 +397,13 @@ def main():
         log(f"istft: {ist['frames_per_s'] / 1e6:.1f} M frames/s, kernel {ist['kernel_ms']:.3f} ms, "
             f"{ist['roofline']['achieved']:.0f} GB/s algorithmic ({100 * ist['roofline']['frac']:.1f} % of HBM)"
             + (f"; cpu {ist['cpu_baseline']['value']:.0f} frames/s" if "cpu_baseline" in ist else ""))
+
+    if not args.no_onset:
+        ons = onset_leg(torch, B.total_input, 3 if args.profile else 20, not args.no_cpu and not args.profile)
+        extra["onset_strength"] = ons
+        log(f"onset strength: {ons['frames_per_s'] / 1e6:.1f} M frames/s, {ons['step_ms']:.3f} ms per clip, "
+            f"{ons['roofline']['achieved']:.0f} GB/s algorithmic"
+            + (f"; cpu {ons['cpu_baseline']['value']:.0f} frames/s" if "cpu_baseline" in ons else ""))
 
     cpu = None
     if not args.no_cpu and not args.profile:

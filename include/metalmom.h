@@ -170,6 +170,21 @@ typedef struct {
 int32_t mm_stft_complex(mm_context ctx, const float* signal_data, int64_t signal_length,
                         int32_t sample_rate, const MMSTFTParams* params, MMBuffer* out);
 
+/* mm_onset_strength on device memory: d_signal holds signal_length samples,
+ * d_out receives the [rows, cols] result (rows 1 or n_mels, cols frames;
+ * both reported through out_rows / out_cols, which may be NULL; size d_out
+ * for n_mels x (frames of the clip) floats when unsure). Enqueued on `stream`
+ * without a host synchronisation after the first call for a clip length and
+ * parameter set (which uploads the clip's work list and, on first use of a
+ * configuration, builds its tables). Needs a live context (it holds the mel
+ * spectrogram in between); calls on one context must not overlap. A
+ * non-power-of-two n_fft is rejected (-1). */
+int32_t mm_onset_strength_device(mm_context ctx, const float* d_signal, int64_t signal_length,
+                                 int32_t sample_rate, int32_t n_fft, int32_t hop_length,
+                                 int32_t n_mels, float f_min, float f_max, int32_t center,
+                                 int32_t aggregate, float* d_out, int64_t* out_rows,
+                                 int64_t* out_cols, void* stream);
+
 /* Number of samples mm_istft returns for this shape, or -1 if it would
  * reject it. */
 int64_t mm_istft_length(int32_t n_freqs, int32_t n_frames, int32_t hop_length, int32_t center,

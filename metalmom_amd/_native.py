@@ -71,6 +71,8 @@ SIGNATURES = {
     "mm_istft": (_I32, [_P, _F, _I64, _I32, _I32, _I32, _I32, _I32, _I32, _I64, _BUF]),
     "mm_onset_strength": (_I32, [_P, _F, _I64, _I32, _I32, _I32, _I32, C.c_float, C.c_float, _I32, _I32, _BUF]),
     "mm_istft_length": (_I64, [_I32, _I32, _I32, _I32, _I64]),
+    "mm_onset_strength_device": (_I32, [_P, _P, _I64, _I32, _I32, _I32, _I32, C.c_float, C.c_float, _I32, _I32,
+                                        _P, C.POINTER(_I64), C.POINTER(_I64), _P]),
     "mm_istft_device": (_I32, [_P, _P, _I32, _I32, _I32, _I32, _I32, _I32, _I32, _I64, _P, _P]),
     "mm_spectral": (_I32, [_P, C.POINTER(MMSpecParams), _F, _I64, _BUF]),
     "mm_spectral_batch": (_I32, [_P, C.POINTER(MMSpecParams), C.POINTER(_F), C.POINTER(_I64), _I32,

@@ -170,6 +170,31 @@ int32_t mm_stft(mm_context ctx, const float* signal_data, int64_t signal_length,
   return run_to_buffer(ctx, r, signal_data, signal_length, out);
 }
 
+int32_t mm_onset_strength_device(mm_context ctx, const float* d_signal, int64_t signal_length, int32_t sample_rate,
+                                 int32_t n_fft, int32_t hop_length, int32_t n_mels, float f_min, float f_max,
+                                 int32_t center, int32_t aggregate, float* d_out, int64_t* out_rows,
+                                 int64_t* out_cols, void* stream) {
+  if (!ctx || !is_live(ctx) || !d_signal || signal_length <= 0 || !d_out) return MM_ERR_INVALID_INPUT;
+  if (sample_rate <= 0 || n_mels < 1 || !pow2(n_fft)) return MM_ERR_INVALID_INPUT;
+  SpecRequest r;
+  r.out_kind = OUT_MEL;
+  r.sr = sample_rate;
+  r.n_fft = n_fft;
+  r.hop = hop_length > 0 ? hop_length : 512;
+  r.win = n_fft;
+  r.center = center != 0;
+  r.power = 2.0f;
+  r.n_mels = n_mels;
+  r.fmin = f_min;
+  r.fmax = f_max > 0.0f ? f_max : (float)sample_rate / 2.0f;
+  const OnsetShape o = onset_shape(r, signal_length, aggregate != 0, 1);
+  if (out_rows) *out_rows = o.rows;
+  if (out_cols) *out_cols = o.cols;
+  Lane* lane = static_cast<Ctx*>(ctx)->get();
+  if (!lane) return ERR_NO_GPU;
+  return enqueue_onset(lane, r, d_signal, signal_length, aggregate != 0, 1, d_out, static_cast<hipStream_t>(stream));
+}
+
 int64_t mm_istft_length(int32_t n_freqs, int32_t n_frames, int32_t hop_length, int32_t center,
                         int64_t output_length) {
   IstftShape sh;

@@ -101,6 +101,7 @@ int32_t create_batch(const MMSpecParams* params, const int64_t* lengths, int32_t
   build_meta(b->r, lengths, n_clips, fused ? L.tile_frames : kTileFrames, &b->m);
   const size_t mb = meta_bytes(b->m);
   if (b->lane->d_meta.ensure(mb) != hipSuccess || b->lane->h_meta.ensure(mb) != hipSuccess) return ERR_ALLOC;
+  b->lane->meta_key.clear();
   pack_meta(b->m, b->lane->h_meta.p);
   if (hipMemcpyAsync(b->lane->d_meta.p, b->lane->h_meta.p, mb, hipMemcpyHostToDevice, b->lane->stream) !=
           hipSuccess ||

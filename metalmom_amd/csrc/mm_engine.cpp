@@ -92,6 +92,7 @@ void HostBuf::release() {
   n = 0;
 }
 Lane::~Lane() {
+  d_small.release();
   d_in.release();
   d_out.release();
   d_meta.release();
@@ -262,6 +263,7 @@ Lane* Engine::acquire_lane() {
     if (!free_lanes_.empty()) {
       Lane* l = free_lanes_.back();
       free_lanes_.pop_back();
+      l->meta_key.clear();   // the previous user may have left another work list in d_meta
       return l;
     }
   }
@@ -644,6 +646,7 @@ int32_t run_single(Lane* lane, const SpecRequest& r, const float* signal, int64_
   MM_HIP(lane->h_in.ensure(in_bytes + mb));
   MM_HIP(lane->d_in.ensure(in_bytes));
   MM_HIP(lane->d_meta.ensure(mb));
+  lane->meta_key.clear();   // d_meta now holds this call's work list
   MM_HIP(lane->d_out.ensure(out_bytes));
   MM_HIP(lane->h_out.ensure(out_bytes));
   std::memcpy(lane->h_in.p, signal, in_bytes);
@@ -665,25 +668,28 @@ int32_t run_single(Lane* lane, const SpecRequest& r, const float* signal, int64_
   return OK;
 }
 
-int32_t run_onset(Lane* lane, const SpecRequest& r, const float* signal, int64_t len, int aggregate, int lag,
-                  float** out, int64_t* rows, int64_t* cols) {
-  *out = nullptr;
-  const int T = (int)frame_count(len, r.n_fft, r.hop, r.center);
-  const int bands = r.n_mels;
-  *rows = aggregate ? 1 : bands;
-  if (T <= lag) {   // OnsetDetection.swift:63-71: one zero column
-    *cols = 1;
-    float* z = static_cast<float*>(std::calloc((size_t)*rows, sizeof(float)));
-    if (!z) return ERR_ALLOC;
-    *out = z;
+OnsetShape onset_shape(const SpecRequest& r, int64_t len, int aggregate, int lag) {
+  OnsetShape o;
+  o.T = (int)frame_count(len, r.n_fft, r.hop, r.center);
+  o.rows = aggregate ? 1 : r.n_mels;
+  if (o.T <= lag) {   // OnsetDetection.swift:63-71: one zero column
+    o.cols = 1;
+    return o;
+  }
+  o.pad = lag + (r.center ? r.n_fft / (2 * r.hop) : 0);
+  o.cols = r.center ? std::min(o.T - lag + o.pad, o.T) : o.T - lag + o.pad;
+  return o;
+}
+
+int32_t enqueue_onset(Lane* lane, const SpecRequest& r, const float* d_x, int64_t len, int aggregate, int lag,
+                      float* d_res, hipStream_t st) {
+  const OnsetShape o = onset_shape(r, len, aggregate, lag);
+  if (o.T <= lag) {
+    MM_HIP(launch_fill_u32(reinterpret_cast<unsigned*>(d_res), o.rows, 0u, st));
     return OK;
   }
-  const int pad = lag + (r.center ? r.n_fft / (2 * r.hop) : 0);
-  const int n_out = r.center ? std::min(T - lag + pad, T) : T - lag + pad;
-  *cols = n_out;
   Engine* eng = Engine::get();
   if (!eng) return ERR_NO_GPU;
-  hipStream_t st = lane->stream;
   const Tables* tb = nullptr;
   int32_t s = eng->tables_for(r, st, &tb);
   if (s != OK) return s;
@@ -691,39 +697,64 @@ int32_t run_onset(Lane* lane, const SpecRequest& r, const float* signal, int64_t
   const bool fused = tile_layout(r.n_fft, r.out_kind, r.precision, r.n_mels, 0, &L);
   BatchMeta m;
   build_meta(r, &len, 1, fused ? L.tile_frames : kTileFrames, &m);
-  // every buffer is sized before the first launch: d_in = signal | max key,
-  // d_out = mel (then its dB, in place) | result
-  const size_t in_bytes = (size_t)len * 4, key_off = (in_bytes + 255) & ~(size_t)255;
-  const size_t mel_bytes = (size_t)m.total_out * 4, res_off = (mel_bytes + 255) & ~(size_t)255;
-  const size_t res_bytes = (size_t)*rows * n_out * 4;
   const size_t mb = meta_bytes(m);
-  MM_HIP(lane->h_in.ensure(in_bytes + mb));
-  MM_HIP(lane->d_in.ensure(key_off + 256));
-  MM_HIP(lane->d_meta.ensure(mb));
-  MM_HIP(lane->d_out.ensure(res_off + res_bytes));
+  // every buffer is sized before the first launch
+  MM_HIP(lane->d_out.ensure((size_t)m.total_out * 4));   // mel, then its dB in place
+  MM_HIP(lane->d_small.ensure(256));                      // maximum key
+  const std::string key = std::to_string(len) + ":" + tune_key(r, "") + ":" + std::to_string(r.center) + ":" +
+                          std::to_string(r.win) + ":" + std::to_string(r.sr) + ":" + std::to_string(r.fmin) + ":" +
+                          std::to_string(r.fmax) + ":" + std::to_string(fused ? L.tile_frames : kTileFrames);
+  if (lane->meta_key != key) {
+    MM_HIP(lane->h_meta.ensure(mb));
+    MM_HIP(lane->d_meta.ensure(mb));
+    pack_meta(m, lane->h_meta.p);
+    MM_HIP(hipMemcpyAsync(lane->d_meta.p, lane->h_meta.p, mb, hipMemcpyHostToDevice, st));
+    MM_HIP(hipStreamSynchronize(st));
+    lane->meta_key = key;
+  }
+  const MetaPtrs mp = meta_ptrs(m, lane->d_meta.p);
+  s = enqueue_batch(eng, r, tb, m, mp, d_x, lane->d_out.p, &lane->d_scratch, &lane->d_scratch2, 0, st);
+  if (s != OK) return s;
+  float* mel = static_cast<float*>(lane->d_out.p);
+  unsigned* mkey = static_cast<unsigned*>(lane->d_small.p);
+  MM_HIP(launch_fill_u32(mkey, 1, float_to_key(-INFINITY), st));
+  MM_HIP(launch_db(mel, (int64_t)r.n_mels * o.T, 10.0f, 1.0f, 1e-10f, mel, mkey, st));   // Scaling.swift:73-114
+  MM_HIP(launch_onset(mel, r.n_mels, o.T, mkey, 80.0f, lag, o.pad, o.cols, aggregate, d_res, st));
+  return OK;
+}
+
+int32_t run_onset(Lane* lane, const SpecRequest& r, const float* signal, int64_t len, int aggregate, int lag,
+                  float** out, int64_t* rows, int64_t* cols) {
+  *out = nullptr;
+  const OnsetShape o = onset_shape(r, len, aggregate, lag);
+  *rows = o.rows;
+  *cols = o.cols;
+  const size_t res_bytes = (size_t)o.rows * o.cols * 4;
+  if (o.T <= lag) {
+    float* z = static_cast<float*>(std::calloc((size_t)o.rows, sizeof(float)));
+    if (!z) return ERR_ALLOC;
+    *out = z;
+    return OK;
+  }
+  hipStream_t st = lane->stream;
+  const size_t in_bytes = (size_t)len * 4, res_off = (in_bytes + 255) & ~(size_t)255;
+  MM_HIP(lane->h_in.ensure(in_bytes));
+  MM_HIP(lane->d_in.ensure(res_off + res_bytes));   // signal | result
   MM_HIP(lane->h_out.ensure(res_bytes));
   std::memcpy(lane->h_in.p, signal, in_bytes);
-  pack_meta(m, static_cast<char*>(lane->h_in.p) + in_bytes);
   MM_HIP(hipMemcpyAsync(lane->d_in.p, lane->h_in.p, in_bytes, hipMemcpyHostToDevice, st));
-  MM_HIP(hipMemcpyAsync(lane->d_meta.p, static_cast<char*>(lane->h_in.p) + in_bytes, mb, hipMemcpyHostToDevice, st));
-  const MetaPtrs mp = meta_ptrs(m, lane->d_meta.p);
-  s = enqueue_batch(eng, r, tb, m, mp, lane->d_in.p, lane->d_out.p, &lane->d_scratch, &lane->d_scratch2, 0, st);
+  float* res = reinterpret_cast<float*>(static_cast<char*>(lane->d_in.p) + res_off);
+  const int32_t s = enqueue_onset(lane, r, static_cast<const float*>(lane->d_in.p), len, aggregate, lag, res, st);
   if (s != OK) {
     (void)hipStreamSynchronize(st);
     return s;
   }
-  float* mel = static_cast<float*>(lane->d_out.p);
-  float* res = reinterpret_cast<float*>(static_cast<char*>(lane->d_out.p) + res_off);
-  unsigned* key = reinterpret_cast<unsigned*>(static_cast<char*>(lane->d_in.p) + key_off);
-  MM_HIP(launch_fill_u32(key, 1, float_to_key(-INFINITY), st));
-  MM_HIP(launch_db(mel, (int64_t)bands * T, 10.0f, 1.0f, 1e-10f, mel, key, st));   // Scaling.swift:73-114
-  MM_HIP(launch_onset(mel, bands, T, key, 80.0f, lag, pad, n_out, aggregate, res, st));
   MM_HIP(hipMemcpyAsync(lane->h_out.p, res, res_bytes, hipMemcpyDeviceToHost, st));
   MM_HIP(hipStreamSynchronize(st));
-  float* o = static_cast<float*>(std::malloc(res_bytes));
-  if (!o) return ERR_ALLOC;
-  std::memcpy(o, lane->h_out.p, res_bytes);
-  *out = o;
+  float* h = static_cast<float*>(std::malloc(res_bytes));
+  if (!h) return ERR_ALLOC;
+  std::memcpy(h, lane->h_out.p, res_bytes);
+  *out = h;
   return OK;
 }
 

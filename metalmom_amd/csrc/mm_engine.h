@@ -62,8 +62,9 @@ struct HostBuf {
 // One stream plus its workspaces. Pooled: contexts borrow one on first use.
 struct Lane {
   hipStream_t stream = nullptr;
-  DevBuf d_in, d_out, d_meta, d_scratch, d_scratch2;
+  DevBuf d_in, d_out, d_meta, d_scratch, d_scratch2, d_small;
   HostBuf h_in, h_out, h_meta;
+  std::string meta_key;   // work list now in d_meta for device-resident single-clip calls ("" = none)
   ~Lane();
 };
 
@@ -158,8 +159,20 @@ int32_t run_db(Lane* lane, const float* data, int64_t n, float mult, float ref, 
 
 // Onset strength of one clip (OnsetDetection.swift:23-128): r is the power-2
 // mel request (win = n_fft); aggregate -> [1, frames], else [n_mels, frames].
-// One zero column when the clip has at most `lag` frames. Synchronous; *out
-// is malloc'd.
+// One zero column when the clip has at most `lag` frames.
+struct OnsetShape {
+  int T = 0;       // STFT frames
+  int pad = 0;     // leading zeros
+  int rows = 0;    // 1 (aggregate) or n_mels
+  int cols = 0;    // output frames
+};
+OnsetShape onset_shape(const SpecRequest& r, int64_t len, int aggregate, int lag);
+// Device signal d_x (len floats) -> d_res (rows x cols floats), enqueued on
+// st; the mel spectrogram and the maximum key live in the lane's buffers and
+// the clip's work list is uploaded once per (length, request).
+int32_t enqueue_onset(Lane* lane, const SpecRequest& r, const float* d_x, int64_t len, int aggregate, int lag,
+                      float* d_res, hipStream_t st);
+// Host in, host out through the lane (synchronous); *out is malloc'd.
 int32_t run_onset(Lane* lane, const SpecRequest& r, const float* signal, int64_t len, int aggregate, int lag,
                   float** out, int64_t* rows, int64_t* cols);
 

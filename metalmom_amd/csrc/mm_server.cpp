@@ -255,6 +255,7 @@ class Server {
     chk(L->h_in.ensure(s.in_bytes + s.meta_bytes_));
     chk(L->d_in.ensure(s.in_bytes));
     chk(L->d_meta.ensure(s.meta_bytes_));
+    L->meta_key.clear();
     chk(L->d_out.ensure(s.out_bytes));
     chk(L->h_out.ensure(s.out_bytes));
     if (st_code == OK) {

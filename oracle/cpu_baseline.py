@@ -79,3 +79,29 @@ class IstftF32:
         m = max(min(n, total - start), 0)
         res[:m] = out[start:start + m]
         return res
+
+
+class OnsetF32:
+    """Float32 onset strength (``Rhythm/OnsetDetection.swift:23-128``): power
+    mel spectrogram, whole-matrix ``power_to_db(top_db 80)``, lag-1 half-wave
+    rectified difference averaged over the bands, centre padding."""
+
+    def __init__(self, sr=22050, n_fft=2048, hop=512, n_mels=128):
+        self.n_fft, self.hop = n_fft, hop
+        self.win = o.hann_periodic(n_fft).astype(np.float32)
+        self.fb = o.mel_filterbank(sr, n_fft, n_mels).astype(np.float32)
+
+    def __call__(self, y: np.ndarray) -> np.ndarray:
+        y = np.asarray(y, dtype=np.float32)
+        p = self.n_fft // 2
+        x = np.pad(y, (p, p))
+        T = 1 + (len(x) - self.n_fft) // self.hop
+        frames = np.lib.stride_tricks.as_strided(
+            x, shape=(T, self.n_fft), strides=(x.strides[0] * self.hop, x.strides[0]))
+        X = np.fft.rfft(frames * self.win, axis=1)
+        P = (X.real * X.real + X.imag * X.imag).astype(np.float32)
+        db = 10.0 * np.log10(np.maximum(self.fb @ P.T, np.float32(1e-10)))
+        db = np.maximum(db, db.max() - np.float32(80.0))
+        flux = np.maximum(np.float32(0), db[:, 1:] - db[:, :-1]).mean(axis=0)
+        pad = 1 + self.n_fft // (2 * self.hop)
+        return np.concatenate([np.zeros(pad, np.float32), flux])[:T]

@@ -68,3 +68,36 @@ def test_onset_strength_silence_single_frame_and_repeatability():
     assert one.shape == (1,) and one[0] == 0
     y = _clip(200000, 22050, 5)
     assert np.array_equal(mm.onset_strength(y=y), mm.onset_strength(y=y))
+
+
+def test_onset_device_entry_matches_host_entry():
+    """mm_onset_strength_device on a torch tensor equals the host entry
+    bitwise, also after another call on the same context replaced the
+    clip work list it caches."""
+    import ctypes as C
+    import torch
+    import metalmom_amd as mm
+    from metalmom_amd._native import MMBuffer, MMSTFTParams, lib
+    y = _clip(300000, 22050, 11)
+    host = mm.onset_strength(y=y)
+    d_y = torch.from_numpy(y).cuda()
+    d_o = torch.full((128 * 600,), float("nan"), device="cuda")
+    rows, cols = C.c_int64(0), C.c_int64(0)
+    st = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    ctx = lib.mm_init()
+    try:
+        def run():
+            s = lib.mm_onset_strength_device(ctx, C.c_void_p(d_y.data_ptr()), len(y), 22050, 2048, 512, 128, 0.0, 0.0,
+                                             1, 1, C.c_void_p(d_o.data_ptr()), C.byref(rows), C.byref(cols), st)
+            assert s == 0
+            torch.cuda.synchronize()
+            return d_o[:rows.value * cols.value].cpu().numpy().copy()
+        a = run()
+        assert (rows.value, cols.value) == (1, host.size) and np.array_equal(a, host)
+        out = MMBuffer()   # a host STFT on the same context rewrites its work-list buffer
+        assert lib.mm_stft(ctx, y.ctypes.data_as(C.POINTER(C.c_float)), 5000, 22050,
+                           C.byref(MMSTFTParams(2048, 512, 2048, 1)), C.byref(out)) == 0
+        lib.mm_buffer_free(C.byref(out))
+        assert np.array_equal(run(), host)
+    finally:
+        lib.mm_destroy(ctx)
