@@ -112,6 +112,18 @@ int32_t mm_mel_spectrogram(mm_context ctx, const float* signal_data, int64_t sig
                            int32_t n_mels, float f_min, float f_max,
                            MMBuffer* out);
 
+/* Replaces Bridge.swift:494-545 (mm_chroma_stft), reference metalmom.h:69-72:
+ * chroma filterbank (Chroma.swift:158-275: Gaussian pitch-class bumps, L2
+ * column norm, octave weighting around octave 5, rows starting at C, tuning
+ * 0) times the power spectrogram, [n_chroma, n_frames], no frame
+ * normalisation. hop <= 0 -> n_fft/4, win <= 0 -> n_fft; a non-power-of-two
+ * n_fft gives [n_chroma, 0]. Rejected (-1): n_chroma outside 1..256 (no
+ * kernel), sample_rate <= 0. */
+int32_t mm_chroma_stft(mm_context ctx, const float* signal_data, int64_t signal_length,
+                       int32_t sample_rate, int32_t n_fft, int32_t hop_length,
+                       int32_t win_length, int32_t n_chroma, int32_t center,
+                       MMBuffer* out);
+
 /* Replaces Bridge.swift:402-493 (mm_mfcc): [min(n_mfcc, n_mels), n_frames];
  * power 2 mel, power_to_db(ref 1, amin 1e-10, top_db 80), DCT-II ortho. */
 int32_t mm_mfcc(mm_context ctx, const float* signal_data, int64_t signal_length,

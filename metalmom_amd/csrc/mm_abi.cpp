@@ -6,6 +6,7 @@
 //   mm_stft                          :44-140
 //   mm_istft                         :144-221
 //   mm_onset_strength                :1319-1358
+//   mm_chroma_stft                   :494-545
 //   mm_amplitude_to_db               :225-263
 //   mm_power_to_db                   :267-306
 //   mm_mel_spectrogram               :310-400
@@ -333,6 +334,30 @@ int32_t mm_mel_spectrogram(mm_context ctx, const float* signal_data, int64_t sig
   if (r.hop <= 0) return MM_ERR_INVALID_INPUT;
   if (!pow2(n_fft)) {  // the reference core returns an empty [n_mels, 0] result
     fill_shape(out, {n_mels, 0}, 0, 0, nullptr);
+    return MM_OK;
+  }
+  return run_to_buffer(ctx, r, signal_data, signal_length, out);
+}
+
+int32_t mm_chroma_stft(mm_context ctx, const float* signal_data, int64_t signal_length, int32_t sample_rate,
+                       int32_t n_fft, int32_t hop_length, int32_t win_length, int32_t n_chroma, int32_t center,
+                       MMBuffer* out) {
+  if (!ctx || !signal_data || signal_length <= 0 || !out) return MM_ERR_INVALID_INPUT;
+  if (n_fft <= 0 || n_chroma < 1 || n_chroma > 256 || sample_rate <= 0) return MM_ERR_INVALID_INPUT;
+  SpecRequest r;
+  r.out_kind = OUT_MEL;   // the banded contraction with the chroma filterbank
+  r.fb_kind = FB_CHROMA;
+  r.tuning = 0.0f;        // Bridge.swift:526-534 passes no tuning (Chroma.stft default 0)
+  r.sr = sample_rate;
+  r.n_fft = n_fft;
+  r.hop = hop_length > 0 ? hop_length : n_fft / 4;   // Bridge.swift:521-524
+  r.win = win_length > 0 ? win_length : n_fft;
+  r.center = center != 0;
+  r.power = 2.0f;
+  r.n_mels = n_chroma;
+  if (r.hop <= 0) return MM_ERR_INVALID_INPUT;
+  if (!pow2(n_fft)) {   // the STFT gives no frames: [n_chroma, 0]
+    fill_shape(out, {n_chroma, 0}, 0, 0, nullptr);
     return MM_OK;
   }
   return run_to_buffer(ctx, r, signal_data, signal_length, out);

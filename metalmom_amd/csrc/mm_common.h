@@ -22,6 +22,11 @@ enum InDtype : int { IN_F32 = 0, IN_F16 = 1 };
 
 enum PadMode : int { PAD_CONSTANT = 0, PAD_REFLECT = 1 };
 
+// Filterbank of the OUT_MEL / OUT_MFCC contraction: Slaney mel
+// (FilterBank.swift:23-73) or the Gaussian chroma filterbank
+// (Chroma.swift:158-275; n_mels = n_chroma rows).
+enum FbKind : int { FB_MEL = 0, FB_CHROMA = 1 };
+
 constexpr int kTileFrames = 16;   // frames per tile = MFMA N dimension
 constexpr int kMelBlock = 8;      // mel rows per filterbank block of the fused contraction
 constexpr int kMaxMelBlocks = 32; // n_mels <= 256 on the fused path

@@ -126,8 +126,11 @@ static TableKey key_of(const SpecRequest& r) {
   const bool mel = (r.out_kind == OUT_MEL || r.out_kind == OUT_MFCC);
   k.n_mels = mel ? r.n_mels : 0;
   k.n_mfcc = (r.out_kind == OUT_MFCC) ? std::min(r.n_mfcc, r.n_mels) : 0;
-  k.fmin = mel ? r.fmin : 0.0f;
-  k.fmax = mel ? r.fmax : 0.0f;
+  const bool chroma = mel && r.fb_kind == FB_CHROMA;
+  k.fmin = (mel && !chroma) ? r.fmin : 0.0f;
+  k.fmax = (mel && !chroma) ? r.fmax : 0.0f;
+  k.fb_kind = mel ? r.fb_kind : FB_MEL;
+  k.tuning = chroma ? r.tuning : 0.0f;
   return k;
 }
 
@@ -301,6 +304,7 @@ std::string tune_key(const SpecRequest& r, const std::string& arch) {
   std::ostringstream o;
   o << arch << ":k" << r.out_kind << ":n" << r.n_fft << ":h" << r.hop << ":m" << r.n_mels << ":c"
     << ((r.out_kind == OUT_MFCC) ? r.n_mfcc : 0) << ":p" << r.precision << ":i" << r.in_dtype;
+  if (r.fb_kind != FB_MEL) o << ":f" << r.fb_kind;
   return o.str();
 }
 

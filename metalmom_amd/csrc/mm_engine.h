@@ -35,6 +35,8 @@ struct SpecRequest {
   float top_db = 80.0f;   // <= 0: no clamp
   int precision = PREC_F32;
   int in_dtype = IN_F32;
+  int fb_kind = FB_MEL;   // FB_CHROMA: n_mels chroma rows (fmin / fmax unused)
+  float tuning = 0.0f;    // chroma tuning, fractional chroma bins
 };
 
 int64_t frame_count(int64_t n, int n_fft, int hop, int center);

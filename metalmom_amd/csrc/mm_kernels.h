@@ -8,9 +8,11 @@ namespace mm {
 struct TableKey {
   int sr, n_fft, win, n_mels, n_mfcc;
   float fmin, fmax;
+  int fb_kind = FB_MEL;   // filterbank of the n_mels rows
+  float tuning = 0.0f;    // chroma: deviation from A440 in fractional chroma bins
   bool operator==(const TableKey& o) const {
     return sr == o.sr && n_fft == o.n_fft && win == o.win && n_mels == o.n_mels && n_mfcc == o.n_mfcc &&
-           fmin == o.fmin && fmax == o.fmax;
+           fmin == o.fmin && fmax == o.fmax && fb_kind == o.fb_kind && tuning == o.tuning;
   }
 };
 

@@ -849,6 +849,56 @@ __global__ void k_tab_mel(float* fb32, uint16_t* fbb, uint16_t* fbh, int fbs, in
   fbh[i] = f2h(wf);
 }
 
+// Chroma filterbank (Chroma.swift:158-275, librosa filters.chroma with
+// ctroct 5, octwidth 2, L2 column norm, base C), one thread per column k of
+// the fbs-wide table: FFT bin k > 0 at n_chroma log2(k sr / n_fft / (A440/16))
+// fractional chroma bins, DC 1.5 octaves below bin 1, width
+// max(next - this, 1) (1 for bin n_fft - 1), Gaussian bumps of the wrapped
+// distance, column L2 norm, octave weight, rows rolled by -3 n_chroma / 12.
+// Rows >= n_chroma and columns > n_fft/2 are zero. FP64 throughout.
+__global__ void k_tab_chroma(float* fb32, uint16_t* fbb, uint16_t* fbh, int fbs, int rows, int sr, int n_fft,
+                             int n_chroma, double tuning) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= fbs) return;
+  const int F = n_fft / 2 + 1;
+  double w[kMaxMelBlocks * kMelBlock];
+  const int nc = min(n_chroma, kMaxMelBlocks * kMelBlock);
+  for (int c = 0; c < nc; ++c) w[c] = 0.0;
+  if (k < F && nc > 0) {
+    const double a16 = 440.0 * exp2(tuning / n_chroma) / 16.0;
+    auto frq = [&](int b) -> double {
+      if (b == 0) return n_chroma * log2(1.0 * sr / n_fft / a16) - 1.5 * n_chroma;
+      return n_chroma * log2((double)b * sr / n_fft / a16);
+    };
+    const double fk = frq(k);
+    const double width = (k < n_fft - 1) ? fmax(frq(k + 1) - fk, 1.0) : 1.0;
+    const double half = rint(n_chroma / 2.0);
+    double norm = 0.0;
+    for (int c = 0; c < nc; ++c) {
+      double d = fmod(fk - c + half + 10.0 * n_chroma, (double)n_chroma);
+      if (d < 0.0) d += n_chroma;
+      d -= half;
+      const double s = 2.0 * d / width;
+      w[c] = exp(-0.5 * s * s);
+      norm += w[c] * w[c];
+    }
+    norm = sqrt(norm);
+    const double oct = fk / n_chroma - 5.0;
+    const double ow = exp(-0.5 * (oct / 2.0) * (oct / 2.0));
+    for (int c = 0; c < nc; ++c) w[c] = (norm > 0.0 ? w[c] / norm : w[c]) * ow;
+  }
+  const int shift = 3 * (n_chroma / 12);
+  for (int r = 0; r < rows; ++r) {
+    double v = 0.0;
+    if (r < nc) v = w[(r + shift) % nc];   // row r holds chroma (r + shift) mod n_chroma
+    const int64_t i = (int64_t)r * fbs + k;
+    const float vf = (float)v;
+    fb32[i] = vf;
+    fbb[i] = f2bf(vf);
+    fbh[i] = f2h(vf);
+  }
+}
+
 // per block of 8 mel rows: first nonzero bin and chunk counts (16 and 32 bins)
 __global__ void k_tab_bands(const float* fb32, int fbs, int F, int nmh, int2* band32, int2* band16) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
@@ -940,8 +990,12 @@ hipError_t gen_tables(const TableKey& key, const DeviceTables& t, hipStream_t st
   }
   const int rows = t.nmb * 16;
   if (rows > 0) {
-    hipLaunchKernelGGL(k_tab_mel, dim3(nblk((int64_t)rows * t.fbs, 256)), dim3(256), 0, st, t.fb32, t.fb16, t.fbh,
-                       t.fbs, rows, key.sr, key.n_fft, key.n_mels, (double)key.fmin, (double)key.fmax);
+    if (key.fb_kind == FB_CHROMA)
+      hipLaunchKernelGGL(k_tab_chroma, dim3(nblk(t.fbs, 64)), dim3(64), 0, st, t.fb32, t.fb16, t.fbh, t.fbs, rows,
+                         key.sr, key.n_fft, key.n_mels, (double)key.tuning);
+    else
+      hipLaunchKernelGGL(k_tab_mel, dim3(nblk((int64_t)rows * t.fbs, 256)), dim3(256), 0, st, t.fb32, t.fb16,
+                         t.fbh, t.fbs, rows, key.sr, key.n_fft, key.n_mels, (double)key.fmin, (double)key.fmax);
     hipLaunchKernelGGL(k_tab_bands, dim3(nblk(t.nmh, 64)), dim3(64), 0, st, t.fb32, t.fbs, F, t.nmh, t.band32,
                        t.band16);
     hipLaunchKernelGGL(k_tab_assign, dim3(1), dim3(64), 0, st, t.band32, t.band16, t.nmh, t.fbs, t.assign,

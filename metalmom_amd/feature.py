@@ -1,8 +1,8 @@
 """Feature entry points, mirroring the reference ``python/metalmom/feature.py``.
 
 ``melspectrogram`` (``feature.py:8-91``), ``mfcc`` (``:94-167``),
-``amplitude_to_db`` (``:170-220``) and ``power_to_db`` (``:223-270``) keep the
-reference signatures, defaults (hop 512 for mel/MFCC, ``n_mfcc=20``) and error
+``amplitude_to_db`` (``:170-220``), ``power_to_db`` (``:223-270``) and
+``chroma_stft`` (``:273-360``) keep the reference signatures, defaults (hop 512 for mel/MFCC, ``n_mfcc=20``) and error
 behaviour (``RuntimeError("mm_... failed with status s")``). Audio input runs
 on the GPU through libmetalmom.so. When a precomputed ``S`` is given, the
 reference applies the filterbank / DCT in NumPy/SciPy (``feature.py:50-57,
@@ -103,3 +103,73 @@ def amplitude_to_db(S, ref=1.0, amin=1e-5, top_db=80.0):
 def power_to_db(S, ref=1.0, amin=1e-10, top_db=80.0):
     """``10 log10(max(S, amin) / ref)``, clipped to ``max - top_db``."""
     return _db("mm_power_to_db", S, ref, amin, top_db)
+
+
+def _chroma_filterbank(sr, n_fft, n_chroma=12, tuning=0.0, ctroct=5.0, octwidth=2.0):
+    """Chroma filterbank ``(n_chroma, n_fft // 2 + 1)`` for the ``S=`` path,
+    as the reference's wrapper builds it in Python (``feature.py:364-420``,
+    librosa's ``filters.chroma``): Gaussian pitch-class bumps, L2 column norm,
+    octave weighting, rows starting at C."""
+    freqs = np.arange(1, n_fft, dtype=np.float64) * sr / n_fft
+    a440 = 440.0 * 2.0 ** (tuning / n_chroma)
+    frq = n_chroma * np.log2(freqs / (a440 / 16.0))
+    frq = np.concatenate(([frq[0] - 1.5 * n_chroma], frq))
+    width = np.concatenate((np.maximum(frq[1:] - frq[:-1], 1.0), [1.0]))
+    half = np.round(n_chroma / 2.0)
+    D = np.subtract.outer(frq, np.arange(n_chroma, dtype=np.float64)).T
+    D = np.remainder(D + half + 10 * n_chroma, n_chroma) - half
+    w = np.exp(-0.5 * (2.0 * D / width[None, :]) ** 2)
+    w = w / np.maximum(np.sqrt((w * w).sum(axis=0)), np.finfo(np.float64).tiny)
+    w = w * np.exp(-0.5 * ((frq / n_chroma - ctroct) / octwidth) ** 2)[None, :]
+    w = np.roll(w, -3 * (n_chroma // 12), axis=0)
+    return np.ascontiguousarray(w[:, :n_fft // 2 + 1], dtype=np.float32)
+
+
+def _normalize_frames(chroma, norm=2.0):
+    """Per-frame normalisation of the reference wrapper (``feature.py:1150-1177``):
+    L1, L2, L-inf or a general p-norm; frames with norm <= 1e-10 unchanged."""
+    chroma = np.array(chroma, copy=True)
+    if norm == 2.0:
+        n = np.sqrt(np.sum(chroma ** 2, axis=0))
+    elif norm == 1.0:
+        n = np.sum(np.abs(chroma), axis=0)
+    elif norm == np.inf:
+        n = np.max(np.abs(chroma), axis=0) if chroma.shape[0] else np.zeros(chroma.shape[1])
+    else:
+        n = np.linalg.norm(chroma, ord=norm, axis=0)
+    ok = n > 1e-10
+    chroma[:, ok] /= n[ok]
+    return chroma
+
+
+def chroma_stft(y=None, sr=22050, S=None, n_fft=2048, hop_length=None, win_length=None, n_chroma=12, center=True,
+                norm=None, tuning=0.0, **kwargs):
+    """STFT chroma ``(n_chroma, n_frames)`` (the reference's ``feature.py:273-360``):
+    the chroma filterbank times the power spectrogram on the GPU
+    (``mm_chroma_stft``; default ``hop_length`` 512, ``win_length`` n_fft),
+    then the optional per-frame ``norm`` (1, 2, inf). ``S`` (a precomputed
+    power spectrogram) is multiplied by the filterbank directly, and only
+    that path uses ``tuning``, as in the reference."""
+    if S is not None:
+        S = np.ascontiguousarray(S, dtype=np.float32)
+        fb = _chroma_filterbank(sr, (S.shape[0] - 1) * 2, n_chroma, tuning)
+        chroma = fb @ S
+        return _normalize_frames(chroma, norm) if norm is not None else chroma
+    if y is None:
+        raise ValueError("Either y or S must be provided")
+    y = np.ascontiguousarray(y, dtype=np.float32)
+    hop = hop_length if hop_length is not None else 512
+    win = win_length if win_length is not None else n_fft
+    ctx = lib.mm_init()
+    if not ctx:
+        raise RuntimeError("Failed to initialize MetalMom context")
+    try:
+        out = MMBuffer()
+        status = lib.mm_chroma_stft(ctx, float_ptr(y), len(y), sr, n_fft, hop, win, n_chroma,
+                                    1 if center else 0, C.byref(out))
+        if status != 0:
+            raise RuntimeError(f"mm_chroma_stft failed with status {status}")
+        result = buffer_to_numpy(out)
+        return _normalize_frames(result, norm) if norm is not None else result
+    finally:
+        lib.mm_destroy(ctx)

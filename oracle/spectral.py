@@ -19,6 +19,7 @@ __all__ = [
     "hz_to_mel", "mel_to_hz", "mel_filterbank", "melspectrogram",
     "linear_to_db", "power_to_db", "amplitude_to_db",
     "dct_ortho_basis", "mfcc", "mfcc_from_logmel", "onset_strength",
+    "chroma_filterbank", "chroma_stft", "normalize_frames",
 ]
 
 
@@ -375,6 +376,68 @@ def onset_strength(y, sr=22050, n_fft=2048, hop=512, n_mels=128, fmin=0.0, fmax=
     return out
 
 
+# ---------------------------------------------------------------------------
+# Chroma (STFT power -> Gaussian pitch-class filterbank)
+# ---------------------------------------------------------------------------
+
+def chroma_filterbank(sr, n_fft, n_chroma=12, tuning=0.0, ctroct=5.0, octwidth=2.0) -> np.ndarray:
+    """Chroma filterbank ``[n_chroma, n_fft//2 + 1]``.
+
+    Restates ``Features/Chroma.swift:158-275`` (librosa's ``filters.chroma``):
+    FFT bin k > 0 sits at ``n_chroma log2(k sr / n_fft / (A440 / 16))``
+    fractional chroma bins (``A440 = 440 * 2^(tuning / n_chroma)``), the DC bin
+    1.5 octaves below bin 1; bin widths ``max(next - this, 1)`` (1 for the
+    last of the n_fft bins); Gaussian bumps ``exp(-0.5 (2 D / width)^2)`` of
+    the distance D wrapped to ``[-n_chroma/2, n_chroma/2)``; every column
+    L2-normalised; octave weighting ``exp(-0.5 ((octave - ctroct)/octwidth)^2)``;
+    rows rolled to start at C (``-3 n_chroma / 12``); cut to n_fft/2 + 1 bins.
+    """
+    k = np.arange(1, n_fft, dtype=np.float64)
+    a440 = 440.0 * 2.0 ** (tuning / n_chroma)
+    frq = np.empty(n_fft)
+    frq[1:] = n_chroma * np.log2(k * sr / n_fft / (a440 / 16.0))
+    frq[0] = frq[1] - 1.5 * n_chroma
+    width = np.ones(n_fft)
+    width[:-1] = np.maximum(frq[1:] - frq[:-1], 1.0)
+    half = np.round(n_chroma / 2.0)
+    D = frq[None, :] - np.arange(n_chroma, dtype=np.float64)[:, None]
+    D = np.remainder(D + half + 10 * n_chroma, n_chroma) - half
+    w = np.exp(-0.5 * (2.0 * D / width[None, :]) ** 2)
+    norm = np.sqrt((w * w).sum(axis=0))
+    w = w / np.where(norm > 0, norm, 1.0)
+    w = w * np.exp(-0.5 * ((frq / n_chroma - ctroct) / octwidth) ** 2)[None, :]
+    w = np.roll(w, -3 * (n_chroma // 12), axis=0)
+    return w[:, :n_fft // 2 + 1].copy()
+
+
+def normalize_frames(C, norm):
+    """Per-frame (column) normalisation of the reference wrapper
+    (``python/metalmom/feature.py:1150-1177``): L1, L2 or L-inf norm, columns
+    whose norm is at most 1e-10 left unchanged."""
+    C = np.array(C, dtype=np.float64, copy=True)
+    if norm == 2.0:
+        n = np.sqrt((C * C).sum(axis=0))
+    elif norm == 1.0:
+        n = np.abs(C).sum(axis=0)
+    elif norm == np.inf:
+        n = np.abs(C).max(axis=0) if C.shape[0] else np.zeros(C.shape[1])
+    else:
+        n = (np.abs(C) ** norm).sum(axis=0) ** (1.0 / norm)
+    ok = n > 1e-10
+    C[:, ok] /= n[ok]
+    return C
+
+
+def chroma_stft(y, sr=22050, n_fft=2048, hop=512, win_length=None, n_chroma=12, center=True, norm=None,
+                tuning=0.0) -> np.ndarray:
+    """``chroma_filterbank @ |STFT|^2``, shape ``[n_chroma, n_frames]``
+    (``Features/Chroma.swift:30-156``), then ``normalize_frames`` when ``norm``
+    is given (the reference's Python wrapper, ``feature.py:273-360``)."""
+    P = power_spectrogram(y, n_fft, hop, win_length, center)
+    C = chroma_filterbank(sr, n_fft, n_chroma, tuning) @ P
+    return normalize_frames(C, norm) if norm is not None else C
+
+
 PINS = {
     "hann_periodic": "known answer hann(4) = [0, .5, 1, .5] (WindowTests.swift:60-66); closed form",
     "prepare_window": "centre zero-pad / centre truncation indices checked by hand on small cases",
@@ -389,6 +452,9 @@ PINS = {
     "amplitude_to_db": "goldens stft_440hz_amplitude_db{,_ref1}.npy",
     "dct_ortho_basis": "scipy.fft.dct(type=2, norm='ortho'); orthonormality",
     "mfcc": "goldens mfcc_440hz_*.npy",
+    "chroma_filterbank": "through the chroma goldens; unit column norms before octave weighting; roll puts C first",
+    "chroma_stft": "goldens chroma_stft_440hz_{no_norm,default,1024}.npy (librosa.feature.chroma_stft)",
+    "normalize_frames": "L1 / L2 / Linf norms of the columns equal 1; zero columns unchanged",
     "onset_strength": "golden onset_strength_440hz_default.npy (librosa.onset.onset_strength); silence -> 0; "
                       "non-negative; shape / padding rules; planted mistakes (no rectification, pad width) fail",
 }

@@ -175,3 +175,20 @@ def test_onset_strength_validation_and_zero_columns():
         assert s == 0 and out.ndim == 2 and out.shape[0] == rows and out.shape[1] == 1 and out.count == rows
         assert not np.ctypeslib.as_array(out.data, shape=(rows,)).any()
         lib.mm_buffer_free(C.byref(out))
+
+
+def test_chroma_precomputed_spectrogram_path_matches_goldens(signal_440):
+    """The S= path multiplies a precomputed power spectrogram by the chroma
+    filterbank in NumPy, with tuning, as the reference wrapper does
+    (feature.py:313-323); with librosa's estimated tunings of the golden
+    signal (0.02 at n_fft 2048, -0.07 at 1024) it reproduces the goldens."""
+    from conftest import load_golden
+    from oracle import spectral as o
+    import metalmom_amd as mm
+    for name, n_fft, tuning, norm in [("chroma_stft_440hz_no_norm", 2048, 0.02, None),
+                                      ("chroma_stft_440hz_default", 2048, 0.02, np.inf),
+                                      ("chroma_stft_440hz_1024", 1024, -0.07, None)]:
+        G = load_golden(name)
+        P = o.power_spectrogram(signal_440, n_fft, 512).astype(np.float32)
+        C = mm.feature.chroma_stft(S=P, sr=22050, tuning=tuning, norm=norm)
+        assert C.shape == G.shape and np.abs(C - G).max() <= 1e-5 * np.abs(G).max()

@@ -331,3 +331,49 @@ def test_onset_strength_planted_mistakes_are_caught(signal_440):
     short_pad = np.concatenate([np.zeros(1), rect])[:44]
     assert np.abs(np.concatenate([np.zeros(3), rect])[:44] - G).max() < 1e-6
     assert np.abs(no_rect - G).max() > 1e-3 and np.abs(short_pad - G).max() > 1e-3
+
+
+# -- chroma (Chroma.swift:30-275) ---------------------------------------------
+
+@pytest.mark.parametrize("name,kw", [
+    ("chroma_stft_440hz_no_norm", dict(tuning=0.02)),
+    ("chroma_stft_440hz_default", dict(tuning=0.02, norm=np.inf)),
+    ("chroma_stft_440hz_1024", dict(n_fft=1024, tuning=-0.07)),
+])
+def test_chroma_goldens(signal_440, name, kw):
+    """librosa.feature.chroma_stft estimated the tuning of the 440 Hz golden
+    signal on its 0.01-bin grid (0.02 at n_fft 2048, -0.07 at 1024); with
+    those tunings the restated filterbank reproduces the goldens to float32
+    rounding (at tuning 0, which the reference uses, they differ by 1-7 %)."""
+    G = load_golden(name)
+    C = o.chroma_stft(signal_440, 22050, **kw)
+    assert C.shape == G.shape == (12, 44)
+    assert np.abs(C - G).max() <= 2e-6 * np.abs(G).max()
+    kw0 = dict(kw, tuning=0.0)
+    assert np.abs(o.chroma_stft(signal_440, 22050, **kw0) - G).max() > 1e-2 * np.abs(G).max()
+
+
+def test_chroma_filterbank_properties():
+    fb = o.chroma_filterbank(22050, 2048)
+    assert fb.shape == (12, 1025) and (fb > 0).all()
+    # before octave weighting every column has unit L2 norm
+    k = np.arange(1, 2048)
+    frq = np.empty(2048)
+    frq[1:] = 12 * np.log2(k * 22050 / 2048 / 27.5)
+    frq[0] = frq[1] - 18
+    ow = np.exp(-0.5 * ((frq / 12 - 5.0) / 2.0) ** 2)[:1025]
+    assert np.allclose(np.sqrt(((fb / ow) ** 2).sum(axis=0)), 1.0)
+    # a 261.63 Hz (C4) partial lands in row 0 (C) after the roll; 440 Hz in row 9 (A)
+    for f, row in [(261.63, 0), (440.0, 9)]:
+        assert np.argmax(fb[:, int(round(f * 2048 / 22050))]) == row
+
+
+def test_normalize_frames_norms():
+    rng = np.random.default_rng(4)
+    C = np.abs(rng.standard_normal((12, 20)))
+    C[:, 3] = 0
+    for p, f in [(1.0, lambda c: np.abs(c).sum(0)), (2.0, lambda c: np.sqrt((c * c).sum(0))),
+                 (np.inf, lambda c: np.abs(c).max(0))]:
+        N = o.normalize_frames(C, p)
+        n = f(N)
+        assert np.allclose(np.delete(n, 3), 1.0) and not N[:, 3].any()
