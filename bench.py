@@ -241,6 +241,35 @@ def onset_leg(torch, n_samples, iters, with_cpu=False):
     return res
 
 
+def chroma_leg(torch, lens, d_in, steps):
+    """Batched STFT chroma (12 bins, no frame normalisation) of the uneven
+    batch already in HBM, FP32 and BF16 contraction: one persistent launch
+    per step replayed from a hipGraph; fused-kernel time from HIP events."""
+    from metalmom_amd.batch import SpectralBatch
+    res = {}
+    for prec in ("f32", "bf16"):
+        B = SpectralBatch(lens, kind="chroma", sr=22050, n_fft=2048, hop_length=512, n_mels=12, precision=prec)
+        out = B.empty_output()
+        B.autotune(d_in, out, iters=2)
+        B.run(d_in, out, graph=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            B.run(d_in, out, graph=True)
+        torch.cuda.synchronize()
+        fps = B.total_frames * steps / (time.perf_counter() - t0)
+        k_ms, _ = B.time(d_in, out, iters=10)
+        bytes_launch = 4 * B.total_input + 4 * 12 * B.total_frames
+        gbs = bytes_launch / (k_ms * 1e-3) / 1e9
+        res[prec] = {"frames_per_s": fps, "kernel_ms": k_ms,
+                     "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                  "frac": gbs / HBM_PEAK_GBS, "bytes_per_launch": bytes_launch}}
+        B.close()
+    res["workload"] = ("uneven batch (256 clips, 336,013 frames), n_fft 2048, hop 512, 12 chroma bins, "
+                       "dense chroma filterbank on the fused banded contraction")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -256,6 +285,7 @@ def main():
                     help="persistent workgroups per CU (skips the autotuner; 0 = autotune)")
     ap.add_argument("--no-istft", action="store_true", help="skip the inverse-STFT leg")
     ap.add_argument("--no-onset", action="store_true", help="skip the onset-strength leg")
+    ap.add_argument("--no-chroma", action="store_true", help="skip the chroma leg")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="profiling aid: also run one device-to-device copy of this many bytes, "
                          "so HBM counters can be scaled by a kernel of known traffic")
@@ -404,6 +434,13 @@ def main():
         log(f"onset strength: {ons['frames_per_s'] / 1e6:.1f} M frames/s, {ons['step_ms']:.3f} ms per clip, "
             f"{ons['roofline']['achieved']:.0f} GB/s algorithmic"
             + (f"; cpu {ons['cpu_baseline']['value']:.0f} frames/s" if "cpu_baseline" in ons else ""))
+
+    if not args.no_chroma and not args.profile:
+        chr_ = chroma_leg(torch, lens, d_in, max(3, steps // 2))
+        extra["chroma"] = chr_
+        log(f"chroma batched: fp32 {chr_['f32']['frames_per_s'] / 1e6:.1f} M frames/s "
+            f"(kernel {chr_['f32']['kernel_ms']:.3f} ms), bf16 {chr_['bf16']['frames_per_s'] / 1e6:.1f} M frames/s "
+            f"(kernel {chr_['bf16']['kernel_ms']:.3f} ms)")
 
     cpu = None
     if not args.no_cpu and not args.profile:

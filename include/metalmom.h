@@ -145,6 +145,7 @@ void mm_buffer_free(MMBuffer* buf);
 #define MM_KIND_STFT_COMPLEX 2
 #define MM_KIND_MEL 3
 #define MM_KIND_MFCC 4
+#define MM_KIND_CHROMA 5 /* STFT chroma (mm_chroma_stft): n_mels = n_chroma (1..256), f_min / f_max unused */
 
 #define MM_PREC_F32 0   /* FP32 mel/DCT contraction (FP32 MFMA) */
 #define MM_PREC_BF16 1  /* BF16 MFMA, FP32 accumulation */

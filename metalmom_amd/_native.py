@@ -22,7 +22,7 @@ MM_ERR_METAL_UNAVAILABLE = -2
 MM_ERR_ALLOC_FAILED = -3
 MM_ERR_INTERNAL = -4
 
-KIND_STFT_MAG, KIND_STFT_POWER, KIND_STFT_COMPLEX, KIND_MEL, KIND_MFCC = range(5)
+KIND_STFT_MAG, KIND_STFT_POWER, KIND_STFT_COMPLEX, KIND_MEL, KIND_MFCC, KIND_CHROMA = range(6)
 PREC_F32, PREC_BF16, PREC_F16 = range(3)
 INPUT_F32, INPUT_F16 = range(2)
 PAD_CONSTANT, PAD_REFLECT = range(2)

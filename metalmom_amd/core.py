@@ -78,8 +78,10 @@ def spectral(y, kind="mfcc", sr=22050, n_fft=2048, hop_length=512, win_length=No
              top_db=80.0, precision="f32"):
     """One clip with every option of ``MMSpecParams`` (additive API).
 
-    ``kind`` is one of ``stft``, ``power``, ``complex``, ``mel``, ``mfcc``;
-    ``precision`` one of ``f32``, ``bf16``, ``f16`` (mel and DCT contraction).
+    ``kind`` is one of ``stft``, ``power``, ``complex``, ``mel``, ``mfcc``,
+    ``chroma`` (``n_mels`` chroma bins, tuning 0, no frame normalisation);
+    ``precision`` one of ``f32``, ``bf16``, ``f16`` (mel / chroma and DCT
+    contraction).
     """
     y = np.ascontiguousarray(y, dtype=np.float32)
     p = make_params(kind, sr, n_fft, hop_length, win_length, center, pad_mode, power, n_mels,
@@ -99,7 +101,7 @@ def spectral(y, kind="mfcc", sr=22050, n_fft=2048, hop_length=512, win_length=No
 
 _KINDS = {"stft": N.KIND_STFT_MAG, "magnitude": N.KIND_STFT_MAG, "power": N.KIND_STFT_POWER,
           "complex": N.KIND_STFT_COMPLEX, "mel": N.KIND_MEL, "melspectrogram": N.KIND_MEL,
-          "mfcc": N.KIND_MFCC}
+          "mfcc": N.KIND_MFCC, "chroma": N.KIND_CHROMA, "chroma_stft": N.KIND_CHROMA}
 _PRECS = {"f32": N.PREC_F32, "fp32": N.PREC_F32, "bf16": N.PREC_BF16, "f16": N.PREC_F16,
           "fp16": N.PREC_F16}
 _INPUTS = {"f32": N.INPUT_F32, "float32": N.INPUT_F32, "f16": N.INPUT_F16, "float16": N.INPUT_F16}

@@ -101,35 +101,6 @@ int32_t db_to_buffer(void* ctx, const float* data, int64_t count, float mult, fl
   return OK;
 }
 
-// Spec params -> request, applying the drop-in sentinels.
-int32_t to_request(const MMSpecParams* p, SpecRequest* r) {
-  if (!p) return ERR_INVALID;
-  if (p->kind < MM_KIND_STFT_MAG || p->kind > MM_KIND_MFCC) return ERR_INVALID;
-  if (!pow2(p->n_fft)) return ERR_INVALID;
-  r->out_kind = p->kind;
-  r->sr = p->sample_rate;
-  r->n_fft = p->n_fft;
-  r->hop = p->hop_length > 0 ? p->hop_length : p->n_fft / 4;
-  r->win = p->win_length > 0 ? p->win_length : p->n_fft;
-  r->center = p->center != 0;
-  r->pad_mode = p->pad_mode;
-  r->power = p->power;
-  r->n_mels = p->n_mels;
-  r->fmin = p->f_min;
-  r->fmax = p->f_max > 0.0f ? p->f_max : (float)p->sample_rate / 2.0f;
-  r->n_mfcc = p->n_mfcc;
-  r->top_db = p->top_db;
-  r->precision = p->precision;
-  r->in_dtype = p->input_dtype;
-  if (r->hop <= 0) return ERR_INVALID;
-  if (r->pad_mode != PAD_CONSTANT && r->pad_mode != PAD_REFLECT) return ERR_INVALID;
-  if (r->precision < PREC_F32 || r->precision > PREC_F16) return ERR_INVALID;
-  if (r->in_dtype != IN_F32 && r->in_dtype != IN_F16) return ERR_INVALID;
-  if ((r->out_kind == MM_KIND_MEL || r->out_kind == MM_KIND_MFCC) && r->n_mels < 0) return ERR_INVALID;
-  if (r->out_kind == MM_KIND_MFCC && r->n_mfcc < 0) return ERR_INVALID;
-  if (r->out_kind <= MM_KIND_STFT_COMPLEX && r->precision != PREC_F32) return ERR_INVALID;
-  return OK;
-}
 
 }  // namespace
 
@@ -405,7 +376,7 @@ int32_t mm_spectral(mm_context ctx, const MMSpecParams* params, const float* sig
                     MMBuffer* out) {
   if (!ctx || !signal_data || signal_length <= 0 || !out) return MM_ERR_INVALID_INPUT;
   SpecRequest r;
-  int32_t s = to_request(params, &r);
+  int32_t s = request_from_params(params, &r);
   if (s != OK) return s;
   if (r.in_dtype != IN_F32) return MM_ERR_INVALID_INPUT;
   return run_to_buffer(ctx, r, signal_data, signal_length, out);

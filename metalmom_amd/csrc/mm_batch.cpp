@@ -45,41 +45,6 @@ struct Batch {
   }
 };
 
-int32_t to_request(const MMSpecParams* p, SpecRequest* r) {
-  if (!p) return ERR_INVALID;
-  const int n_fft = p->n_fft;
-  if (n_fft <= 0 || (n_fft & (n_fft - 1))) return ERR_INVALID;
-  if (p->kind < MM_KIND_STFT_MAG || p->kind > MM_KIND_MFCC) return ERR_INVALID;
-  r->out_kind = p->kind;
-  r->sr = p->sample_rate;
-  r->n_fft = n_fft;
-  r->hop = p->hop_length > 0 ? p->hop_length : n_fft / 4;
-  r->win = p->win_length > 0 ? p->win_length : n_fft;
-  r->center = p->center != 0;
-  r->pad_mode = p->pad_mode;
-  r->power = p->power;
-  r->n_mels = p->n_mels;
-  r->fmin = p->f_min;
-  r->fmax = p->f_max > 0.0f ? p->f_max : (float)p->sample_rate / 2.0f;
-  r->n_mfcc = p->n_mfcc;
-  r->top_db = p->top_db;
-  r->precision = p->precision;
-  r->in_dtype = p->input_dtype;
-  if (r->hop <= 0) return ERR_INVALID;
-  if (r->pad_mode != PAD_CONSTANT && r->pad_mode != PAD_REFLECT) return ERR_INVALID;
-  if (r->precision < PREC_F32 || r->precision > PREC_F16) return ERR_INVALID;
-  if (r->in_dtype != IN_F32 && r->in_dtype != IN_F16) return ERR_INVALID;
-  if (r->out_kind >= OUT_MEL && r->n_mels < 0) return ERR_INVALID;
-  if (r->out_kind == OUT_MFCC && r->n_mfcc < 0) return ERR_INVALID;
-  if (r->out_kind <= OUT_STFT_COMPLEX && r->precision != PREC_F32) return ERR_INVALID;
-  if (r->precision != PREC_F32 || r->in_dtype != IN_F32) {
-    // 16-bit paths exist only in the fused kernel
-    TileLayout L;
-    if (!tile_layout(r->n_fft, r->out_kind, r->precision, r->n_mels, std::min(r->n_mfcc, r->n_mels), &L))
-      return ERR_INVALID;
-  }
-  return OK;
-}
 
 int32_t create_batch(const MMSpecParams* params, const int64_t* lengths, int32_t n_clips, Batch** out) {
   if (!lengths || n_clips <= 0 || !out) return ERR_INVALID;
@@ -89,7 +54,7 @@ int32_t create_batch(const MMSpecParams* params, const int64_t* lengths, int32_t
   Engine* e = Engine::get();
   if (!e) return ERR_NO_GPU;
   auto b = std::make_unique<Batch>();
-  int32_t s = to_request(params, &b->r);
+  int32_t s = request_from_params(params, &b->r);
   if (s != OK) return s;
   b->lane = e->acquire_lane();
   if (!b->lane) return ERR_INTERNAL;

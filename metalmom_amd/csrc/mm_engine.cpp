@@ -3,6 +3,8 @@
 // generic-kernel execution, and the autotuner's persisted choices.
 #include "mm_engine.h"
 
+#include "../../include/metalmom.h"
+
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -786,6 +788,50 @@ int32_t run_db(Lane* lane, const float* data, int64_t n, float mult, float ref, 
   if (!res) return ERR_ALLOC;
   std::memcpy(res, lane->h_out.p, bytes);
   *out = res;
+  return OK;
+}
+
+// ---------------------------------------------------------------------------
+// additive API requests
+// ---------------------------------------------------------------------------
+
+int32_t request_from_params(const MMSpecParams* p, SpecRequest* r) {
+  if (!p) return ERR_INVALID;
+  const int n_fft = p->n_fft;
+  if (n_fft <= 0 || (n_fft & (n_fft - 1))) return ERR_INVALID;
+  if (p->kind < MM_KIND_STFT_MAG || p->kind > MM_KIND_CHROMA) return ERR_INVALID;
+  const bool chroma = p->kind == MM_KIND_CHROMA;
+  r->out_kind = chroma ? OUT_MEL : p->kind;   // chroma: the banded contraction with the chroma table
+  r->fb_kind = chroma ? FB_CHROMA : FB_MEL;
+  r->tuning = 0.0f;
+  r->sr = p->sample_rate;
+  r->n_fft = n_fft;
+  r->hop = p->hop_length > 0 ? p->hop_length : n_fft / 4;
+  r->win = p->win_length > 0 ? p->win_length : n_fft;
+  r->center = p->center != 0;
+  r->pad_mode = p->pad_mode;
+  r->power = p->power;
+  r->n_mels = p->n_mels;
+  r->fmin = p->f_min;
+  r->fmax = p->f_max > 0.0f ? p->f_max : (float)p->sample_rate / 2.0f;
+  r->n_mfcc = p->n_mfcc;
+  r->top_db = p->top_db;
+  r->precision = p->precision;
+  r->in_dtype = p->input_dtype;
+  if (r->hop <= 0) return ERR_INVALID;
+  if (r->pad_mode != PAD_CONSTANT && r->pad_mode != PAD_REFLECT) return ERR_INVALID;
+  if (r->precision < PREC_F32 || r->precision > PREC_F16) return ERR_INVALID;
+  if (r->in_dtype != IN_F32 && r->in_dtype != IN_F16) return ERR_INVALID;
+  if (r->out_kind >= OUT_MEL && r->n_mels < 0) return ERR_INVALID;
+  if (chroma && (r->n_mels < 1 || r->n_mels > 256 || r->sr <= 0)) return ERR_INVALID;
+  if (r->out_kind == OUT_MFCC && r->n_mfcc < 0) return ERR_INVALID;
+  if (r->out_kind <= OUT_STFT_COMPLEX && r->precision != PREC_F32) return ERR_INVALID;
+  if (r->precision != PREC_F32 || r->in_dtype != IN_F32) {
+    // 16-bit contractions and FP16 input exist only in the fused kernel
+    TileLayout L;
+    if (!tile_layout(r->n_fft, r->out_kind, r->precision, r->n_mels, std::min(r->n_mfcc, r->n_mels), &L))
+      return ERR_INVALID;
+  }
   return OK;
 }
 

@@ -8,6 +8,7 @@
 #include <string>
 #include <vector>
 
+#include "../../include/metalmom.h"
 #include "mm_common.h"
 #include "mm_kernels.h"
 
@@ -158,6 +159,11 @@ int32_t run_single(Lane* lane, const SpecRequest& r, const float* signal, int64_
 
 int32_t run_db(Lane* lane, const float* data, int64_t n, float mult, float ref, float amin, float top_db,
                float** out);
+
+// MMSpecParams of the additive API -> request, with its sentinels (hop <= 0
+// -> n_fft/4, win <= 0 -> n_fft, f_max <= 0 -> sr/2) and validation; 16-bit
+// contractions and FP16 input only where the fused kernel covers them.
+int32_t request_from_params(const MMSpecParams* p, SpecRequest* r);
 
 // Onset strength of one clip (OnsetDetection.swift:23-128): r is the power-2
 // mel request (win = n_fft); aggregate -> [1, frames], else [n_mels, frames].

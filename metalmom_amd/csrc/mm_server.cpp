@@ -357,38 +357,6 @@ class Server {
   std::thread th_;
 };
 
-int32_t spec_to_request(const MMSpecParams* p, SpecRequest* r) {
-  if (!p) return ERR_INVALID;
-  const int n_fft = p->n_fft;
-  if (n_fft <= 0 || (n_fft & (n_fft - 1))) return ERR_INVALID;
-  if (p->kind < MM_KIND_STFT_MAG || p->kind > MM_KIND_MFCC) return ERR_INVALID;
-  if (p->input_dtype != MM_INPUT_F32) return ERR_INVALID;
-  r->out_kind = p->kind;
-  r->sr = p->sample_rate;
-  r->n_fft = n_fft;
-  r->hop = p->hop_length > 0 ? p->hop_length : n_fft / 4;
-  r->win = p->win_length > 0 ? p->win_length : n_fft;
-  r->center = p->center != 0;
-  r->pad_mode = p->pad_mode;
-  r->power = p->power;
-  r->n_mels = p->n_mels;
-  r->fmin = p->f_min;
-  r->fmax = p->f_max > 0.0f ? p->f_max : (float)p->sample_rate / 2.0f;
-  r->n_mfcc = p->n_mfcc;
-  r->top_db = p->top_db;
-  r->precision = p->precision;
-  r->in_dtype = IN_F32;
-  if (r->hop <= 0 || r->pad_mode < 0 || r->pad_mode > 1 || r->precision < 0 || r->precision > 2) return ERR_INVALID;
-  if (r->out_kind >= OUT_MEL && r->n_mels < 0) return ERR_INVALID;
-  if (r->out_kind == OUT_MFCC && r->n_mfcc < 0) return ERR_INVALID;
-  if (r->out_kind <= OUT_STFT_COMPLEX && r->precision != PREC_F32) return ERR_INVALID;
-  if (r->precision != PREC_F32) {
-    TileLayout L;
-    if (!tile_layout(r->n_fft, r->out_kind, r->precision, r->n_mels, std::min(r->n_mfcc, r->n_mels), &L))
-      return ERR_INVALID;
-  }
-  return OK;
-}
 
 }  // namespace
 
@@ -410,7 +378,8 @@ int32_t mm_server_create(mm_context ctx, int64_t hbm_budget_bytes, int32_t max_b
 int64_t mm_server_submit(mm_server srv, const MMSpecParams* params, const float* signal_data, int64_t signal_length) {
   if (!srv || !signal_data || signal_length <= 0) return MM_ERR_INVALID_INPUT;
   SpecRequest r;
-  const int32_t s = spec_to_request(params, &r);
+  int32_t s = request_from_params(params, &r);
+  if (s == OK && r.in_dtype != IN_F32) s = ERR_INVALID;   // host clips are float32
   if (s != OK) return s;
   return static_cast<Server*>(srv)->submit(r, signal_data, signal_length);
 }

@@ -60,3 +60,28 @@ def test_chroma_non_power_of_two_is_empty():
     import metalmom_amd as mm
     C = mm.chroma_stft(y=np.ones(5000, np.float32), n_fft=1000)
     assert C.shape == (12, 0)
+
+
+def test_chroma_batch_equals_single_clips_and_bf16_error():
+    """kind='chroma' in a device-resident batch: every clip bitwise equal to
+    mm_chroma_stft alone (FP32); the BF16 contraction within 1e-2 of the
+    frame's largest chroma value (bf16 weights and power, FP32 accumulation)."""
+    import torch
+    import metalmom_amd as mm
+    from metalmom_amd.batch import SpectralBatch
+    clips = [_tones(n, 22050, n) for n in (30000, 77777, 5000, 150000)]
+    lens = [len(c) for c in clips]
+    for prec in ("f32", "bf16"):
+        B = SpectralBatch(lens, kind="chroma", sr=22050, n_fft=2048, hop_length=512, n_mels=12, precision=prec)
+        d_in = B.pack(clips)
+        out = B.empty_output()
+        B.run(d_in, out)
+        torch.cuda.synchronize()
+        for c, r in zip(clips, B.unpack(out)):
+            single = mm.chroma_stft(y=c, sr=22050, hop_length=512)
+            assert r.shape == single.shape
+            if prec == "f32":
+                assert np.array_equal(r, single)
+            else:
+                assert (np.abs(r - single) <= 1e-2 * single.max(axis=0, keepdims=True) + 1e-30).all()
+        B.close()
