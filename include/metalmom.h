@@ -124,6 +124,28 @@ int32_t mm_chroma_stft(mm_context ctx, const float* signal_data, int64_t signal_
                        int32_t win_length, int32_t n_chroma, int32_t center,
                        MMBuffer* out);
 
+/* Replace Bridge.swift:621-813 (mm_spectral_centroid / _bandwidth / _rolloff
+ * / _flatness), reference metalmom.h:76-90: per-frame reductions of the
+ * magnitude STFT (SpectralDescriptors.swift), 1-D [n_frames]. Frequencies
+ * are Float(k) * sr / n_fft. centroid: sum f S / sum S (0 without energy);
+ * bandwidth: (sum S |f - centroid|^p / sum S)^(1/p); rolloff: first bin
+ * whose cumulative magnitude reaches roll_percent of the total; flatness:
+ * geometric / arithmetic mean of max(S^2, 1e-10) (0 when sum S < 1e-10).
+ * hop <= 0 -> n_fft/4, win <= 0 -> n_fft; a non-power-of-two n_fft (no STFT
+ * frames) gives an empty result. */
+int32_t mm_spectral_centroid(mm_context ctx, const float* signal_data, int64_t signal_length,
+                             int32_t sample_rate, int32_t n_fft, int32_t hop_length,
+                             int32_t win_length, int32_t center, MMBuffer* out);
+int32_t mm_spectral_bandwidth(mm_context ctx, const float* signal_data, int64_t signal_length,
+                              int32_t sample_rate, int32_t n_fft, int32_t hop_length,
+                              int32_t win_length, int32_t center, float p, MMBuffer* out);
+int32_t mm_spectral_rolloff(mm_context ctx, const float* signal_data, int64_t signal_length,
+                            int32_t sample_rate, int32_t n_fft, int32_t hop_length,
+                            int32_t win_length, int32_t center, float roll_percent, MMBuffer* out);
+int32_t mm_spectral_flatness(mm_context ctx, const float* signal_data, int64_t signal_length,
+                             int32_t sample_rate, int32_t n_fft, int32_t hop_length,
+                             int32_t win_length, int32_t center, MMBuffer* out);
+
 /* Replaces Bridge.swift:402-493 (mm_mfcc): [min(n_mfcc, n_mels), n_frames];
  * power 2 mel, power_to_db(ref 1, amin 1e-10, top_db 80), DCT-II ortho. */
 int32_t mm_mfcc(mm_context ctx, const float* signal_data, int64_t signal_length,

@@ -11,6 +11,7 @@ batches (``batch.SpectralBatch``) and a request server (``serve``).
 __version__ = "0.1.0"
 
 from .core import stft, stft_complex, istft, spectral  # noqa: F401
-from .feature import (amplitude_to_db, power_to_db, melspectrogram, mfcc, chroma_stft)  # noqa: F401
+from .feature import (amplitude_to_db, power_to_db, melspectrogram, mfcc, chroma_stft,  # noqa: F401
+                      spectral_centroid, spectral_bandwidth, spectral_rolloff, spectral_flatness)
 from . import feature  # noqa: F401
 from .onset import onset_strength  # noqa: F401

@@ -69,6 +69,10 @@ SIGNATURES = {
     "mm_buffer_free": (None, [_BUF]),
     "mm_stft_complex": (_I32, [_P, _F, _I64, _I32, C.POINTER(MMSTFTParams), _BUF]),
     "mm_istft": (_I32, [_P, _F, _I64, _I32, _I32, _I32, _I32, _I32, _I32, _I64, _BUF]),
+    "mm_spectral_centroid": (_I32, [_P, _F, _I64, _I32, _I32, _I32, _I32, _I32, _BUF]),
+    "mm_spectral_bandwidth": (_I32, [_P, _F, _I64, _I32, _I32, _I32, _I32, _I32, C.c_float, _BUF]),
+    "mm_spectral_rolloff": (_I32, [_P, _F, _I64, _I32, _I32, _I32, _I32, _I32, C.c_float, _BUF]),
+    "mm_spectral_flatness": (_I32, [_P, _F, _I64, _I32, _I32, _I32, _I32, _I32, _BUF]),
     "mm_chroma_stft": (_I32, [_P, _F, _I64, _I32, _I32, _I32, _I32, _I32, _I32, _BUF]),
     "mm_onset_strength": (_I32, [_P, _F, _I64, _I32, _I32, _I32, _I32, C.c_float, C.c_float, _I32, _I32, _BUF]),
     "mm_istft_length": (_I64, [_I32, _I32, _I32, _I32, _I64]),

@@ -7,6 +7,7 @@
 //   mm_istft                         :144-221
 //   mm_onset_strength                :1319-1358
 //   mm_chroma_stft                   :494-545
+//   mm_spectral_{centroid,bandwidth,rolloff,flatness}  :621-813
 //   mm_amplitude_to_db               :225-263
 //   mm_power_to_db                   :267-306
 //   mm_mel_spectrogram               :310-400
@@ -78,6 +79,35 @@ int32_t run_to_buffer(void* ctx, const SpecRequest& r, const float* sig, int64_t
   const int32_t s = run_single(lane, r, sig, len, &data, &rr, &cc);
   if (s != OK) return s;
   fill_shape(out, {rr, cc}, dtype, rr * cc, data);
+  return OK;
+}
+
+// Spectral descriptor of one clip into a 1-D [frames] buffer (Bridge.swift:621-813):
+// hop <= 0 -> n_fft/4, win <= 0 -> n_fft (STFT.compute defaults); a
+// non-power-of-two n_fft has no frames (empty result).
+int32_t descriptor_to_buffer(void* ctx, const float* sig, int64_t len, int32_t sr, int32_t n_fft, int32_t hop,
+                             int32_t win, int32_t center, int kind, float param, MMBuffer* out) {
+  if (!ctx || !sig || len <= 0 || !out) return MM_ERR_INVALID_INPUT;
+  if (n_fft <= 0) return MM_ERR_INVALID_INPUT;
+  SpecRequest r;
+  r.out_kind = OUT_STFT_MAG;
+  r.sr = sr;
+  r.n_fft = n_fft;
+  r.hop = hop > 0 ? hop : n_fft / 4;
+  r.win = win > 0 ? win : n_fft;
+  r.center = center != 0;
+  if (r.hop <= 0) return MM_ERR_INVALID_INPUT;
+  if (!pow2(n_fft) || frame_count(len, n_fft, r.hop, r.center) == 0) {
+    fill_shape(out, {0}, 0, 0, nullptr);
+    return OK;
+  }
+  Lane* lane = static_cast<Ctx*>(ctx)->get();
+  if (!lane) return ERR_NO_GPU;
+  float* data = nullptr;
+  int64_t T = 0;
+  const int32_t s = run_descriptor(lane, r, sig, len, kind, param, &data, &T);
+  if (s != OK) return s;
+  fill_shape(out, {T}, 0, T, data);
   return OK;
 }
 
@@ -332,6 +362,32 @@ int32_t mm_chroma_stft(mm_context ctx, const float* signal_data, int64_t signal_
     return MM_OK;
   }
   return run_to_buffer(ctx, r, signal_data, signal_length, out);
+}
+
+int32_t mm_spectral_centroid(mm_context ctx, const float* signal_data, int64_t signal_length, int32_t sample_rate,
+                             int32_t n_fft, int32_t hop_length, int32_t win_length, int32_t center, MMBuffer* out) {
+  return descriptor_to_buffer(ctx, signal_data, signal_length, sample_rate, n_fft, hop_length, win_length, center,
+                              DESC_CENTROID, 0.0f, out);
+}
+
+int32_t mm_spectral_bandwidth(mm_context ctx, const float* signal_data, int64_t signal_length, int32_t sample_rate,
+                              int32_t n_fft, int32_t hop_length, int32_t win_length, int32_t center, float p,
+                              MMBuffer* out) {
+  return descriptor_to_buffer(ctx, signal_data, signal_length, sample_rate, n_fft, hop_length, win_length, center,
+                              DESC_BANDWIDTH, p, out);
+}
+
+int32_t mm_spectral_rolloff(mm_context ctx, const float* signal_data, int64_t signal_length, int32_t sample_rate,
+                            int32_t n_fft, int32_t hop_length, int32_t win_length, int32_t center, float roll_percent,
+                            MMBuffer* out) {
+  return descriptor_to_buffer(ctx, signal_data, signal_length, sample_rate, n_fft, hop_length, win_length, center,
+                              DESC_ROLLOFF, roll_percent, out);
+}
+
+int32_t mm_spectral_flatness(mm_context ctx, const float* signal_data, int64_t signal_length, int32_t sample_rate,
+                             int32_t n_fft, int32_t hop_length, int32_t win_length, int32_t center, MMBuffer* out) {
+  return descriptor_to_buffer(ctx, signal_data, signal_length, sample_rate, n_fft, hop_length, win_length, center,
+                              DESC_FLATNESS, 0.0f, out);
 }
 
 int32_t mm_mfcc(mm_context ctx, const float* signal_data, int64_t signal_length, int32_t sample_rate, int32_t n_mfcc,

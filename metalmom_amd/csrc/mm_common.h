@@ -27,6 +27,9 @@ enum PadMode : int { PAD_CONSTANT = 0, PAD_REFLECT = 1 };
 // (Chroma.swift:158-275; n_mels = n_chroma rows).
 enum FbKind : int { FB_MEL = 0, FB_CHROMA = 1 };
 
+// Per-frame reductions of the magnitude spectrogram (SpectralDescriptors.swift).
+enum DescKind : int { DESC_CENTROID = 0, DESC_BANDWIDTH = 1, DESC_ROLLOFF = 2, DESC_FLATNESS = 3 };
+
 constexpr int kTileFrames = 16;   // frames per tile = MFMA N dimension
 constexpr int kMelBlock = 8;      // mel rows per filterbank block of the fused contraction
 constexpr int kMaxMelBlocks = 32; // n_mels <= 256 on the fused path

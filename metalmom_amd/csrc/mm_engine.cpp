@@ -674,6 +674,56 @@ int32_t run_single(Lane* lane, const SpecRequest& r, const float* signal, int64_
   return OK;
 }
 
+int32_t run_descriptor(Lane* lane, const SpecRequest& r, const float* signal, int64_t len, int kind, float param,
+                       float** out, int64_t* frames) {
+  *out = nullptr;
+  const int T = (int)frame_count(len, r.n_fft, r.hop, r.center);
+  *frames = T;
+  if (T == 0) return OK;
+  Engine* eng = Engine::get();
+  if (!eng) return ERR_NO_GPU;
+  hipStream_t st = lane->stream;
+  const Tables* tb = nullptr;
+  int32_t s = eng->tables_for(r, st, &tb);
+  if (s != OK) return s;
+  TileLayout L;
+  const bool fused = tile_layout(r.n_fft, r.out_kind, r.precision, 0, 0, &L);
+  BatchMeta m;
+  build_meta(r, &len, 1, fused ? L.tile_frames : kTileFrames, &m);
+  // every buffer is sized before the first launch: d_in = signal | result,
+  // d_out = magnitude spectrogram
+  const int F = r.n_fft / 2 + 1;
+  const size_t in_bytes = (size_t)len * 4, res_off = (in_bytes + 255) & ~(size_t)255;
+  const size_t res_bytes = (size_t)T * 4;
+  const size_t mb = meta_bytes(m);
+  MM_HIP(lane->h_in.ensure(in_bytes + mb));
+  MM_HIP(lane->d_in.ensure(res_off + res_bytes));
+  MM_HIP(lane->d_meta.ensure(mb));
+  lane->meta_key.clear();   // d_meta now holds this call's work list
+  MM_HIP(lane->d_out.ensure((size_t)m.total_out * 4));
+  MM_HIP(lane->h_out.ensure(res_bytes));
+  std::memcpy(lane->h_in.p, signal, in_bytes);
+  pack_meta(m, static_cast<char*>(lane->h_in.p) + in_bytes);
+  MM_HIP(hipMemcpyAsync(lane->d_in.p, lane->h_in.p, in_bytes, hipMemcpyHostToDevice, st));
+  MM_HIP(hipMemcpyAsync(lane->d_meta.p, static_cast<char*>(lane->h_in.p) + in_bytes, mb, hipMemcpyHostToDevice, st));
+  const MetaPtrs mp = meta_ptrs(m, lane->d_meta.p);
+  s = enqueue_batch(eng, r, tb, m, mp, lane->d_in.p, lane->d_out.p, &lane->d_scratch, &lane->d_scratch2, 0, st);
+  if (s != OK) {
+    (void)hipStreamSynchronize(st);
+    return s;
+  }
+  float* res = reinterpret_cast<float*>(static_cast<char*>(lane->d_in.p) + res_off);
+  MM_HIP(launch_descriptor(static_cast<const float*>(lane->d_out.p), F, T, kind, (float)r.sr, (float)r.n_fft, param,
+                           res, st));
+  MM_HIP(hipMemcpyAsync(lane->h_out.p, res, res_bytes, hipMemcpyDeviceToHost, st));
+  MM_HIP(hipStreamSynchronize(st));
+  float* h = static_cast<float*>(std::malloc(res_bytes));
+  if (!h) return ERR_ALLOC;
+  std::memcpy(h, lane->h_out.p, res_bytes);
+  *out = h;
+  return OK;
+}
+
 OnsetShape onset_shape(const SpecRequest& r, int64_t len, int aggregate, int lag) {
   OnsetShape o;
   o.T = (int)frame_count(len, r.n_fft, r.hop, r.center);

@@ -62,6 +62,11 @@ hipError_t launch_generic_dct(const float* logmel, int n_mels, int T, const floa
 hipError_t launch_onset(const float* db, int n_bands, int T, const unsigned* max_key, float top_db, int lag, int pad,
                         int n_out, int aggregate, float* out, hipStream_t st);
 
+// Spectral descriptor (DescKind) of every frame of S [F, T] -> out [T];
+// param: bandwidth p, rolloff roll_percent.
+hipError_t launch_descriptor(const float* S, int F, int T, int kind, float srf, float nfftf, float param,
+                             float* out, hipStream_t st);
+
 // ---- inverse STFT ----
 struct IstftArgs {
   const float2* X;       // [n_fft/2 + 1][ld] complex, row-major (the reference's interleaved floats at ld = T)

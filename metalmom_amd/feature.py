@@ -1,8 +1,9 @@
 """Feature entry points, mirroring the reference ``python/metalmom/feature.py``.
 
 ``melspectrogram`` (``feature.py:8-91``), ``mfcc`` (``:94-167``),
-``amplitude_to_db`` (``:170-220``), ``power_to_db`` (``:223-270``) and
-``chroma_stft`` (``:273-360``) keep the reference signatures, defaults (hop 512 for mel/MFCC, ``n_mfcc=20``) and error
+``amplitude_to_db`` (``:170-220``), ``power_to_db`` (``:223-270``),
+``chroma_stft`` (``:273-360``) and the spectral descriptors (``:612-945``)
+keep the reference signatures, defaults (hop 512 for mel/MFCC, ``n_mfcc=20``) and error
 behaviour (``RuntimeError("mm_... failed with status s")``). Audio input runs
 on the GPU through libmetalmom.so. When a precomputed ``S`` is given, the
 reference applies the filterbank / DCT in NumPy/SciPy (``feature.py:50-57,
@@ -173,3 +174,95 @@ def chroma_stft(y=None, sr=22050, S=None, n_fft=2048, hop_length=None, win_lengt
         return _normalize_frames(result, norm) if norm is not None else result
     finally:
         lib.mm_destroy(ctx)
+
+
+def _descriptor(fn_name, y, sr, n_fft, hop_length, win_length, center, *extra):
+    y = np.ascontiguousarray(y, dtype=np.float32)
+    hop = hop_length if hop_length is not None else 512
+    win = win_length if win_length is not None else n_fft
+    ctx = lib.mm_init()
+    if not ctx:
+        raise RuntimeError("Failed to initialize MetalMom context")
+    try:
+        out = MMBuffer()
+        status = getattr(lib, fn_name)(ctx, float_ptr(y), len(y), sr, n_fft, hop, win, 1 if center else 0,
+                                       *extra, C.byref(out))
+        if status != 0:
+            raise RuntimeError(f"{fn_name} failed with status {status}")
+        return buffer_to_numpy(out).reshape(1, -1)
+    finally:
+        lib.mm_destroy(ctx)
+
+
+def _freqs(n_freqs, sr, n_fft):
+    return np.arange(n_freqs, dtype=np.float32) * sr / n_fft
+
+
+def spectral_centroid(y=None, sr=22050, S=None, n_fft=2048, hop_length=None, win_length=None, center=True,
+                      freq=None, **kwargs):
+    """Spectral centroid ``(1, n_frames)`` (the reference's ``feature.py:612-672``):
+    ``sum f S / sum S`` per frame of the magnitude STFT, on the GPU for audio
+    (default ``hop_length`` 512); ``S`` (a magnitude spectrogram) in NumPy."""
+    if S is not None:
+        S = np.ascontiguousarray(S, dtype=np.float32)
+        f = _freqs(S.shape[0], sr, (S.shape[0] - 1) * 2)
+        total = S.sum(axis=0, keepdims=True)
+        total = np.where(total > 0, total, 1.0)
+        return (f[:, None] * S).sum(axis=0, keepdims=True) / total
+    if y is None:
+        raise ValueError("Either y or S must be provided")
+    return _descriptor("mm_spectral_centroid", y, sr, n_fft, hop_length, win_length, center)
+
+
+def spectral_bandwidth(y=None, sr=22050, S=None, n_fft=2048, hop_length=None, win_length=None, center=True,
+                       freq=None, p=2, **kwargs):
+    """Spectral bandwidth ``(1, n_frames)`` (``feature.py:675-740``):
+    ``(sum S |f - centroid|^p / sum S)^(1/p)``."""
+    if S is not None:
+        S = np.ascontiguousarray(S, dtype=np.float32)
+        n_fft_s = (S.shape[0] - 1) * 2
+        f = _freqs(S.shape[0], sr, n_fft_s)
+        c = spectral_centroid(S=S, sr=sr, n_fft=n_fft_s)
+        total = S.sum(axis=0, keepdims=True)
+        total = np.where(total > 0, total, 1.0)
+        return ((S * np.abs(f[:, None] - c) ** p).sum(axis=0, keepdims=True) / total) ** (1.0 / p)
+    if y is None:
+        raise ValueError("Either y or S must be provided")
+    return _descriptor("mm_spectral_bandwidth", y, sr, n_fft, hop_length, win_length, center, float(p))
+
+
+def spectral_rolloff(y=None, sr=22050, S=None, n_fft=2048, hop_length=None, win_length=None, center=True,
+                     roll_percent=0.85, **kwargs):
+    """Spectral rolloff ``(1, n_frames)`` (``feature.py:808-880``): frequency of
+    the first bin where the cumulative magnitude reaches ``roll_percent`` of
+    the frame's total."""
+    if S is not None:
+        S = np.ascontiguousarray(S, dtype=np.float32)
+        f = _freqs(S.shape[0], sr, (S.shape[0] - 1) * 2)
+        total = S.sum(axis=0, keepdims=True)
+        thr = roll_percent * total
+        cum = np.cumsum(S, axis=0)
+        res = np.zeros((1, S.shape[1]), dtype=np.float32)
+        for t in range(S.shape[1]):
+            if total[0, t] > 0:
+                res[0, t] = f[min(int(np.searchsorted(cum[:, t], thr[0, t])), S.shape[0] - 1)]
+        return res
+    if y is None:
+        raise ValueError("Either y or S must be provided")
+    return _descriptor("mm_spectral_rolloff", y, sr, n_fft, hop_length, win_length, center, float(roll_percent))
+
+
+def spectral_flatness(y=None, sr=22050, S=None, n_fft=2048, hop_length=None, win_length=None, center=True,
+                      power=2.0, amin=1e-10, **kwargs):
+    """Spectral flatness ``(1, n_frames)`` (``feature.py:883-945``): geometric over
+    arithmetic mean of ``max(S^2, 1e-10)``; the audio path uses power 2 and
+    amin 1e-10 as the reference's bridge does, ``S`` uses ``amin`` as given."""
+    if S is not None:
+        S = np.ascontiguousarray(S, dtype=np.float32)
+        Sf = np.maximum(S, amin)
+        raw = S.sum(axis=0)
+        res = np.where(raw > amin, np.exp(np.mean(np.log(Sf), axis=0)) / np.mean(Sf, axis=0), 0.0)
+        return res.reshape(1, -1).astype(np.float32)
+    if y is None:
+        raise ValueError("Either y or S must be provided")
+    return _descriptor("mm_spectral_flatness", y, sr, n_fft, hop_length, win_length, center)

@@ -20,6 +20,7 @@ __all__ = [
     "linear_to_db", "power_to_db", "amplitude_to_db",
     "dct_ortho_basis", "mfcc", "mfcc_from_logmel", "onset_strength",
     "chroma_filterbank", "chroma_stft", "normalize_frames",
+    "spectral_centroid", "spectral_bandwidth", "spectral_rolloff", "spectral_flatness",
 ]
 
 
@@ -438,6 +439,65 @@ def chroma_stft(y, sr=22050, n_fft=2048, hop=512, win_length=None, n_chroma=12, 
     return normalize_frames(C, norm) if norm is not None else C
 
 
+# ---------------------------------------------------------------------------
+# Spectral descriptors (per-frame reductions of |STFT|)
+# ---------------------------------------------------------------------------
+
+def _freqs(n_freqs, sr, n_fft):
+    return np.arange(n_freqs, dtype=np.float64) * sr / n_fft
+
+
+def spectral_centroid(y, sr=22050, n_fft=2048, hop=512, win_length=None, center=True) -> np.ndarray:
+    """``sum_k f_k S_k / sum_k S_k`` per frame of the magnitude STFT, 0 for a
+    frame without energy; shape ``[1, frames]``
+    (``Features/SpectralDescriptors.swift:25-56``)."""
+    S = stft_magnitude(y, n_fft, hop, win_length, center)
+    f = _freqs(S.shape[0], sr, n_fft)
+    tot = S.sum(axis=0)
+    c = np.where(tot > 0, (f[:, None] * S).sum(axis=0) / np.where(tot > 0, tot, 1.0), 0.0)
+    return c[None, :]
+
+
+def spectral_bandwidth(y, sr=22050, n_fft=2048, hop=512, win_length=None, center=True, p=2.0) -> np.ndarray:
+    """``(sum_k S_k |f_k - centroid|^p / sum_k S_k)^(1/p)`` per frame, 0
+    without energy (``SpectralDescriptors.swift:70-111``)."""
+    S = stft_magnitude(y, n_fft, hop, win_length, center)
+    f = _freqs(S.shape[0], sr, n_fft)
+    c = spectral_centroid(y, sr, n_fft, hop, win_length, center)[0]
+    tot = S.sum(axis=0)
+    m = (S * np.abs(f[:, None] - c[None, :]) ** p).sum(axis=0)
+    bw = np.where(tot > 0, (m / np.where(tot > 0, tot, 1.0)) ** (1.0 / p), 0.0)
+    return bw[None, :]
+
+
+def spectral_rolloff(y, sr=22050, n_fft=2048, hop=512, win_length=None, center=True,
+                     roll_percent=0.85) -> np.ndarray:
+    """Frequency of the first bin at which the cumulative magnitude reaches
+    ``roll_percent`` of the frame's total, 0 without energy
+    (``SpectralDescriptors.swift:228-270``)."""
+    S = stft_magnitude(y, n_fft, hop, win_length, center)
+    f = _freqs(S.shape[0], sr, n_fft)
+    tot = S.sum(axis=0)
+    cum = np.cumsum(S, axis=0)
+    out = np.zeros(S.shape[1])
+    for t in range(S.shape[1]):
+        if tot[t] > 0:
+            out[t] = f[int(np.argmax(cum[:, t] >= roll_percent * tot[t]))]
+    return out[None, :]
+
+
+def spectral_flatness(y, n_fft=2048, hop=512, win_length=None, center=True, power=2.0, amin=1e-10) -> np.ndarray:
+    """Geometric over arithmetic mean of ``max(S^power, amin)`` per frame, 0
+    for a frame whose magnitude sum is below ``amin``
+    (``SpectralDescriptors.swift:295-340``)."""
+    S = stft_magnitude(y, n_fft, hop, win_length, center)
+    P = np.maximum(S ** power, amin)
+    am = P.mean(axis=0)
+    gm = np.exp(np.log(P).mean(axis=0))
+    out = np.where((S.sum(axis=0) >= amin) & (am > amin), gm / am, 0.0)
+    return out[None, :]
+
+
 PINS = {
     "hann_periodic": "known answer hann(4) = [0, .5, 1, .5] (WindowTests.swift:60-66); closed form",
     "prepare_window": "centre zero-pad / centre truncation indices checked by hand on small cases",
@@ -455,6 +515,10 @@ PINS = {
     "chroma_filterbank": "through the chroma goldens; unit column norms before octave weighting; roll puts C first",
     "chroma_stft": "goldens chroma_stft_440hz_{no_norm,default,1024}.npy (librosa.feature.chroma_stft)",
     "normalize_frames": "L1 / L2 / Linf norms of the columns equal 1; zero columns unchanged",
+    "spectral_centroid": "golden spectral_centroid_440hz_default.npy; pure tone -> its frequency; silence -> 0",
+    "spectral_bandwidth": "golden spectral_bandwidth_440hz_default.npy; a single bin has bandwidth 0",
+    "spectral_rolloff": "golden spectral_rolloff_440hz_default.npy; brute-force cumulative search",
+    "spectral_flatness": "golden spectral_flatness_440hz_default.npy; white noise near 1, tone near 0, flat -> 1",
     "onset_strength": "golden onset_strength_440hz_default.npy (librosa.onset.onset_strength); silence -> 0; "
                       "non-negative; shape / padding rules; planted mistakes (no rectification, pad width) fail",
 }

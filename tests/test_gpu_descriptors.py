@@ -1,0 +1,93 @@
+"""GPU parity of the spectral descriptors (mm_spectral_centroid / _bandwidth /
+_rolloff / _flatness, SpectralDescriptors.swift) against the reference's
+librosa goldens and the float64 oracle.
+
+Tolerances: centroid and bandwidth are ratios of FP32 sums over the FP32
+magnitude spectrum: 1e-5 relative to the largest value. Rolloff is a bin
+frequency: equal to the oracle on at least 97 % of frames and never more
+than one bin (sr / n_fft) away (a cumulative sum within rounding of the
+threshold may cross one bin later or earlier), compared as bin indices
+since the GPU writes the frequency in FP32 as the reference does. Flatness of noise-like
+frames: 1e-4 relative. Flatness of a pure tone is set by the FP32 FFT's
+noise floor against the 1e-10 power floor (most bins hold only rounding
+noise), so the 440 Hz golden is checked to within a factor of 2; the same
+noise, weighted by the squared distance from the centroid, puts the pure
+tone's bandwidth at 1e-3 relative (noise-like frames stay at 1e-5).
+"""
+import numpy as np
+import pytest
+
+from conftest import load_golden
+from oracle import spectral as o
+
+pytestmark = pytest.mark.gpu
+
+
+def _mix(n, sr, seed):
+    rng = np.random.default_rng(seed)
+    t = np.arange(n) / sr
+    x = 0.3 * rng.standard_normal(n)
+    for f in (180.0, 700.0, 2500.0):
+        x += 0.3 * np.sin(2 * np.pi * f * t * (1 + 0.1 * t))
+    return x.astype(np.float32)
+
+
+def _check(name, got, ref, sr, n_fft):
+    assert got.shape == ref.shape
+    if name == "spectral_rolloff":   # compared as bin indices (the GPU writes Float(k) * sr / n_fft in FP32)
+        d = np.abs(np.rint(got * n_fft / sr) - np.rint(ref * n_fft / sr))
+        assert (d <= 1).all() and (d == 0).mean() >= 0.97, (d.max(), (d == 0).mean())
+    elif name == "spectral_flatness":
+        assert np.abs(got - ref).max() <= 1e-4 * np.abs(ref).max()
+    else:
+        assert np.abs(got - ref).max() <= 1e-5 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("name", ["spectral_centroid", "spectral_bandwidth", "spectral_rolloff", "spectral_flatness"])
+def test_descriptor_goldens(signal_440, name):
+    import metalmom_amd as mm
+    y = signal_440.astype(np.float32)
+    got = getattr(mm, name)(y=y, sr=22050)
+    G = load_golden(f"{name}_440hz_default")
+    assert got.shape == G.shape == (1, 44) and got.dtype == np.float32
+    if name == "spectral_flatness":
+        assert (got > 0).all() and (got < 2 * G).all() and (got > 0.5 * G).all()
+    elif name == "spectral_bandwidth":
+        assert np.abs(got - G).max() <= 1e-3 * np.abs(G).max()
+    else:
+        _check(name, got.astype(np.float64), G.astype(np.float64), 22050, 2048)
+
+
+@pytest.mark.parametrize("name", ["spectral_centroid", "spectral_bandwidth", "spectral_rolloff", "spectral_flatness"])
+@pytest.mark.parametrize("sr,n,kw", [
+    (22050, 80000, {}),
+    (44100, 90000, {"n_fft": 4096, "hop_length": 1024}),
+    (22050, 50000, {"n_fft": 1024, "hop_length": 300, "win_length": 800, "center": False}),
+    (22050, 900000, {}),
+], ids=["default", "44k-4096", "1024-hop300-uncentred", "long"])
+def test_descriptor_matches_oracle(name, sr, n, kw):
+    import metalmom_amd as mm
+    print(f"{name} sr={sr} n={n} {kw}", flush=True)
+    y = _mix(n, sr, n % 83)
+    got = getattr(mm, name)(y=y, sr=sr, **kw).astype(np.float64)
+    okw = dict(n_fft=kw.get("n_fft", 2048), hop=kw.get("hop_length", 512), win_length=kw.get("win_length"),
+               center=kw.get("center", True))
+    ref = getattr(o, name)(y.astype(np.float64), **okw) if name == "spectral_flatness" else \
+        getattr(o, name)(y.astype(np.float64), sr, **okw)
+    _check(name, got, ref, sr, okw["n_fft"])
+
+
+def test_descriptor_params_and_edges():
+    import metalmom_amd as mm
+    y = _mix(60000, 22050, 3)
+    for p in (1.0, 3.0):
+        got = mm.spectral_bandwidth(y=y, p=p).astype(np.float64)
+        ref = o.spectral_bandwidth(y.astype(np.float64), p=p)
+        assert np.abs(got - ref).max() <= 1e-5 * np.abs(ref).max()
+    got = mm.spectral_rolloff(y=y, roll_percent=0.5).astype(np.float64)
+    ref = o.spectral_rolloff(y.astype(np.float64), roll_percent=0.5)
+    _check("spectral_rolloff", got, ref, 22050, 2048)
+    z = np.zeros(30000, np.float32)
+    for name in ("spectral_centroid", "spectral_bandwidth", "spectral_rolloff", "spectral_flatness"):
+        assert not getattr(mm, name)(y=z).any()                   # silence: 0 (no energy)
+        assert getattr(mm, name)(y=z, n_fft=1000).shape == (1, 0)  # no STFT frames

@@ -377,3 +377,41 @@ def test_normalize_frames_norms():
         N = o.normalize_frames(C, p)
         n = f(N)
         assert np.allclose(np.delete(n, 3), 1.0) and not N[:, 3].any()
+
+
+# -- spectral descriptors (SpectralDescriptors.swift) ---------------------------
+
+@pytest.mark.parametrize("name,tol", [("spectral_centroid", 1e-7), ("spectral_bandwidth", 1e-7),
+                                      ("spectral_rolloff", 0.0), ("spectral_flatness", 2e-6)])
+def test_descriptor_goldens(signal_440, name, tol):
+    G = load_golden(f"{name}_440hz_default")
+    fn = getattr(o, name)
+    v = fn(signal_440) if name == "spectral_flatness" else fn(signal_440, 22050)
+    assert v.shape == G.shape == (1, 44)
+    assert np.abs(v - G).max() <= tol * np.abs(G).max()
+
+
+def test_descriptor_known_answers():
+    sr, n = 22050, 22050
+    k = 100                                   # a tone exactly on bin 100 of a 2048-point frame
+    f0 = k * sr / 2048
+    t = np.arange(n) / sr
+    tone = np.sin(2 * np.pi * f0 * t)
+    c = o.spectral_centroid(tone, sr)[0, 4:-4]
+    assert np.allclose(c, f0, rtol=1e-3)      # Hann leakage is symmetric about the tone
+    assert (o.spectral_bandwidth(tone, sr)[0, 4:-4] < 30.0).all()   # within the window's main lobe
+    assert (o.spectral_flatness(tone)[0, 4:-4] < 1e-3).all()
+    noise = np.random.default_rng(0).standard_normal(n)
+    assert (o.spectral_flatness(noise)[0, 4:-4] > 0.3).all()
+    z = np.zeros(n)
+    assert not o.spectral_centroid(z).any() and not o.spectral_rolloff(z).any()
+    assert not o.spectral_flatness(z).any() and not o.spectral_bandwidth(z).any()
+    # rolloff against a direct search on one frame
+    S = o.stft_magnitude(noise, 2048, 512)
+    fr = 10
+    cum, tot = 0.0, S[:, fr].sum()
+    for kk in range(S.shape[0]):
+        cum += S[kk, fr]
+        if cum >= 0.85 * tot:
+            break
+    assert o.spectral_rolloff(noise, sr)[0, fr] == kk * sr / 2048
