@@ -270,6 +270,45 @@ def chroma_leg(torch, lens, d_in, steps):
     return res
 
 
+def descriptor_leg(torch, n_samples, iters):
+    """Spectral centroid / bandwidth / rolloff / flatness
+    (mm_spectral_descriptor_device) of the uneven batch joined into one clip
+    on device-resident data: n_fft 2048, hop 512, centred. Time per
+    descriptor from HIP events on the launch stream (fused STFT kernel writing
+    the magnitude spectrogram, then the per-frame reduction)."""
+    import ctypes as C
+    from metalmom_amd._native import lib
+    from metalmom_amd.workloads import uneven_batch
+    y = np.concatenate(uneven_batch())[:n_samples]
+    X = torch.from_numpy(y).cuda()
+    T = 1 + len(y) // 512
+    Y = torch.empty(T, device="cuda")
+    st = torch.cuda.Stream()
+    ctx = lib.mm_init()
+    res = {"workload": f"one clip of {len(y)} samples (the uneven batch joined), n_fft 2048, hop 512, centred",
+           "frames": T}
+    for kind, name, param in [(0, "centroid", 0.0), (1, "bandwidth", 2.0), (2, "rolloff", 0.85), (3, "flatness", 0.0)]:
+        def run():
+            s = lib.mm_spectral_descriptor_device(ctx, C.c_void_p(X.data_ptr()), len(y), 22050, 2048, 512, 2048, 1,
+                                                  kind, param, C.c_void_p(Y.data_ptr()), None,
+                                                  C.c_void_p(st.cuda_stream))
+            if s != 0:
+                raise RuntimeError(f"mm_spectral_descriptor_device failed with status {s}")
+        run()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(iters):
+            run()
+        e1.record(st)
+        e1.synchronize()
+        ms = e0.elapsed_time(e1) / iters
+        res[name] = {"step_ms": ms, "frames_per_s": T / (ms * 1e-3)}
+    lib.mm_destroy(ctx)
+    del X, Y
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -286,6 +325,7 @@ def main():
     ap.add_argument("--no-istft", action="store_true", help="skip the inverse-STFT leg")
     ap.add_argument("--no-onset", action="store_true", help="skip the onset-strength leg")
     ap.add_argument("--no-chroma", action="store_true", help="skip the chroma leg")
+    ap.add_argument("--no-descriptors", action="store_true", help="skip the spectral-descriptor leg")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="profiling aid: also run one device-to-device copy of this many bytes, "
                          "so HBM counters can be scaled by a kernel of known traffic")
@@ -441,6 +481,12 @@ def main():
         log(f"chroma batched: fp32 {chr_['f32']['frames_per_s'] / 1e6:.1f} M frames/s "
             f"(kernel {chr_['f32']['kernel_ms']:.3f} ms), bf16 {chr_['bf16']['frames_per_s'] / 1e6:.1f} M frames/s "
             f"(kernel {chr_['bf16']['kernel_ms']:.3f} ms)")
+
+    if not args.no_descriptors and not args.profile:
+        dsc = descriptor_leg(torch, B.total_input, max(3, steps // 2))
+        extra["spectral_descriptors"] = dsc
+        log("spectral descriptors: " + ", ".join(f"{k} {dsc[k]['frames_per_s'] / 1e6:.1f} M frames/s"
+                                                 for k in ("centroid", "bandwidth", "rolloff", "flatness")))
 
     cpu = None
     if not args.no_cpu and not args.profile:

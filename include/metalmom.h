@@ -205,6 +205,22 @@ typedef struct {
 int32_t mm_stft_complex(mm_context ctx, const float* signal_data, int64_t signal_length,
                         int32_t sample_rate, const MMSTFTParams* params, MMBuffer* out);
 
+/* Spectral descriptor kinds of mm_spectral_descriptor_device (param:
+ * bandwidth p, rolloff roll_percent; unused otherwise). */
+#define MM_DESC_CENTROID 0
+#define MM_DESC_BANDWIDTH 1
+#define MM_DESC_ROLLOFF 2
+#define MM_DESC_FLATNESS 3
+/* The mm_spectral_* descriptors on device memory: d_signal holds
+ * signal_length samples, d_out receives one value per STFT frame (count
+ * through out_frames, which may be NULL). Enqueued on `stream` without a host
+ * synchronisation after the first call for a clip length and parameter set.
+ * Needs a live context (it holds the magnitude spectrogram in between); calls
+ * on one context must not overlap. A non-power-of-two n_fft is rejected. */
+int32_t mm_spectral_descriptor_device(mm_context ctx, const float* d_signal, int64_t signal_length,
+                                      int32_t sample_rate, int32_t n_fft, int32_t hop_length,
+                                      int32_t win_length, int32_t center, int32_t kind, float param,
+                                      float* d_out, int64_t* out_frames, void* stream);
 /* mm_onset_strength on device memory: d_signal holds signal_length samples,
  * d_out receives the [rows, cols] result (rows 1 or n_mels, cols frames;
  * both reported through out_rows / out_cols, which may be NULL; size d_out

@@ -172,6 +172,25 @@ int32_t mm_stft(mm_context ctx, const float* signal_data, int64_t signal_length,
   return run_to_buffer(ctx, r, signal_data, signal_length, out);
 }
 
+int32_t mm_spectral_descriptor_device(mm_context ctx, const float* d_signal, int64_t signal_length,
+                                      int32_t sample_rate, int32_t n_fft, int32_t hop_length, int32_t win_length,
+                                      int32_t center, int32_t kind, float param, float* d_out, int64_t* out_frames,
+                                      void* stream) {
+  if (!ctx || !is_live(ctx) || !d_signal || signal_length <= 0 || !d_out) return MM_ERR_INVALID_INPUT;
+  if (!pow2(n_fft) || kind < MM_DESC_CENTROID || kind > MM_DESC_FLATNESS) return MM_ERR_INVALID_INPUT;
+  SpecRequest r;
+  r.out_kind = OUT_STFT_MAG;
+  r.sr = sample_rate;
+  r.n_fft = n_fft;
+  r.hop = hop_length > 0 ? hop_length : n_fft / 4;
+  r.win = win_length > 0 ? win_length : n_fft;
+  r.center = center != 0;
+  if (out_frames) *out_frames = frame_count(signal_length, n_fft, r.hop, r.center);
+  Lane* lane = static_cast<Ctx*>(ctx)->get();
+  if (!lane) return ERR_NO_GPU;
+  return enqueue_descriptor(lane, r, d_signal, signal_length, kind, param, d_out, static_cast<hipStream_t>(stream));
+}
+
 int32_t mm_onset_strength_device(mm_context ctx, const float* d_signal, int64_t signal_length, int32_t sample_rate,
                                  int32_t n_fft, int32_t hop_length, int32_t n_mels, float f_min, float f_max,
                                  int32_t center, int32_t aggregate, float* d_out, int64_t* out_rows,

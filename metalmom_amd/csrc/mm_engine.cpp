@@ -674,47 +674,68 @@ int32_t run_single(Lane* lane, const SpecRequest& r, const float* signal, int64_
   return OK;
 }
 
+// the clip's work list in the lane's d_meta, uploaded once per (length, request)
+static int32_t cached_meta(Lane* lane, const SpecRequest& r, int64_t len, int tile_frames, const BatchMeta& m,
+                           hipStream_t st) {
+  const std::string key = std::to_string(len) + ":" + tune_key(r, "") + ":" + std::to_string(r.center) + ":" +
+                          std::to_string(r.win) + ":" + std::to_string(r.sr) + ":" + std::to_string(r.fmin) + ":" +
+                          std::to_string(r.fmax) + ":" + std::to_string(r.tuning) + ":" + std::to_string(tile_frames);
+  if (lane->meta_key == key) return OK;
+  const size_t mb = meta_bytes(m);
+  MM_HIP(lane->h_meta.ensure(mb));
+  MM_HIP(lane->d_meta.ensure(mb));
+  pack_meta(m, lane->h_meta.p);
+  MM_HIP(hipMemcpyAsync(lane->d_meta.p, lane->h_meta.p, mb, hipMemcpyHostToDevice, st));
+  MM_HIP(hipStreamSynchronize(st));
+  lane->meta_key = key;
+  return OK;
+}
+
+int32_t enqueue_descriptor(Lane* lane, const SpecRequest& r, const float* d_x, int64_t len, int kind, float param,
+                           float* d_res, hipStream_t st) {
+  const int T = (int)frame_count(len, r.n_fft, r.hop, r.center);
+  if (T == 0) return OK;
+  Engine* eng = Engine::get();
+  if (!eng) return ERR_NO_GPU;
+  const Tables* tb = nullptr;
+  int32_t s = eng->tables_for(r, st, &tb);
+  if (s != OK) return s;
+  TileLayout L;
+  const bool fused = tile_layout(r.n_fft, r.out_kind, r.precision, 0, 0, &L);
+  const int tf = fused ? L.tile_frames : kTileFrames;
+  BatchMeta m;
+  build_meta(r, &len, 1, tf, &m);
+  MM_HIP(lane->d_out.ensure((size_t)m.total_out * 4));   // magnitude spectrogram [F, T]
+  s = cached_meta(lane, r, len, tf, m, st);
+  if (s != OK) return s;
+  const MetaPtrs mp = meta_ptrs(m, lane->d_meta.p);
+  s = enqueue_batch(eng, r, tb, m, mp, d_x, lane->d_out.p, &lane->d_scratch, &lane->d_scratch2, 0, st);
+  if (s != OK) return s;
+  MM_HIP(launch_descriptor(static_cast<const float*>(lane->d_out.p), r.n_fft / 2 + 1, T, kind, (float)r.sr,
+                           (float)r.n_fft, param, d_res, st));
+  return OK;
+}
+
 int32_t run_descriptor(Lane* lane, const SpecRequest& r, const float* signal, int64_t len, int kind, float param,
                        float** out, int64_t* frames) {
   *out = nullptr;
   const int T = (int)frame_count(len, r.n_fft, r.hop, r.center);
   *frames = T;
   if (T == 0) return OK;
-  Engine* eng = Engine::get();
-  if (!eng) return ERR_NO_GPU;
   hipStream_t st = lane->stream;
-  const Tables* tb = nullptr;
-  int32_t s = eng->tables_for(r, st, &tb);
-  if (s != OK) return s;
-  TileLayout L;
-  const bool fused = tile_layout(r.n_fft, r.out_kind, r.precision, 0, 0, &L);
-  BatchMeta m;
-  build_meta(r, &len, 1, fused ? L.tile_frames : kTileFrames, &m);
-  // every buffer is sized before the first launch: d_in = signal | result,
-  // d_out = magnitude spectrogram
-  const int F = r.n_fft / 2 + 1;
   const size_t in_bytes = (size_t)len * 4, res_off = (in_bytes + 255) & ~(size_t)255;
   const size_t res_bytes = (size_t)T * 4;
-  const size_t mb = meta_bytes(m);
-  MM_HIP(lane->h_in.ensure(in_bytes + mb));
-  MM_HIP(lane->d_in.ensure(res_off + res_bytes));
-  MM_HIP(lane->d_meta.ensure(mb));
-  lane->meta_key.clear();   // d_meta now holds this call's work list
-  MM_HIP(lane->d_out.ensure((size_t)m.total_out * 4));
+  MM_HIP(lane->h_in.ensure(in_bytes));
+  MM_HIP(lane->d_in.ensure(res_off + res_bytes));   // signal | result
   MM_HIP(lane->h_out.ensure(res_bytes));
   std::memcpy(lane->h_in.p, signal, in_bytes);
-  pack_meta(m, static_cast<char*>(lane->h_in.p) + in_bytes);
   MM_HIP(hipMemcpyAsync(lane->d_in.p, lane->h_in.p, in_bytes, hipMemcpyHostToDevice, st));
-  MM_HIP(hipMemcpyAsync(lane->d_meta.p, static_cast<char*>(lane->h_in.p) + in_bytes, mb, hipMemcpyHostToDevice, st));
-  const MetaPtrs mp = meta_ptrs(m, lane->d_meta.p);
-  s = enqueue_batch(eng, r, tb, m, mp, lane->d_in.p, lane->d_out.p, &lane->d_scratch, &lane->d_scratch2, 0, st);
+  float* res = reinterpret_cast<float*>(static_cast<char*>(lane->d_in.p) + res_off);
+  const int32_t s = enqueue_descriptor(lane, r, static_cast<const float*>(lane->d_in.p), len, kind, param, res, st);
   if (s != OK) {
     (void)hipStreamSynchronize(st);
     return s;
   }
-  float* res = reinterpret_cast<float*>(static_cast<char*>(lane->d_in.p) + res_off);
-  MM_HIP(launch_descriptor(static_cast<const float*>(lane->d_out.p), F, T, kind, (float)r.sr, (float)r.n_fft, param,
-                           res, st));
   MM_HIP(hipMemcpyAsync(lane->h_out.p, res, res_bytes, hipMemcpyDeviceToHost, st));
   MM_HIP(hipStreamSynchronize(st));
   float* h = static_cast<float*>(std::malloc(res_bytes));
@@ -753,21 +774,11 @@ int32_t enqueue_onset(Lane* lane, const SpecRequest& r, const float* d_x, int64_
   const bool fused = tile_layout(r.n_fft, r.out_kind, r.precision, r.n_mels, 0, &L);
   BatchMeta m;
   build_meta(r, &len, 1, fused ? L.tile_frames : kTileFrames, &m);
-  const size_t mb = meta_bytes(m);
   // every buffer is sized before the first launch
   MM_HIP(lane->d_out.ensure((size_t)m.total_out * 4));   // mel, then its dB in place
   MM_HIP(lane->d_small.ensure(256));                      // maximum key
-  const std::string key = std::to_string(len) + ":" + tune_key(r, "") + ":" + std::to_string(r.center) + ":" +
-                          std::to_string(r.win) + ":" + std::to_string(r.sr) + ":" + std::to_string(r.fmin) + ":" +
-                          std::to_string(r.fmax) + ":" + std::to_string(fused ? L.tile_frames : kTileFrames);
-  if (lane->meta_key != key) {
-    MM_HIP(lane->h_meta.ensure(mb));
-    MM_HIP(lane->d_meta.ensure(mb));
-    pack_meta(m, lane->h_meta.p);
-    MM_HIP(hipMemcpyAsync(lane->d_meta.p, lane->h_meta.p, mb, hipMemcpyHostToDevice, st));
-    MM_HIP(hipStreamSynchronize(st));
-    lane->meta_key = key;
-  }
+  s = cached_meta(lane, r, len, fused ? L.tile_frames : kTileFrames, m, st);
+  if (s != OK) return s;
   const MetaPtrs mp = meta_ptrs(m, lane->d_meta.p);
   s = enqueue_batch(eng, r, tb, m, mp, d_x, lane->d_out.p, &lane->d_scratch, &lane->d_scratch2, 0, st);
   if (s != OK) return s;

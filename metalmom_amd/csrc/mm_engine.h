@@ -180,6 +180,11 @@ OnsetShape onset_shape(const SpecRequest& r, int64_t len, int aggregate, int lag
 // the clip's work list is uploaded once per (length, request).
 int32_t enqueue_onset(Lane* lane, const SpecRequest& r, const float* d_x, int64_t len, int aggregate, int lag,
                       float* d_res, hipStream_t st);
+// Device signal d_x (len floats) -> descriptor per frame in d_res, enqueued on
+// st; the magnitude spectrogram lives in the lane's buffers, the clip's work
+// list is uploaded once per (length, request).
+int32_t enqueue_descriptor(Lane* lane, const SpecRequest& r, const float* d_x, int64_t len, int kind, float param,
+                           float* d_res, hipStream_t st);
 // Spectral descriptor of one clip: r is the magnitude STFT request; *out
 // (malloc'd) gets one value per frame, *frames their count. Synchronous.
 int32_t run_descriptor(Lane* lane, const SpecRequest& r, const float* signal, int64_t len, int kind, float param,

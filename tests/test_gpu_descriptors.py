@@ -91,3 +91,28 @@ def test_descriptor_params_and_edges():
     for name in ("spectral_centroid", "spectral_bandwidth", "spectral_rolloff", "spectral_flatness"):
         assert not getattr(mm, name)(y=z).any()                   # silence: 0 (no energy)
         assert getattr(mm, name)(y=z, n_fft=1000).shape == (1, 0)  # no STFT frames
+
+
+def test_descriptor_device_entry_matches_host_entry():
+    import ctypes as C
+    import torch
+    import metalmom_amd as mm
+    from metalmom_amd._native import lib
+    y = _mix(250000, 22050, 9)
+    d_y = torch.from_numpy(y).cuda()
+    T = 1 + len(y) // 512
+    d_o = torch.full((T,), float("nan"), device="cuda")
+    n = C.c_int64(0)
+    ctx = lib.mm_init()
+    try:
+        for kind, name, param, kw in [(0, "spectral_centroid", 0.0, {}), (1, "spectral_bandwidth", 2.0, {}),
+                                      (2, "spectral_rolloff", 0.85, {}), (3, "spectral_flatness", 0.0, {}),
+                                      (0, "spectral_centroid", 0.0, {})]:
+            s = lib.mm_spectral_descriptor_device(ctx, C.c_void_p(d_y.data_ptr()), len(y), 22050, 2048, 512, 2048, 1,
+                                                  kind, param, C.c_void_p(d_o.data_ptr()), C.byref(n),
+                                                  C.c_void_p(torch.cuda.current_stream().cuda_stream))
+            assert s == 0 and n.value == T
+            torch.cuda.synchronize()
+            assert np.array_equal(d_o.cpu().numpy()[None, :], getattr(mm, name)(y=y, **kw))
+    finally:
+        lib.mm_destroy(ctx)
