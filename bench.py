@@ -241,7 +241,25 @@ def onset_leg(torch, n_samples, iters, with_cpu=False):
     return res
 
 
-def chroma_leg(torch, lens, d_in, steps):
+def cpu_time_1thread(fn, warm, sample, frames, what):
+    """One-thread CPU baseline entry: fn timed on `sample` after a warm-up
+    call, frames/s over `frames`."""
+    try:
+        from threadpoolctl import threadpool_limits
+        limiter = threadpool_limits(limits=1)
+    except Exception:
+        limiter = None
+    fn(warm)
+    t0 = time.perf_counter()
+    fn(sample)
+    dt = time.perf_counter() - t0
+    if limiter is not None and hasattr(limiter, "unregister"):
+        limiter.unregister()
+    return {"value": frames / dt, "unit": "frames/s", "cores": 1, "kind": "port",
+            "sample": f"{what}, {dt:.1f} s, 1 thread", "host_cpus": os.cpu_count()}
+
+
+def chroma_leg(torch, lens, d_in, steps, clips=None):
     """Batched STFT chroma (12 bins, no frame normalisation) of the uneven
     batch already in HBM, FP32 and BF16 contraction: one persistent launch
     per step replayed from a hipGraph; fused-kernel time from HIP events."""
@@ -267,10 +285,17 @@ def chroma_leg(torch, lens, d_in, steps):
         B.close()
     res["workload"] = ("uneven batch (256 clips, 336,013 frames), n_fft 2048, hop 512, 12 chroma bins, "
                        "dense chroma filterbank on the fused banded contraction")
+    if clips is not None:
+        from oracle.cpu_baseline import ChromaF32
+        sample = np.concatenate(clips)[:20_000_000]
+        fr = 1 + len(sample) // 512
+        res["cpu_baseline"] = cpu_time_1thread(ChromaF32(), sample[:22050], sample, fr,
+                                               f"first {len(sample)} samples of the batch joined ({fr} frames), "
+                                               f"float32 numpy rfft + FB @ |X|^2")
     return res
 
 
-def descriptor_leg(torch, n_samples, iters):
+def descriptor_leg(torch, n_samples, iters, with_cpu=False):
     """Spectral centroid / bandwidth / rolloff / flatness
     (mm_spectral_descriptor_device) of the uneven batch joined into one clip
     on device-resident data: n_fft 2048, hop 512, centred. Time per
@@ -306,6 +331,13 @@ def descriptor_leg(torch, n_samples, iters):
         res[name] = {"step_ms": ms, "frames_per_s": T / (ms * 1e-3)}
     lib.mm_destroy(ctx)
     del X, Y
+    if with_cpu:
+        from oracle.cpu_baseline import CentroidF32
+        sample = y[:20_000_000]
+        fr = 1 + len(sample) // 512
+        res["cpu_baseline_centroid"] = cpu_time_1thread(CentroidF32(), sample[:22050], sample, fr,
+                                                        f"first {len(sample)} samples of the clip ({fr} frames), "
+                                                        f"float32 numpy rfft + |X| + centroid")
     return res
 
 
@@ -476,14 +508,14 @@ def main():
             + (f"; cpu {ons['cpu_baseline']['value']:.0f} frames/s" if "cpu_baseline" in ons else ""))
 
     if not args.no_chroma and not args.profile:
-        chr_ = chroma_leg(torch, lens, d_in, max(3, steps // 2))
+        chr_ = chroma_leg(torch, lens, d_in, max(3, steps // 2), None if args.no_cpu else clips)
         extra["chroma"] = chr_
         log(f"chroma batched: fp32 {chr_['f32']['frames_per_s'] / 1e6:.1f} M frames/s "
             f"(kernel {chr_['f32']['kernel_ms']:.3f} ms), bf16 {chr_['bf16']['frames_per_s'] / 1e6:.1f} M frames/s "
             f"(kernel {chr_['bf16']['kernel_ms']:.3f} ms)")
 
     if not args.no_descriptors and not args.profile:
-        dsc = descriptor_leg(torch, B.total_input, max(3, steps // 2))
+        dsc = descriptor_leg(torch, B.total_input, max(3, steps // 2), not args.no_cpu)
         extra["spectral_descriptors"] = dsc
         log("spectral descriptors: " + ", ".join(f"{k} {dsc[k]['frames_per_s'] / 1e6:.1f} M frames/s"
                                                  for k in ("centroid", "bandwidth", "rolloff", "flatness")))

@@ -105,3 +105,42 @@ class OnsetF32:
         flux = np.maximum(np.float32(0), db[:, 1:] - db[:, :-1]).mean(axis=0)
         pad = 1 + self.n_fft // (2 * self.hop)
         return np.concatenate([np.zeros(pad, np.float32), flux])[:T]
+
+
+def _magnitude_f32(y, n_fft, hop, win):
+    y = np.asarray(y, dtype=np.float32)
+    p = n_fft // 2
+    x = np.pad(y, (p, p))
+    T = 1 + (len(x) - n_fft) // hop
+    frames = np.lib.stride_tricks.as_strided(x, shape=(T, n_fft), strides=(x.strides[0] * hop, x.strides[0]))
+    X = np.fft.rfft(frames * win, axis=1)
+    return np.abs(X).astype(np.float32).T            # [F, T]
+
+
+class ChromaF32:
+    """Float32 STFT chroma (``Features/Chroma.swift:30-156``): power spectrogram
+    times the chroma filterbank (``vDSP_mmul``), no frame normalisation."""
+
+    def __init__(self, sr=22050, n_fft=2048, hop=512, n_chroma=12):
+        self.n_fft, self.hop = n_fft, hop
+        self.win = o.hann_periodic(n_fft).astype(np.float32)
+        self.fb = o.chroma_filterbank(sr, n_fft, n_chroma).astype(np.float32)
+
+    def __call__(self, y):
+        S = _magnitude_f32(y, self.n_fft, self.hop, self.win)
+        return self.fb @ (S * S)
+
+
+class CentroidF32:
+    """Float32 spectral centroid (``SpectralDescriptors.swift:25-56``) of the
+    magnitude STFT."""
+
+    def __init__(self, sr=22050, n_fft=2048, hop=512):
+        self.n_fft, self.hop = n_fft, hop
+        self.win = o.hann_periodic(n_fft).astype(np.float32)
+        self.f = (np.arange(n_fft // 2 + 1, dtype=np.float32) * np.float32(sr) / np.float32(n_fft))
+
+    def __call__(self, y):
+        S = _magnitude_f32(y, self.n_fft, self.hop, self.win)
+        tot = S.sum(axis=0)
+        return np.where(tot > 0, (self.f[:, None] * S).sum(axis=0) / np.where(tot > 0, tot, 1), 0)
