@@ -66,7 +66,7 @@ bool same_plan(const SpecRequest& a, const SpecRequest& b) {
   return a.out_kind == b.out_kind && a.sr == b.sr && a.n_fft == b.n_fft && a.hop == b.hop && a.win == b.win &&
          a.center == b.center && a.pad_mode == b.pad_mode && a.power == b.power && a.n_mels == b.n_mels &&
          a.fmin == b.fmin && a.fmax == b.fmax && a.n_mfcc == b.n_mfcc && a.top_db == b.top_db &&
-         a.precision == b.precision && a.in_dtype == b.in_dtype;
+         a.precision == b.precision && a.in_dtype == b.in_dtype && a.fb_kind == b.fb_kind && a.tuning == b.tuning;
 }
 
 size_t a256(size_t x) { return (x + 255) & ~(size_t)255; }

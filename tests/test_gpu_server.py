@@ -44,6 +44,23 @@ def test_mixed_requests_equal_alone(sources):
         assert lat > 0
 
 
+def test_chroma_and_mel_of_equal_shape_stay_apart(sources):
+    """Chroma is the banded contraction with another filterbank: requests of
+    both kinds with the same n_mels, n_fft and hop in one step must be
+    launched as separate plans."""
+    from metalmom_amd.serve import SpectralServer
+    import metalmom_amd
+    y = sources[22050][:5 * 22050]
+    srv = SpectralServer(hbm_budget_bytes=2 << 30)
+    ids = [srv.submit(y, srv.params(k, sr=22050, n_mels=12)) for k in ("mel", "chroma", "mel", "chroma")]
+    outs = [srv.wait(i)[0] for i in ids]
+    srv.close()
+    mel = metalmom_amd.spectral(y, kind="mel", sr=22050, n_fft=2048, hop_length=512, n_mels=12)
+    chroma = metalmom_amd.chroma_stft(y=y, sr=22050, n_fft=2048, hop_length=512)
+    for k, arr in zip(("mel", "chroma", "mel", "chroma"), outs):
+        assert np.array_equal(arr, mel if k == "mel" else chroma), k
+
+
 def test_results_match_oracle(sources):
     from metalmom_amd.serve import SpectralServer
     y = sources[22050][:3 * 22050]
