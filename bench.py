@@ -349,7 +349,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain single-clip leg")
     ap.add_argument("--profile", action="store_true", help="short run for a profiler")
-    ap.add_argument("--serve-rates", default="1000,5000",
+    ap.add_argument("--serve-rates", default="1000,5000,8000",
                     help="Poisson request rates (req/s) for the serving leg; empty to skip")
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--grid", type=int, default=0,
@@ -483,7 +483,9 @@ def main():
         sources = make_sources()
         for rate in [float(r) for r in args.serve_rates.split(",") if r]:
             srv = SpectralServer(hbm_budget_bytes=32 << 30)
-            open_loop(srv, rate=min(rate, 200.0), duration=0.25, seed=99, sources=sources)  # warm-up
+            # warm-up at the measured rate: grows the lane buffers and the
+            # process heap that results are copied into to their steady size
+            open_loop(srv, rate=rate, duration=0.5, seed=99, sources=sources)
             r = open_loop(srv, rate=rate, duration=args.serve_seconds, seed=4, sources=sources)
             st = srv.stats()
             srv.close()

@@ -57,10 +57,18 @@ int out_elem_floats(const SpecRequest& r) { return r.out_kind == OUT_STFT_COMPLE
 
 hipError_t DevBuf::ensure(size_t bytes) {
   if (bytes <= n && p) return hipSuccess;
+  // grow by at least half the old size, so that a stream of growing requests
+  // reallocates (and synchronises) a logarithmic number of times
+  const size_t old = n;
   release();
-  size_t want = std::max<size_t>(bytes, 256);
+  size_t want = std::max<size_t>(std::max<size_t>(bytes, old + old / 2), 256);
   want = (want + (1 << 20) - 1) & ~((size_t)(1 << 20) - 1);
   hipError_t e = hipMalloc(&p, want);
+  const size_t exact = (std::max<size_t>(bytes, 1) + (1 << 20) - 1) & ~((size_t)(1 << 20) - 1);
+  if (e != hipSuccess && want > exact) {  // no room for the headroom: the exact size
+    want = exact;
+    e = hipMalloc(&p, exact);
+  }
   if (e != hipSuccess) {
     p = nullptr;
     n = 0;
@@ -76,10 +84,18 @@ void DevBuf::release() {
 }
 hipError_t HostBuf::ensure(size_t bytes) {
   if (bytes <= n && p) return hipSuccess;
+  // grow by at least half the old size, so that a stream of growing requests
+  // reallocates (and synchronises) a logarithmic number of times
+  const size_t old = n;
   release();
-  size_t want = std::max<size_t>(bytes, 4096);
+  size_t want = std::max<size_t>(std::max<size_t>(bytes, old + old / 2), 4096);
   want = (want + (1 << 20) - 1) & ~((size_t)(1 << 20) - 1);
   hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
+  const size_t exact = (std::max<size_t>(bytes, 1) + (1 << 20) - 1) & ~((size_t)(1 << 20) - 1);
+  if (e != hipSuccess && want > exact) {  // no room for the headroom: the exact size
+    want = exact;
+    e = hipHostMalloc(&p, exact, hipHostMallocDefault);
+  }
   if (e != hipSuccess) {
     p = nullptr;
     n = 0;

@@ -8,6 +8,12 @@
 // others run on the GPU join the next step and overlap with them (H2D of step
 // n+1 and D2H of step n-1 run next to the kernels of step n). Device state
 // (tables, work lists, lane buffers) stays resident between steps.
+//
+// Host copies stay off the scheduler thread: submit() copies the clip straight
+// into a pinned input ring (one copy, on the client's thread), from which the
+// step's H2D copies read; each step's D2H lands in a pinned output ring, and
+// the client's poll()/wait() copies its own result out. When a ring is full the
+// request falls back to heap memory and the lane's staging buffers.
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -35,10 +41,79 @@ double now_s() {
   return duration_cast<duration<double>>(steady_clock::now().time_since_epoch()).count();
 }
 
+// FIFO allocations out of one pinned block. A range becomes reusable once it
+// and every range allocated before it have been released.
+class PinnedRing {
+ public:
+  ~PinnedRing() {
+    if (base_) (void)hipHostFree(base_);
+  }
+  bool init(size_t cap) {
+    if (hipHostMalloc(reinterpret_cast<void**>(&base_), cap, hipHostMallocDefault) != hipSuccess) {
+      base_ = nullptr;
+      return false;
+    }
+    cap_ = cap;
+    return true;
+  }
+  char* alloc(size_t n) {
+    n = (n + 255) & ~(size_t)255;
+    std::lock_guard<std::mutex> g(mu_);
+    if (!base_ || n == 0 || n > cap_) return nullptr;
+    size_t off;
+    if (live_.empty()) {
+      off = 0;
+    } else {
+      const size_t tail = live_.front().off, head = live_.back().off + live_.back().len;
+      if (head > tail) {  // live range [tail, head): room at the end, else at the start
+        if (cap_ - head >= n) off = head;
+        else if (tail >= n) off = 0;
+        else return nullptr;
+      } else {  // wrapped: live ranges [tail, cap) and [0, head)
+        if (tail - head >= n) off = head;
+        else return nullptr;
+      }
+    }
+    live_.push_back({off, n, false});
+    return base_ + off;
+  }
+  void release(const char* p) {
+    std::lock_guard<std::mutex> g(mu_);
+    const size_t off = (size_t)(p - base_);
+    for (auto& b : live_)
+      if (b.off == off && !b.freed) {
+        b.freed = true;
+        break;
+      }
+    while (!live_.empty() && live_.front().freed) live_.pop_front();
+  }
+
+ private:
+  struct Range {
+    size_t off, len;
+    bool freed;
+  };
+  std::mutex mu_;
+  char* base_ = nullptr;
+  size_t cap_ = 0;
+  std::deque<Range> live_;
+};
+
+// One step's results in the output ring, released when the last of its
+// requests has been collected.
+struct OutBlock {
+  char* p = nullptr;
+  int refs = 0;
+};
+
 struct Req {
   int64_t id = 0;
   SpecRequest r;
-  std::vector<float> sig;
+  int64_t len = 0;
+  const float* pin = nullptr;  // clip in the input ring, or
+  std::vector<float> sig;      // on the heap when the ring was full
+  const char* res = nullptr;   // result in the output ring (block blk), or `out`
+  OutBlock* blk = nullptr;
   double t_submit = 0, t_done = 0;
   int state = 0;  // 0 queued, 1 on device, 2 done
   int32_t status = OK;
@@ -57,6 +132,7 @@ struct Group {
 struct Step {
   Lane* lane = nullptr;
   hipEvent_t done = nullptr;
+  OutBlock* blk = nullptr;  // null: results staged in the lane's h_out
   std::vector<Group> groups;
   size_t in_bytes = 0, out_bytes = 0, meta_bytes_ = 0, dev_bytes = 0;
   int32_t status = OK;
@@ -84,6 +160,11 @@ class Server {
   int32_t start() {
     Engine* e = Engine::get();
     if (!e) return ERR_NO_GPU;
+    // a quarter of the HBM budget each, 32..512 MiB; a ring that cannot be
+    // pinned leaves every request on the heap path
+    const size_t ring = std::min<size_t>(std::max<size_t>(budget_ / 4, (size_t)32 << 20), (size_t)512 << 20);
+    (void)in_ring_.init(ring);
+    (void)out_ring_.init(ring);
     for (int i = 0; i < kLanes; ++i) {
       Lane* l = e->acquire_lane();
       if (!l) return ERR_INTERNAL;
@@ -104,6 +185,7 @@ class Server {
     for (auto& s : inflight_) {
       (void)hipEventSynchronize(s.done);
       (void)hipEventDestroy(s.done);
+      delete s.blk;
       free_lanes_.push_back(s.lane);
     }
     inflight_.clear();
@@ -111,6 +193,7 @@ class Server {
     free_lanes_.clear();
     for (auto& kv : reqs_) {
       if (kv.second->out) std::free(kv.second->out);
+      drop_result(kv.second);
       delete kv.second;
     }
     reqs_.clear();
@@ -119,7 +202,14 @@ class Server {
   int64_t submit(const SpecRequest& r, const float* sig, int64_t len) {
     auto* q = new Req();
     q->r = r;
-    q->sig.assign(sig, sig + len);
+    q->len = len;
+    char* pin = in_ring_.alloc((size_t)len * 4);
+    if (pin) {
+      std::memcpy(pin, sig, (size_t)len * 4);
+      q->pin = reinterpret_cast<const float*>(pin);
+    } else {
+      q->sig.assign(sig, sig + len);
+    }
     q->dev_bytes = request_bytes(r, len);
     q->t_submit = now_s();
     std::lock_guard<std::mutex> g(mu_);
@@ -128,6 +218,7 @@ class Server {
       q->status = ERR_ALLOC;
       q->state = 2;
       q->t_done = q->t_submit;
+      release_input(q);
     } else {
       queue_.push_back(q);
     }
@@ -149,6 +240,15 @@ class Server {
     g.unlock();
     if (latency) *latency = q->t_done - q->t_submit;
     int32_t s = q->status;
+    if (s == OK && q->res) {  // copy the result out of the output ring
+      const size_t n = (size_t)(q->rows * q->cols * out_elem_floats(q->r)) * 4;
+      if (out) {
+        q->out = static_cast<float*>(std::malloc(n));
+        if (q->out) std::memcpy(q->out, q->res, n);
+        else s = ERR_ALLOC;
+      }
+      drop_result(q);
+    }
     if (s == OK && out) {
       std::memset(out, 0, sizeof(*out));
       out->ndim = 2;
@@ -234,9 +334,10 @@ class Server {
     Lane* L = s.lane;
     hipStream_t st = L->stream;
     // layout of inputs, outputs and work lists in the lane buffers
+    size_t heap_bytes = 0;
     for (auto& G : s.groups) {
       std::vector<int64_t> lens;
-      for (Req* q : G.reqs) lens.push_back((int64_t)q->sig.size());
+      for (Req* q : G.reqs) lens.push_back(q->len);
       TileLayout TL;
       const bool fused =
           tile_layout(G.r.n_fft, G.r.out_kind, G.r.precision, G.r.n_mels, std::min(G.r.n_mfcc, G.r.n_mels), &TL);
@@ -247,26 +348,45 @@ class Server {
       s.out_bytes += a256((size_t)G.m.total_out * 4);
       G.meta_off = s.meta_bytes_;
       s.meta_bytes_ += a256(meta_bytes(G.m));
+      for (const Req* q : G.reqs)
+        if (!q->pin) heap_bytes += a256((size_t)q->len * 4);
     }
     int32_t st_code = OK;
     auto chk = [&](hipError_t e) {
       if (e != hipSuccess && st_code == OK) st_code = ERR_INTERNAL;
     };
-    chk(L->h_in.ensure(s.in_bytes + s.meta_bytes_));
+    // lane staging: work lists, then the clips that are not in the input ring
+    chk(L->h_in.ensure(s.meta_bytes_ + heap_bytes));
     chk(L->d_in.ensure(s.in_bytes));
     chk(L->d_meta.ensure(s.meta_bytes_));
     L->meta_key.clear();
     chk(L->d_out.ensure(s.out_bytes));
-    chk(L->h_out.ensure(s.out_bytes));
+    if (s.out_bytes > 0) {
+      if (char* p = out_ring_.alloc(s.out_bytes)) {
+        s.blk = new OutBlock();
+        s.blk->p = p;
+      }
+    }
+    if (!s.blk) chk(L->h_out.ensure(s.out_bytes));
     if (st_code == OK) {
       char* hin = static_cast<char*>(L->h_in.p);
+      char* heap_stage = hin + s.meta_bytes_;
+      char* din = static_cast<char*>(L->d_in.p);
       for (auto& G : s.groups) {
-        for (size_t i = 0; i < G.reqs.size(); ++i)
-          std::memcpy(hin + G.in_off + (size_t)G.m.clip_off[i] * 4, G.reqs[i]->sig.data(), G.reqs[i]->sig.size() * 4);
-        pack_meta(G.m, hin + s.in_bytes + G.meta_off);
+        for (size_t i = 0; i < G.reqs.size(); ++i) {
+          const Req* q = G.reqs[i];
+          const size_t off = G.in_off + (size_t)G.m.clip_off[i] * 4, n = (size_t)q->len * 4;
+          const void* src = q->pin;
+          if (!src) {  // heap clip: stage it in the lane's pinned buffer
+            std::memcpy(heap_stage, q->sig.data(), n);
+            src = heap_stage;
+            heap_stage += a256(n);
+          }
+          chk(hipMemcpyAsync(din + off, src, n, hipMemcpyHostToDevice, st));
+        }
+        pack_meta(G.m, hin + G.meta_off);
       }
-      chk(hipMemcpyAsync(L->d_in.p, hin, s.in_bytes, hipMemcpyHostToDevice, st));
-      chk(hipMemcpyAsync(L->d_meta.p, hin + s.in_bytes, s.meta_bytes_, hipMemcpyHostToDevice, st));
+      chk(hipMemcpyAsync(L->d_meta.p, hin, s.meta_bytes_, hipMemcpyHostToDevice, st));
       for (auto& G : s.groups) {
         if (st_code != OK) break;
         const Tables* tb = nullptr;
@@ -278,7 +398,7 @@ class Server {
         }
         if (r != OK) st_code = r;
       }
-      chk(hipMemcpyAsync(L->h_out.p, L->d_out.p, s.out_bytes, hipMemcpyDeviceToHost, st));
+      chk(hipMemcpyAsync(s.blk ? s.blk->p : L->h_out.p, L->d_out.p, s.out_bytes, hipMemcpyDeviceToHost, st));
     }
     chk(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
     if (s.done) chk(hipEventRecord(s.done, st));
@@ -286,6 +406,29 @@ class Server {
     std::lock_guard<std::mutex> g(mu_);
     ++steps_;
     inflight_.push_back(std::move(s));
+  }
+
+  void release_input(Req* q) {
+    if (q->pin) in_ring_.release(reinterpret_cast<const char*>(q->pin));
+    q->pin = nullptr;
+    std::vector<float>().swap(q->sig);
+  }
+
+  // Lets go of q's share of its step's output block.
+  void drop_result(Req* q) {
+    OutBlock* b = q->blk;
+    q->blk = nullptr;
+    q->res = nullptr;
+    if (!b) return;
+    bool last;
+    {
+      std::lock_guard<std::mutex> g(blk_mu_);
+      last = --b->refs == 0;
+    }
+    if (last) {
+      out_ring_.release(b->p);
+      delete b;
+    }
   }
 
   // Completes finished steps (the oldest first). block: wait for the oldest.
@@ -303,9 +446,10 @@ class Server {
       } else if (hipEventQuery(s->done) == hipErrorNotReady) {
         return;
       }
-      const char* hout = static_cast<const char*>(s->lane->h_out.p);
+      const char* hout = s->blk ? s->blk->p : static_cast<const char*>(s->lane->h_out.p);
       const double t = now_s();
       int64_t frames = 0;
+      int refs = 0;
       for (auto& G : s->groups) {
         const int ef = out_elem_floats(G.r);
         const int64_t rows = out_rows(G.r);
@@ -316,13 +460,33 @@ class Server {
           frames += q->cols;
           q->status = s->status;
           const size_t n = (size_t)(rows * q->cols * ef) * 4;
+          const char* src = hout + G.out_off + (size_t)G.m.out_off[i] * ef * 4;
           if (q->status == OK && n > 0) {
-            q->out = static_cast<float*>(std::malloc(n));
-            if (!q->out) q->status = ERR_ALLOC;
-            else std::memcpy(q->out, hout + G.out_off + (size_t)G.m.out_off[i] * ef * 4, n);
+            if (s->blk) {  // the client copies it out of the ring
+              q->res = src;
+              q->blk = s->blk;
+              ++refs;
+            } else {
+              q->out = static_cast<float*>(std::malloc(n));
+              if (!q->out) q->status = ERR_ALLOC;
+              else std::memcpy(q->out, src, n);
+            }
           }
-          std::vector<float>().swap(q->sig);
+          release_input(q);
         }
+      }
+      if (s->blk) {
+        bool unused;
+        {
+          std::lock_guard<std::mutex> g(blk_mu_);
+          s->blk->refs += refs;
+          unused = s->blk->refs == 0;
+        }
+        if (unused) {
+          out_ring_.release(s->blk->p);
+          delete s->blk;
+        }
+        s->blk = nullptr;
       }
       (void)hipEventDestroy(s->done);
       {
@@ -352,6 +516,8 @@ class Server {
   std::deque<Step> inflight_;
   std::vector<Lane*> free_lanes_;
   size_t in_use_ = 0;
+  PinnedRing in_ring_, out_ring_;
+  std::mutex blk_mu_;
   int64_t next_id_ = 1, submitted_ = 0, completed_ = 0, steps_ = 0, frames_ = 0;
   bool stopping_ = false;
   std::thread th_;
