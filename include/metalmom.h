@@ -146,6 +146,17 @@ int32_t mm_spectral_flatness(mm_context ctx, const float* signal_data, int64_t s
                              int32_t sample_rate, int32_t n_fft, int32_t hop_length,
                              int32_t win_length, int32_t center, MMBuffer* out);
 
+/* Replaces Bridge.swift:1087-1104 (mm_resample), reference metalmom.h:134-135:
+ * windowed-sinc sample-rate conversion (Resample.swift:12-96; sinc x Kaiser
+ * beta 6, 64 zero crossings, 512-point table with linear interpolation, the
+ * filter widened by target/source when downsampling), 1-D output of
+ * ceil(signal_length * target_sr / source_sr) samples; equal rates return a
+ * copy of the input. ctx may be NULL (the reference does not check it).
+ * Rejected (-1) where the reference would divide by zero or trap:
+ * source_sr <= 0 or target_sr <= 0. */
+int32_t mm_resample(mm_context ctx, const float* signal_data, int64_t signal_length,
+                    int32_t source_sr, int32_t target_sr, MMBuffer* out);
+
 /* Replaces Bridge.swift:402-493 (mm_mfcc): [min(n_mfcc, n_mels), n_frames];
  * power 2 mel, power_to_db(ref 1, amin 1e-10, top_db 80), DCT-II ortho. */
 int32_t mm_mfcc(mm_context ctx, const float* signal_data, int64_t signal_length,
@@ -235,6 +246,17 @@ int32_t mm_onset_strength_device(mm_context ctx, const float* d_signal, int64_t 
                                  int32_t n_mels, float f_min, float f_max, int32_t center,
                                  int32_t aggregate, float* d_out, int64_t* out_rows,
                                  int64_t* out_cols, void* stream);
+
+/* Number of samples mm_resample returns (ceil(n * target / source) in
+ * double; n for equal rates), or -1 for a rate <= 0 or n < 0. */
+int64_t mm_resample_length(int64_t signal_length, int32_t source_sr, int32_t target_sr);
+/* mm_resample on device memory: d_signal holds signal_length samples, d_out
+ * mm_resample_length(...) floats (it must not overlap d_signal); enqueued on
+ * `stream` (a hipStream_t; NULL = the null stream) without a host
+ * synchronisation after the first call, which builds the filter table. ctx
+ * may be NULL. */
+int32_t mm_resample_device(mm_context ctx, const float* d_signal, int64_t signal_length,
+                           int32_t source_sr, int32_t target_sr, float* d_out, void* stream);
 
 /* Number of samples mm_istft returns for this shape, or -1 if it would
  * reject it. */

@@ -76,6 +76,9 @@ SIGNATURES = {
     "mm_chroma_stft": (_I32, [_P, _F, _I64, _I32, _I32, _I32, _I32, _I32, _I32, _BUF]),
     "mm_onset_strength": (_I32, [_P, _F, _I64, _I32, _I32, _I32, _I32, C.c_float, C.c_float, _I32, _I32, _BUF]),
     "mm_istft_length": (_I64, [_I32, _I32, _I32, _I32, _I64]),
+    "mm_resample": (_I32, [_P, _F, _I64, _I32, _I32, _BUF]),
+    "mm_resample_length": (_I64, [_I64, _I32, _I32]),
+    "mm_resample_device": (_I32, [_P, _P, _I64, _I32, _I32, _P, _P]),
     "mm_spectral_descriptor_device": (_I32, [_P, _P, _I64, _I32, _I32, _I32, _I32, _I32, _I32, C.c_float, _P,
                                              C.POINTER(_I64), _P]),
     "mm_onset_strength_device": (_I32, [_P, _P, _I64, _I32, _I32, _I32, _I32, C.c_float, C.c_float, _I32, _I32,

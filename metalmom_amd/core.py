@@ -73,6 +73,29 @@ def istft(stft_matrix, hop_length=None, win_length=None, center=True, length=Non
         lib.mm_destroy(ctx)
 
 
+def resample(y, orig_sr, target_sr, **kwargs):
+    """Resamples a 1-D signal from ``orig_sr`` to ``target_sr`` (the
+    reference's ``core.py:115-160``): Kaiser-windowed sinc interpolation
+    (``mm_resample``) on the GPU. Equal rates return the input as float32
+    without a copy. Returns a 1-D float32 array of ``ceil(len(y) * target_sr
+    / orig_sr)`` samples.
+    """
+    if orig_sr == target_sr:
+        return np.asarray(y, dtype=np.float32)
+    y = np.ascontiguousarray(y, dtype=np.float32)
+    ctx = lib.mm_init()
+    if not ctx:
+        raise RuntimeError("Failed to initialize MetalMom context")
+    try:
+        out = MMBuffer()
+        status = lib.mm_resample(ctx, float_ptr(y), len(y), int(orig_sr), int(target_sr), C.byref(out))
+        if status != 0:
+            raise RuntimeError(f"mm_resample failed with status {status}")
+        return buffer_to_numpy(out)
+    finally:
+        lib.mm_destroy(ctx)
+
+
 def spectral(y, kind="mfcc", sr=22050, n_fft=2048, hop_length=512, win_length=None, center=True,
              pad_mode="constant", power=2.0, n_mels=128, fmin=0.0, fmax=None, n_mfcc=13,
              top_db=80.0, precision="f32"):

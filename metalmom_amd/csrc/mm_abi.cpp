@@ -12,6 +12,7 @@
 //   mm_power_to_db                   :267-306
 //   mm_mel_spectrogram               :310-400
 //   mm_mfcc                          :402-493
+//   mm_resample                      :1087-1104
 //   mm_buffer_free                   :3695-3711
 #include <cmath>
 #include <cstdlib>
@@ -321,6 +322,51 @@ int32_t mm_onset_strength(mm_context ctx, const float* signal_data, int64_t sign
   if (s != OK) return s;
   fill_shape(out, {rows, cols}, 0, rows * cols, data);
   return OK;
+}
+
+int32_t mm_resample(mm_context ctx, const float* signal_data, int64_t signal_length, int32_t source_sr,
+                    int32_t target_sr, MMBuffer* out) {
+  // Bridge.swift:1095-1097 checks the signal, its length and out, not the context
+  if (!signal_data || signal_length <= 0 || !out) return MM_ERR_INVALID_INPUT;
+  if (source_sr <= 0 || target_sr <= 0) return MM_ERR_INVALID_INPUT;
+  if (source_sr == target_sr) {   // Resample.swift:13: the signal itself
+    float* c = static_cast<float*>(std::malloc((size_t)signal_length * sizeof(float)));
+    if (!c) return MM_ERR_ALLOC_FAILED;
+    std::memcpy(c, signal_data, (size_t)signal_length * sizeof(float));
+    fill_shape(out, {signal_length}, 0, signal_length, c);
+    return OK;
+  }
+  Lane* lane = nullptr;
+  if (ctx && is_live(ctx)) lane = static_cast<Ctx*>(ctx)->get();
+  Engine* e = Engine::get();
+  if (!e) return ERR_NO_GPU;
+  bool borrowed = false;
+  if (!lane) {
+    lane = e->acquire_lane();
+    borrowed = true;
+    if (!lane) return ERR_INTERNAL;
+  }
+  float* data = nullptr;
+  int64_t n = 0;
+  const int32_t s = run_resample(lane, signal_data, signal_length, source_sr, target_sr, &data, &n);
+  if (borrowed) e->release_lane(lane);
+  if (s != OK) return s;
+  fill_shape(out, {n}, 0, n, data);
+  return OK;
+}
+
+int64_t mm_resample_length(int64_t signal_length, int32_t source_sr, int32_t target_sr) {
+  if (signal_length < 0 || source_sr <= 0 || target_sr <= 0) return -1;
+  return resample_length(signal_length, source_sr, target_sr);
+}
+
+int32_t mm_resample_device(mm_context ctx, const float* d_signal, int64_t signal_length, int32_t source_sr,
+                           int32_t target_sr, float* d_out, void* stream) {
+  (void)ctx;
+  if (!d_signal || signal_length <= 0 || !d_out) return MM_ERR_INVALID_INPUT;
+  if (source_sr <= 0 || target_sr <= 0) return MM_ERR_INVALID_INPUT;
+  if (!Engine::get()) return ERR_NO_GPU;
+  return enqueue_resample(d_signal, signal_length, source_sr, target_sr, d_out, static_cast<hipStream_t>(stream));
 }
 
 int32_t mm_amplitude_to_db(mm_context ctx, const float* data, int64_t count, float ref, float amin, float top_db,

@@ -318,6 +318,17 @@ int32_t Engine::tables_for(const SpecRequest& r, hipStream_t st, const Tables** 
   return OK;
 }
 
+int32_t Engine::resample_table(hipStream_t st, const float** out) {
+  std::lock_guard<std::mutex> g(mu_);
+  if (!rs_table_.p) {
+    MM_HIP(rs_table_.ensure((size_t)resample_table_len() * sizeof(float)));
+    MM_HIP(launch_tab_resample(static_cast<float*>(rs_table_.p), st));
+    MM_HIP(hipStreamSynchronize(st));   // shared by every stream from here on
+  }
+  *out = static_cast<const float*>(rs_table_.p);
+  return OK;
+}
+
 std::string tune_key(const SpecRequest& r, const std::string& arch) {
   std::ostringstream o;
   o << arch << ":k" << r.out_kind << ":n" << r.n_fft << ":h" << r.hop << ":m" << r.n_mels << ":c"
@@ -748,6 +759,57 @@ int32_t run_descriptor(Lane* lane, const SpecRequest& r, const float* signal, in
   MM_HIP(hipMemcpyAsync(lane->d_in.p, lane->h_in.p, in_bytes, hipMemcpyHostToDevice, st));
   float* res = reinterpret_cast<float*>(static_cast<char*>(lane->d_in.p) + res_off);
   const int32_t s = enqueue_descriptor(lane, r, static_cast<const float*>(lane->d_in.p), len, kind, param, res, st);
+  if (s != OK) {
+    (void)hipStreamSynchronize(st);
+    return s;
+  }
+  MM_HIP(hipMemcpyAsync(lane->h_out.p, res, res_bytes, hipMemcpyDeviceToHost, st));
+  MM_HIP(hipStreamSynchronize(st));
+  float* h = static_cast<float*>(std::malloc(res_bytes));
+  if (!h) return ERR_ALLOC;
+  std::memcpy(h, lane->h_out.p, res_bytes);
+  *out = h;
+  return OK;
+}
+
+int64_t resample_length(int64_t n, int source_sr, int target_sr) {
+  if (source_sr == target_sr) return n;
+  if (n <= 0) return 0;
+  return (int64_t)std::ceil((double)n * ((double)target_sr / (double)source_sr));   // Resample.swift:21-22
+}
+
+int32_t enqueue_resample(const float* d_x, int64_t n, int source_sr, int target_sr, float* d_y, hipStream_t st) {
+  if (n <= 0) return OK;
+  if (source_sr == target_sr) {   // Resample.swift:13: the signal unchanged
+    MM_HIP(hipMemcpyAsync(d_y, d_x, (size_t)n * sizeof(float), hipMemcpyDeviceToDevice, st));
+    return OK;
+  }
+  Engine* e = Engine::get();
+  if (!e) return ERR_NO_GPU;
+  const float* table = nullptr;
+  const int32_t s = e->resample_table(st, &table);
+  if (s != OK) return s;
+  MM_HIP(launch_resample(d_x, n, d_y, resample_length(n, source_sr, target_sr), source_sr, target_sr, table,
+                         e->num_cus(), st));
+  return OK;
+}
+
+int32_t run_resample(Lane* lane, const float* x, int64_t n, int source_sr, int target_sr, float** out,
+                     int64_t* n_out) {
+  *out = nullptr;
+  const int64_t m = resample_length(n, source_sr, target_sr);
+  *n_out = m;
+  if (m <= 0) return OK;
+  hipStream_t st = lane->stream;
+  const size_t in_bytes = (size_t)n * 4, res_off = (in_bytes + 255) & ~(size_t)255;
+  const size_t res_bytes = (size_t)m * 4;
+  MM_HIP(lane->h_in.ensure(in_bytes));
+  MM_HIP(lane->d_in.ensure(res_off + res_bytes));   // signal | result
+  MM_HIP(lane->h_out.ensure(res_bytes));
+  std::memcpy(lane->h_in.p, x, in_bytes);
+  MM_HIP(hipMemcpyAsync(lane->d_in.p, lane->h_in.p, in_bytes, hipMemcpyHostToDevice, st));
+  float* res = reinterpret_cast<float*>(static_cast<char*>(lane->d_in.p) + res_off);
+  const int32_t s = enqueue_resample(static_cast<const float*>(lane->d_in.p), n, source_sr, target_sr, res, st);
   if (s != OK) {
     (void)hipStreamSynchronize(st);
     return s;

@@ -85,6 +85,8 @@ class Engine {
 
   // Tables for a request (generated on first use, then shared).
   int32_t tables_for(const SpecRequest& r, hipStream_t st, const Tables** out);
+  // Filter table of the sample-rate converter (generated on first use).
+  int32_t resample_table(hipStream_t st, const float** out);
 
   // Persistent grid size: autotuned if known, else a default.
   int grid_for(const SpecRequest& r, int n_tiles, const TileLayout& L);
@@ -102,6 +104,7 @@ class Engine {
   std::mutex mu_;
   std::vector<Lane*> free_lanes_;
   std::vector<std::unique_ptr<Tables>> tables_;
+  DevBuf rs_table_;
   struct TuneEntry {
     std::string key;
     int wg_per_cu;
@@ -190,6 +193,15 @@ int32_t enqueue_descriptor(Lane* lane, const SpecRequest& r, const float* d_x, i
 int32_t run_descriptor(Lane* lane, const SpecRequest& r, const float* signal, int64_t len, int kind, float param,
                        float** out, int64_t* frames);
 // Host in, host out through the lane (synchronous); *out is malloc'd.
+// Sample-rate conversion (Resample.swift:12-96): output length
+// ceil(n * target / source) in double, n when the rates agree.
+int64_t resample_length(int64_t n, int source_sr, int target_sr);
+// Enqueues d_y[resample_length(n)] = d_x[n] converted on st (a device copy
+// when the rates agree).
+int32_t enqueue_resample(const float* d_x, int64_t n, int source_sr, int target_sr, float* d_y, hipStream_t st);
+// Host clip -> malloc'd result of *n_out samples (nullptr when empty).
+int32_t run_resample(Lane* lane, const float* x, int64_t n, int source_sr, int target_sr, float** out,
+                     int64_t* n_out);
 int32_t run_onset(Lane* lane, const SpecRequest& r, const float* signal, int64_t len, int aggregate, int lag,
                   float** out, int64_t* rows, int64_t* cols);
 

@@ -67,6 +67,16 @@ hipError_t launch_onset(const float* db, int n_bands, int T, const unsigned* max
 hipError_t launch_descriptor(const float* S, int F, int T, int kind, float srf, float nfftf, float param,
                              float* out, hipStream_t st);
 
+// ---- sample-rate conversion ----
+// FP32 filter table (sinc x Kaiser, 64 zero crossings, 512 points per
+// crossing); resample_table_len() floats.
+hipError_t launch_tab_resample(float* table, hipStream_t st);
+int resample_table_len();
+// y[n_out] = x[n] converted from source_sr to target_sr (rates differ, both
+// > 0); grid: persistent workgroups (one per CU).
+hipError_t launch_resample(const float* x, int64_t n, float* y, int64_t n_out, int source_sr, int target_sr,
+                           const float* table, int grid, hipStream_t st);
+
 // ---- inverse STFT ----
 struct IstftArgs {
   const float2* X;       // [n_fft/2 + 1][ld] complex, row-major (the reference's interleaved floats at ld = T)

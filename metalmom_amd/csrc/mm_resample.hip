@@ -1,0 +1,141 @@
+// Windowed-sinc sample-rate conversion on gfx950.
+//
+// Reference semantics followed (zakkeown/MetalMom):
+//   Resample.resample     Sources/MetalMomCore/Audio/Resample.swift:12-96
+//   Resample.besselI0     Sources/MetalMomCore/Audio/Resample.swift:100-112
+//   C ABI mm_resample     Sources/MetalMomBridge/Bridge.swift:1087-1104
+//
+// Output sample i sits at input position p = i / ratio (ratio = target /
+// source, both in double). It is scale * sum_j x[j] * w_j over j in
+// [floor(p) - support + 1, floor(p) + support] clipped to the signal, where
+// scale = min(1, ratio), support = ceil(64 / scale), and w_j interpolates the
+// filter table linearly at |j - p| * scale * 512; taps whose table index
+// reaches the last entry are skipped and an output whose weights sum to zero
+// or less is 0. The table holds sinc(x) * Kaiser(beta 6, 64 zero crossings) at
+// x = i / 512, computed in FP64 and rounded to FP32.
+//
+// Kernel: persistent, one workgroup of 1024 threads per CU. The workgroup
+// copies the 128 KB table into LDS once, then takes chunks of 1024 consecutive
+// outputs (one per thread); the chunk's input window (1023 / ratio + 2 support
+// samples) is staged in LDS too when it fits in 24 KB, so every tap reads LDS.
+// Tap positions are formed in FP64 exactly as the reference forms them
+// (j - p is exact in double, times the filter scale), the weights and sums in
+// FP32. The work per output is 2 * support taps of ~13 VALU operations and
+// three LDS reads: compute-bound, not HBM-bound (8 bytes of HBM per output).
+#include "mm_common.h"
+#include "mm_kernels.h"
+
+namespace mm {
+
+namespace {
+
+constexpr int kZeros = 64;
+constexpr int kOver = 512;
+constexpr int kTabLen = kZeros * kOver + 1;   // 32769
+constexpr int kTabLds = 32772;                // padded to a 16-byte multiple
+constexpr int kRsThreads = 1024;
+constexpr int kWinCap = 6144;                 // staged input samples (24 KB)
+
+__device__ double bessel_i0(double x) {
+  double sum = 1.0, term = 1.0;
+  const double half = x / 2.0;
+  for (int k = 1; k < 50; ++k) {
+    term *= half / (double)k;
+    const double t2 = term * term;
+    sum += t2;
+    if (t2 < sum * 1e-20) break;
+  }
+  return sum;
+}
+
+}  // namespace
+
+__global__ void k_tab_resample(float* __restrict__ T) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= kTabLen) return;
+  const double pi = 3.14159265358979323846;
+  const double x = (double)i / (double)kOver;
+  const double s = (i == 0) ? 1.0 : sin(pi * x) / (pi * x);
+  const double t = x / (double)kZeros;
+  const double k = (t >= 1.0) ? 0.0 : bessel_i0(6.0 * sqrt(1.0 - t * t)) / bessel_i0(6.0);
+  T[i] = (float)(s * k);
+}
+
+__global__ __launch_bounds__(kRsThreads) void k_resample(const float* __restrict__ x, int64_t n,
+                                                        float* __restrict__ y, int64_t n_out, double ratio,
+                                                        double fscale, float scale, int support, int win_cap,
+                                                        const float* __restrict__ table) {
+  extern __shared__ float lds[];
+  float* T = lds;
+  float* W = lds + kTabLds;
+  for (int i = threadIdx.x; i < kTabLen; i += kRsThreads) T[i] = table[i];
+  const int64_t n_chunks = (n_out + kRsThreads - 1) / kRsThreads;
+  for (int64_t c = blockIdx.x; c < n_chunks; c += gridDim.x) {
+    const int64_t i0 = c * kRsThreads;
+    const int64_t i1 = (i0 + kRsThreads < n_out ? i0 + kRsThreads : n_out) - 1;
+    // input window of the chunk: floor(i / ratio) is non-decreasing in i
+    int64_t lo = (int64_t)((double)i0 / ratio) - support + 1;
+    lo = lo < 0 ? 0 : lo;
+    int64_t hi = (int64_t)((double)i1 / ratio) + support;
+    hi = hi > n - 1 ? n - 1 : hi;
+    const int64_t span = hi - lo + 1;
+    const bool staged = span <= win_cap;
+    __syncthreads();   // the table is loaded; the previous chunk's reads of W are done
+    if (staged)
+      for (int k = threadIdx.x; k < span; k += kRsThreads) W[k] = x[lo + k];
+    __syncthreads();
+    const int64_t i = i0 + threadIdx.x;
+    if (i <= i1) {
+      const double p = (double)i / ratio;
+      const int64_t k = (int64_t)p;
+      const int64_t js = (k - support + 1 < 0) ? 0 : k - support + 1;
+      const int64_t je = (k + support > n - 1) ? n - 1 : k + support;
+      const float* src = staged ? W + (js - lo) : x + js;
+      const int taps = (int)(je - js + 1);
+      double jd = (double)js;   // j as a double (exact), so j - p rounds as the reference's does
+      float sum = 0.0f, wsum = 0.0f;
+      for (int m = 0; m < taps; ++m, jd += 1.0) {
+        const double fp = fabs(jd - p) * fscale;
+        const double fl = floor(fp);
+        const int fi = (int)fl;
+        if (fi < kTabLen - 1) {
+          const float fr = (float)(fp - fl);
+          const float w = T[fi] * (1.0f - fr) + T[fi + 1] * fr;
+          sum += src[m] * w;
+          wsum += w;
+        }
+      }
+      y[i] = wsum > 0.0f ? sum * scale : 0.0f;
+    }
+  }
+}
+
+hipError_t launch_tab_resample(float* table, hipStream_t st) {
+  hipLaunchKernelGGL(k_tab_resample, dim3((kTabLen + 255) / 256), dim3(256), 0, st, table);
+  return hipGetLastError();
+}
+
+int resample_table_len() { return kTabLen; }
+
+hipError_t launch_resample(const float* x, int64_t n, float* y, int64_t n_out, int source_sr, int target_sr,
+                           const float* table, int grid, hipStream_t st) {
+  const double ratio = (double)target_sr / (double)source_sr;
+  const double scale = ratio < 1.0 ? ratio : 1.0;
+  const int support = (int)std::ceil((double)kZeros / scale);
+  const int64_t n_chunks = (n_out + kRsThreads - 1) / kRsThreads;
+  if (n_chunks <= 0) return hipSuccess;
+  const int g = (int)std::min<int64_t>(std::max(grid, 1), n_chunks);
+  // the table must fit; the staged window takes what is left, up to 24 KB
+  const int64_t room = (int64_t)lds_limit_bytes() / 4 - kTabLds;
+  if (room < 0) return hipErrorInvalidConfiguration;
+  const int win_cap = (int)std::min<int64_t>(room, kWinCap);
+  static const hipError_t attr = hipFuncSetAttribute(
+      reinterpret_cast<const void*>(k_resample), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_limit_bytes());
+  if (attr != hipSuccess) return attr;
+  const size_t lds = (size_t)(kTabLds + win_cap) * sizeof(float);
+  hipLaunchKernelGGL(k_resample, dim3(g), dim3(kRsThreads), lds, st, x, n, y, n_out, ratio, scale * (double)kOver,
+                     (float)scale, support, win_cap, table);
+  return hipGetLastError();
+}
+
+}  // namespace mm

@@ -192,3 +192,37 @@ def test_chroma_precomputed_spectrogram_path_matches_goldens(signal_440):
         P = o.power_spectrogram(signal_440, n_fft, 512).astype(np.float32)
         C = mm.feature.chroma_stft(S=P, sr=22050, tuning=tuning, norm=norm)
         assert C.shape == G.shape and np.abs(C - G).max() <= 1e-5 * np.abs(G).max()
+
+
+# -- mm_resample (Bridge.swift:1087-1104, Resample.swift:12-96) ----------------
+
+def test_resample_length_and_validation():
+    lib = _lib()
+    from oracle import ingest as g
+    for n, a, b in [(44100, 44100, 22050), (22050, 22050, 44100), (1000, 48000, 22050), (7, 16000, 16000),
+                    (1, 44100, 8000), (0, 44100, 22050)]:
+        assert lib.mm_resample_length(n, a, b) == g.resample_length(n, a, b)
+    assert lib.mm_resample_length(10, 0, 22050) == -1 and lib.mm_resample_length(-1, 1, 2) == -1
+    from metalmom_amd._native import MMBuffer, float_ptr
+    y = np.ones(100, np.float32)
+    out = MMBuffer()
+    assert lib.mm_resample(None, None, 100, 44100, 22050, C.byref(out)) == -1
+    assert lib.mm_resample(None, float_ptr(y), 0, 44100, 22050, C.byref(out)) == -1
+    assert lib.mm_resample(None, float_ptr(y), 100, 0, 22050, C.byref(out)) == -1
+    assert lib.mm_resample(None, float_ptr(y), 100, 44100, -5, C.byref(out)) == -1
+    assert lib.mm_resample(None, float_ptr(y), 100, 44100, 22050, None) == -1
+
+
+def test_resample_identity_needs_no_gpu():
+    """test_resample.py:5-8 / ResampleTests.swift:5-10: equal rates return the
+    signal itself."""
+    import metalmom_amd as mm
+    from metalmom_amd._native import MMBuffer, float_ptr
+    y = np.sin(np.arange(22050, dtype=np.float32) * 440.0 * 2 * np.pi / 22050)
+    assert np.array_equal(mm.resample(y, 22050, 22050), y)
+    lib = _lib()
+    out = MMBuffer()
+    assert lib.mm_resample(None, float_ptr(y), len(y), 22050, 22050, C.byref(out)) == 0
+    from metalmom_amd._buffer import buffer_to_numpy
+    r = buffer_to_numpy(out)
+    assert r.shape == y.shape and np.array_equal(r, y)
