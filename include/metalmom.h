@@ -146,6 +146,23 @@ int32_t mm_spectral_flatness(mm_context ctx, const float* signal_data, int64_t s
                              int32_t sample_rate, int32_t n_fft, int32_t hop_length,
                              int32_t win_length, int32_t center, MMBuffer* out);
 
+/* Replace Bridge.swift:1008-1085 (mm_get_duration, mm_get_sample_rate,
+ * mm_load), reference metalmom.h:125-131. The reference decodes through
+ * AVFoundation; this library reads RIFF/WAVE files: PCM u8 / s16 / s24 / s32
+ * (also WAVE_FORMAT_EXTENSIBLE) and IEEE float32 / float64, any channel
+ * count; other files give MM_ERR_INVALID_INPUT. mm_load (AudioIO.swift:16-131):
+ * sr <= 0 keeps the file's rate, start = trunc(offset * file rate), duration
+ * > 0 reads at most trunc(duration * file rate) frames, no frames left gives
+ * an empty signal; 1-D float32 output (integer PCM scaled by 2^-(bits-1));
+ * channel 0 of a multichannel file, or the mean of its channels when mono !=
+ * 0; another rate is reached with mm_resample's converter on the GPU (the
+ * reference uses AVAudioConverter); *out_sr (may be NULL) receives the output
+ * rate. ctx may be NULL. A negative offset is rejected (-1). */
+int32_t mm_get_duration(mm_context ctx, const char* path, float* out_duration);
+int32_t mm_get_sample_rate(mm_context ctx, const char* path, int32_t* out_sr);
+int32_t mm_load(mm_context ctx, const char* path, int32_t sr, int32_t mono, float offset,
+                float duration, MMBuffer* out, int32_t* out_sr);
+
 /* Replaces Bridge.swift:1087-1104 (mm_resample), reference metalmom.h:134-135:
  * windowed-sinc sample-rate conversion (Resample.swift:12-96; sinc x Kaiser
  * beta 6, 64 zero crossings, 512-point table with linear interpolation, the
@@ -319,6 +336,22 @@ int32_t mm_server_create(mm_context ctx, int64_t hbm_budget_bytes, int32_t max_b
 /* Copies the clip and queues it; returns the request id (> 0) or a status. */
 int64_t mm_server_submit(mm_server s, const MMSpecParams* params, const float* signal_data,
                          int64_t signal_length);
+/* Interleaved little-endian PCM sample formats (integers scaled by
+ * 2^-(bits-1), unsigned 8-bit centred at 128). */
+#define MM_PCM_U8 1
+#define MM_PCM_S16 2
+#define MM_PCM_S24 3 /* packed, 3 bytes per sample */
+#define MM_PCM_S32 4
+#define MM_PCM_F32 5
+#define MM_PCM_F64 6
+/* Raw-PCM request: `frames` interleaved frames of `channels` (1..64) samples
+ * in `pcm_format` at source_sr. On the GPU, in the request's step, each frame
+ * becomes the mean of its channels (mono != 0) or its first channel, and the
+ * signal is resampled to params->sample_rate with mm_resample's converter when
+ * the rates differ; the result is the analysis of that clip, bitwise equal to
+ * running it alone. Copies the data and queues it; returns the request id. */
+int64_t mm_server_submit_pcm(mm_server s, const MMSpecParams* params, const void* pcm, int64_t frames,
+                             int32_t channels, int32_t pcm_format, int32_t source_sr, int32_t mono);
 /* 1: done (out filled, latency = completion - submission in seconds, id
  * released), 0: pending, < 0: the request's error status. */
 int32_t mm_server_poll(mm_server s, int64_t request_id, MMBuffer* out, double* latency_s);

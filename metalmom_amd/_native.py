@@ -24,6 +24,7 @@ MM_ERR_INTERNAL = -4
 
 KIND_STFT_MAG, KIND_STFT_POWER, KIND_STFT_COMPLEX, KIND_MEL, KIND_MFCC, KIND_CHROMA = range(6)
 PREC_F32, PREC_BF16, PREC_F16 = range(3)
+PCM_U8, PCM_S16, PCM_S24, PCM_S32, PCM_F32, PCM_F64 = range(1, 7)   # MM_PCM_*
 INPUT_F32, INPUT_F16 = range(2)
 PAD_CONSTANT, PAD_REFLECT = range(2)
 
@@ -77,6 +78,9 @@ SIGNATURES = {
     "mm_onset_strength": (_I32, [_P, _F, _I64, _I32, _I32, _I32, _I32, C.c_float, C.c_float, _I32, _I32, _BUF]),
     "mm_istft_length": (_I64, [_I32, _I32, _I32, _I32, _I64]),
     "mm_resample": (_I32, [_P, _F, _I64, _I32, _I32, _BUF]),
+    "mm_get_duration": (_I32, [_P, C.c_char_p, C.POINTER(C.c_float)]),
+    "mm_get_sample_rate": (_I32, [_P, C.c_char_p, C.POINTER(_I32)]),
+    "mm_load": (_I32, [_P, C.c_char_p, _I32, _I32, C.c_float, C.c_float, _BUF, C.POINTER(_I32)]),
     "mm_resample_length": (_I64, [_I64, _I32, _I32]),
     "mm_resample_device": (_I32, [_P, _P, _I64, _I32, _I32, _P, _P]),
     "mm_spectral_descriptor_device": (_I32, [_P, _P, _I64, _I32, _I32, _I32, _I32, _I32, _I32, C.c_float, _P,
@@ -101,6 +105,7 @@ SIGNATURES = {
     "mm_debug_set_phase_buffer": (_I32, [_P]),
     "mm_server_create": (_I32, [_P, _I64, _I32, C.POINTER(_P)]),
     "mm_server_submit": (_I64, [_P, C.POINTER(MMSpecParams), _F, _I64]),
+    "mm_server_submit_pcm": (_I64, [_P, C.POINTER(MMSpecParams), _P, _I64, _I32, _I32, _I32, _I32]),
     "mm_server_poll": (_I32, [_P, _I64, _BUF, C.POINTER(C.c_double)]),
     "mm_server_wait": (_I32, [_P, _I64, _BUF, C.POINTER(C.c_double)]),
     "mm_server_stats": (_I32, [_P, C.POINTER(_I64), C.POINTER(_I64), C.POINTER(_I64), C.POINTER(_I64)]),

@@ -1,7 +1,8 @@
-"""STFT and iSTFT entry points, mirroring the reference ``python/metalmom/core.py:198-318``."""
+"""STFT, iSTFT, audio loading and resampling entry points, mirroring the reference ``python/metalmom/core.py``."""
 from __future__ import annotations
 
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -69,6 +70,62 @@ def istft(stft_matrix, hop_length=None, win_length=None, center=True, length=Non
         if status != 0:
             raise RuntimeError(f"mm_istft failed with status {status}")
         return buffer_to_numpy(out)
+    finally:
+        lib.mm_destroy(ctx)
+
+
+def load(path, sr=22050, mono=True, offset=0.0, duration=None, **kwargs):
+    """Loads an audio file (the reference's ``core.py:8-58``): returns
+    ``(y, sr)`` with ``y`` a 1-D float32 array. ``sr=None`` keeps the file's
+    rate; another rate is reached with the GPU resampler. ``mono`` averages
+    the channels (otherwise channel 0 is returned); ``offset`` and
+    ``duration`` are in seconds. Reads RIFF/WAVE files (integer PCM of 8 to
+    32 bits, float32 / float64).
+    """
+    path_bytes = os.fsencode(path)
+    c_sr = int(sr) if sr is not None else 0
+    c_dur = float(duration) if duration is not None else 0.0
+    ctx = lib.mm_init()
+    if not ctx:
+        raise RuntimeError("Failed to initialize MetalMom context")
+    try:
+        out = MMBuffer()
+        out_sr = C.c_int32()
+        status = lib.mm_load(ctx, path_bytes, c_sr, 1 if mono else 0, float(offset), c_dur, C.byref(out),
+                             C.byref(out_sr))
+        if status != 0:
+            raise RuntimeError(f"mm_load failed with status {status}")
+        return buffer_to_numpy(out), int(out_sr.value)
+    finally:
+        lib.mm_destroy(ctx)
+
+
+def get_duration(path, **kwargs):
+    """Duration of an audio file in seconds (the reference's ``core.py:61-85``)."""
+    ctx = lib.mm_init()
+    if not ctx:
+        raise RuntimeError("Failed to initialize MetalMom context")
+    try:
+        d = C.c_float()
+        status = lib.mm_get_duration(ctx, os.fsencode(path), C.byref(d))
+        if status != 0:
+            raise RuntimeError(f"mm_get_duration failed with status {status}")
+        return float(d.value)
+    finally:
+        lib.mm_destroy(ctx)
+
+
+def get_samplerate(path, **kwargs):
+    """Sample rate of an audio file in Hz (the reference's ``core.py:88-112``)."""
+    ctx = lib.mm_init()
+    if not ctx:
+        raise RuntimeError("Failed to initialize MetalMom context")
+    try:
+        r = C.c_int32()
+        status = lib.mm_get_sample_rate(ctx, os.fsencode(path), C.byref(r))
+        if status != 0:
+            raise RuntimeError(f"mm_get_sample_rate failed with status {status}")
+        return int(r.value)
     finally:
         lib.mm_destroy(ctx)
 

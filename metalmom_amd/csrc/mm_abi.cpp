@@ -135,6 +135,10 @@ int32_t db_to_buffer(void* ctx, const float* data, int64_t count, float mult, fl
 
 }  // namespace
 
+namespace mm {
+Lane* ctx_lane(void* ctx) { return (ctx && is_live(ctx)) ? static_cast<Ctx*>(ctx)->get() : nullptr; }
+}  // namespace mm
+
 extern "C" {
 
 mm_context mm_init(void) {

@@ -30,6 +30,13 @@ enum FbKind : int { FB_MEL = 0, FB_CHROMA = 1 };
 // Per-frame reductions of the magnitude spectrogram (SpectralDescriptors.swift).
 enum DescKind : int { DESC_CENTROID = 0, DESC_BANDWIDTH = 1, DESC_ROLLOFF = 2, DESC_FLATNESS = 3 };
 
+// Interleaved little-endian PCM sample formats (the values of MM_PCM_*).
+enum PcmFormat : int { PCM_U8 = 1, PCM_S16 = 2, PCM_S24 = 3, PCM_S32 = 4, PCM_F32 = 5, PCM_F64 = 6 };
+__host__ __device__ inline int pcm_bytes(int fmt) {
+  return fmt == PCM_U8 ? 1 : fmt == PCM_S16 ? 2 : fmt == PCM_S24 ? 3 : fmt == PCM_F64 ? 8 : 4;
+}
+inline bool pcm_valid(int fmt) { return fmt >= PCM_U8 && fmt <= PCM_F64; }
+
 constexpr int kTileFrames = 16;   // frames per tile = MFMA N dimension
 constexpr int kMelBlock = 8;      // mel rows per filterbank block of the fused contraction
 constexpr int kMaxMelBlocks = 32; // n_mels <= 256 on the fused path

@@ -17,17 +17,6 @@
 
 namespace mm {
 
-#define MM_HIP(expr)                                 \
-  do {                                               \
-    hipError_t e_ = (expr);                          \
-    if (e_ != hipSuccess) return hip_status(e_);     \
-  } while (0)
-
-static int32_t hip_status(hipError_t e) {
-  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return ERR_ALLOC;
-  if (e == hipErrorNoDevice || e == hipErrorInvalidDevice) return ERR_NO_GPU;
-  return ERR_INTERNAL;
-}
 
 // ---------------------------------------------------------------------------
 // shapes

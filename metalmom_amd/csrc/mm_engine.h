@@ -193,6 +193,25 @@ int32_t enqueue_descriptor(Lane* lane, const SpecRequest& r, const float* d_x, i
 int32_t run_descriptor(Lane* lane, const SpecRequest& r, const float* signal, int64_t len, int kind, float param,
                        float** out, int64_t* frames);
 // Host in, host out through the lane (synchronous); *out is malloc'd.
+// Status of a failed HIP call; MM_HIP returns it from the enclosing function.
+#define MM_HIP(expr)                                 \
+  do {                                               \
+    hipError_t e_ = (expr);                          \
+    if (e_ != hipSuccess) return hip_status(e_);     \
+  } while (0)
+
+inline int32_t hip_status(hipError_t e) {
+  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return ERR_ALLOC;
+  if (e == hipErrorNoDevice || e == hipErrorInvalidDevice) return ERR_NO_GPU;
+  return ERR_INTERNAL;
+}
+
+// The lane of a live mm_context (nullptr for NULL or a destroyed context).
+Lane* ctx_lane(void* ctx);
+// Interleaved PCM (PcmFormat) -> one channel (mean when mix, else channel 0)
+// -> resampled from source_sr to target_sr, all on the GPU; malloc'd result.
+int32_t run_pcm_resample(Lane* lane, const void* raw, int64_t frames, int channels, int fmt, int mix,
+                         int source_sr, int target_sr, float** out, int64_t* n_out);
 // Sample-rate conversion (Resample.swift:12-96): output length
 // ceil(n * target / source) in double, n when the rates agree.
 int64_t resample_length(int64_t n, int source_sr, int target_sr);

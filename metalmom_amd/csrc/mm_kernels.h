@@ -67,6 +67,12 @@ hipError_t launch_onset(const float* db, int n_bands, int T, const unsigned* max
 hipError_t launch_descriptor(const float* S, int F, int T, int kind, float srf, float nfftf, float param,
                              float* out, hipStream_t st);
 
+// ---- PCM ingest ----
+// out[frames] = interleaved PCM (PcmFormat) -> float32: the mean of the
+// channels (mix) or the first channel.
+hipError_t launch_pcm_to_mono(const void* raw, int64_t frames, int channels, int fmt, int mix, float* out,
+                              int n_cu, hipStream_t st);
+
 // ---- sample-rate conversion ----
 // FP32 filter table (sinc x Kaiser, 64 zero crossings, 512 points per
 // crossing); resample_table_len() floats.

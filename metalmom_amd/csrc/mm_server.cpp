@@ -109,9 +109,13 @@ struct OutBlock {
 struct Req {
   int64_t id = 0;
   SpecRequest r;
-  int64_t len = 0;
-  const float* pin = nullptr;  // clip in the input ring, or
-  std::vector<float> sig;      // on the heap when the ring was full
+  int64_t len = 0;             // samples the analysis sees (after PCM conversion and resampling)
+  int fmt = 0;                 // 0: float32 samples; else the PcmFormat of interleaved PCM
+  int channels = 1, mix = 0, src_sr = 0;
+  int64_t frames = 0;          // PCM frames (fmt != 0)
+  size_t data_bytes = 0;       // bytes as submitted
+  const char* pin = nullptr;   // submitted data in the input ring, or
+  std::vector<char> heap;      // on the heap when the ring was full
   const char* res = nullptr;   // result in the output ring (block blk), or `out`
   OutBlock* blk = nullptr;
   double t_submit = 0, t_done = 0;
@@ -150,6 +154,13 @@ size_t a256(size_t x) { return (x + 255) & ~(size_t)255; }
 size_t request_bytes(const SpecRequest& r, int64_t len) {
   const int64_t T = frame_count(len, r.n_fft, r.hop, r.center);
   return a256((size_t)len * 4) + a256((size_t)(out_rows(r) * T * out_elem_floats(r)) * 4) + 1024;
+}
+
+// Device staging of a PCM request: its raw bytes, and its converted samples
+// when they still have to be resampled.
+size_t pcm_stage_bytes(const Req& q) {
+  if (!q.fmt) return 0;
+  return a256(q.data_bytes) + (q.src_sr != q.r.sr ? a256((size_t)q.frames * 4) : 0);
 }
 
 class Server {
@@ -199,18 +210,28 @@ class Server {
     reqs_.clear();
   }
 
-  int64_t submit(const SpecRequest& r, const float* sig, int64_t len) {
+  // A float32 clip of len samples, or (fmt != 0) `frames` frames of
+  // interleaved PCM at src_sr that become a clip of len samples at r.sr.
+  int64_t submit(const SpecRequest& r, const void* data, int64_t len, int fmt = 0, int64_t frames = 0,
+                 int channels = 1, int mix = 0, int src_sr = 0) {
     auto* q = new Req();
     q->r = r;
     q->len = len;
-    char* pin = in_ring_.alloc((size_t)len * 4);
+    q->fmt = fmt;
+    q->frames = frames;
+    q->channels = channels;
+    q->mix = mix;
+    q->src_sr = src_sr;
+    q->data_bytes = fmt ? (size_t)frames * channels * pcm_bytes(fmt) : (size_t)len * 4;
+    char* pin = in_ring_.alloc(q->data_bytes);
     if (pin) {
-      std::memcpy(pin, sig, (size_t)len * 4);
-      q->pin = reinterpret_cast<const float*>(pin);
+      std::memcpy(pin, data, q->data_bytes);
+      q->pin = pin;
     } else {
-      q->sig.assign(sig, sig + len);
+      const char* b = static_cast<const char*>(data);
+      q->heap.assign(b, b + q->data_bytes);
     }
-    q->dev_bytes = request_bytes(r, len);
+    q->dev_bytes = request_bytes(r, len) + pcm_stage_bytes(*q);
     q->t_submit = now_s();
     std::lock_guard<std::mutex> g(mu_);
     q->id = next_id_++;
@@ -334,7 +355,7 @@ class Server {
     Lane* L = s.lane;
     hipStream_t st = L->stream;
     // layout of inputs, outputs and work lists in the lane buffers
-    size_t heap_bytes = 0;
+    size_t heap_bytes = 0, stage_bytes = 0;
     for (auto& G : s.groups) {
       std::vector<int64_t> lens;
       for (Req* q : G.reqs) lens.push_back(q->len);
@@ -348,8 +369,10 @@ class Server {
       s.out_bytes += a256((size_t)G.m.total_out * 4);
       G.meta_off = s.meta_bytes_;
       s.meta_bytes_ += a256(meta_bytes(G.m));
-      for (const Req* q : G.reqs)
-        if (!q->pin) heap_bytes += a256((size_t)q->len * 4);
+      for (const Req* q : G.reqs) {
+        if (!q->pin) heap_bytes += a256(q->data_bytes);
+        stage_bytes += pcm_stage_bytes(*q);
+      }
     }
     int32_t st_code = OK;
     auto chk = [&](hipError_t e) {
@@ -357,7 +380,7 @@ class Server {
     };
     // lane staging: work lists, then the clips that are not in the input ring
     chk(L->h_in.ensure(s.meta_bytes_ + heap_bytes));
-    chk(L->d_in.ensure(s.in_bytes));
+    chk(L->d_in.ensure(s.in_bytes + stage_bytes));   // clips | PCM staging
     chk(L->d_meta.ensure(s.meta_bytes_));
     L->meta_key.clear();
     chk(L->d_out.ensure(s.out_bytes));
@@ -372,17 +395,36 @@ class Server {
       char* hin = static_cast<char*>(L->h_in.p);
       char* heap_stage = hin + s.meta_bytes_;
       char* din = static_cast<char*>(L->d_in.p);
+      char* dstage = din + s.in_bytes;
       for (auto& G : s.groups) {
         for (size_t i = 0; i < G.reqs.size(); ++i) {
           const Req* q = G.reqs[i];
-          const size_t off = G.in_off + (size_t)G.m.clip_off[i] * 4, n = (size_t)q->len * 4;
+          char* dst = din + G.in_off + (size_t)G.m.clip_off[i] * 4;
           const void* src = q->pin;
-          if (!src) {  // heap clip: stage it in the lane's pinned buffer
-            std::memcpy(heap_stage, q->sig.data(), n);
+          if (!src) {  // heap data: stage it in the lane's pinned buffer
+            std::memcpy(heap_stage, q->heap.data(), q->data_bytes);
             src = heap_stage;
-            heap_stage += a256(n);
+            heap_stage += a256(q->data_bytes);
           }
-          chk(hipMemcpyAsync(din + off, src, n, hipMemcpyHostToDevice, st));
+          if (!q->fmt) {
+            chk(hipMemcpyAsync(dst, src, q->data_bytes, hipMemcpyHostToDevice, st));
+            continue;
+          }
+          // PCM: raw bytes -> one float channel (-> resampled) into the clip's slot
+          char* raw = dstage;
+          dstage += a256(q->data_bytes);
+          chk(hipMemcpyAsync(raw, src, q->data_bytes, hipMemcpyHostToDevice, st));
+          if (q->src_sr == G.r.sr) {
+            chk(launch_pcm_to_mono(raw, q->frames, q->channels, q->fmt, q->mix, reinterpret_cast<float*>(dst),
+                                   eng->num_cus(), st));
+          } else {
+            float* mono = reinterpret_cast<float*>(dstage);
+            dstage += a256((size_t)q->frames * 4);
+            chk(launch_pcm_to_mono(raw, q->frames, q->channels, q->fmt, q->mix, mono, eng->num_cus(), st));
+            if (enqueue_resample(mono, q->frames, q->src_sr, G.r.sr, reinterpret_cast<float*>(dst), st) != OK &&
+                st_code == OK)
+              st_code = ERR_INTERNAL;
+          }
         }
         pack_meta(G.m, hin + G.meta_off);
       }
@@ -409,9 +451,9 @@ class Server {
   }
 
   void release_input(Req* q) {
-    if (q->pin) in_ring_.release(reinterpret_cast<const char*>(q->pin));
+    if (q->pin) in_ring_.release(q->pin);
     q->pin = nullptr;
-    std::vector<float>().swap(q->sig);
+    std::vector<char>().swap(q->heap);
   }
 
   // Lets go of q's share of its step's output block.
@@ -548,6 +590,20 @@ int64_t mm_server_submit(mm_server srv, const MMSpecParams* params, const float*
   if (s == OK && r.in_dtype != IN_F32) s = ERR_INVALID;   // host clips are float32
   if (s != OK) return s;
   return static_cast<Server*>(srv)->submit(r, signal_data, signal_length);
+}
+
+int64_t mm_server_submit_pcm(mm_server srv, const MMSpecParams* params, const void* pcm, int64_t frames,
+                             int32_t channels, int32_t pcm_format, int32_t source_sr, int32_t mono) {
+  if (!srv || !pcm || frames <= 0 || channels < 1 || channels > 64 || !pcm_valid(pcm_format) || source_sr <= 0)
+    return MM_ERR_INVALID_INPUT;
+  SpecRequest r;
+  int32_t s = request_from_params(params, &r);
+  if (s == OK && r.in_dtype != IN_F32) s = ERR_INVALID;   // the analysis input is float32
+  if (s != OK) return s;
+  const int64_t len = resample_length(frames, source_sr, r.sr);
+  if (len <= 0) return MM_ERR_INVALID_INPUT;
+  return static_cast<Server*>(srv)->submit(r, pcm, len, pcm_format, frames, channels,
+                                           (mono != 0 && channels > 1) ? 1 : 0, source_sr);
 }
 
 int32_t mm_server_poll(mm_server srv, int64_t request_id, MMBuffer* out, double* latency_s) {

@@ -54,6 +54,21 @@ class SpectralServer:
             raise RuntimeError(f"mm_server_submit failed with status {rid}")
         return rid
 
+    def submit_pcm(self, pcm, channels, pcm_format, source_sr, params, mono=True) -> int:
+        """Queues raw interleaved PCM (``pcm_format`` one of ``_native.PCM_*``,
+        ``channels`` samples per frame, at ``source_sr``); the GPU converts it
+        to one channel and resamples it to the analysis rate of ``params``."""
+        buf = np.ascontiguousarray(pcm)
+        width = {1: 1, 2: 2, 3: 3, 4: 4, 5: 4, 6: 8}[int(pcm_format)]
+        frames = buf.nbytes // (width * channels)
+        if buf.nbytes != frames * width * channels:
+            raise ValueError("PCM buffer is not a whole number of frames")
+        rid = lib.mm_server_submit_pcm(self._h, C.byref(params), C.c_void_p(buf.ctypes.data), frames,
+                                       int(channels), int(pcm_format), int(source_sr), 1 if mono else 0)
+        if rid <= 0:
+            raise RuntimeError(f"mm_server_submit_pcm failed with status {rid}")
+        return rid
+
     def wait(self, rid: int):
         """Blocks until request ``rid`` is done; returns (array, latency_s)."""
         out, lat = MMBuffer(), C.c_double()
