@@ -341,6 +341,58 @@ def descriptor_leg(torch, n_samples, iters, with_cpu=False):
     return res
 
 
+def resample_leg(torch, n_samples, iters, with_cpu=False):
+    """Sample-rate conversion (mm_resample_device) of the uneven batch joined
+    into one clip, taken as 44.1 kHz audio and converted to 22.05 kHz (the
+    server's two rates): 256 taps per output. Kernel time from HIP events on
+    the launch stream; algorithmic HBM bytes = 4 per input + 4 per output
+    sample (the kernel is bound by its FP64 tap positions and LDS reads, not
+    by HBM)."""
+    import ctypes as C
+    from metalmom_amd._native import lib
+    from metalmom_amd.workloads import uneven_batch
+    y = np.concatenate(uneven_batch())[:n_samples]
+    src, dst = 44100, 22050
+    m = lib.mm_resample_length(len(y), src, dst)
+    X = torch.from_numpy(y).cuda()
+    Y = torch.empty(m, device="cuda")
+    st = torch.cuda.Stream()
+
+    def run():
+        s = lib.mm_resample_device(None, C.c_void_p(X.data_ptr()), len(y), src, dst, C.c_void_p(Y.data_ptr()),
+                                   C.c_void_p(st.cuda_stream))
+        if s != 0:
+            raise RuntimeError(f"mm_resample_device failed with status {s}")
+    run()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(iters):
+        run()
+    e1.record(st)
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / iters
+    taps = 2 * int(np.ceil(64 / min(1.0, dst / src)))
+    bytes_launch = 4 * (len(y) + m)
+    gbs = bytes_launch / (ms * 1e-3) / 1e9
+    res = {"workload": f"one clip of {len(y)} samples (the uneven batch joined) at 44.1 kHz -> 22.05 kHz, "
+                       f"{m} output samples, {taps} taps each",
+           "kernel_ms": ms, "out_samples_per_s": m / (ms * 1e-3), "in_samples_per_s": len(y) / (ms * 1e-3),
+           "taps_per_s": m * taps / (ms * 1e-3),
+           "hbm": {"achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+                   "bytes_per_launch": bytes_launch}}
+    del X, Y
+    if with_cpu:
+        from oracle.cpu_baseline import ResampleF32
+        sample = y[:2_000_000]
+        ms_ = len(sample) // 2
+        r = cpu_time_1thread(ResampleF32(src, dst), sample[:20000], sample, ms_,
+                             f"first {len(sample)} samples ({ms_} outputs), float32 numpy restatement")
+        r["unit"] = "output samples/s"
+        res["cpu_baseline"] = r
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -358,6 +410,7 @@ def main():
     ap.add_argument("--no-onset", action="store_true", help="skip the onset-strength leg")
     ap.add_argument("--no-chroma", action="store_true", help="skip the chroma leg")
     ap.add_argument("--no-descriptors", action="store_true", help="skip the spectral-descriptor leg")
+    ap.add_argument("--no-resample", action="store_true", help="skip the sample-rate conversion leg")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="profiling aid: also run one device-to-device copy of this many bytes, "
                          "so HBM counters can be scaled by a kernel of known traffic")
@@ -521,6 +574,14 @@ def main():
         extra["spectral_descriptors"] = dsc
         log("spectral descriptors: " + ", ".join(f"{k} {dsc[k]['frames_per_s'] / 1e6:.1f} M frames/s"
                                                  for k in ("centroid", "bandwidth", "rolloff", "flatness")))
+
+    if not args.no_resample:
+        rs = resample_leg(torch, B.total_input, 3 if args.profile else max(3, steps // 2),
+                          not args.no_cpu and not args.profile)
+        extra["resample"] = rs
+        log(f"resample 44.1 -> 22.05 kHz: {rs['out_samples_per_s'] / 1e9:.2f} G output samples/s, "
+            f"kernel {rs['kernel_ms']:.3f} ms, {rs['taps_per_s'] / 1e12:.2f} T taps/s"
+            + (f"; cpu {rs['cpu_baseline']['value'] / 1e6:.2f} M samples/s" if "cpu_baseline" in rs else ""))
 
     cpu = None
     if not args.no_cpu and not args.profile:

@@ -318,6 +318,30 @@ int32_t Engine::resample_table(hipStream_t st, const float** out) {
   return OK;
 }
 
+int32_t Engine::resample_weights(int source_sr, int target_sr, hipStream_t st, const float** out) {
+  *out = nullptr;
+  int P, taps, stride;
+  if (!resample_poly_shape(source_sr, target_sr, &P, &taps, &stride)) return OK;
+  const float* table = nullptr;
+  int32_t s = resample_table(st, &table);
+  if (s != OK) return s;
+  std::lock_guard<std::mutex> g(mu_);
+  for (auto& e : poly_)
+    if (e->src == source_sr && e->dst == target_sr) {
+      *out = static_cast<const float*>(e->w.p);
+      return OK;
+    }
+  auto e = std::make_unique<PolyEntry>();
+  e->src = source_sr;
+  e->dst = target_sr;
+  MM_HIP(e->w.ensure((size_t)P * stride * sizeof(float)));
+  MM_HIP(launch_poly_table(source_sr, target_sr, table, static_cast<float*>(e->w.p), st));
+  MM_HIP(hipStreamSynchronize(st));   // shared by every stream from here on
+  *out = static_cast<const float*>(e->w.p);
+  poly_.push_back(std::move(e));
+  return OK;
+}
+
 std::string tune_key(const SpecRequest& r, const std::string& arch) {
   std::ostringstream o;
   o << arch << ":k" << r.out_kind << ":n" << r.n_fft << ":h" << r.hop << ":m" << r.n_mels << ":c"
@@ -775,11 +799,18 @@ int32_t enqueue_resample(const float* d_x, int64_t n, int source_sr, int target_
   }
   Engine* e = Engine::get();
   if (!e) return ERR_NO_GPU;
-  const float* table = nullptr;
-  const int32_t s = e->resample_table(st, &table);
+  const int64_t m = resample_length(n, source_sr, target_sr);
+  const float* w = nullptr;
+  int32_t s = e->resample_weights(source_sr, target_sr, st, &w);
   if (s != OK) return s;
-  MM_HIP(launch_resample(d_x, n, d_y, resample_length(n, source_sr, target_sr), source_sr, target_sr, table,
-                         e->num_cus(), st));
+  if (w) {   // polyphase: weight rows in LDS
+    MM_HIP(launch_resample_poly(d_x, n, d_y, m, source_sr, target_sr, w, e->num_cus(), st));
+    return OK;
+  }
+  const float* table = nullptr;
+  s = e->resample_table(st, &table);
+  if (s != OK) return s;
+  MM_HIP(launch_resample(d_x, n, d_y, m, source_sr, target_sr, table, e->num_cus(), st));
   return OK;
 }
 

@@ -87,6 +87,9 @@ class Engine {
   int32_t tables_for(const SpecRequest& r, hipStream_t st, const Tables** out);
   // Filter table of the sample-rate converter (generated on first use).
   int32_t resample_table(hipStream_t st, const float** out);
+  // Polyphase weight rows of a rate pair (built on first use; nullptr when
+  // they do not fit the polyphase kernel).
+  int32_t resample_weights(int source_sr, int target_sr, hipStream_t st, const float** out);
 
   // Persistent grid size: autotuned if known, else a default.
   int grid_for(const SpecRequest& r, int n_tiles, const TileLayout& L);
@@ -105,6 +108,11 @@ class Engine {
   std::vector<Lane*> free_lanes_;
   std::vector<std::unique_ptr<Tables>> tables_;
   DevBuf rs_table_;
+  struct PolyEntry {
+    int src, dst;
+    DevBuf w;
+  };
+  std::vector<std::unique_ptr<PolyEntry>> poly_;
   struct TuneEntry {
     std::string key;
     int wg_per_cu;

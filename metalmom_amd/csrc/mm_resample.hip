@@ -22,6 +22,15 @@
 // (j - p is exact in double, times the filter scale), the weights and sums in
 // FP32. The work per output is 2 * support taps of ~13 VALU operations and
 // three LDS reads: compute-bound, not HBM-bound (8 bytes of HBM per output).
+//
+// Polyphase path: with g = gcd(source, target), output i sits (i * source
+// mod target) / g steps of g / target past input floor(i * source / target),
+// so the tap weights take only P = target / g distinct rows ("phases"). When
+// P * 2 * support weights fit in LDS they are computed once per launch
+// (k_poly_table, from the same FP32 table and interpolation, the tap distance
+// taken exactly from integers) and each tap is then one LDS weight read, one
+// LDS sample read and one FMA; the sums run over the same taps in the same
+// order. Otherwise the kernel above forms the weights per tap.
 #include "mm_common.h"
 #include "mm_kernels.h"
 
@@ -35,6 +44,8 @@ constexpr int kTabLen = kZeros * kOver + 1;   // 32769
 constexpr int kTabLds = 32772;                // padded to a 16-byte multiple
 constexpr int kRsThreads = 1024;
 constexpr int kWinCap = 6144;                 // staged input samples (24 KB)
+constexpr int kWinCapPoly = 8192;             // staged input samples of the polyphase path (32 KB)
+constexpr int64_t kPolyMaxFloats = 28672;     // polyphase weight rows in LDS (112 KB)
 
 __device__ double bessel_i0(double x) {
   double sum = 1.0, term = 1.0;
@@ -110,12 +121,138 @@ __global__ __launch_bounds__(kRsThreads) void k_resample(const float* __restrict
   }
 }
 
+// Weight rows of the polyphase path: W[phase * stride + m] for the tap
+// j = floor(p) - support + 1 + m of an output in that phase; the tap distance
+// |j - p| * scale * 512 = |(m - support + 1) * target - phase * g| * 512 / D
+// with D = max(source, target), split exactly into table index and fraction.
+__global__ void k_poly_table(const float* __restrict__ table, int target, int g, int D, int support, int taps,
+                             int stride, int phases, float* __restrict__ W) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= phases * stride) return;
+  const int ph = e / stride, m = e - ph * stride;
+  float w = 0.0f;
+  if (m < taps) {
+    int64_t s = (int64_t)(m - support + 1) * target - (int64_t)ph * g;
+    s = s < 0 ? -s : s;
+    const int64_t N = s * kOver;
+    const int64_t q = N / D, r = N - q * D;
+    if (q < kTabLen - 1) {
+      const float fr = (float)((double)r / (double)D);
+      w = table[q] * (1.0f - fr) + table[q + 1] * fr;
+    }
+  }
+  W[e] = w;
+}
+
+__global__ __launch_bounds__(kRsThreads) void k_resample_poly(const float* __restrict__ x, int64_t n,
+                                                             float* __restrict__ y, int64_t n_out, int source,
+                                                             int target, int g, int support, int taps, int stride,
+                                                             int w_floats, int win_cap, float scale,
+                                                             const float* __restrict__ wtab) {
+  extern __shared__ float lds[];
+  float* Wt = lds;
+  float* X = lds + ((w_floats + 3) & ~3);
+  for (int i = threadIdx.x; i < w_floats; i += kRsThreads) Wt[i] = wtab[i];
+  const int64_t n_chunks = (n_out + kRsThreads - 1) / kRsThreads;
+  for (int64_t c = blockIdx.x; c < n_chunks; c += gridDim.x) {
+    const int64_t i0 = c * kRsThreads;
+    const int64_t i1 = (i0 + kRsThreads < n_out ? i0 + kRsThreads : n_out) - 1;
+    int64_t lo = (i0 * source) / target - support + 1;
+    lo = lo < 0 ? 0 : lo;
+    int64_t hi = (i1 * source) / target + support;
+    hi = hi > n - 1 ? n - 1 : hi;
+    const int64_t span = hi - lo + 1;
+    const bool staged = span <= win_cap;
+    __syncthreads();   // the weights are loaded; the previous chunk's reads of X are done
+    if (staged)
+      for (int k = threadIdx.x; k < span; k += kRsThreads) X[k] = x[lo + k];
+    __syncthreads();
+    const int64_t i = i0 + threadIdx.x;
+    if (i <= i1) {
+      const int64_t A = i * source;
+      const int64_t k = A / target;
+      const int ph = (int)((A - k * target) / g);
+      const int64_t j0 = k - support + 1;   // the tap of m = 0
+      const int m0 = j0 < 0 ? (int)-j0 : 0;
+      const int m1 = (j0 + taps - 1 > n - 1) ? (int)(n - 1 - j0) : taps - 1;
+      const float* w = Wt + ph * stride;
+      const float* src = staged ? X + (j0 - lo) : x + j0;
+      float sum = 0.0f, wsum = 0.0f;
+      for (int m = m0; m <= m1; ++m) {
+        const float wm = w[m];
+        sum += src[m] * wm;
+        wsum += wm;
+      }
+      y[i] = wsum > 0.0f ? sum * scale : 0.0f;
+    }
+  }
+}
+
 hipError_t launch_tab_resample(float* table, hipStream_t st) {
   hipLaunchKernelGGL(k_tab_resample, dim3((kTabLen + 255) / 256), dim3(256), 0, st, table);
   return hipGetLastError();
 }
 
 int resample_table_len() { return kTabLen; }
+
+namespace {
+int64_t gcd64(int64_t a, int64_t b) {
+  while (b) {
+    const int64_t t = a % b;
+    a = b;
+    b = t;
+  }
+  return a;
+}
+}  // namespace
+
+bool resample_poly_shape(int source_sr, int target_sr, int* phases, int* taps, int* stride) {
+  const double ratio = (double)target_sr / (double)source_sr;
+  const double scale = ratio < 1.0 ? ratio : 1.0;
+  const int support = (int)std::ceil((double)kZeros / scale);
+  const int64_t g = gcd64(source_sr, target_sr);
+  const int64_t P = target_sr / g;
+  const int t = 2 * support, st = t | 1;   // odd row stride: rows of different phases fall in different banks
+  *phases = (int)std::min<int64_t>(P, 1 << 30);
+  *taps = t;
+  *stride = st;
+  // the weights plus at least a 1024-output window must fit in LDS
+  return P * st <= kPolyMaxFloats && (int64_t)source_sr * 1023 / target_sr + t + 1 <= kWinCapPoly &&
+         (size_t)(P * st + kWinCapPoly + 4) * sizeof(float) <= lds_limit_bytes();
+}
+
+hipError_t launch_poly_table(int source_sr, int target_sr, const float* table, float* wtab, hipStream_t st) {
+  int P, taps, stride;
+  if (!resample_poly_shape(source_sr, target_sr, &P, &taps, &stride)) return hipErrorInvalidConfiguration;
+  const int g = (int)gcd64(source_sr, target_sr);
+  const int D = std::max(source_sr, target_sr);
+  const int wf = P * stride;
+  hipLaunchKernelGGL(k_poly_table, dim3((wf + 255) / 256), dim3(256), 0, st, table, target_sr, g, D, taps / 2, taps,
+                     stride, P, wtab);
+  return hipGetLastError();
+}
+
+hipError_t launch_resample_poly(const float* x, int64_t n, float* y, int64_t n_out, int source_sr, int target_sr,
+                                const float* wtab, int grid, hipStream_t st) {
+  int P, taps, stride;
+  if (!resample_poly_shape(source_sr, target_sr, &P, &taps, &stride)) return hipErrorInvalidConfiguration;
+  const int g = (int)gcd64(source_sr, target_sr);
+  const int support = taps / 2;
+  const int wf = P * stride;
+  const int64_t n_chunks = (n_out + kRsThreads - 1) / kRsThreads;
+  if (n_chunks <= 0) return hipSuccess;
+  const int gr = (int)std::min<int64_t>(std::max(grid, 1), n_chunks);
+  static const hipError_t attr = hipFuncSetAttribute(reinterpret_cast<const void*>(k_resample_poly),
+                                                     hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                     (int)lds_limit_bytes());
+  if (attr != hipSuccess) return attr;
+  const size_t lds = (size_t)(((wf + 3) & ~3) + kWinCapPoly) * sizeof(float);
+  const double ratio = (double)target_sr / (double)source_sr;
+  const float scale = (float)(ratio < 1.0 ? ratio : 1.0);
+  hipLaunchKernelGGL(k_resample_poly, dim3(gr), dim3(kRsThreads), lds, st, x, n, y, n_out, source_sr, target_sr,
+                     g, support, taps, stride, wf, kWinCapPoly, scale, wtab);
+  return hipGetLastError();
+}
 
 hipError_t launch_resample(const float* x, int64_t n, float* y, int64_t n_out, int source_sr, int target_sr,
                            const float* table, int grid, hipStream_t st) {

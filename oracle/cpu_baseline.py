@@ -131,6 +131,44 @@ class ChromaF32:
         return self.fb @ (S * S)
 
 
+class ResampleF32:
+    """Float32 restatement of the windowed-sinc converter
+    (``Audio/Resample.swift:12-96``): tap positions in float64 as the
+    reference forms them, float32 table, weights and sums, vectorised over
+    chunks of outputs."""
+
+    def __init__(self, source_sr=44100, target_sr=22050, chunk=8192):
+        from . import ingest as g
+        self.src, self.dst, self.chunk = source_sr, target_sr, chunk
+        self.T = g.resample_table().astype(np.float32)
+        self.ratio = target_sr / source_sr
+        self.scale = min(1.0, self.ratio)
+        self.support = int(np.ceil(g.NUM_ZEROS / self.scale))
+        self.n_out = g.resample_length
+
+    def __call__(self, y):
+        y = np.asarray(y, dtype=np.float32)
+        n = y.size
+        m = self.n_out(n, self.src, self.dst)
+        out = np.zeros(m, np.float32)
+        offs = np.arange(-self.support + 1, self.support + 1)
+        fs = self.scale * 512.0
+        for c0 in range(0, m, self.chunk):
+            i = np.arange(c0, min(m, c0 + self.chunk))
+            pos = i / self.ratio
+            j = pos.astype(np.int64)[:, None] + offs[None, :]
+            fp = np.abs(j - pos[:, None]) * fs
+            fi = fp.astype(np.int64)
+            use = (j >= 0) & (j < n) & (fi < self.T.size - 1)
+            fi = np.where(use, fi, 0)
+            fr = (fp - fi).astype(np.float32)
+            w = np.where(use, self.T[fi] * (np.float32(1) - fr) + self.T[fi + 1] * fr, np.float32(0))
+            s = (y[np.clip(j, 0, n - 1)] * w).sum(axis=1, dtype=np.float32)
+            ws = w.sum(axis=1, dtype=np.float32)
+            out[i] = np.where(ws > 0, s * np.float32(self.scale), np.float32(0))
+        return out
+
+
 class CentroidF32:
     """Float32 spectral centroid (``SpectralDescriptors.swift:25-56``) of the
     magnitude STFT."""

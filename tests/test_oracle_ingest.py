@@ -145,3 +145,13 @@ def test_to_mono():
     assert np.array_equal(g.to_mono(x, 2, True), [2.0, 0.0, 0.5])
     assert np.array_equal(g.to_mono(x, 2, False), [1.0, -2.0, 0.5])
     assert np.array_equal(g.to_mono(np.arange(4.0), 1, True), np.arange(4.0))
+
+
+def test_cpu_baseline_port_matches_oracle():
+    """The float32 restatement timed as the CPU baseline computes the same
+    conversion (FP32 rounding only)."""
+    from oracle.cpu_baseline import ResampleF32
+    y = np.random.default_rng(0).standard_normal(20000).astype(np.float32)
+    a = ResampleF32(44100, 22050)(y)
+    b = g.resample(y.astype(np.float64), 44100, 22050)
+    assert a.shape == b.shape and np.abs(a - b).max() < 1e-5
