@@ -11,6 +11,7 @@
 // order the reference accumulates them in. Consecutive threads read
 // consecutive frames of a bin row, so every load is coalesced.
 #include "mm_common.h"
+#include "mm_fft_dev.h"
 #include "mm_kernels.h"
 
 namespace mm {
@@ -75,6 +76,153 @@ __global__ void k_descriptor(const float* __restrict__ S, int F, int T, int kind
     }
   }
   out[t] = y;
+}
+
+// Fused form for one clip: each wave computes the FFT of one frame with the
+// last pass in registers (the fused tile kernel's register-split FFT), forms
+// |X[k]| in registers (lane l holds k = l + 64 q + BL r; lane 0 also the
+// Nyquist bin) and reduces it with wave sums, so the magnitude spectrogram
+// never reaches HBM. Rolloff finds the first bin whose cumulative sum reaches
+// the threshold by a binary search over k (each probe one masked wave sum).
+// Sums are per-lane then across lanes instead of in bin order; the
+// descriptors agree with the two-pass kernel above to FP32 rounding.
+template <int LOG2M, int NW>
+__global__ __launch_bounds__(NW * 64) void k_desc_fused(TileArgs a, int kind, float srf, float nfftf, float param,
+                                                       float* __restrict__ out) {
+  using S = FftShape<LOG2M>;
+  constexpr int M = S::M, QL = S::QL, RL = S::RL, BL = S::BL;
+  extern __shared__ float lds_f[];
+  const int wave = threadIdx.x >> 6, lane0 = threadIdx.x & 63;
+  float* sf = lds_f + wave * (M + M / 16);
+  const int t_begin = blockIdx.x * a.tiles_per_wg;
+  const int t_end = min(a.n_tiles, t_begin + a.tiles_per_wg);
+  const float2 pf[S::Q0][S::R0] = {};
+  for (int tile = t_begin; tile < t_end; ++tile) {
+    int lane = lane0;
+    __asm__ __volatile__("" : "+v"(lane));
+    const int clip = ldc(a.tile_clip, tile);
+    const int fr = ldc(a.tile_f0, tile) + wave;
+    if (fr >= ldc(a.clip_frames, clip)) continue;   // wave-uniform; the FFT uses only this wave's LDS
+    float2 z[QL][RL];
+    frame_fft_split<LOG2M, float>(a, clip, fr, sf, lane, a.window, a.twp, z, pf, false);
+    float2 x[QL][RL];
+    float nyq;
+    real_split_regs<LOG2M>(z, a.twN, lane, x, nyq);
+    float v[QL][RL];
+#pragma unroll
+    for (int q = 0; q < QL; ++q)
+#pragma unroll
+      for (int r = 0; r < RL; ++r) v[q][r] = 0.5f * sqrtf(fmaf(x[q][r].x, x[q][r].x, x[q][r].y * x[q][r].y));
+    const float vn = (lane == 0) ? 0.5f * fabsf(nyq) : 0.0f;   // bin M
+    auto freq = [&](int k) { return (float)k * srf / nfftf; };
+    auto bin = [&](int q, int r) { return lane + 64 * q + BL * r; };
+    float y = 0.0f;
+    if (kind == DESC_CENTROID || kind == DESC_BANDWIDTH) {
+      float ws = (lane == 0) ? freq(M) * vn : 0.0f, tot = vn;
+#pragma unroll
+      for (int q = 0; q < QL; ++q)
+#pragma unroll
+        for (int r = 0; r < RL; ++r) {
+          ws = fmaf(freq(bin(q, r)), v[q][r], ws);
+          tot += v[q][r];
+        }
+      ws = wave_sum(ws);
+      tot = wave_sum(tot);
+      const float c = tot > 0.0f ? ws / tot : 0.0f;
+      if (kind == DESC_CENTROID) {
+        y = c;
+      } else {
+        const bool p2 = (param == 2.0f);
+        auto dev = [&](int k, float val) {
+          const float d = fabsf(freq(k) - c);
+          return val * (p2 ? d * d : powf(d, param));
+        };
+        float m = (lane == 0) ? dev(M, vn) : 0.0f;
+#pragma unroll
+        for (int q = 0; q < QL; ++q)
+#pragma unroll
+          for (int r = 0; r < RL; ++r) m += dev(bin(q, r), v[q][r]);
+        m = wave_sum(m);
+        y = tot > 0.0f ? (p2 ? sqrtf(m / tot) : powf(m / tot, 1.0f / param)) : 0.0f;
+      }
+    } else if (kind == DESC_ROLLOFF) {
+      float tot = vn;
+#pragma unroll
+      for (int q = 0; q < QL; ++q)
+#pragma unroll
+        for (int r = 0; r < RL; ++r) tot += v[q][r];
+      tot = wave_sum(tot);
+      if (tot > 0.0f) {
+        const float thr = param * tot;
+        auto cum = [&](int K) {   // sum of bins 0..K
+          float c = (K >= M) ? vn : 0.0f;
+#pragma unroll
+          for (int q = 0; q < QL; ++q)
+#pragma unroll
+            for (int r = 0; r < RL; ++r) c += (bin(q, r) <= K) ? v[q][r] : 0.0f;
+          return wave_sum(c);
+        };
+        int lo = 0, hi = M;
+        while (lo < hi) {   // wave-uniform: every lane holds the same sums
+          const int mid = (lo + hi) >> 1;
+          if (cum(mid) >= thr) hi = mid;
+          else lo = mid + 1;
+        }
+        y = (cum(lo) >= thr) ? freq(lo) : 0.0f;
+      }
+    } else {   // flatness: power 2, amin 1e-10
+      const float amin = 1e-10f;
+      float raw = vn, ls = 0.0f, as = 0.0f;
+      if (lane == 0) {
+        const float pn = fmaxf(vn * vn, amin);
+        ls = logf(pn);
+        as = pn;
+      }
+#pragma unroll
+      for (int q = 0; q < QL; ++q)
+#pragma unroll
+        for (int r = 0; r < RL; ++r) {
+          const float pv = fmaxf(v[q][r] * v[q][r], amin);
+          raw += v[q][r];
+          ls += logf(pv);
+          as += pv;
+        }
+      raw = wave_sum(raw);
+      ls = wave_sum(ls);
+      as = wave_sum(as);
+      if (raw >= amin) {
+        const float am = as / (float)(M + 1);
+        const float gm = expf(ls / (float)(M + 1));
+        y = am > amin ? gm / am : 0.0f;
+      }
+    }
+    if (lane == 0) out[fr] = y;
+  }
+}
+
+template <int LOG2M, int NW>
+static hipError_t launch_desc_fused_t(const TileArgs& a, int kind, float srf, float nfftf, float param, float* out,
+                                      int grid, hipStream_t st) {
+  constexpr int M = 1 << LOG2M;
+  const size_t lds = (size_t)NW * (M + M / 16) * sizeof(float);
+  static const hipError_t attr = hipFuncSetAttribute(reinterpret_cast<const void*>(k_desc_fused<LOG2M, NW>),
+                                                     hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                     (int)lds_limit_bytes());
+  if (attr != hipSuccess) return attr;
+  hipLaunchKernelGGL((k_desc_fused<LOG2M, NW>), dim3(grid), dim3(NW * 64), lds, st, a, kind, srf, nfftf, param, out);
+  return hipGetLastError();
+}
+
+int desc_fused_waves(int n_fft) { return (n_fft == 1024 || n_fft == 2048) ? 16 : n_fft == 4096 ? 8 : 0; }
+
+hipError_t launch_desc_fused(const TileArgs& a, int kind, float srf, float nfftf, float param, float* out, int grid,
+                             hipStream_t st) {
+  switch (a.n_fft) {
+    case 1024: return launch_desc_fused_t<9, 16>(a, kind, srf, nfftf, param, out, grid, st);
+    case 2048: return launch_desc_fused_t<10, 16>(a, kind, srf, nfftf, param, out, grid, st);
+    case 4096: return launch_desc_fused_t<11, 8>(a, kind, srf, nfftf, param, out, grid, st);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 hipError_t launch_descriptor(const float* S, int F, int T, int kind, float srf, float nfftf, float param,

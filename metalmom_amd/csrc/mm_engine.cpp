@@ -742,6 +742,20 @@ int32_t enqueue_descriptor(Lane* lane, const SpecRequest& r, const float* d_x, i
   if (s != OK) return s;
   TileLayout L;
   const bool fused = tile_layout(r.n_fft, r.out_kind, r.precision, 0, 0, &L);
+  const int nw = desc_fused_waves(r.n_fft);
+  if (fused && nw > 0 && r.in_dtype == IN_F32) {
+    // FFT and reduction in one kernel: one frame per wave, tiles of nw frames
+    BatchMeta m;
+    build_meta(r, &len, 1, nw, &m);
+    s = cached_meta(lane, r, len, nw, m, st);
+    if (s != OK) return s;
+    const MetaPtrs mp = meta_ptrs(m, lane->d_meta.p);
+    const int grid = std::max(1, std::min(m.n_tiles, eng->num_cus()));
+    const TileArgs a = make_args(r, tb, m, mp, d_x, d_res, L, grid);
+    const int g = (m.n_tiles + a.tiles_per_wg - 1) / a.tiles_per_wg;
+    MM_HIP(launch_desc_fused(a, kind, (float)r.sr, (float)r.n_fft, param, d_res, g, st));
+    return OK;
+  }
   const int tf = fused ? L.tile_frames : kTileFrames;
   BatchMeta m;
   build_meta(r, &len, 1, tf, &m);

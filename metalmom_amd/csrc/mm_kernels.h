@@ -66,6 +66,12 @@ hipError_t launch_onset(const float* db, int n_bands, int T, const unsigned* max
 // param: bandwidth p, rolloff roll_percent.
 hipError_t launch_descriptor(const float* S, int F, int T, int kind, float srf, float nfftf, float param,
                              float* out, hipStream_t st);
+// Fused descriptor of one clip (FFT + reduction per wave, no spectrogram in
+// HBM) for n_fft 1024 / 2048 / 4096: desc_fused_waves gives the waves per
+// workgroup (tiles of that many frames; 0 = no fused kernel); out[T].
+int desc_fused_waves(int n_fft);
+hipError_t launch_desc_fused(const TileArgs& a, int kind, float srf, float nfftf, float param, float* out, int grid,
+                             hipStream_t st);
 
 // ---- PCM ingest ----
 // out[frames] = interleaved PCM (PcmFormat) -> float32: the mean of the

@@ -64,7 +64,8 @@ def test_descriptor_goldens(signal_440, name):
     (44100, 90000, {"n_fft": 4096, "hop_length": 1024}),
     (22050, 50000, {"n_fft": 1024, "hop_length": 300, "win_length": 800, "center": False}),
     (22050, 900000, {}),
-], ids=["default", "44k-4096", "1024-hop300-uncentred", "long"])
+    (22050, 40000, {"n_fft": 512, "hop_length": 128}),
+], ids=["default", "44k-4096", "1024-hop300-uncentred", "long", "512-two-pass"])
 def test_descriptor_matches_oracle(name, sr, n, kw):
     import metalmom_amd as mm
     print(f"{name} sr={sr} n={n} {kw}", flush=True)
