@@ -318,8 +318,9 @@ int32_t Engine::resample_table(hipStream_t st, const float** out) {
   return OK;
 }
 
-int32_t Engine::resample_weights(int source_sr, int target_sr, hipStream_t st, const float** out) {
+int32_t Engine::resample_weights(int source_sr, int target_sr, hipStream_t st, const float** out, float* wsum1) {
   *out = nullptr;
+  *wsum1 = 0.0f;
   int P, taps, stride;
   if (!resample_poly_shape(source_sr, target_sr, &P, &taps, &stride)) return OK;
   const float* table = nullptr;
@@ -329,6 +330,7 @@ int32_t Engine::resample_weights(int source_sr, int target_sr, hipStream_t st, c
   for (auto& e : poly_)
     if (e->src == source_sr && e->dst == target_sr) {
       *out = static_cast<const float*>(e->w.p);
+      *wsum1 = e->wsum1;
       return OK;
     }
   auto e = std::make_unique<PolyEntry>();
@@ -337,7 +339,14 @@ int32_t Engine::resample_weights(int source_sr, int target_sr, hipStream_t st, c
   MM_HIP(e->w.ensure((size_t)P * stride * sizeof(float)));
   MM_HIP(launch_poly_table(source_sr, target_sr, table, static_cast<float*>(e->w.p), st));
   MM_HIP(hipStreamSynchronize(st));   // shared by every stream from here on
+  e->wsum1 = 0.0f;
+  if (P == 1) {   // the row's sum, in tap order
+    std::vector<float> row(taps);
+    MM_HIP(hipMemcpy(row.data(), e->w.p, (size_t)taps * sizeof(float), hipMemcpyDeviceToHost));
+    for (float v : row) e->wsum1 += v;
+  }
   *out = static_cast<const float*>(e->w.p);
+  *wsum1 = e->wsum1;
   poly_.push_back(std::move(e));
   return OK;
 }
@@ -815,10 +824,11 @@ int32_t enqueue_resample(const float* d_x, int64_t n, int source_sr, int target_
   if (!e) return ERR_NO_GPU;
   const int64_t m = resample_length(n, source_sr, target_sr);
   const float* w = nullptr;
-  int32_t s = e->resample_weights(source_sr, target_sr, st, &w);
+  float wsum1 = 0.0f;
+  int32_t s = e->resample_weights(source_sr, target_sr, st, &w, &wsum1);
   if (s != OK) return s;
   if (w) {   // polyphase: weight rows in LDS
-    MM_HIP(launch_resample_poly(d_x, n, d_y, m, source_sr, target_sr, w, e->num_cus(), st));
+    MM_HIP(launch_resample_poly(d_x, n, d_y, m, source_sr, target_sr, w, wsum1, e->num_cus(), st));
     return OK;
   }
   const float* table = nullptr;

@@ -89,7 +89,9 @@ class Engine {
   int32_t resample_table(hipStream_t st, const float** out);
   // Polyphase weight rows of a rate pair (built on first use; nullptr when
   // they do not fit the polyphase kernel).
-  int32_t resample_weights(int source_sr, int target_sr, hipStream_t st, const float** out);
+  // they do not fit the polyphase kernel); *wsum1: the sum of the row when
+  // there is one phase.
+  int32_t resample_weights(int source_sr, int target_sr, hipStream_t st, const float** out, float* wsum1);
 
   // Persistent grid size: autotuned if known, else a default.
   int grid_for(const SpecRequest& r, int n_tiles, const TileLayout& L);
@@ -110,6 +112,7 @@ class Engine {
   DevBuf rs_table_;
   struct PolyEntry {
     int src, dst;
+    float wsum1;
     DevBuf w;
   };
   std::vector<std::unique_ptr<PolyEntry>> poly_;

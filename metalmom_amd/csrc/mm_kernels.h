@@ -94,8 +94,10 @@ hipError_t launch_resample(const float* x, int64_t n, float* y, int64_t n_out, i
 // them.
 bool resample_poly_shape(int source_sr, int target_sr, int* phases, int* taps, int* stride);
 hipError_t launch_poly_table(int source_sr, int target_sr, const float* table, float* wtab, hipStream_t st);
+// wsum1: the sum of the weight row when there is one phase (its sign decides
+// whether interior outputs are 0, as the per-output weight sum would).
 hipError_t launch_resample_poly(const float* x, int64_t n, float* y, int64_t n_out, int source_sr, int target_sr,
-                                const float* wtab, int grid, hipStream_t st);
+                                const float* wtab, float wsum1, int grid, hipStream_t st);
 
 // ---- inverse STFT ----
 struct IstftArgs {

@@ -147,7 +147,7 @@ __global__ void k_poly_table(const float* __restrict__ table, int target, int g,
 __global__ __launch_bounds__(kRsThreads) void k_resample_poly(const float* __restrict__ x, int64_t n,
                                                              float* __restrict__ y, int64_t n_out, int source,
                                                              int target, int g, int support, int taps, int stride,
-                                                             int w_floats, int win_cap, float scale,
+                                                             int w_floats, int win_cap, float scale, float wsum1,
                                                              const float* __restrict__ wtab) {
   extern __shared__ float lds[];
   float* Wt = lds;
@@ -167,7 +167,23 @@ __global__ __launch_bounds__(kRsThreads) void k_resample_poly(const float* __res
     if (staged)
       for (int k = threadIdx.x; k < span; k += kRsThreads) X[k] = x[lo + k];
     __syncthreads();
+    // one phase and every tap of every output of the chunk inside the signal:
+    // the weight row is the same for all lanes and read with scalar loads
+    // from global memory (wave-uniform addresses), all taps used
+    const bool uniform_row = staged && target / g == 1 && (i0 * source) / target - support + 1 >= 0 &&
+                             (i1 * source) / target + support <= n - 1;
     const int64_t i = i0 + threadIdx.x;
+    if (uniform_row) {
+      if (i <= i1) {
+        const int64_t k = (i * source) / target;
+        const float* src = X + (k - support + 1 - lo);
+        float sum = 0.0f;
+#pragma unroll 16
+        for (int m = 0; m < taps; ++m) sum = fmaf(src[m], wtab[m], sum);
+        y[i] = wsum1 > 0.0f ? sum * scale : 0.0f;
+      }
+      continue;
+    }
     if (i <= i1) {
       const int64_t A = i * source;
       const int64_t k = A / target;
@@ -233,7 +249,7 @@ hipError_t launch_poly_table(int source_sr, int target_sr, const float* table, f
 }
 
 hipError_t launch_resample_poly(const float* x, int64_t n, float* y, int64_t n_out, int source_sr, int target_sr,
-                                const float* wtab, int grid, hipStream_t st) {
+                                const float* wtab, float wsum1, int grid, hipStream_t st) {
   int P, taps, stride;
   if (!resample_poly_shape(source_sr, target_sr, &P, &taps, &stride)) return hipErrorInvalidConfiguration;
   const int g = (int)gcd64(source_sr, target_sr);
@@ -250,7 +266,7 @@ hipError_t launch_resample_poly(const float* x, int64_t n, float* y, int64_t n_o
   const double ratio = (double)target_sr / (double)source_sr;
   const float scale = (float)(ratio < 1.0 ? ratio : 1.0);
   hipLaunchKernelGGL(k_resample_poly, dim3(gr), dim3(kRsThreads), lds, st, x, n, y, n_out, source_sr, target_sr,
-                     g, support, taps, stride, wf, kWinCapPoly, scale, wtab);
+                     g, support, taps, stride, wf, kWinCapPoly, scale, wsum1, wtab);
   return hipGetLastError();
 }
 
