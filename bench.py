@@ -404,6 +404,9 @@ def main():
     ap.add_argument("--serve-rates", default="1000,5000,8000",
                     help="Poisson request rates (req/s) for the serving leg; empty to skip")
     ap.add_argument("--serve-seconds", type=float, default=2.0)
+    ap.add_argument("--serve-pcm-rates", default="1000,5000",
+                    help="rates for the raw-PCM serving leg (int16 stereo at 22.05 / 44.1 kHz, analysed at "
+                         "22.05 kHz); empty to skip")
     ap.add_argument("--grid", type=int, default=0,
                     help="persistent workgroups per CU (skips the autotuner; 0 = autotune)")
     ap.add_argument("--no-istft", action="store_true", help="skip the inverse-STFT leg")
@@ -546,6 +549,22 @@ def main():
                             "requests": r["n_requests"], "p50_ms": r["p50"] * 1e3, "p99_ms": r["p99"] * 1e3,
                             "frames_per_s": r["frames_per_s"], "device_steps": st["steps"]})
             log(f"serving @{rate:.0f} req/s: p50 {r['p50'] * 1e3:.2f} ms, p99 {r['p99'] * 1e3:.2f} ms, "
+                f"{r['frames_per_s'] / 1e6:.2f} M frames/s, {st['steps']} steps for {r['n_requests']} requests")
+        # the same schedule as raw int16 stereo PCM at the source rates, mixed
+        # down and resampled to 22.05 kHz on the GPU inside each step
+        from metalmom_amd.serve import pcm_sources
+        psrc = pcm_sources(sources) if args.serve_pcm_rates else None
+        for rate in [float(r) for r in args.serve_pcm_rates.split(",") if r]:
+            srv = SpectralServer(hbm_budget_bytes=32 << 30)
+            open_loop(srv, rate=rate, duration=0.5, seed=99, sources=sources, pcm=psrc)
+            r = open_loop(srv, rate=rate, duration=args.serve_seconds, seed=4, sources=sources, pcm=psrc)
+            st = srv.stats()
+            srv.close()
+            serving.append({"rate_req_s": rate, "input": "int16 stereo PCM, resampled to 22.05 kHz on the GPU",
+                            "achieved_rate": r["achieved_rate"], "requests": r["n_requests"],
+                            "p50_ms": r["p50"] * 1e3, "p99_ms": r["p99"] * 1e3,
+                            "frames_per_s": r["frames_per_s"], "device_steps": st["steps"]})
+            log(f"serving PCM @{rate:.0f} req/s: p50 {r['p50'] * 1e3:.2f} ms, p99 {r['p99'] * 1e3:.2f} ms, "
                 f"{r['frames_per_s'] / 1e6:.2f} M frames/s, {st['steps']} steps for {r['n_requests']} requests")
 
     if not args.no_istft:

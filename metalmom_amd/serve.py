@@ -128,16 +128,34 @@ def make_schedule(rate: float, duration: float, seed: int = 4):
     return [(float(t[i]), KINDS[kind[i]], int(sr[i]), int(lens[i]), int(offs[i])) for i in range(n)]
 
 
+def pcm_sources(sources):
+    """int16 stereo versions of the float sources (right channel at half the
+    left's level), as a WAV file at the source's rate would hold them."""
+    out = {}
+    for sr, x in sources.items():
+        left = np.clip(np.round(x * 16384.0), -32768, 32767)
+        out[sr] = np.ascontiguousarray(np.stack([left, np.round(left * 0.5)], axis=1).astype("<i2"))
+    return out
+
+
 def open_loop(server: SpectralServer, rate: float, duration: float = 2.0, seed: int = 4,
-              sources=None, keep_outputs: bool = False):
+              sources=None, keep_outputs: bool = False, pcm=None, analysis_sr: int = 22050):
     """Submits a Poisson schedule in real time and collects every result.
+
+    With ``pcm`` (from ``pcm_sources``) every clip is submitted as int16
+    stereo PCM at its source rate and analysed at ``analysis_sr``: the GPU
+    mixes it to mono and resamples it inside the request's step.
 
     Returns dict(latencies, p50, p99, frames, wall, frames_per_s, achieved_rate,
     outputs (if kept), schedule).
     """
     sources = sources or make_sources(seed=seed)
     sched = make_schedule(rate, duration, seed)
-    params = {(k, sr): server.params(k, sr=sr) for k in KINDS for sr in SRS}
+    if pcm is not None:
+        from ._native import PCM_S16
+        params = {(k, sr): server.params(k, sr=analysis_sr) for k in KINDS for sr in SRS}
+    else:
+        params = {(k, sr): server.params(k, sr=sr) for k in KINDS for sr in SRS}
     ids = [None] * len(sched)
     ready = threading.Semaphore(0)
     lat = [0.0] * len(sched)
@@ -160,7 +178,10 @@ def open_loop(server: SpectralServer, rate: float, duration: float = 2.0, seed: 
         dt = t0 + ta - time.perf_counter()
         if dt > 0:
             time.sleep(dt)
-        ids[i] = server.submit(sources[sr][off:off + n], params[(kind, sr)])
+        if pcm is not None:
+            ids[i] = server.submit_pcm(pcm[sr][off:off + n], 2, PCM_S16, sr, params[(kind, sr)])
+        else:
+            ids[i] = server.submit(sources[sr][off:off + n], params[(kind, sr)])
         ready.release()
     th.join()
     wall = time.perf_counter() - t0

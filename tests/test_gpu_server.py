@@ -99,3 +99,23 @@ def test_open_loop_latency_report(sources):
     assert r["n_requests"] >= 50
     assert 0 < r["p50"] <= r["p99"] < 5.0
     assert r["frames"] > 0
+
+
+def test_open_loop_pcm_requests():
+    """The serving schedule as int16 stereo PCM at 22.05 / 44.1 kHz, analysed
+    at 22.05 kHz: every request completes, and one kept result equals the
+    same clip mixed down, resampled and analysed alone."""
+    import metalmom_amd as mm
+    from metalmom_amd.serve import SpectralServer, make_sources, open_loop, pcm_sources
+    src = make_sources()   # the schedule's offsets reach 330 s into each source
+    psrc = pcm_sources(src)
+    srv = SpectralServer()
+    r = open_loop(srv, rate=100.0, duration=0.3, seed=3, sources=src, pcm=psrc, keep_outputs=True)
+    srv.close()
+    assert r["n_requests"] >= 10 and 0 < r["p50"] <= r["p99"] < 5.0
+    t, kind, sr, n, off = r["schedule"][0]
+    x = psrc[sr][off:off + n]
+    clip = ((x[:, 0].astype(np.float64) / 32768.0 + x[:, 1].astype(np.float64) / 32768.0) / 2).astype(np.float32)
+    clip = mm.resample(clip, sr, 22050)
+    alone = mm.spectral(clip, kind=kind, sr=22050, n_fft=2048, hop_length=512)
+    assert np.array_equal(r["outputs"][0], alone)
