@@ -119,3 +119,30 @@ def test_open_loop_pcm_requests():
     clip = mm.resample(clip, sr, 22050)
     alone = mm.spectral(clip, kind=kind, sr=22050, n_fft=2048, hop_length=512)
     assert np.array_equal(r["outputs"][0], alone)
+
+
+def test_full_rings_fall_back_to_heap_paths():
+    """More input than the pinned input ring holds (32 MiB at this budget)
+    sends later clips down the heap path; results left unpolled fill the
+    output ring, so later steps stage results in the lane buffers. Every
+    result still equals the clip run alone."""
+    import metalmom_amd as mm
+    from metalmom_amd.serve import SpectralServer
+    srv = SpectralServer(hbm_budget_bytes=64 << 20, max_batch=4)
+    try:
+        rng = np.random.default_rng(11)
+        clips = [(0.3 * rng.standard_normal(2_000_000 + 1000 * i)).astype(np.float32) for i in range(12)]  # 8 MB each
+        p = srv.params("stft", sr=22050)
+        ids = [srv.submit(c, p) for c in clips]
+        # let every request finish before collecting any result
+        import time
+        t0 = time.time()
+        while srv.stats()["completed"] < len(ids) and time.time() - t0 < 60:
+            time.sleep(0.01)
+        assert srv.stats()["completed"] == len(ids)
+        for rid, c in zip(ids, clips):
+            got, _ = srv.wait(rid)
+            alone = mm.spectral(c, kind="stft", sr=22050, n_fft=2048, hop_length=512)
+            assert np.array_equal(got, alone)
+    finally:
+        srv.close()
