@@ -329,6 +329,23 @@ def descriptor_leg(torch, n_samples, iters, with_cpu=False):
         e1.synchronize()
         ms = e0.elapsed_time(e1) / iters
         res[name] = {"step_ms": ms, "frames_per_s": T / (ms * 1e-3)}
+    # spectral contrast: 7 octave bands (magnitude STFT, then one thread per (frame, band))
+    Yc = torch.empty(7 * T, device="cuda")
+
+    def runc():
+        s = lib.mm_spectral_contrast_device(ctx, C.c_void_p(X.data_ptr()), len(y), 22050, 2048, 512, 2048, 1, 6,
+                                            200.0, C.c_void_p(Yc.data_ptr()), None, C.c_void_p(st.cuda_stream))
+        if s != 0:
+            raise RuntimeError(f"mm_spectral_contrast_device failed with status {s}")
+    runc()
+    torch.cuda.synchronize()
+    e0.record(st)
+    for _ in range(iters):
+        runc()
+    e1.record(st)
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / iters
+    res["contrast"] = {"step_ms": ms, "frames_per_s": T / (ms * 1e-3)}
     lib.mm_destroy(ctx)
     del X, Y
     if with_cpu:
@@ -614,7 +631,8 @@ def main():
         dsc = descriptor_leg(torch, B.total_input, max(3, steps // 2), not args.no_cpu)
         extra["spectral_descriptors"] = dsc
         log("spectral descriptors: " + ", ".join(f"{k} {dsc[k]['frames_per_s'] / 1e6:.1f} M frames/s"
-                                                 for k in ("centroid", "bandwidth", "rolloff", "flatness")))
+                                                 for k in ("centroid", "bandwidth", "rolloff", "flatness",
+                                                           "contrast")))
 
     if not args.no_resample:
         rs = resample_leg(torch, B.total_input, 3 if args.profile else max(3, steps // 2),

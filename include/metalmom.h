@@ -139,6 +139,19 @@ int32_t mm_spectral_centroid(mm_context ctx, const float* signal_data, int64_t s
 int32_t mm_spectral_bandwidth(mm_context ctx, const float* signal_data, int64_t signal_length,
                               int32_t sample_rate, int32_t n_fft, int32_t hop_length,
                               int32_t win_length, int32_t center, float p, MMBuffer* out);
+/* Replaces Bridge.swift:698-737 (mm_spectral_contrast), reference
+ * metalmom.h:82-84: per frame and band (bins [0, fmin), then octaves from
+ * f_min, the last up to sr/2; SpectralDescriptors.swift:137-204),
+ * 10 log10(mean of the top q magnitudes) - 10 log10(mean of the bottom q),
+ * q = max(1, round(0.02 * bins)), each floored at 1e-10; an empty band gives
+ * 0. [n_bands + 1, n_frames]. hop <= 0 -> n_fft/4, win <= 0 -> n_fft; a
+ * non-power-of-two n_fft gives [n_bands + 1, 0]. Rejected (-1) where the
+ * reference traps or divides by zero: n_bands < 1, sample_rate <= 0; and
+ * n_bands > 62. */
+int32_t mm_spectral_contrast(mm_context ctx, const float* signal_data, int64_t signal_length,
+                             int32_t sample_rate, int32_t n_fft, int32_t hop_length,
+                             int32_t win_length, int32_t center, int32_t n_bands, float f_min,
+                             MMBuffer* out);
 int32_t mm_spectral_rolloff(mm_context ctx, const float* signal_data, int64_t signal_length,
                             int32_t sample_rate, int32_t n_fft, int32_t hop_length,
                             int32_t win_length, int32_t center, float roll_percent, MMBuffer* out);
@@ -249,6 +262,15 @@ int32_t mm_spectral_descriptor_device(mm_context ctx, const float* d_signal, int
                                       int32_t sample_rate, int32_t n_fft, int32_t hop_length,
                                       int32_t win_length, int32_t center, int32_t kind, float param,
                                       float* d_out, int64_t* out_frames, void* stream);
+/* mm_spectral_contrast on device memory: d_out receives [n_bands + 1,
+ * frames] (frames through out_frames, which may be NULL). Enqueued on
+ * `stream` without a host synchronisation after the first call for a clip
+ * length and parameter set. Needs a live context; calls on one context must
+ * not overlap. A non-power-of-two n_fft is rejected. */
+int32_t mm_spectral_contrast_device(mm_context ctx, const float* d_signal, int64_t signal_length,
+                                    int32_t sample_rate, int32_t n_fft, int32_t hop_length,
+                                    int32_t win_length, int32_t center, int32_t n_bands, float f_min,
+                                    float* d_out, int64_t* out_frames, void* stream);
 /* mm_onset_strength on device memory: d_signal holds signal_length samples,
  * d_out receives the [rows, cols] result (rows 1 or n_mels, cols frames;
  * both reported through out_rows / out_cols, which may be NULL; size d_out

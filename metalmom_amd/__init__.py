@@ -13,6 +13,7 @@ __version__ = "0.1.0"
 
 from .core import stft, stft_complex, istft, spectral, resample, load, get_duration, get_samplerate  # noqa: F401
 from .feature import (amplitude_to_db, power_to_db, melspectrogram, mfcc, chroma_stft,  # noqa: F401
-                      spectral_centroid, spectral_bandwidth, spectral_rolloff, spectral_flatness)
+                      spectral_centroid, spectral_bandwidth, spectral_contrast, spectral_rolloff,
+                      spectral_flatness)
 from . import feature  # noqa: F401
 from .onset import onset_strength  # noqa: F401

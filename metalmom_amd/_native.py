@@ -73,6 +73,7 @@ SIGNATURES = {
     "mm_spectral_centroid": (_I32, [_P, _F, _I64, _I32, _I32, _I32, _I32, _I32, _BUF]),
     "mm_spectral_bandwidth": (_I32, [_P, _F, _I64, _I32, _I32, _I32, _I32, _I32, C.c_float, _BUF]),
     "mm_spectral_rolloff": (_I32, [_P, _F, _I64, _I32, _I32, _I32, _I32, _I32, C.c_float, _BUF]),
+    "mm_spectral_contrast": (_I32, [_P, _F, _I64, _I32, _I32, _I32, _I32, _I32, _I32, C.c_float, _BUF]),
     "mm_spectral_flatness": (_I32, [_P, _F, _I64, _I32, _I32, _I32, _I32, _I32, _BUF]),
     "mm_chroma_stft": (_I32, [_P, _F, _I64, _I32, _I32, _I32, _I32, _I32, _I32, _BUF]),
     "mm_onset_strength": (_I32, [_P, _F, _I64, _I32, _I32, _I32, _I32, C.c_float, C.c_float, _I32, _I32, _BUF]),
@@ -85,6 +86,8 @@ SIGNATURES = {
     "mm_resample_device": (_I32, [_P, _P, _I64, _I32, _I32, _P, _P]),
     "mm_spectral_descriptor_device": (_I32, [_P, _P, _I64, _I32, _I32, _I32, _I32, _I32, _I32, C.c_float, _P,
                                              C.POINTER(_I64), _P]),
+    "mm_spectral_contrast_device": (_I32, [_P, _P, _I64, _I32, _I32, _I32, _I32, _I32, _I32, C.c_float, _P,
+                                           C.POINTER(_I64), _P]),
     "mm_onset_strength_device": (_I32, [_P, _P, _I64, _I32, _I32, _I32, _I32, C.c_float, C.c_float, _I32, _I32,
                                         _P, C.POINTER(_I64), C.POINTER(_I64), _P]),
     "mm_istft_device": (_I32, [_P, _P, _I32, _I32, _I32, _I32, _I32, _I32, _I32, _I64, _P, _P]),

@@ -7,7 +7,7 @@
 //   mm_istft                         :144-221
 //   mm_onset_strength                :1319-1358
 //   mm_chroma_stft                   :494-545
-//   mm_spectral_{centroid,bandwidth,rolloff,flatness}  :621-813
+//   mm_spectral_{centroid,bandwidth,contrast,rolloff,flatness}  :621-813
 //   mm_amplitude_to_db               :225-263
 //   mm_power_to_db                   :267-306
 //   mm_mel_spectrogram               :310-400
@@ -194,6 +194,25 @@ int32_t mm_spectral_descriptor_device(mm_context ctx, const float* d_signal, int
   Lane* lane = static_cast<Ctx*>(ctx)->get();
   if (!lane) return ERR_NO_GPU;
   return enqueue_descriptor(lane, r, d_signal, signal_length, kind, param, d_out, static_cast<hipStream_t>(stream));
+}
+
+int32_t mm_spectral_contrast_device(mm_context ctx, const float* d_signal, int64_t signal_length,
+                                    int32_t sample_rate, int32_t n_fft, int32_t hop_length, int32_t win_length,
+                                    int32_t center, int32_t n_bands, float f_min, float* d_out, int64_t* out_frames,
+                                    void* stream) {
+  if (!ctx || !is_live(ctx) || !d_signal || signal_length <= 0 || !d_out) return MM_ERR_INVALID_INPUT;
+  if (!pow2(n_fft) || sample_rate <= 0 || n_bands < 1 || n_bands + 1 > kMaxContrastBands) return MM_ERR_INVALID_INPUT;
+  SpecRequest r;
+  r.out_kind = OUT_STFT_MAG;
+  r.sr = sample_rate;
+  r.n_fft = n_fft;
+  r.hop = hop_length > 0 ? hop_length : n_fft / 4;
+  r.win = win_length > 0 ? win_length : n_fft;
+  r.center = center != 0;
+  if (out_frames) *out_frames = frame_count(signal_length, n_fft, r.hop, r.center);
+  Lane* lane = static_cast<Ctx*>(ctx)->get();
+  if (!lane) return ERR_NO_GPU;
+  return enqueue_contrast(lane, r, d_signal, signal_length, n_bands, f_min, d_out, static_cast<hipStream_t>(stream));
 }
 
 int32_t mm_onset_strength_device(mm_context ctx, const float* d_signal, int64_t signal_length, int32_t sample_rate,
@@ -444,6 +463,35 @@ int32_t mm_spectral_bandwidth(mm_context ctx, const float* signal_data, int64_t 
                               MMBuffer* out) {
   return descriptor_to_buffer(ctx, signal_data, signal_length, sample_rate, n_fft, hop_length, win_length, center,
                               DESC_BANDWIDTH, p, out);
+}
+
+int32_t mm_spectral_contrast(mm_context ctx, const float* signal_data, int64_t signal_length, int32_t sample_rate,
+                             int32_t n_fft, int32_t hop_length, int32_t win_length, int32_t center, int32_t n_bands,
+                             float f_min, MMBuffer* out) {
+  if (!ctx || !signal_data || signal_length <= 0 || !out) return MM_ERR_INVALID_INPUT;   // Bridge.swift:711-716
+  // n_bands < 1 traps in the reference (the range 1...n_bands); sr <= 0 has no bin width
+  if (n_fft <= 0 || sample_rate <= 0 || n_bands < 1 || n_bands + 1 > kMaxContrastBands) return MM_ERR_INVALID_INPUT;
+  SpecRequest r;
+  r.out_kind = OUT_STFT_MAG;
+  r.sr = sample_rate;
+  r.n_fft = n_fft;
+  r.hop = hop_length > 0 ? hop_length : n_fft / 4;
+  r.win = win_length > 0 ? win_length : n_fft;
+  r.center = center != 0;
+  if (r.hop <= 0) return MM_ERR_INVALID_INPUT;
+  const int64_t nb = n_bands + 1;
+  if (!pow2(n_fft) || frame_count(signal_length, n_fft, r.hop, r.center) == 0) {   // no STFT frames
+    fill_shape(out, {nb, 0}, 0, 0, nullptr);
+    return OK;
+  }
+  Lane* lane = static_cast<Ctx*>(ctx)->get();
+  if (!lane) return ERR_NO_GPU;
+  float* data = nullptr;
+  int64_t T = 0;
+  const int32_t s = run_contrast(lane, r, signal_data, signal_length, n_bands, f_min, &data, &T);
+  if (s != OK) return s;
+  fill_shape(out, {nb, T}, 0, nb * T, data);
+  return OK;
 }
 
 int32_t mm_spectral_rolloff(mm_context ctx, const float* signal_data, int64_t signal_length, int32_t sample_rate,

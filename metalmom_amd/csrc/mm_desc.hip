@@ -78,6 +78,135 @@ __global__ void k_descriptor(const float* __restrict__ S, int F, int T, int kind
   out[t] = y;
 }
 
+// Spectral contrast (SpectralDescriptors.swift:137-204): one thread per
+// (frame, band) of S [F, T]. The mean of the q lowest and of the q highest
+// magnitudes of the band equal those of the sorted band; they are kept in two
+// register lists (q <= KMAX, inserted into only when a value beats the
+// current q-th one) and summed in ascending order as the reference sums its
+// sorted band. Larger q (n_fft above 4096) finds the q-th value by
+// bisection over the float bit patterns (magnitudes are >= 0, so bit order is
+// value order) and sums around it.
+constexpr int kContrastKmax = 32;
+
+// l[0..k) holds the k lowest values in descending order (HIGH false) or the
+// k highest in ascending order (HIGH true), so l[0] is always the value a
+// newcomer must beat; v beats it: drop l[0] and insert v in order. Every
+// index is a compile-time constant (the lists stay in registers).
+template <int KMAX, bool HIGH>
+__device__ __forceinline__ void keep_extreme(float (&l)[KMAX], float v, int k) {
+  const float edge = HIGH ? INFINITY : -INFINITY;   // stands for l[k]
+#pragma unroll
+  for (int i = 0; i < KMAX; ++i) {
+    if (i < k) {
+      const float nx = (i + 1 < k) ? l[i + 1] : edge;
+      const bool nx_first = HIGH ? (nx < v) : (nx > v);
+      const bool v_here = (i == 0) || (HIGH ? (l[i] < v) : (l[i] > v));
+      l[i] = nx_first ? nx : (v_here ? v : l[i]);
+    }
+  }
+}
+
+// Sum of the k lowest (HIGH false) or highest values of col[t * T], t < n.
+template <bool HIGH>
+__device__ float sum_extreme_bisect(const float* col, int T, int n, int k) {
+  auto bits = [&](int t) { return __float_as_uint(col[(int64_t)t * T]); };
+  unsigned lo = 0, hi = 0x7f800000u;
+  while (lo < hi) {
+    const unsigned mid = HIGH ? lo + (hi - lo + 1) / 2 : lo + (hi - lo) / 2;
+    int c = 0;
+    for (int t = 0; t < n; ++t) c += HIGH ? (bits(t) >= mid) : (bits(t) <= mid);
+    if (HIGH) {
+      if (c >= k) lo = mid;
+      else hi = mid - 1;
+    } else {
+      if (c >= k) hi = mid;
+      else lo = mid + 1;
+    }
+  }
+  const float x = __uint_as_float(lo);   // the k-th value
+  float s = 0.0f;
+  int c = 0;
+  for (int t = 0; t < n; ++t) {
+    const float v = col[(int64_t)t * T];
+    if (HIGH ? (v > x) : (v < x)) {
+      s += v;
+      ++c;
+    }
+  }
+  return s + (float)(k - c) * x;
+}
+
+// Sums of the k lowest and k highest values of col[t * T], t < n, k <= KM,
+// each in ascending order.
+template <int KM>
+__device__ __forceinline__ void band_extremes(const float* col, int T, int n, int k, float& valley, float& peak) {
+  float lo[KM], hi[KM];
+#pragma unroll
+  for (int i = 0; i < KM; ++i) {
+    lo[i] = INFINITY;
+    hi[i] = -INFINITY;
+  }
+  for (int t = 0; t < n; ++t) {
+    const float v = col[(int64_t)t * T];
+    if (v < lo[0]) keep_extreme<KM, false>(lo, v, k);
+    if (v > hi[0]) keep_extreme<KM, true>(hi, v, k);
+  }
+  valley = 0.0f;
+  peak = 0.0f;
+#pragma unroll
+  for (int i = KM - 1; i >= 0; --i)
+    if (i < k) valley += lo[i];
+#pragma unroll
+  for (int i = 0; i < KM; ++i)
+    if (i < k) peak += hi[i];
+}
+
+__global__ void k_contrast(const float* __restrict__ S, int T, const ContrastBands* __restrict__ cb,
+                           float* __restrict__ out) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y;
+  if (f >= T) return;
+  const int e0 = cb->edge[b], n = cb->edge[b + 1] - e0;
+  float y = 0.0f;
+  if (n > 0) {
+    const int k = cb->q[b];
+    const float* col = S + (int64_t)e0 * T + f;
+    float valley = 0.0f, peak = 0.0f;
+    // k is the same for the whole block (one band): the smallest list that holds it
+    if (k <= 2) {
+      band_extremes<2>(col, T, n, k, valley, peak);
+    } else if (k <= 4) {
+      band_extremes<4>(col, T, n, k, valley, peak);
+    } else if (k <= 8) {
+      band_extremes<8>(col, T, n, k, valley, peak);
+    } else if (k <= 16) {
+      band_extremes<16>(col, T, n, k, valley, peak);
+    } else if (k <= kContrastKmax) {
+      band_extremes<kContrastKmax>(col, T, n, k, valley, peak);
+    } else {
+      valley = sum_extreme_bisect<false>(col, T, n, k);
+      peak = sum_extreme_bisect<true>(col, T, n, k);
+    }
+    valley /= (float)k;
+    peak /= (float)k;
+    {
+      // both dB values rounded before the difference, as the reference forms
+      // them (an FMA would leave one product unrounded: equal bands not 0)
+#pragma clang fp contract(off)
+      const float pdb = 10.0f * log10f(fmaxf(peak, 1e-10f));
+      const float vdb = 10.0f * log10f(fmaxf(valley, 1e-10f));
+      y = pdb - vdb;
+    }
+  }
+  out[(int64_t)b * T + f] = y;
+}
+
+hipError_t launch_contrast(const float* S, int T, int nb, const ContrastBands* d_cb, float* out, hipStream_t st) {
+  if (T <= 0 || nb <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_contrast, dim3((unsigned)((T + 255) / 256), (unsigned)nb), dim3(256), 0, st, S, T, d_cb, out);
+  return hipGetLastError();
+}
+
 // Fused form for one clip: each wave computes the FFT of one frame with the
 // last pass in registers (the fused tile kernel's register-split FFT), forms
 // |X[k]| in registers (lane l holds k = l + 64 q + BL r; lane 0 also the

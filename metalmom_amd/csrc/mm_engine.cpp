@@ -108,6 +108,7 @@ Lane::~Lane() {
   h_in.release();
   h_out.release();
   h_meta.release();
+  h_small.release();
   if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -854,6 +855,95 @@ int32_t run_resample(Lane* lane, const float* x, int64_t n, int source_sr, int t
   MM_HIP(hipMemcpyAsync(lane->d_in.p, lane->h_in.p, in_bytes, hipMemcpyHostToDevice, st));
   float* res = reinterpret_cast<float*>(static_cast<char*>(lane->d_in.p) + res_off);
   const int32_t s = enqueue_resample(static_cast<const float*>(lane->d_in.p), n, source_sr, target_sr, res, st);
+  if (s != OK) {
+    (void)hipStreamSynchronize(st);
+    return s;
+  }
+  MM_HIP(hipMemcpyAsync(lane->h_out.p, res, res_bytes, hipMemcpyDeviceToHost, st));
+  MM_HIP(hipStreamSynchronize(st));
+  float* h = static_cast<float*>(std::malloc(res_bytes));
+  if (!h) return ERR_ALLOC;
+  std::memcpy(h, lane->h_out.p, res_bytes);
+  *out = h;
+  return OK;
+}
+
+bool contrast_bands(int sr, int n_fft, int n_freqs, int n_bands, float fmin, ContrastBands* cb) {
+  if (n_bands < 1 || n_bands + 1 > kMaxContrastBands) return false;
+  // float32 as SpectralDescriptors.swift:145-158 forms them; roundf rounds half away from zero
+  const float res = (float)sr / (float)n_fft;
+  cb->nb = n_bands + 1;
+  for (int b = 0; b <= n_bands + 1; ++b) {
+    const float hz = (b == 0) ? 0.0f : (b <= n_bands) ? fmin * powf(2.0f, (float)(b - 1)) : (float)sr / 2.0f;
+    cb->edge[b] = std::min((int)roundf(hz / res), n_freqs);
+  }
+  for (int b = 0; b <= n_bands; ++b) {
+    const int n = cb->edge[b + 1] - cb->edge[b];
+    cb->q[b] = n > 0 ? std::max(1, (int)roundf(0.02f * (float)n)) : 0;   // quantile 0.02 (:180-181)
+  }
+  return true;
+}
+
+int32_t enqueue_contrast(Lane* lane, const SpecRequest& r, const float* d_x, int64_t len, int n_bands, float fmin,
+                         float* d_res, hipStream_t st) {
+  const int T = (int)frame_count(len, r.n_fft, r.hop, r.center);
+  if (T == 0) return OK;
+  ContrastBands cb;
+  if (!contrast_bands(r.sr, r.n_fft, r.n_fft / 2 + 1, n_bands, fmin, &cb)) return ERR_INVALID;
+  Engine* eng = Engine::get();
+  if (!eng) return ERR_NO_GPU;
+  const Tables* tb = nullptr;
+  int32_t s = eng->tables_for(r, st, &tb);
+  if (s != OK) return s;
+  TileLayout L;
+  const bool fused = tile_layout(r.n_fft, r.out_kind, r.precision, 0, 0, &L);
+  const int tf = fused ? L.tile_frames : kTileFrames;
+  BatchMeta m;
+  build_meta(r, &len, 1, tf, &m);
+  MM_HIP(lane->d_out.ensure((size_t)m.total_out * 4));   // magnitude spectrogram [F, T]
+  s = cached_meta(lane, r, len, tf, m, st);
+  if (s != OK) return s;
+  // band table in d_small after the onset key slot, uploaded once per configuration
+  constexpr size_t kBandsOff = 256;
+  const void* before = lane->d_small.p;
+  MM_HIP(lane->d_small.ensure(kBandsOff + sizeof(ContrastBands)));
+  if (lane->d_small.p != before) lane->bands_key.clear();
+  unsigned fbits;
+  std::memcpy(&fbits, &fmin, 4);
+  const std::string key = std::to_string(r.sr) + ":" + std::to_string(r.n_fft) + ":" + std::to_string(n_bands) +
+                          ":" + std::to_string(fbits);
+  auto* d_cb = reinterpret_cast<const ContrastBands*>(static_cast<char*>(lane->d_small.p) + kBandsOff);
+  if (lane->bands_key != key) {
+    MM_HIP(lane->h_small.ensure(sizeof(ContrastBands)));
+    std::memcpy(lane->h_small.p, &cb, sizeof(ContrastBands));
+    MM_HIP(hipMemcpyAsync(const_cast<ContrastBands*>(d_cb), lane->h_small.p, sizeof(ContrastBands),
+                          hipMemcpyHostToDevice, st));
+    MM_HIP(hipStreamSynchronize(st));
+    lane->bands_key = key;
+  }
+  const MetaPtrs mp = meta_ptrs(m, lane->d_meta.p);
+  s = enqueue_batch(eng, r, tb, m, mp, d_x, lane->d_out.p, &lane->d_scratch, &lane->d_scratch2, 0, st);
+  if (s != OK) return s;
+  MM_HIP(launch_contrast(static_cast<const float*>(lane->d_out.p), T, cb.nb, d_cb, d_res, st));
+  return OK;
+}
+
+int32_t run_contrast(Lane* lane, const SpecRequest& r, const float* signal, int64_t len, int n_bands, float fmin,
+                     float** out, int64_t* frames) {
+  *out = nullptr;
+  const int T = (int)frame_count(len, r.n_fft, r.hop, r.center);
+  *frames = T;
+  if (T == 0) return OK;
+  hipStream_t st = lane->stream;
+  const size_t in_bytes = (size_t)len * 4, res_off = (in_bytes + 255) & ~(size_t)255;
+  const size_t res_bytes = (size_t)(n_bands + 1) * T * 4;
+  MM_HIP(lane->h_in.ensure(in_bytes));
+  MM_HIP(lane->d_in.ensure(res_off + res_bytes));   // signal | result
+  MM_HIP(lane->h_out.ensure(res_bytes));
+  std::memcpy(lane->h_in.p, signal, in_bytes);
+  MM_HIP(hipMemcpyAsync(lane->d_in.p, lane->h_in.p, in_bytes, hipMemcpyHostToDevice, st));
+  float* res = reinterpret_cast<float*>(static_cast<char*>(lane->d_in.p) + res_off);
+  const int32_t s = enqueue_contrast(lane, r, static_cast<const float*>(lane->d_in.p), len, n_bands, fmin, res, st);
   if (s != OK) {
     (void)hipStreamSynchronize(st);
     return s;

@@ -66,8 +66,9 @@ struct HostBuf {
 struct Lane {
   hipStream_t stream = nullptr;
   DevBuf d_in, d_out, d_meta, d_scratch, d_scratch2, d_small;
-  HostBuf h_in, h_out, h_meta;
+  HostBuf h_in, h_out, h_meta, h_small;
   std::string meta_key;   // work list now in d_meta for device-resident single-clip calls ("" = none)
+  std::string bands_key;  // spectral contrast band table now in d_small (at kBandsOff), "" = none
   ~Lane();
 };
 
@@ -216,6 +217,7 @@ inline int32_t hip_status(hipError_t e) {
   if (e == hipErrorNoDevice || e == hipErrorInvalidDevice) return ERR_NO_GPU;
   return ERR_INTERNAL;
 }
+inline int32_t hip_status_or_ok(hipError_t e) { return e == hipSuccess ? OK : hip_status(e); }
 
 // The lane of a live mm_context (nullptr for NULL or a destroyed context).
 Lane* ctx_lane(void* ctx);
@@ -232,6 +234,16 @@ int32_t enqueue_resample(const float* d_x, int64_t n, int source_sr, int target_
 // Host clip -> malloc'd result of *n_out samples (nullptr when empty).
 int32_t run_resample(Lane* lane, const float* x, int64_t n, int source_sr, int target_sr, float** out,
                      int64_t* n_out);
+// Spectral contrast (SpectralDescriptors.swift:137-204) of one host clip:
+// bands from contrast_bands, result [n_bands + 1, frames] (malloc'd).
+bool contrast_bands(int sr, int n_fft, int n_freqs, int n_bands, float fmin, ContrastBands* cb);
+int32_t run_contrast(Lane* lane, const SpecRequest& r, const float* signal, int64_t len, int n_bands, float fmin,
+                     float** out, int64_t* frames);
+// The same on device memory: d_res [n_bands + 1, frames], enqueued on st
+// (the band table is uploaded, with a host synchronisation, once per
+// configuration on a lane).
+int32_t enqueue_contrast(Lane* lane, const SpecRequest& r, const float* d_x, int64_t len, int n_bands, float fmin,
+                         float* d_res, hipStream_t st);
 int32_t run_onset(Lane* lane, const SpecRequest& r, const float* signal, int64_t len, int aggregate, int lag,
                   float** out, int64_t* rows, int64_t* cols);
 

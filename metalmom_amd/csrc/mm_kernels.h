@@ -66,6 +66,18 @@ hipError_t launch_onset(const float* db, int n_bands, int T, const unsigned* max
 // param: bandwidth p, rolloff roll_percent.
 hipError_t launch_descriptor(const float* S, int F, int T, int kind, float srf, float nfftf, float param,
                              float* out, hipStream_t st);
+// Spectral contrast bands: nb = n_bands + 1 bands, band b = bins
+// [edge[b], edge[b+1]), q[b] values averaged at each end.
+constexpr int kMaxContrastBands = 63;
+struct ContrastBands {
+  int nb;
+  int edge[kMaxContrastBands + 1];
+  int q[kMaxContrastBands];
+};
+// out [nb, T] (row-major) from the magnitude spectrogram S [F, T]; d_cb: the
+// band table in device memory.
+hipError_t launch_contrast(const float* S, int T, int nb, const ContrastBands* d_cb, float* out, hipStream_t st);
+
 // Fused descriptor of one clip (FFT + reduction per wave, no spectrogram in
 // HBM) for n_fft 1024 / 2048 / 4096: desc_fused_waves gives the waves per
 // workgroup (tiles of that many frames; 0 = no fused kernel); out[T].

@@ -231,6 +231,35 @@ def spectral_bandwidth(y=None, sr=22050, S=None, n_fft=2048, hop_length=None, wi
     return _descriptor("mm_spectral_bandwidth", y, sr, n_fft, hop_length, win_length, center, float(p))
 
 
+def spectral_contrast(y=None, sr=22050, S=None, n_fft=2048, hop_length=None, win_length=None, center=True,
+                      n_bands=6, fmin=200.0, quantile=0.02, **kwargs):
+    """Spectral contrast ``(n_bands + 1, n_frames)`` (the reference's
+    ``feature.py:744-806``): per octave band of the magnitude STFT, dB of the
+    mean of its top 2 % over the mean of its bottom 2 % (on the GPU; default
+    ``hop_length`` 512). As in the reference, ``quantile`` is not passed to
+    the C ABI (which always uses 0.02) and ``S`` is not supported.
+    """
+    if S is not None:
+        raise NotImplementedError("spectral_contrast with pre-computed S not yet supported")
+    if y is None:
+        raise ValueError("Either y or S must be provided")
+    y = np.ascontiguousarray(y, dtype=np.float32)
+    hop = hop_length if hop_length is not None else 512
+    win = win_length if win_length is not None else n_fft
+    ctx = lib.mm_init()
+    if not ctx:
+        raise RuntimeError("Failed to initialize MetalMom context")
+    try:
+        out = MMBuffer()
+        status = lib.mm_spectral_contrast(ctx, float_ptr(y), len(y), sr, n_fft, hop, win, 1 if center else 0,
+                                          int(n_bands), float(fmin), C.byref(out))
+        if status != 0:
+            raise RuntimeError(f"mm_spectral_contrast failed with status {status}")
+        return buffer_to_numpy(out)
+    finally:
+        lib.mm_destroy(ctx)
+
+
 def spectral_rolloff(y=None, sr=22050, S=None, n_fft=2048, hop_length=None, win_length=None, center=True,
                      roll_percent=0.85, **kwargs):
     """Spectral rolloff ``(1, n_frames)`` (``feature.py:808-880``): frequency of

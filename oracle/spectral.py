@@ -21,6 +21,7 @@ __all__ = [
     "dct_ortho_basis", "mfcc", "mfcc_from_logmel", "onset_strength",
     "chroma_filterbank", "chroma_stft", "normalize_frames",
     "spectral_centroid", "spectral_bandwidth", "spectral_rolloff", "spectral_flatness",
+    "contrast_bands", "contrast_from_magnitude", "spectral_contrast",
 ]
 
 
@@ -498,6 +499,56 @@ def spectral_flatness(y, n_fft=2048, hop=512, win_length=None, center=True, powe
     return out[None, :]
 
 
+def _round_f32(x):
+    """Swift's ``roundf`` (half away from zero) on a float32 value >= 0."""
+    return int(np.floor(np.float32(x) + np.float32(0.5)))
+
+
+def contrast_bands(sr: int, n_fft: int, n_freqs: int, n_bands: int = 6, fmin: float = 200.0,
+                   quantile: float = 0.02):
+    """Bin edges and per-band quantile counts of spectral contrast, in float32
+    as the reference forms them (``SpectralDescriptors.swift:142-158, 180-181``):
+    edges 0, fmin 2^(b-1) for b = 1..n_bands, sr/2, each ``min(round(edge /
+    (sr / n_fft)), n_freqs)``; a band of ``size`` bins averages
+    ``max(1, round(quantile * size))`` values at each end."""
+    f32 = np.float32
+    res = f32(sr) / f32(n_fft)
+    hz = [f32(0)] + [f32(fmin) * f32(2.0) ** f32(b - 1) for b in range(1, n_bands + 1)] + [f32(sr) / f32(2)]
+    edges = [min(_round_f32(f32(h) / res), n_freqs) for h in hz]
+    q = [max(1, _round_f32(f32(quantile) * f32(e1 - e0))) if e1 > e0 else 0 for e0, e1 in zip(edges[:-1], edges[1:])]
+    return edges, q
+
+
+def contrast_from_magnitude(S, sr: int, n_fft: int, n_bands: int = 6, fmin: float = 200.0,
+                            quantile: float = 0.02) -> np.ndarray:
+    """Spectral contrast of a magnitude spectrogram ``[F, T]``
+    (``SpectralDescriptors.swift:137-204``): per band and frame, the bins
+    sorted ascending, valley = mean of the lowest q, peak = mean of the
+    highest q, output ``10 log10(max(peak, 1e-10)) - 10 log10(max(valley,
+    1e-10))``; an empty band gives 0. ``[n_bands + 1, T]``."""
+    S = np.asarray(S, dtype=np.float64)
+    F, T = S.shape
+    edges, q = contrast_bands(sr, n_fft, F, n_bands, fmin, quantile)
+    out = np.zeros((n_bands + 1, T))
+    for b in range(n_bands + 1):
+        e0, e1 = edges[b], edges[b + 1]
+        if e1 <= e0:
+            continue
+        srt = np.sort(S[e0:e1], axis=0)
+        valley = srt[:q[b]].mean(axis=0)
+        peak = srt[e1 - e0 - q[b]:].mean(axis=0)
+        out[b] = 10 * np.log10(np.maximum(peak, 1e-10)) - 10 * np.log10(np.maximum(valley, 1e-10))
+    return out
+
+
+def spectral_contrast(y, sr=22050, n_fft=2048, hop=512, win_length=None, center=True, n_bands=6,
+                      fmin=200.0, quantile=0.02) -> np.ndarray:
+    """Spectral contrast of a signal: ``contrast_from_magnitude`` of the
+    magnitude STFT (``Bridge.swift:698-737``)."""
+    return contrast_from_magnitude(stft_magnitude(y, n_fft, hop, win_length, center), sr, n_fft, n_bands,
+                                   fmin, quantile)
+
+
 PINS = {
     "hann_periodic": "known answer hann(4) = [0, .5, 1, .5] (WindowTests.swift:60-66); closed form",
     "prepare_window": "centre zero-pad / centre truncation indices checked by hand on small cases",
@@ -519,6 +570,12 @@ PINS = {
     "spectral_bandwidth": "golden spectral_bandwidth_440hz_default.npy; a single bin has bandwidth 0",
     "spectral_rolloff": "golden spectral_rolloff_440hz_default.npy; brute-force cumulative search",
     "spectral_flatness": "golden spectral_flatness_440hz_default.npy; white noise near 1, tone near 0, flat -> 1",
+    "contrast_bands": "edges and counts worked by hand for sr 22050, n_fft 2048 and 4096 (round half away)",
+    "contrast_from_magnitude": "hand cases (a band holding 1..100 gives 10 log10(99.5 / 1.5)); a flat band gives 0; "
+                               "bin order within a band does not matter; planted mistakes (peak and valley "
+                               "swapped, q rounded down) fail",
+    "spectral_contrast": "golden spectral_contrast_440hz_default.npy (librosa's algorithm) at the reference "
+                         "test's tolerance (rtol 0.5, atol 15 dB, test_parity_spectral.py:158-172)",
     "onset_strength": "golden onset_strength_440hz_default.npy (librosa.onset.onset_strength); silence -> 0; "
                       "non-negative; shape / padding rules; planted mistakes (no rectification, pad width) fail",
 }

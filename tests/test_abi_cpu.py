@@ -226,3 +226,25 @@ def test_resample_identity_needs_no_gpu():
     from metalmom_amd._buffer import buffer_to_numpy
     r = buffer_to_numpy(out)
     assert r.shape == y.shape and np.array_equal(r, y)
+
+
+def test_contrast_validation_needs_no_gpu():
+    """Bridge.swift:711-716 (ctx, signal, length, out); n_bands < 1 traps in
+    the reference and sr <= 0 has no bin width: -1 here. A non-power-of-two
+    n_fft has no STFT frames: [n_bands + 1, 0]."""
+    lib = _lib()
+    from metalmom_amd._native import MMBuffer, float_ptr
+    y = np.ones(4096, np.float32)
+    out = MMBuffer()
+    assert lib.mm_spectral_contrast(None, float_ptr(y), 4096, 22050, 2048, 512, 2048, 1, 6, 200.0, C.byref(out)) == -1
+    ctx = lib.mm_init()
+    try:
+        assert lib.mm_spectral_contrast(ctx, float_ptr(y), 4096, 22050, 2048, 512, 2048, 1, 0, 200.0,
+                                        C.byref(out)) == -1
+        assert lib.mm_spectral_contrast(ctx, float_ptr(y), 4096, 0, 2048, 512, 2048, 1, 6, 200.0, C.byref(out)) == -1
+        assert lib.mm_spectral_contrast(ctx, float_ptr(y), 4096, 22050, 1000, 512, 1000, 1, 6, 200.0,
+                                        C.byref(out)) == 0
+        assert out.ndim == 2 and out.shape[0] == 7 and out.shape[1] == 0 and out.count == 0
+        lib.mm_buffer_free(C.byref(out))
+    finally:
+        lib.mm_destroy(ctx)

@@ -117,3 +117,69 @@ def test_descriptor_device_entry_matches_host_entry():
             assert np.array_equal(d_o.cpu().numpy()[None, :], getattr(mm, name)(y=y, **kw))
     finally:
         lib.mm_destroy(ctx)
+
+
+# -- spectral contrast (mm_spectral_contrast, SpectralDescriptors.swift:137-204)
+
+def test_contrast_golden_at_reference_tolerance(signal_440):
+    """The librosa golden at the reference's own tolerance (rtol 0.5, atol
+    15 dB; test_parity_spectral.py:158-172), and the oracle closely."""
+    import metalmom_amd as mm
+    G = load_golden("spectral_contrast_440hz_default")
+    got = mm.feature.spectral_contrast(y=signal_440, sr=22050)
+    assert got.shape == G.shape and np.all(np.isfinite(got))
+    np.testing.assert_allclose(got, G, rtol=0.5, atol=15.0)
+
+
+@pytest.mark.parametrize("sr,n,kw", [
+    (22050, 80000, {}),
+    (44100, 120000, {"n_fft": 4096, "hop_length": 1024}),                          # top band q = 29
+    (44100, 150000, {"n_fft": 8192, "hop_length": 2048}),                          # top band q = 58: bisection
+    (22050, 50000, {"n_fft": 1024, "hop_length": 300, "center": False, "n_bands": 4, "fmin": 100.0}),
+], ids=["default", "4096", "8192-bisection", "1024-4bands-uncentred"])
+def test_contrast_matches_oracle(sr, n, kw):
+    """Noise-rich signals, so every band's lowest magnitudes sit far above
+    the FP32 FFT's rounding floor: the dB difference of two FP32 means
+    agrees with the float64 oracle to 0.01 dB."""
+    import metalmom_amd as mm
+    print(f"contrast sr={sr} n={n} {kw}", flush=True)
+    y = _mix(n, sr, n % 89)
+    got = mm.feature.spectral_contrast(y=y, sr=sr, **kw).astype(np.float64)
+    ref = o.spectral_contrast(y.astype(np.float64), sr, n_fft=kw.get("n_fft", 2048), hop=kw.get("hop_length", 512),
+                              center=kw.get("center", True), n_bands=kw.get("n_bands", 6),
+                              fmin=kw.get("fmin", 200.0))
+    assert got.shape == ref.shape
+    assert np.abs(got - ref).max() <= 0.01, float(np.abs(got - ref).max())
+
+
+def test_contrast_edges():
+    import metalmom_amd as mm
+    z = np.zeros(30000, np.float32)
+    assert not mm.feature.spectral_contrast(y=z).any()                        # silence: 0 dB in every band
+    assert mm.feature.spectral_contrast(y=z, n_fft=1000).shape == (7, 0)      # no STFT frames
+    c = mm.feature.spectral_contrast(y=_mix(20000, 8000, 1), sr=8000, n_fft=512, n_bands=7)
+    assert c.shape[0] == 8 and not c[-1].any()                                 # octaves past Nyquist: empty
+
+
+def test_contrast_device_entry_matches_host_entry():
+    import ctypes as C
+    import torch
+    import metalmom_amd as mm
+    from metalmom_amd._native import lib
+    y = _mix(70000, 22050, 4)
+    host = mm.feature.spectral_contrast(y=y, sr=22050, n_bands=5, fmin=150.0)
+    X = torch.from_numpy(y.copy()).cuda()
+    T = host.shape[1]
+    Y = torch.full((6 * T,), float("nan"), device="cuda")
+    fr = C.c_int64()
+    ctx = lib.mm_init()
+    try:
+        for _ in range(2):   # the second call reuses the uploaded band table
+            s = lib.mm_spectral_contrast_device(ctx, C.c_void_p(X.data_ptr()), len(y), 22050, 2048, 512, 2048, 1, 5,
+                                                150.0, C.c_void_p(Y.data_ptr()), C.byref(fr),
+                                                C.c_void_p(torch.cuda.current_stream().cuda_stream))
+            assert s == 0 and fr.value == T
+            torch.cuda.synchronize()
+            assert np.array_equal(Y.cpu().numpy().reshape(6, T), host)
+    finally:
+        lib.mm_destroy(ctx)

@@ -415,3 +415,53 @@ def test_descriptor_known_answers():
         if cum >= 0.85 * tot:
             break
     assert o.spectral_rolloff(noise, sr)[0, fr] == kk * sr / 2048
+
+
+# -- spectral contrast (SpectralDescriptors.swift:137-204) ---------------------
+
+def test_contrast_bands_by_hand():
+    """sr 22050, n_fft 2048: bin width 10.7666 Hz; 200, 400, ..., 6400 Hz ->
+    18.58, 37.15, 74.3, 148.6, 297.2, 594.4 -> 19, 37, 74, 149, 297, 594;
+    Nyquist -> 1024. Band sizes 19, 18, 37, 75, 148, 297, 430 -> 2 % of them
+    rounded half away from zero, at least 1: 1, 1, 1, 2, 3, 6, 9."""
+    assert o.contrast_bands(22050, 2048, 1025) == ([0, 19, 37, 74, 149, 297, 594, 1024], [1, 1, 1, 2, 3, 6, 9])
+    e, q = o.contrast_bands(22050, 4096, 2049)
+    assert e == [0, 37, 74, 149, 297, 594, 1189, 2048] and q == [1, 1, 2, 3, 6, 12, 17]
+    e, q = o.contrast_bands(8000, 512, 257, n_bands=7)   # upper octaves past Nyquist: empty bands
+    assert e[-3:] == [257, 257, 256] and q[-2:] == [0, 0]
+
+
+def test_contrast_known_answers():
+    rng = np.random.default_rng(2)
+    S = np.ones((1025, 3))
+    S[149:297, 0] = rng.permutation(np.arange(1.0, 149.0))   # band 4: 148 bins, q = 3
+    c = o.contrast_from_magnitude(S, 22050, 2048)
+    assert abs(c[4, 0] - 10 * np.log10(147.0 / 2.0)) < 1e-12    # mean(146..148) / mean(1..3)
+    assert np.all(c[:, 1:] == 0) and np.all(np.delete(c[:, 0], 4) == 0)   # flat bands: 0
+    S2 = S.copy()
+    S2[149:297, 0] = S2[149:297, 0][::-1]                          # order within a band is irrelevant
+    assert np.array_equal(o.contrast_from_magnitude(S2, 22050, 2048), c)
+    z = o.contrast_from_magnitude(np.zeros((1025, 2)), 22050, 2048)
+    assert np.all(z == 0)                                          # 10 log10(1e-10) - itself
+
+
+def test_contrast_planted_mistakes_are_caught():
+    """Swapping peak and valley flips the sign; rounding the 75-bin band's
+    1.5 down keeps one bin per end instead of two."""
+    S = np.ones((1025, 1))
+    S[74:149, 0] = np.arange(1.0, 76.0)
+    good = o.contrast_from_magnitude(S, 22050, 2048)[3, 0]
+    assert abs(good - 10 * np.log10(74.5 / 1.5)) < 1e-12
+    assert abs(-good - good) > 1.0                                  # swapped
+    assert abs(10 * np.log10(75.0 / 1.0) - good) > 1.0             # q = floor(1.5) = 1
+
+
+def test_contrast_golden_at_reference_tolerance(signal_440):
+    """librosa's spectral_contrast (the golden) differs from the reference's
+    algorithm; the reference checks itself against it at rtol 0.5, atol 15 dB
+    (test_parity_spectral.py:158-172), and so does the oracle."""
+    G = load_golden("spectral_contrast_440hz_default")
+    c = o.spectral_contrast(signal_440, 22050)
+    assert c.shape == G.shape
+    np.testing.assert_allclose(c, G, rtol=0.5, atol=15.0)
+    assert np.all(np.isfinite(c))
