@@ -16,6 +16,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
+#include <new>
 #include <vector>
 
 #include "../../include/metalmom.h"
@@ -216,20 +218,32 @@ int32_t mm_get_sample_rate(mm_context ctx, const char* path, int32_t* out_sr) {
   return MM_OK;
 }
 
+static int32_t load_wav(mm_context ctx, const char* path, int32_t sr, int32_t mono, float offset, float duration,
+                        MMBuffer* out, int32_t* out_sr);
+
 int32_t mm_load(mm_context ctx, const char* path, int32_t sr, int32_t mono, float offset, float duration,
                 MMBuffer* out, int32_t* out_sr) {
+  try {
+    return load_wav(ctx, path, sr, mono, offset, duration, out, out_sr);
+  } catch (const std::bad_alloc&) {   // no exception leaves the C ABI
+    return MM_ERR_ALLOC_FAILED;
+  }
+}
+
+static int32_t load_wav(mm_context ctx, const char* path, int32_t sr, int32_t mono, float offset, float duration,
+                        MMBuffer* out, int32_t* out_sr) {
   if (!path || !out) return MM_ERR_INVALID_INPUT;   // Bridge.swift:1058-1060
   if (!(offset >= 0.0f)) return MM_ERR_INVALID_INPUT;
   WavInfo w;
   FILE* f = nullptr;
   if (!wav_open(path, &w, &f)) return MM_ERR_INVALID_INPUT;
+  std::unique_ptr<FILE, int (*)(FILE*)> file(f, &std::fclose);   // closed on every path, exceptions included
   const int target = sr > 0 ? sr : w.sr;
   const int64_t start = (int64_t)((double)offset * (double)w.sr);   // AudioIO.swift:31-33
   const int64_t avail = w.frames - start;
   int64_t n = avail;
   if (duration > 0.0f) n = (int64_t)std::min((double)avail, (double)duration * (double)w.sr);
   if (avail <= 0 || n <= 0) {   // AudioIO.swift:34-36
-    std::fclose(f);
     empty_signal(out);
     if (out_sr) *out_sr = target;
     return MM_OK;
@@ -239,7 +253,7 @@ int32_t mm_load(mm_context ctx, const char* path, int32_t sr, int32_t mono, floa
   std::vector<unsigned char> raw((size_t)n * ba);
   const bool ok = fseeko(f, w.data_off + start * (int64_t)ba, SEEK_SET) == 0 &&
                   std::fread(raw.data(), 1, raw.size(), f) == raw.size();
-  std::fclose(f);
+  file.reset();
   if (!ok) return MM_ERR_INVALID_INPUT;
   const int mix = (mono != 0 && w.channels > 1) ? 1 : 0;
   float* data = nullptr;

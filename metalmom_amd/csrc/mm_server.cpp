@@ -23,6 +23,7 @@
 #include <deque>
 #include <map>
 #include <mutex>
+#include <new>
 #include <thread>
 #include <unordered_map>
 #include <vector>
@@ -589,7 +590,11 @@ int64_t mm_server_submit(mm_server srv, const MMSpecParams* params, const float*
   int32_t s = request_from_params(params, &r);
   if (s == OK && r.in_dtype != IN_F32) s = ERR_INVALID;   // host clips are float32
   if (s != OK) return s;
-  return static_cast<Server*>(srv)->submit(r, signal_data, signal_length);
+  try {
+    return static_cast<Server*>(srv)->submit(r, signal_data, signal_length);
+  } catch (const std::bad_alloc&) {   // no exception leaves the C ABI
+    return MM_ERR_ALLOC_FAILED;
+  }
 }
 
 int64_t mm_server_submit_pcm(mm_server srv, const MMSpecParams* params, const void* pcm, int64_t frames,
@@ -602,8 +607,12 @@ int64_t mm_server_submit_pcm(mm_server srv, const MMSpecParams* params, const vo
   if (s != OK) return s;
   const int64_t len = resample_length(frames, source_sr, r.sr);
   if (len <= 0) return MM_ERR_INVALID_INPUT;
-  return static_cast<Server*>(srv)->submit(r, pcm, len, pcm_format, frames, channels,
-                                           (mono != 0 && channels > 1) ? 1 : 0, source_sr);
+  try {
+    return static_cast<Server*>(srv)->submit(r, pcm, len, pcm_format, frames, channels,
+                                             (mono != 0 && channels > 1) ? 1 : 0, source_sr);
+  } catch (const std::bad_alloc&) {   // no exception leaves the C ABI
+    return MM_ERR_ALLOC_FAILED;
+  }
 }
 
 int32_t mm_server_poll(mm_server srv, int64_t request_id, MMBuffer* out, double* latency_s) {
