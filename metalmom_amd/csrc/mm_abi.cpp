@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <new>
 #include <mutex>
 #include <unordered_set>
 #include <vector>
@@ -141,14 +142,16 @@ Lane* ctx_lane(void* ctx) { return (ctx && is_live(ctx)) ? static_cast<Ctx*>(ctx
 
 extern "C" {
 
-mm_context mm_init(void) {
+mm_context mm_init(void) try {
   Ctx* c = new Ctx();
   std::lock_guard<std::mutex> g(g_reg_mu);
   registry().insert(c);
   return c;
+} catch (...) {   // no exception leaves the C ABI
+  return nullptr;
 }
 
-void mm_destroy(mm_context ctx) {
+void mm_destroy(mm_context ctx) try {
   if (!ctx) return;
   {
     std::lock_guard<std::mutex> g(g_reg_mu);
@@ -160,10 +163,11 @@ void mm_destroy(mm_context ctx) {
     Engine::get()->release_lane(c->lane);
   }
   delete c;
+} catch (...) {   // no exception leaves the C ABI
 }
 
 int32_t mm_stft(mm_context ctx, const float* signal_data, int64_t signal_length, int32_t sample_rate,
-                const MMSTFTParams* params, MMBuffer* out) {
+                const MMSTFTParams* params, MMBuffer* out) try {
   if (!ctx || !signal_data || signal_length <= 0 || !params || !out) return MM_ERR_INVALID_INPUT;
   if (!pow2(params->n_fft)) return MM_ERR_INVALID_INPUT;
   if (params->hop_length <= 0) return MM_ERR_INVALID_INPUT;  // the reference divides by it
@@ -175,12 +179,16 @@ int32_t mm_stft(mm_context ctx, const float* signal_data, int64_t signal_length,
   r.win = params->win_length;  // taken verbatim; <= 0 gives an all-zero window
   r.center = params->center != 0;
   return run_to_buffer(ctx, r, signal_data, signal_length, out);
+} catch (const std::bad_alloc&) {   // no exception leaves the C ABI
+  return MM_ERR_ALLOC_FAILED;
+} catch (...) {
+  return MM_ERR_INTERNAL;
 }
 
 int32_t mm_spectral_descriptor_device(mm_context ctx, const float* d_signal, int64_t signal_length,
                                       int32_t sample_rate, int32_t n_fft, int32_t hop_length, int32_t win_length,
                                       int32_t center, int32_t kind, float param, float* d_out, int64_t* out_frames,
-                                      void* stream) {
+                                      void* stream) try {
   if (!ctx || !is_live(ctx) || !d_signal || signal_length <= 0 || !d_out) return MM_ERR_INVALID_INPUT;
   if (!pow2(n_fft) || kind < MM_DESC_CENTROID || kind > MM_DESC_FLATNESS) return MM_ERR_INVALID_INPUT;
   SpecRequest r;
@@ -194,12 +202,16 @@ int32_t mm_spectral_descriptor_device(mm_context ctx, const float* d_signal, int
   Lane* lane = static_cast<Ctx*>(ctx)->get();
   if (!lane) return ERR_NO_GPU;
   return enqueue_descriptor(lane, r, d_signal, signal_length, kind, param, d_out, static_cast<hipStream_t>(stream));
+} catch (const std::bad_alloc&) {   // no exception leaves the C ABI
+  return MM_ERR_ALLOC_FAILED;
+} catch (...) {
+  return MM_ERR_INTERNAL;
 }
 
 int32_t mm_spectral_contrast_device(mm_context ctx, const float* d_signal, int64_t signal_length,
                                     int32_t sample_rate, int32_t n_fft, int32_t hop_length, int32_t win_length,
                                     int32_t center, int32_t n_bands, float f_min, float* d_out, int64_t* out_frames,
-                                    void* stream) {
+                                    void* stream) try {
   if (!ctx || !is_live(ctx) || !d_signal || signal_length <= 0 || !d_out) return MM_ERR_INVALID_INPUT;
   if (!pow2(n_fft) || sample_rate <= 0 || n_bands < 1 || n_bands + 1 > kMaxContrastBands) return MM_ERR_INVALID_INPUT;
   SpecRequest r;
@@ -213,12 +225,16 @@ int32_t mm_spectral_contrast_device(mm_context ctx, const float* d_signal, int64
   Lane* lane = static_cast<Ctx*>(ctx)->get();
   if (!lane) return ERR_NO_GPU;
   return enqueue_contrast(lane, r, d_signal, signal_length, n_bands, f_min, d_out, static_cast<hipStream_t>(stream));
+} catch (const std::bad_alloc&) {   // no exception leaves the C ABI
+  return MM_ERR_ALLOC_FAILED;
+} catch (...) {
+  return MM_ERR_INTERNAL;
 }
 
 int32_t mm_onset_strength_device(mm_context ctx, const float* d_signal, int64_t signal_length, int32_t sample_rate,
                                  int32_t n_fft, int32_t hop_length, int32_t n_mels, float f_min, float f_max,
                                  int32_t center, int32_t aggregate, float* d_out, int64_t* out_rows,
-                                 int64_t* out_cols, void* stream) {
+                                 int64_t* out_cols, void* stream) try {
   if (!ctx || !is_live(ctx) || !d_signal || signal_length <= 0 || !d_out) return MM_ERR_INVALID_INPUT;
   if (sample_rate <= 0 || n_mels < 1 || !pow2(n_fft)) return MM_ERR_INVALID_INPUT;
   SpecRequest r;
@@ -238,18 +254,26 @@ int32_t mm_onset_strength_device(mm_context ctx, const float* d_signal, int64_t 
   Lane* lane = static_cast<Ctx*>(ctx)->get();
   if (!lane) return ERR_NO_GPU;
   return enqueue_onset(lane, r, d_signal, signal_length, aggregate != 0, 1, d_out, static_cast<hipStream_t>(stream));
+} catch (const std::bad_alloc&) {   // no exception leaves the C ABI
+  return MM_ERR_ALLOC_FAILED;
+} catch (...) {
+  return MM_ERR_INTERNAL;
 }
 
 int64_t mm_istft_length(int32_t n_freqs, int32_t n_frames, int32_t hop_length, int32_t center,
-                        int64_t output_length) {
+                        int64_t output_length) try {
   IstftShape sh;
   if (istft_shape(n_freqs, n_frames, hop_length, center, output_length, &sh) != OK) return -1;
   return sh.out_len;
+} catch (const std::bad_alloc&) {   // no exception leaves the C ABI
+  return MM_ERR_ALLOC_FAILED;
+} catch (...) {
+  return MM_ERR_INTERNAL;
 }
 
 int32_t mm_istft_device(mm_context ctx, const float* d_stft, int32_t ld_frames, int32_t n_freqs, int32_t n_frames,
                         int32_t sample_rate, int32_t hop_length, int32_t win_length, int32_t center,
-                        int64_t output_length, float* d_out, void* stream) {
+                        int64_t output_length, float* d_out, void* stream) try {
   if (!ctx || !is_live(ctx) || !d_stft || !d_out) return MM_ERR_INVALID_INPUT;
   IstftShape sh;
   if (istft_shape(n_freqs, n_frames, hop_length, center, output_length, &sh) != OK) return MM_ERR_INVALID_INPUT;
@@ -259,10 +283,14 @@ int32_t mm_istft_device(mm_context ctx, const float* d_stft, int32_t ld_frames, 
   if (!lane) return ERR_NO_GPU;
   return enqueue_istft(sh, sample_rate, win_length, d_stft, ld, d_out, &lane->d_scratch,
                        static_cast<hipStream_t>(stream));
+} catch (const std::bad_alloc&) {   // no exception leaves the C ABI
+  return MM_ERR_ALLOC_FAILED;
+} catch (...) {
+  return MM_ERR_INTERNAL;
 }
 
 int32_t mm_stft_complex(mm_context ctx, const float* signal_data, int64_t signal_length, int32_t sample_rate,
-                        const MMSTFTParams* params, MMBuffer* out) {
+                        const MMSTFTParams* params, MMBuffer* out) try {
   if (!ctx || !signal_data || signal_length <= 0 || !params || !out) return MM_ERR_INVALID_INPUT;
   if (!pow2(params->n_fft) || params->hop_length <= 0) return MM_ERR_INVALID_INPUT;
   SpecRequest r;
@@ -273,11 +301,15 @@ int32_t mm_stft_complex(mm_context ctx, const float* signal_data, int64_t signal
   r.win = params->win_length;
   r.center = params->center != 0;
   return run_to_buffer(ctx, r, signal_data, signal_length, out);
+} catch (const std::bad_alloc&) {   // no exception leaves the C ABI
+  return MM_ERR_ALLOC_FAILED;
+} catch (...) {
+  return MM_ERR_INTERNAL;
 }
 
 int32_t mm_istft(mm_context ctx, const float* stft_data, int64_t stft_count, int32_t n_freqs, int32_t n_frames,
                  int32_t sample_rate, int32_t hop_length, int32_t win_length, int32_t center, int64_t output_length,
-                 MMBuffer* out) {
+                 MMBuffer* out) try {
   if (!ctx || !stft_data || stft_count <= 0 || !out) return MM_ERR_INVALID_INPUT;
   if (n_freqs < 1 || n_frames < 1 || stft_count != 2 * (int64_t)n_freqs * n_frames) return MM_ERR_INVALID_INPUT;
   IstftShape sh;
@@ -299,11 +331,15 @@ int32_t mm_istft(mm_context ctx, const float* stft_data, int64_t stft_count, int
   if (s != OK) return s;
   fill_shape(out, {sh.out_len}, 0, sh.out_len, data);
   return OK;
+} catch (const std::bad_alloc&) {   // no exception leaves the C ABI
+  return MM_ERR_ALLOC_FAILED;
+} catch (...) {
+  return MM_ERR_INTERNAL;
 }
 
 int32_t mm_onset_strength(mm_context ctx, const float* signal_data, int64_t signal_length, int32_t sample_rate,
                           int32_t n_fft, int32_t hop_length, int32_t n_mels, float f_min, float f_max, int32_t center,
-                          int32_t aggregate, MMBuffer* out) {
+                          int32_t aggregate, MMBuffer* out) try {
   // Bridge.swift:1334-1336 checks the signal, its length and out, not the context
   if (!signal_data || signal_length <= 0 || !out) return MM_ERR_INVALID_INPUT;
   if (sample_rate <= 0 || n_mels < 1) return MM_ERR_INVALID_INPUT;
@@ -345,10 +381,14 @@ int32_t mm_onset_strength(mm_context ctx, const float* signal_data, int64_t sign
   if (s != OK) return s;
   fill_shape(out, {rows, cols}, 0, rows * cols, data);
   return OK;
+} catch (const std::bad_alloc&) {   // no exception leaves the C ABI
+  return MM_ERR_ALLOC_FAILED;
+} catch (...) {
+  return MM_ERR_INTERNAL;
 }
 
 int32_t mm_resample(mm_context ctx, const float* signal_data, int64_t signal_length, int32_t source_sr,
-                    int32_t target_sr, MMBuffer* out) {
+                    int32_t target_sr, MMBuffer* out) try {
   // Bridge.swift:1095-1097 checks the signal, its length and out, not the context
   if (!signal_data || signal_length <= 0 || !out) return MM_ERR_INVALID_INPUT;
   if (source_sr <= 0 || target_sr <= 0) return MM_ERR_INVALID_INPUT;
@@ -376,37 +416,57 @@ int32_t mm_resample(mm_context ctx, const float* signal_data, int64_t signal_len
   if (s != OK) return s;
   fill_shape(out, {n}, 0, n, data);
   return OK;
+} catch (const std::bad_alloc&) {   // no exception leaves the C ABI
+  return MM_ERR_ALLOC_FAILED;
+} catch (...) {
+  return MM_ERR_INTERNAL;
 }
 
-int64_t mm_resample_length(int64_t signal_length, int32_t source_sr, int32_t target_sr) {
+int64_t mm_resample_length(int64_t signal_length, int32_t source_sr, int32_t target_sr) try {
   if (signal_length < 0 || source_sr <= 0 || target_sr <= 0) return -1;
   return resample_length(signal_length, source_sr, target_sr);
+} catch (const std::bad_alloc&) {   // no exception leaves the C ABI
+  return MM_ERR_ALLOC_FAILED;
+} catch (...) {
+  return MM_ERR_INTERNAL;
 }
 
 int32_t mm_resample_device(mm_context ctx, const float* d_signal, int64_t signal_length, int32_t source_sr,
-                           int32_t target_sr, float* d_out, void* stream) {
+                           int32_t target_sr, float* d_out, void* stream) try {
   (void)ctx;
   if (!d_signal || signal_length <= 0 || !d_out) return MM_ERR_INVALID_INPUT;
   if (source_sr <= 0 || target_sr <= 0) return MM_ERR_INVALID_INPUT;
   if (!Engine::get()) return ERR_NO_GPU;
   return enqueue_resample(d_signal, signal_length, source_sr, target_sr, d_out, static_cast<hipStream_t>(stream));
+} catch (const std::bad_alloc&) {   // no exception leaves the C ABI
+  return MM_ERR_ALLOC_FAILED;
+} catch (...) {
+  return MM_ERR_INTERNAL;
 }
 
 int32_t mm_amplitude_to_db(mm_context ctx, const float* data, int64_t count, float ref, float amin, float top_db,
-                           MMBuffer* out) {
+                           MMBuffer* out) try {
   if (!data || count <= 0 || !out) return MM_ERR_INVALID_INPUT;
   return db_to_buffer(ctx, data, count, 20.0f, ref, amin, top_db, out);
+} catch (const std::bad_alloc&) {   // no exception leaves the C ABI
+  return MM_ERR_ALLOC_FAILED;
+} catch (...) {
+  return MM_ERR_INTERNAL;
 }
 
 int32_t mm_power_to_db(mm_context ctx, const float* data, int64_t count, float ref, float amin, float top_db,
-                       MMBuffer* out) {
+                       MMBuffer* out) try {
   if (!data || count <= 0 || !out) return MM_ERR_INVALID_INPUT;
   return db_to_buffer(ctx, data, count, 10.0f, ref, amin, top_db, out);
+} catch (const std::bad_alloc&) {   // no exception leaves the C ABI
+  return MM_ERR_ALLOC_FAILED;
+} catch (...) {
+  return MM_ERR_INTERNAL;
 }
 
 int32_t mm_mel_spectrogram(mm_context ctx, const float* signal_data, int64_t signal_length, int32_t sample_rate,
                            int32_t n_fft, int32_t hop_length, int32_t win_length, int32_t center, float power,
-                           int32_t n_mels, float f_min, float f_max, MMBuffer* out) {
+                           int32_t n_mels, float f_min, float f_max, MMBuffer* out) try {
   if (!ctx || !signal_data || signal_length <= 0 || !out) return MM_ERR_INVALID_INPUT;
   if (n_fft <= 0 || n_mels < 0) return MM_ERR_INVALID_INPUT;
   SpecRequest r;
@@ -426,11 +486,15 @@ int32_t mm_mel_spectrogram(mm_context ctx, const float* signal_data, int64_t sig
     return MM_OK;
   }
   return run_to_buffer(ctx, r, signal_data, signal_length, out);
+} catch (const std::bad_alloc&) {   // no exception leaves the C ABI
+  return MM_ERR_ALLOC_FAILED;
+} catch (...) {
+  return MM_ERR_INTERNAL;
 }
 
 int32_t mm_chroma_stft(mm_context ctx, const float* signal_data, int64_t signal_length, int32_t sample_rate,
                        int32_t n_fft, int32_t hop_length, int32_t win_length, int32_t n_chroma, int32_t center,
-                       MMBuffer* out) {
+                       MMBuffer* out) try {
   if (!ctx || !signal_data || signal_length <= 0 || !out) return MM_ERR_INVALID_INPUT;
   if (n_fft <= 0 || n_chroma < 1 || n_chroma > 256 || sample_rate <= 0) return MM_ERR_INVALID_INPUT;
   SpecRequest r;
@@ -450,24 +514,36 @@ int32_t mm_chroma_stft(mm_context ctx, const float* signal_data, int64_t signal_
     return MM_OK;
   }
   return run_to_buffer(ctx, r, signal_data, signal_length, out);
+} catch (const std::bad_alloc&) {   // no exception leaves the C ABI
+  return MM_ERR_ALLOC_FAILED;
+} catch (...) {
+  return MM_ERR_INTERNAL;
 }
 
 int32_t mm_spectral_centroid(mm_context ctx, const float* signal_data, int64_t signal_length, int32_t sample_rate,
-                             int32_t n_fft, int32_t hop_length, int32_t win_length, int32_t center, MMBuffer* out) {
+                             int32_t n_fft, int32_t hop_length, int32_t win_length, int32_t center, MMBuffer* out) try {
   return descriptor_to_buffer(ctx, signal_data, signal_length, sample_rate, n_fft, hop_length, win_length, center,
                               DESC_CENTROID, 0.0f, out);
+} catch (const std::bad_alloc&) {   // no exception leaves the C ABI
+  return MM_ERR_ALLOC_FAILED;
+} catch (...) {
+  return MM_ERR_INTERNAL;
 }
 
 int32_t mm_spectral_bandwidth(mm_context ctx, const float* signal_data, int64_t signal_length, int32_t sample_rate,
                               int32_t n_fft, int32_t hop_length, int32_t win_length, int32_t center, float p,
-                              MMBuffer* out) {
+                              MMBuffer* out) try {
   return descriptor_to_buffer(ctx, signal_data, signal_length, sample_rate, n_fft, hop_length, win_length, center,
                               DESC_BANDWIDTH, p, out);
+} catch (const std::bad_alloc&) {   // no exception leaves the C ABI
+  return MM_ERR_ALLOC_FAILED;
+} catch (...) {
+  return MM_ERR_INTERNAL;
 }
 
 int32_t mm_spectral_contrast(mm_context ctx, const float* signal_data, int64_t signal_length, int32_t sample_rate,
                              int32_t n_fft, int32_t hop_length, int32_t win_length, int32_t center, int32_t n_bands,
-                             float f_min, MMBuffer* out) {
+                             float f_min, MMBuffer* out) try {
   if (!ctx || !signal_data || signal_length <= 0 || !out) return MM_ERR_INVALID_INPUT;   // Bridge.swift:711-716
   // n_bands < 1 traps in the reference (the range 1...n_bands); sr <= 0 has no bin width
   if (n_fft <= 0 || sample_rate <= 0 || n_bands < 1 || n_bands + 1 > kMaxContrastBands) return MM_ERR_INVALID_INPUT;
@@ -492,24 +568,36 @@ int32_t mm_spectral_contrast(mm_context ctx, const float* signal_data, int64_t s
   if (s != OK) return s;
   fill_shape(out, {nb, T}, 0, nb * T, data);
   return OK;
+} catch (const std::bad_alloc&) {   // no exception leaves the C ABI
+  return MM_ERR_ALLOC_FAILED;
+} catch (...) {
+  return MM_ERR_INTERNAL;
 }
 
 int32_t mm_spectral_rolloff(mm_context ctx, const float* signal_data, int64_t signal_length, int32_t sample_rate,
                             int32_t n_fft, int32_t hop_length, int32_t win_length, int32_t center, float roll_percent,
-                            MMBuffer* out) {
+                            MMBuffer* out) try {
   return descriptor_to_buffer(ctx, signal_data, signal_length, sample_rate, n_fft, hop_length, win_length, center,
                               DESC_ROLLOFF, roll_percent, out);
+} catch (const std::bad_alloc&) {   // no exception leaves the C ABI
+  return MM_ERR_ALLOC_FAILED;
+} catch (...) {
+  return MM_ERR_INTERNAL;
 }
 
 int32_t mm_spectral_flatness(mm_context ctx, const float* signal_data, int64_t signal_length, int32_t sample_rate,
-                             int32_t n_fft, int32_t hop_length, int32_t win_length, int32_t center, MMBuffer* out) {
+                             int32_t n_fft, int32_t hop_length, int32_t win_length, int32_t center, MMBuffer* out) try {
   return descriptor_to_buffer(ctx, signal_data, signal_length, sample_rate, n_fft, hop_length, win_length, center,
                               DESC_FLATNESS, 0.0f, out);
+} catch (const std::bad_alloc&) {   // no exception leaves the C ABI
+  return MM_ERR_ALLOC_FAILED;
+} catch (...) {
+  return MM_ERR_INTERNAL;
 }
 
 int32_t mm_mfcc(mm_context ctx, const float* signal_data, int64_t signal_length, int32_t sample_rate, int32_t n_mfcc,
                 int32_t n_fft, int32_t hop_length, int32_t win_length, int32_t n_mels, float f_min, float f_max,
-                int32_t center, MMBuffer* out) {
+                int32_t center, MMBuffer* out) try {
   if (!ctx || !signal_data || signal_length <= 0 || !out) return MM_ERR_INVALID_INPUT;
   if (n_fft <= 0 || n_mels < 0 || n_mfcc < 0) return MM_ERR_INVALID_INPUT;
   SpecRequest r;
@@ -533,9 +621,13 @@ int32_t mm_mfcc(mm_context ctx, const float* signal_data, int64_t signal_length,
     return MM_OK;
   }
   return run_to_buffer(ctx, r, signal_data, signal_length, out);
+} catch (const std::bad_alloc&) {   // no exception leaves the C ABI
+  return MM_ERR_ALLOC_FAILED;
+} catch (...) {
+  return MM_ERR_INTERNAL;
 }
 
-void mm_buffer_free(MMBuffer* buf) {
+void mm_buffer_free(MMBuffer* buf) try {
   if (!buf) return;
   if (buf->data) std::free(buf->data);
   buf->data = nullptr;
@@ -543,19 +635,24 @@ void mm_buffer_free(MMBuffer* buf) {
   buf->dtype = 0;
   buf->count = 0;
   for (int i = 0; i < 8; ++i) buf->shape[i] = 0;
+} catch (...) {   // no exception leaves the C ABI
 }
 
 int32_t mm_spectral(mm_context ctx, const MMSpecParams* params, const float* signal_data, int64_t signal_length,
-                    MMBuffer* out) {
+                    MMBuffer* out) try {
   if (!ctx || !signal_data || signal_length <= 0 || !out) return MM_ERR_INVALID_INPUT;
   SpecRequest r;
   int32_t s = request_from_params(params, &r);
   if (s != OK) return s;
   if (r.in_dtype != IN_F32) return MM_ERR_INVALID_INPUT;
   return run_to_buffer(ctx, r, signal_data, signal_length, out);
+} catch (const std::bad_alloc&) {   // no exception leaves the C ABI
+  return MM_ERR_ALLOC_FAILED;
+} catch (...) {
+  return MM_ERR_INTERNAL;
 }
 
-int32_t mm_device_info(char* name, int32_t name_len, char* arch, int32_t arch_len, int32_t* n_cu) {
+int32_t mm_device_info(char* name, int32_t name_len, char* arch, int32_t arch_len, int32_t* n_cu) try {
   Engine* e = Engine::get();
   if (!e) return MM_ERR_METAL_UNAVAILABLE;
   if (name && name_len > 0) {
@@ -568,6 +665,10 @@ int32_t mm_device_info(char* name, int32_t name_len, char* arch, int32_t arch_le
   }
   if (n_cu) *n_cu = e->num_cus();
   return MM_OK;
+} catch (const std::bad_alloc&) {   // no exception leaves the C ABI
+  return MM_ERR_ALLOC_FAILED;
+} catch (...) {
+  return MM_ERR_INTERNAL;
 }
 
 }  // extern "C"

@@ -196,7 +196,7 @@ int32_t run_pcm_resample(Lane* lane, const void* raw, int64_t frames, int channe
 
 extern "C" {
 
-int32_t mm_get_duration(mm_context ctx, const char* path, float* out_duration) {
+int32_t mm_get_duration(mm_context ctx, const char* path, float* out_duration) try {
   (void)ctx;
   if (!path || !out_duration) return MM_ERR_INVALID_INPUT;
   WavInfo w;
@@ -205,9 +205,13 @@ int32_t mm_get_duration(mm_context ctx, const char* path, float* out_duration) {
   std::fclose(f);
   *out_duration = (float)((double)w.frames / (double)w.sr);   // AudioIO.swift:134-138
   return MM_OK;
+} catch (const std::bad_alloc&) {   // no exception leaves the C ABI
+  return MM_ERR_ALLOC_FAILED;
+} catch (...) {
+  return MM_ERR_INTERNAL;
 }
 
-int32_t mm_get_sample_rate(mm_context ctx, const char* path, int32_t* out_sr) {
+int32_t mm_get_sample_rate(mm_context ctx, const char* path, int32_t* out_sr) try {
   (void)ctx;
   if (!path || !out_sr) return MM_ERR_INVALID_INPUT;
   WavInfo w;
@@ -216,6 +220,10 @@ int32_t mm_get_sample_rate(mm_context ctx, const char* path, int32_t* out_sr) {
   std::fclose(f);
   *out_sr = w.sr;
   return MM_OK;
+} catch (const std::bad_alloc&) {   // no exception leaves the C ABI
+  return MM_ERR_ALLOC_FAILED;
+} catch (...) {
+  return MM_ERR_INTERNAL;
 }
 
 static int32_t load_wav(mm_context ctx, const char* path, int32_t sr, int32_t mono, float offset, float duration,

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <new>
 #include <vector>
 
 #include "../../include/metalmom.h"
@@ -90,7 +91,7 @@ int32_t run_batch(Batch* b, const void* d_signal, float* d_out, hipStream_t st, 
 extern "C" {
 
 int32_t mm_batch_create(mm_context ctx, const MMSpecParams* params, const int64_t* lengths, int32_t n_clips,
-                        mm_batch* out) {
+                        mm_batch* out) try {
   (void)ctx;  // plans own their stream and workspace
   if (!out) return MM_ERR_INVALID_INPUT;
   Batch* b = nullptr;
@@ -98,10 +99,14 @@ int32_t mm_batch_create(mm_context ctx, const MMSpecParams* params, const int64_
   if (s != OK) return s;
   *out = b;
   return MM_OK;
+} catch (const std::bad_alloc&) {   // no exception leaves the C ABI
+  return MM_ERR_ALLOC_FAILED;
+} catch (...) {
+  return MM_ERR_INTERNAL;
 }
 
 int32_t mm_batch_info(mm_batch bh, int64_t* total_input_samples, int64_t* total_output_floats,
-                      int64_t* total_frames, int32_t* n_tiles) {
+                      int64_t* total_frames, int32_t* n_tiles) try {
   Batch* b = static_cast<Batch*>(bh);
   if (!b) return MM_ERR_INVALID_INPUT;
   if (total_input_samples) *total_input_samples = b->m.total_in;
@@ -113,10 +118,14 @@ int32_t mm_batch_info(mm_batch bh, int64_t* total_input_samples, int64_t* total_
   }
   if (n_tiles) *n_tiles = b->m.n_tiles;
   return MM_OK;
+} catch (const std::bad_alloc&) {   // no exception leaves the C ABI
+  return MM_ERR_ALLOC_FAILED;
+} catch (...) {
+  return MM_ERR_INTERNAL;
 }
 
 int32_t mm_batch_clip(mm_batch bh, int32_t clip, int64_t* input_offset, int64_t* output_offset, int64_t* rows,
-                      int64_t* frames) {
+                      int64_t* frames) try {
   Batch* b = static_cast<Batch*>(bh);
   if (!b || clip < 0 || clip >= b->m.n_clips) return MM_ERR_INVALID_INPUT;
   if (input_offset) *input_offset = b->m.clip_off[clip];
@@ -124,16 +133,24 @@ int32_t mm_batch_clip(mm_batch bh, int32_t clip, int64_t* input_offset, int64_t*
   if (rows) *rows = out_rows(b->r);
   if (frames) *frames = b->m.clip_frames[clip];
   return MM_OK;
+} catch (const std::bad_alloc&) {   // no exception leaves the C ABI
+  return MM_ERR_ALLOC_FAILED;
+} catch (...) {
+  return MM_ERR_INTERNAL;
 }
 
-int32_t mm_batch_run(mm_batch bh, const void* d_signal, float* d_out, void* hip_stream) {
+int32_t mm_batch_run(mm_batch bh, const void* d_signal, float* d_out, void* hip_stream) try {
   Batch* b = static_cast<Batch*>(bh);
   if (!b || (b->m.total_in > 0 && !d_signal) || (b->m.total_out > 0 && !d_out)) return MM_ERR_INVALID_INPUT;
   hipStream_t st = hip_stream ? static_cast<hipStream_t>(hip_stream) : b->lane->stream;
   return run_batch(b, d_signal, d_out, st);
+} catch (const std::bad_alloc&) {   // no exception leaves the C ABI
+  return MM_ERR_ALLOC_FAILED;
+} catch (...) {
+  return MM_ERR_INTERNAL;
 }
 
-int32_t mm_batch_run_graph(mm_batch bh, const void* d_signal, float* d_out, void* hip_stream) {
+int32_t mm_batch_run_graph(mm_batch bh, const void* d_signal, float* d_out, void* hip_stream) try {
   Batch* b = static_cast<Batch*>(bh);
   if (!b || (b->m.total_in > 0 && !d_signal) || (b->m.total_out > 0 && !d_out)) return MM_ERR_INVALID_INPUT;
   hipStream_t st = hip_stream ? static_cast<hipStream_t>(hip_stream) : b->lane->stream;
@@ -164,9 +181,13 @@ int32_t mm_batch_run_graph(mm_batch bh, const void* d_signal, float* d_out, void
   g.exec = exec;
   b->graphs.push_back(g);
   return hipGraphLaunch(exec, st) == hipSuccess ? MM_OK : MM_ERR_INTERNAL;
+} catch (const std::bad_alloc&) {   // no exception leaves the C ABI
+  return MM_ERR_ALLOC_FAILED;
+} catch (...) {
+  return MM_ERR_INTERNAL;
 }
 
-int32_t mm_batch_set_grid(mm_batch bh, int32_t wg_per_cu) {
+int32_t mm_batch_set_grid(mm_batch bh, int32_t wg_per_cu) try {
   Batch* b = static_cast<Batch*>(bh);
   if (!b || wg_per_cu < 0 || wg_per_cu > 64) return MM_ERR_INVALID_INPUT;
   b->wg_per_cu = wg_per_cu;
@@ -176,10 +197,14 @@ int32_t mm_batch_set_grid(mm_batch bh, int32_t wg_per_cu) {
   }
   b->graphs.clear();
   return MM_OK;
+} catch (const std::bad_alloc&) {   // no exception leaves the C ABI
+  return MM_ERR_ALLOC_FAILED;
+} catch (...) {
+  return MM_ERR_INTERNAL;
 }
 
 int32_t mm_batch_time(mm_batch bh, const void* d_signal, float* d_out, void* hip_stream, int32_t iters,
-                      float* kernel_ms, float* step_ms) {
+                      float* kernel_ms, float* step_ms) try {
   Batch* b = static_cast<Batch*>(bh);
   if (!b || iters <= 0) return MM_ERR_INVALID_INPUT;
   hipStream_t st = hip_stream ? static_cast<hipStream_t>(hip_stream) : b->lane->stream;
@@ -207,10 +232,14 @@ int32_t mm_batch_time(mm_batch bh, const void* d_signal, float* d_out, void* hip
   if (kernel_ms) *kernel_ms = (float)(k / iters);
   if (step_ms) *step_ms = (float)(t / iters);
   return MM_OK;
+} catch (const std::bad_alloc&) {   // no exception leaves the C ABI
+  return MM_ERR_ALLOC_FAILED;
+} catch (...) {
+  return MM_ERR_INTERNAL;
 }
 
 int32_t mm_batch_autotune(mm_batch bh, const void* d_signal, float* d_out, int32_t iters, int32_t* best_wg_per_cu,
-                          float* best_ms) {
+                          float* best_ms) try {
   Batch* b = static_cast<Batch*>(bh);
   if (!b || iters <= 0) return MM_ERR_INVALID_INPUT;
   const int cands[] = {1, 2, 3, 4, 6, 8};
@@ -236,23 +265,35 @@ int32_t mm_batch_autotune(mm_batch bh, const void* d_signal, float* d_out, int32
   if (best_wg_per_cu) *best_wg_per_cu = best;
   if (best_ms) *best_ms = best_t;
   return MM_OK;
+} catch (const std::bad_alloc&) {   // no exception leaves the C ABI
+  return MM_ERR_ALLOC_FAILED;
+} catch (...) {
+  return MM_ERR_INTERNAL;
 }
 
 // Instrumented builds only: device buffer for per-wave phase cycle sums.
-int32_t mm_debug_set_phase_buffer(void* d_buf) {
+int32_t mm_debug_set_phase_buffer(void* d_buf) try {
   set_phase_cycles_buffer(static_cast<long long*>(d_buf));
   return MM_OK;
+} catch (const std::bad_alloc&) {   // no exception leaves the C ABI
+  return MM_ERR_ALLOC_FAILED;
+} catch (...) {
+  return MM_ERR_INTERNAL;
 }
 
-int32_t mm_batch_destroy(mm_batch bh) {
+int32_t mm_batch_destroy(mm_batch bh) try {
   Batch* b = static_cast<Batch*>(bh);
   if (!b) return MM_ERR_INVALID_INPUT;
   delete b;
   return MM_OK;
+} catch (const std::bad_alloc&) {   // no exception leaves the C ABI
+  return MM_ERR_ALLOC_FAILED;
+} catch (...) {
+  return MM_ERR_INTERNAL;
 }
 
 int32_t mm_spectral_batch(mm_context ctx, const MMSpecParams* params, const float* const* clips,
-                          const int64_t* lengths, int32_t n_clips, MMBuffer* outs) {
+                          const int64_t* lengths, int32_t n_clips, MMBuffer* outs) try {
   if (!ctx || !params || !clips || !lengths || n_clips <= 0 || !outs) return MM_ERR_INVALID_INPUT;
   if (params->input_dtype != MM_INPUT_F32) return MM_ERR_INVALID_INPUT;
   for (int i = 0; i < n_clips; ++i)
@@ -300,6 +341,10 @@ int32_t mm_spectral_batch(mm_context ctx, const MMSpecParams* params, const floa
     }
   }
   return MM_OK;
+} catch (const std::bad_alloc&) {   // no exception leaves the C ABI
+  return MM_ERR_ALLOC_FAILED;
+} catch (...) {
+  return MM_ERR_INTERNAL;
 }
 
 }  // extern "C"

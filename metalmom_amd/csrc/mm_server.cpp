@@ -22,6 +22,7 @@
 #include <cstring>
 #include <deque>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <new>
 #include <thread>
@@ -571,17 +572,18 @@ class Server {
 
 extern "C" {
 
-int32_t mm_server_create(mm_context ctx, int64_t hbm_budget_bytes, int32_t max_batch, mm_server* out) {
+int32_t mm_server_create(mm_context ctx, int64_t hbm_budget_bytes, int32_t max_batch, mm_server* out) try {
   (void)ctx;
   if (!out || hbm_budget_bytes <= 0) return MM_ERR_INVALID_INPUT;
-  auto* s = new Server((size_t)hbm_budget_bytes, max_batch > 0 ? max_batch : 256);
-  const int32_t r = s->start();
-  if (r != OK) {
-    delete s;
-    return r;
-  }
-  *out = s;
+  std::unique_ptr<Server> s(new Server((size_t)hbm_budget_bytes, max_batch > 0 ? max_batch : 256));
+  const int32_t r = s->start();   // may throw (thread creation): s is freed either way
+  if (r != OK) return r;
+  *out = s.release();
   return MM_OK;
+} catch (const std::bad_alloc&) {   // no exception leaves the C ABI
+  return MM_ERR_ALLOC_FAILED;
+} catch (...) {
+  return MM_ERR_INTERNAL;
 }
 
 int64_t mm_server_submit(mm_server srv, const MMSpecParams* params, const float* signal_data, int64_t signal_length) {
@@ -615,26 +617,42 @@ int64_t mm_server_submit_pcm(mm_server srv, const MMSpecParams* params, const vo
   }
 }
 
-int32_t mm_server_poll(mm_server srv, int64_t request_id, MMBuffer* out, double* latency_s) {
+int32_t mm_server_poll(mm_server srv, int64_t request_id, MMBuffer* out, double* latency_s) try {
   if (!srv) return MM_ERR_INVALID_INPUT;
   return static_cast<Server*>(srv)->poll(request_id, out, latency_s, false);
+} catch (const std::bad_alloc&) {   // no exception leaves the C ABI
+  return MM_ERR_ALLOC_FAILED;
+} catch (...) {
+  return MM_ERR_INTERNAL;
 }
 
-int32_t mm_server_wait(mm_server srv, int64_t request_id, MMBuffer* out, double* latency_s) {
+int32_t mm_server_wait(mm_server srv, int64_t request_id, MMBuffer* out, double* latency_s) try {
   if (!srv) return MM_ERR_INVALID_INPUT;
   return static_cast<Server*>(srv)->poll(request_id, out, latency_s, true);
+} catch (const std::bad_alloc&) {   // no exception leaves the C ABI
+  return MM_ERR_ALLOC_FAILED;
+} catch (...) {
+  return MM_ERR_INTERNAL;
 }
 
-int32_t mm_server_stats(mm_server srv, int64_t* submitted, int64_t* completed, int64_t* steps, int64_t* frames) {
+int32_t mm_server_stats(mm_server srv, int64_t* submitted, int64_t* completed, int64_t* steps, int64_t* frames) try {
   if (!srv) return MM_ERR_INVALID_INPUT;
   static_cast<Server*>(srv)->stats(submitted, completed, steps, frames);
   return MM_OK;
+} catch (const std::bad_alloc&) {   // no exception leaves the C ABI
+  return MM_ERR_ALLOC_FAILED;
+} catch (...) {
+  return MM_ERR_INTERNAL;
 }
 
-int32_t mm_server_destroy(mm_server srv) {
+int32_t mm_server_destroy(mm_server srv) try {
   if (!srv) return MM_ERR_INVALID_INPUT;
   delete static_cast<Server*>(srv);
   return MM_OK;
+} catch (const std::bad_alloc&) {   // no exception leaves the C ABI
+  return MM_ERR_ALLOC_FAILED;
+} catch (...) {
+  return MM_ERR_INTERNAL;
 }
 
 }  // extern "C"
