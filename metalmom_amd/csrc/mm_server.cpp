@@ -341,6 +341,50 @@ class Server {
       s.lane = free_lanes_.back();
       free_lanes_.pop_back();
     }
+    try {
+      prepare(s, take);
+    } catch (...) {   // an allocation failed: nothing of the step may stay in flight
+      (void)hipStreamSynchronize(s.lane->stream);
+      if (s.blk) {
+        out_ring_.release(s.blk->p);
+        delete s.blk;
+      }
+      if (s.done) (void)hipEventDestroy(s.done);
+      fail(take, s.lane);
+      return;
+    }
+    std::lock_guard<std::mutex> g(mu_);
+    ++steps_;
+    inflight_.push_back(std::move(s));
+  }
+
+  // Ends the requests of a step that could not be launched with ERR_ALLOC and
+  // returns its lane.
+  void fail(std::vector<Req*>& take, Lane* lane) {
+    const double t = now_s();
+    size_t bytes = 0;
+    for (Req* q : take) {
+      release_input(q);
+      bytes += q->dev_bytes;
+    }
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      for (Req* q : take) {
+        q->status = ERR_ALLOC;
+        q->t_done = t;
+        q->state = 2;
+        ++completed_;
+      }
+      in_use_ -= bytes;
+      free_lanes_.push_back(lane);
+    }
+    done_cv_.notify_all();
+    cv_.notify_all();
+  }
+
+  // Lays out the step, stages and enqueues its copies and kernels on the
+  // lane's stream and records its completion event (s.done).
+  void prepare(Step& s, std::vector<Req*>& take) {
     Engine* eng = Engine::get();
     for (Req* q : take) {
       s.dev_bytes += q->dev_bytes;
@@ -447,9 +491,6 @@ class Server {
     chk(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
     if (s.done) chk(hipEventRecord(s.done, st));
     s.status = st_code;
-    std::lock_guard<std::mutex> g(mu_);
-    ++steps_;
-    inflight_.push_back(std::move(s));
   }
 
   void release_input(Req* q) {
