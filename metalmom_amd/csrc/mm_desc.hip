@@ -146,11 +146,21 @@ __device__ __forceinline__ void band_extremes(const float* col, int T, int n, in
     lo[i] = INFINITY;
     hi[i] = -INFINITY;
   }
-  for (int t = 0; t < n; ++t) {
-    const float v = col[(int64_t)t * T];
+  auto take = [&](float v) {
     if (v < lo[0]) keep_extreme<KM, false>(lo, v, k);
     if (v > hi[0]) keep_extreme<KM, true>(hi, v, k);
+  };
+  // four loads in flight before the (branchy) list updates
+  int t = 0;
+  for (; t + 4 <= n; t += 4) {
+    const float v0 = col[(int64_t)t * T], v1 = col[(int64_t)(t + 1) * T];
+    const float v2 = col[(int64_t)(t + 2) * T], v3 = col[(int64_t)(t + 3) * T];
+    take(v0);
+    take(v1);
+    take(v2);
+    take(v3);
   }
+  for (; t < n; ++t) take(col[(int64_t)t * T]);
   valley = 0.0f;
   peak = 0.0f;
 #pragma unroll
