@@ -759,8 +759,16 @@ __global__ void k_db(const float* __restrict__ in, int64_t n, float mult, float 
     out[i] = d;
     lmax = fmaxf(lmax, d);
   }
+  // one atomic per block (same-address atomics serialise in L2): wave
+  // maxima, then the block's 4 waves through LDS
+  __shared__ float red[4];
   lmax = wave_max(lmax);
-  if ((threadIdx.x & 63) == 0 && lmax > -INFINITY) atomicMax(max_key, float_to_key(lmax));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = lmax;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const float m = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+    if (m > -INFINITY) atomicMax(max_key, float_to_key(m));
+  }
 }
 
 __global__ void k_db_clamp(float* __restrict__ x, int64_t n, const unsigned* max_key, float top_db) {
@@ -1187,8 +1195,8 @@ hipError_t launch_generic_mel(const float2* spec, int F, int T, const float* fb,
 hipError_t launch_db(const float* in, int64_t n, float mult, float ref, float amin, float* out, unsigned* max_key,
                      hipStream_t st) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_db, dim3(std::min<unsigned>(nblk(n, 256), 2048)), dim3(256), 0, st, in, n, mult, ref, amin,
-                     out, max_key);
+  hipLaunchKernelGGL(k_db, dim3(std::min<unsigned>(nblk(n, 256), 1024)), dim3(256), 0, st, in, n, mult, ref, amin,
+                     out, max_key);   // 256 threads = the 4 waves k_db reduces over
   return hipGetLastError();
 }
 
