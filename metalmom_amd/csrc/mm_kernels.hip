@@ -754,10 +754,23 @@ __global__ void k_generic_mel(const float2* __restrict__ spec, int F, int T, con
 __global__ void k_db(const float* __restrict__ in, int64_t n, float mult, float ref, float amin,
                      float* __restrict__ out, unsigned* max_key) {
   float lmax = -INFINITY;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const float d = mult * log10f(fmaxf(in[i], amin) / ref);
-    out[i] = d;
+  auto db = [&](float x) {
+    const float d = mult * log10f(fmaxf(x, amin) / ref);
     lmax = fmaxf(lmax, d);
+    return d;
+  };
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15) == 0) {
+    // 16-byte loads and stores, then the tail
+    const int64_t n4 = n / 4;
+    for (int64_t i = t0; i < n4; i += stride) {
+      const float4 v = reinterpret_cast<const float4*>(in)[i];
+      reinterpret_cast<float4*>(out)[i] = make_float4(db(v.x), db(v.y), db(v.z), db(v.w));
+    }
+    for (int64_t i = 4 * n4 + t0; i < n; i += stride) out[i] = db(in[i]);
+  } else {
+    for (int64_t i = t0; i < n; i += stride) out[i] = db(in[i]);
   }
   // one atomic per block (same-address atomics serialise in L2): wave
   // maxima, then the block's 4 waves through LDS

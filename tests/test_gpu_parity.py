@@ -343,3 +343,19 @@ def test_multi_round_mel_blocks(prec, idt):
             d = np.abs(o.power_to_db(m.astype(np.float64)) - Om[i])
             assert d.max() <= 0.05, (rep, i, int(np.argmax(d.max(axis=1))), float(d.max()))
             assert rel_err(f, Of[i]) <= 2e-3, (rep, i)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 7, 9, 1003, 40001])
+def test_db_any_length(n):
+    """power_to_db / amplitude_to_db over lengths that leave 1-3 values after
+    the kernel's 16-byte groups: same values as the oracle, top_db from the
+    maximum over all of them (the maximum may sit in the tail)."""
+    rng = np.random.default_rng(n)
+    x = (rng.random(n) * 10.0).astype(np.float32)
+    x[-1] = 50.0   # the maximum in the last position
+    got = mm().power_to_db(x, top_db=80.0).astype(np.float64).ravel()
+    ref = o.power_to_db(x.astype(np.float64), top_db=80.0).ravel()
+    assert np.abs(got - ref).max() < 1e-3
+    got = mm().amplitude_to_db(x, top_db=40.0).astype(np.float64).ravel()
+    ref = o.amplitude_to_db(x.astype(np.float64), top_db=40.0).ravel()
+    assert np.abs(got - ref).max() < 1e-3
