@@ -337,7 +337,7 @@ int32_t Engine::resample_weights(int source_sr, int target_sr, hipStream_t st, c
   auto e = std::make_unique<PolyEntry>();
   e->src = source_sr;
   e->dst = target_sr;
-  MM_HIP(e->w.ensure((size_t)P * stride * sizeof(float)));
+  MM_HIP(e->w.ensure((size_t)(P / 2 + 1) * stride * sizeof(float)));   // rows 0 .. P/2
   MM_HIP(launch_poly_table(source_sr, target_sr, table, static_cast<float*>(e->w.p), st));
   MM_HIP(hipStreamSynchronize(st));   // shared by every stream from here on
   e->wsum1 = 0.0f;

@@ -100,10 +100,10 @@ int resample_table_len();
 // > 0); grid: persistent workgroups (one per CU).
 hipError_t launch_resample(const float* x, int64_t n, float* y, int64_t n_out, int source_sr, int target_sr,
                            const float* table, int grid, hipStream_t st);
-// Polyphase form: true when this rate pair's weight rows (phases rows of
-// stride floats) fit in LDS; launch_poly_table writes the rows (phases *
-// stride floats) from the filter table, launch_resample_poly converts with
-// them.
+// Polyphase form: true when this rate pair's weight rows (rows 0 .. phases/2
+// of stride floats; the others are their mirror images) fit in LDS;
+// launch_poly_table writes those rows ((phases / 2 + 1) * stride floats)
+// from the filter table, launch_resample_poly converts with them.
 bool resample_poly_shape(int source_sr, int target_sr, int* phases, int* taps, int* stride);
 hipError_t launch_poly_table(int source_sr, int target_sr, const float* table, float* wtab, hipStream_t st);
 // wsum1: the sum of the weight row when there is one phase (its sign decides

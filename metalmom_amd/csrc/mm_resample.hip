@@ -25,12 +25,14 @@
 //
 // Polyphase path: with g = gcd(source, target), output i sits (i * source
 // mod target) / g steps of g / target past input floor(i * source / target),
-// so the tap weights take only P = target / g distinct rows ("phases"). When
-// P * 2 * support weights fit in LDS they are computed once per launch
-// (k_poly_table, from the same FP32 table and interpolation, the tap distance
-// taken exactly from integers) and each tap is then one LDS weight read, one
-// LDS sample read and one FMA; the sums run over the same taps in the same
-// order. Otherwise the kernel above forms the weights per tap.
+// so the tap weights take only P = target / g distinct rows ("phases"). The
+// filter is symmetric, so row P - ph is row ph reversed (bit for bit: the
+// tap distances are the same integers), and rows 0 .. P/2 suffice. When they
+// fit in LDS they are computed once per rate pair (k_poly_table, from the same
+// FP32 table and interpolation, the tap distance taken exactly from integers)
+// and each tap is then one LDS weight read, one LDS sample read and one FMA;
+// the sums run over the same taps in the same order. Otherwise the kernel
+// above forms the weights per tap.
 #include "mm_common.h"
 #include "mm_kernels.h"
 
@@ -45,7 +47,7 @@ constexpr int kTabLds = 32772;                // padded to a 16-byte multiple
 constexpr int kRsThreads = 1024;
 constexpr int kWinCap = 6144;                 // staged input samples (24 KB)
 constexpr int kWinCapPoly = 8192;             // staged input samples of the polyphase path (32 KB)
-constexpr int64_t kPolyMaxFloats = 28672;     // polyphase weight rows in LDS (112 KB)
+constexpr int64_t kPolyMaxFloats = 30000;     // polyphase weight rows in LDS (117 KB)
 
 __device__ double bessel_i0(double x) {
   double sum = 1.0, term = 1.0;
@@ -126,9 +128,9 @@ __global__ __launch_bounds__(kRsThreads) void k_resample(const float* __restrict
 // |j - p| * scale * 512 = |(m - support + 1) * target - phase * g| * 512 / D
 // with D = max(source, target), split exactly into table index and fraction.
 __global__ void k_poly_table(const float* __restrict__ table, int target, int g, int D, int support, int taps,
-                             int stride, int phases, float* __restrict__ W) {
+                             int stride, int rows, float* __restrict__ W) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= phases * stride) return;
+  if (e >= rows * stride) return;
   const int ph = e / stride, m = e - ph * stride;
   float w = 0.0f;
   if (m < taps) {
@@ -147,7 +149,8 @@ __global__ void k_poly_table(const float* __restrict__ table, int target, int g,
 __global__ __launch_bounds__(kRsThreads) void k_resample_poly(const float* __restrict__ x, int64_t n,
                                                              float* __restrict__ y, int64_t n_out, int source,
                                                              int target, int g, int support, int taps, int stride,
-                                                             int w_floats, int win_cap, float scale, float wsum1,
+                                                             int phases, int w_floats, int win_cap, float scale,
+                                                             float wsum1,
                                                              const float* __restrict__ wtab) {
   extern __shared__ float lds[];
   float* Wt = lds;
@@ -191,11 +194,14 @@ __global__ __launch_bounds__(kRsThreads) void k_resample_poly(const float* __res
       const int64_t j0 = k - support + 1;   // the tap of m = 0
       const int m0 = j0 < 0 ? (int)-j0 : 0;
       const int m1 = (j0 + taps - 1 > n - 1) ? (int)(n - 1 - j0) : taps - 1;
-      const float* w = Wt + ph * stride;
+      // phases above P/2 read the mirrored row backwards
+      const bool mir = ph > phases / 2;
+      const float* w = mir ? Wt + (phases - ph) * stride + (taps - 1) : Wt + ph * stride;
+      const int dw = mir ? -1 : 1;
       const float* src = staged ? X + (j0 - lo) : x + j0;
       float sum = 0.0f, wsum = 0.0f;
       for (int m = m0; m <= m1; ++m) {
-        const float wm = w[m];
+        const float wm = w[dw * m];
         sum += src[m] * wm;
         wsum += wm;
       }
@@ -229,12 +235,13 @@ bool resample_poly_shape(int source_sr, int target_sr, int* phases, int* taps, i
   const int64_t g = gcd64(source_sr, target_sr);
   const int64_t P = target_sr / g;
   const int t = 2 * support, st = t | 1;   // odd row stride: rows of different phases fall in different banks
+  const int64_t rows = P / 2 + 1;          // rows 0 .. P/2; the others are their mirror images
   *phases = (int)std::min<int64_t>(P, 1 << 30);
   *taps = t;
   *stride = st;
   // the weights plus at least a 1024-output window must fit in LDS
-  return P * st <= kPolyMaxFloats && (int64_t)source_sr * 1023 / target_sr + t + 1 <= kWinCapPoly &&
-         (size_t)(P * st + kWinCapPoly + 4) * sizeof(float) <= lds_limit_bytes();
+  return rows * st <= kPolyMaxFloats && (int64_t)source_sr * 1023 / target_sr + t + 1 <= kWinCapPoly &&
+         (size_t)(rows * st + kWinCapPoly + 4) * sizeof(float) <= lds_limit_bytes();
 }
 
 hipError_t launch_poly_table(int source_sr, int target_sr, const float* table, float* wtab, hipStream_t st) {
@@ -242,9 +249,10 @@ hipError_t launch_poly_table(int source_sr, int target_sr, const float* table, f
   if (!resample_poly_shape(source_sr, target_sr, &P, &taps, &stride)) return hipErrorInvalidConfiguration;
   const int g = (int)gcd64(source_sr, target_sr);
   const int D = std::max(source_sr, target_sr);
-  const int wf = P * stride;
+  const int rows = P / 2 + 1;
+  const int wf = rows * stride;
   hipLaunchKernelGGL(k_poly_table, dim3((wf + 255) / 256), dim3(256), 0, st, table, target_sr, g, D, taps / 2, taps,
-                     stride, P, wtab);
+                     stride, rows, wtab);
   return hipGetLastError();
 }
 
@@ -254,7 +262,7 @@ hipError_t launch_resample_poly(const float* x, int64_t n, float* y, int64_t n_o
   if (!resample_poly_shape(source_sr, target_sr, &P, &taps, &stride)) return hipErrorInvalidConfiguration;
   const int g = (int)gcd64(source_sr, target_sr);
   const int support = taps / 2;
-  const int wf = P * stride;
+  const int wf = (P / 2 + 1) * stride;
   const int64_t n_chunks = (n_out + kRsThreads - 1) / kRsThreads;
   if (n_chunks <= 0) return hipSuccess;
   const int gr = (int)std::min<int64_t>(std::max(grid, 1), n_chunks);
@@ -266,7 +274,7 @@ hipError_t launch_resample_poly(const float* x, int64_t n, float* y, int64_t n_o
   const double ratio = (double)target_sr / (double)source_sr;
   const float scale = (float)(ratio < 1.0 ? ratio : 1.0);
   hipLaunchKernelGGL(k_resample_poly, dim3(gr), dim3(kRsThreads), lds, st, x, n, y, n_out, source_sr, target_sr,
-                     g, support, taps, stride, wf, kWinCapPoly, scale, wsum1, wtab);
+                     g, support, taps, stride, P, wf, kWinCapPoly, scale, wsum1, wtab);
   return hipGetLastError();
 }
 

@@ -46,11 +46,13 @@ def _check(got, y, src, dst):
     assert err.max() <= tol, (float(err.max()), tol, int(np.argmax(err)))
 
 
-# (source, target, samples): the 44.1 -> 8 kHz window exceeds the LDS stage
-# (the kernel reads the input from global memory there)
+# (source, target, samples): most pairs take the polyphase kernel (weight rows
+# in LDS, the upper half of the phases read mirrored); 44.1 -> 32.001 kHz has
+# too many phases and takes the per-tap kernel
 CASES = [(44100, 22050, 50000), (22050, 44100, 30000), (48000, 22050, 40000), (16000, 22050, 20000),
          (22050, 16000, 20000), (44100, 8000, 30000), (8000, 48000, 5000), (44100, 48000, 30000),
-         (44100, 22050, 1), (44100, 22050, 2), (22050, 44100, 3), (48000, 22050, 129)]
+         (44100, 22050, 1), (44100, 22050, 2), (22050, 44100, 3), (48000, 22050, 129),
+         (44100, 32001, 20000)]   # 10,667 phases: too many rows for LDS, the per-tap kernel
 
 
 @pytest.mark.parametrize("src,dst,n", CASES, ids=[f"{a}-{b}-{n}" for a, b, n in CASES])
