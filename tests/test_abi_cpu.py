@@ -248,3 +248,26 @@ def test_contrast_validation_needs_no_gpu():
         lib.mm_buffer_free(C.byref(out))
     finally:
         lib.mm_destroy(ctx)
+
+
+def test_device_entries_reject_bad_arguments_without_gpu():
+    """The additive device entries validate before touching the device."""
+    lib = _lib()
+    buf = C.c_void_p(1 << 20)   # never dereferenced: every call below fails validation first
+    assert lib.mm_resample_device(None, None, 100, 44100, 22050, buf, None) == -1
+    assert lib.mm_resample_device(None, buf, 0, 44100, 22050, buf, None) == -1
+    assert lib.mm_resample_device(None, buf, 100, 0, 22050, buf, None) == -1
+    assert lib.mm_resample_device(None, buf, 100, 44100, 22050, None, None) == -1
+    assert lib.mm_spectral_contrast_device(None, buf, 4096, 22050, 2048, 512, 2048, 1, 6, 200.0, buf, None,
+                                           None) == -1
+    ctx = lib.mm_init()
+    try:
+        for bad in [dict(n_fft=1000), dict(n_bands=0), dict(n_bands=63), dict(sr=0)]:
+            a = dict(sr=22050, n_fft=2048, n_bands=6)
+            a.update(bad)
+            assert lib.mm_spectral_contrast_device(ctx, buf, 4096, a["sr"], a["n_fft"], 512, 0, 1, a["n_bands"],
+                                                   200.0, buf, None, None) == -1, bad
+        assert lib.mm_spectral_descriptor_device(ctx, buf, 4096, 22050, 2048, 512, 0, 1, 9, 0.0, buf, None,
+                                                 None) == -1   # no such descriptor kind
+    finally:
+        lib.mm_destroy(ctx)
