@@ -183,3 +183,19 @@ def test_contrast_device_entry_matches_host_entry():
             assert np.array_equal(Y.cpu().numpy().reshape(6, T), host)
     finally:
         lib.mm_destroy(ctx)
+
+
+@pytest.mark.parametrize("n", [1, 100, 2047, 2049])
+def test_descriptors_on_tiny_clips(n):
+    """Clips of one or a few frames, mostly centre padding: every descriptor
+    and contrast against the oracle (rolloff by bin, as above)."""
+    import metalmom_amd as mm
+    y = _mix(n, 22050, n)
+    for name in ("spectral_centroid", "spectral_bandwidth", "spectral_rolloff", "spectral_flatness"):
+        got = getattr(mm, name)(y=y, sr=22050).astype(np.float64)
+        ref = getattr(o, name)(y.astype(np.float64)) if name == "spectral_flatness" else \
+            getattr(o, name)(y.astype(np.float64), 22050)
+        _check(name, got, ref, 22050, 2048)
+    got = mm.feature.spectral_contrast(y=y, sr=22050).astype(np.float64)
+    ref = o.spectral_contrast(y.astype(np.float64), 22050)
+    assert got.shape == ref.shape and np.abs(got - ref).max() <= 0.05
