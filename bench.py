@@ -328,7 +328,11 @@ def descriptor_leg(torch, n_samples, iters, with_cpu=False):
         e1.record(st)
         e1.synchronize()
         ms = e0.elapsed_time(e1) / iters
-        res[name] = {"step_ms": ms, "frames_per_s": T / (ms * 1e-3)}
+        # algorithmic bytes: the signal in, one float per frame out (the fused
+        # kernel keeps the spectrum on chip)
+        gbs = (4 * len(y) + 4 * T) / (ms * 1e-3) / 1e9
+        res[name] = {"step_ms": ms, "frames_per_s": T / (ms * 1e-3),
+                     "hbm": {"achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS}}
     # spectral contrast: 7 octave bands (magnitude STFT, then one thread per (frame, band))
     Yc = torch.empty(7 * T, device="cuda")
 
@@ -345,7 +349,9 @@ def descriptor_leg(torch, n_samples, iters, with_cpu=False):
     e1.record(st)
     e1.synchronize()
     ms = e0.elapsed_time(e1) / iters
-    res["contrast"] = {"step_ms": ms, "frames_per_s": T / (ms * 1e-3)}
+    gbs = (4 * len(y) + 4 * 7 * T) / (ms * 1e-3) / 1e9
+    res["contrast"] = {"step_ms": ms, "frames_per_s": T / (ms * 1e-3),
+                       "hbm": {"achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS}}
     lib.mm_destroy(ctx)
     del X, Y
     if with_cpu:
@@ -355,6 +361,12 @@ def descriptor_leg(torch, n_samples, iters, with_cpu=False):
         res["cpu_baseline_centroid"] = cpu_time_1thread(CentroidF32(), sample[:22050], sample, fr,
                                                         f"first {len(sample)} samples of the clip ({fr} frames), "
                                                         f"float32 numpy rfft + |X| + centroid")
+        from oracle.cpu_baseline import ContrastF32
+        small = y[:40_000_000]
+        frc = 1 + len(small) // 512
+        res["cpu_baseline_contrast"] = cpu_time_1thread(ContrastF32(), small[:22050], small, frc,
+                                                        f"first {len(small)} samples of the clip ({frc} frames), "
+                                                        f"float32 numpy rfft + |X| + per-band sort")
     return res
 
 

@@ -169,6 +169,30 @@ class ResampleF32:
         return out
 
 
+class ContrastF32:
+    """Float32 spectral contrast (``SpectralDescriptors.swift:137-204``) of the
+    magnitude STFT: per band, sort, mean of the q lowest and highest, dB
+    difference."""
+
+    def __init__(self, sr=22050, n_fft=2048, hop=512, n_bands=6, fmin=200.0):
+        self.n_fft, self.hop = n_fft, hop
+        self.win = o.hann_periodic(n_fft).astype(np.float32)
+        self.edges, self.q = o.contrast_bands(sr, n_fft, n_fft // 2 + 1, n_bands, fmin)
+
+    def __call__(self, y):
+        S = _magnitude_f32(y, self.n_fft, self.hop, self.win)
+        out = np.zeros((len(self.q), S.shape[1]), np.float32)
+        for b, q in enumerate(self.q):
+            e0, e1 = self.edges[b], self.edges[b + 1]
+            if e1 <= e0:
+                continue
+            srt = np.sort(S[e0:e1], axis=0)
+            valley = srt[:q].mean(axis=0)
+            peak = srt[e1 - e0 - q:].mean(axis=0)
+            out[b] = 10 * np.log10(np.maximum(peak, 1e-10)) - 10 * np.log10(np.maximum(valley, 1e-10))
+        return out
+
+
 class CentroidF32:
     """Float32 spectral centroid (``SpectralDescriptors.swift:25-56``) of the
     magnitude STFT."""

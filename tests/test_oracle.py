@@ -465,3 +465,13 @@ def test_contrast_golden_at_reference_tolerance(signal_440):
     assert c.shape == G.shape
     np.testing.assert_allclose(c, G, rtol=0.5, atol=15.0)
     assert np.all(np.isfinite(c))
+
+
+def test_contrast_cpu_baseline_port_matches_oracle():
+    """The float32 contrast timed as the CPU baseline is the oracle's
+    computation (FP32 rounding only, on a noise-rich signal)."""
+    from oracle.cpu_baseline import ContrastF32
+    y = np.random.default_rng(3).standard_normal(30000).astype(np.float32)
+    a = ContrastF32()(y)
+    b = o.spectral_contrast(y.astype(np.float64), 22050)
+    assert a.shape == b.shape and np.abs(a - b).max() < 0.01
