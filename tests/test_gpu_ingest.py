@@ -84,3 +84,38 @@ def test_server_pcm_requests_equal_alone():
             assert got.shape == alone.shape and np.array_equal(got, alone)
     finally:
         srv.close()
+
+
+@pytest.mark.parametrize("fmt,mono", [("u8", True), ("s24", False), ("s32", True), ("f32", False), ("f64", True)])
+def test_server_pcm_formats(fmt, mono):
+    """Every PCM sample format through the server, mixed down or channel 0,
+    against the same clip converted on the host (oracle), resampled and
+    analysed alone."""
+    import metalmom_amd as mm
+    from metalmom_amd.serve import SpectralServer
+    from metalmom_amd import _native as N
+    rng = np.random.default_rng(len(fmt) + mono)
+    n, ch, src, dst = 40000, 3, 44100, 22050
+    x = np.clip(0.4 * rng.standard_normal((n, ch)), -0.99, 0.99)
+    if fmt == "u8":
+        raw, bits, flt, code = np.round(x * 127 + 128).astype(np.uint8).tobytes(), 8, False, N.PCM_U8
+    elif fmt == "s24":
+        v = np.round(x * 8388607).astype(np.int64).ravel()
+        raw = b"".join(int(a & 0xFFFFFF).to_bytes(3, "little") for a in v)
+        bits, flt, code = 24, False, N.PCM_S24
+    elif fmt == "s32":
+        raw, bits, flt, code = np.round(x * 2147483647).astype("<i4").tobytes(), 32, False, N.PCM_S32
+    elif fmt == "f32":
+        raw, bits, flt, code = x.astype("<f4").tobytes(), 32, True, N.PCM_F32
+    else:
+        raw, bits, flt, code = x.astype("<f8").tobytes(), 64, True, N.PCM_F64
+    srv = SpectralServer(hbm_budget_bytes=1 << 30)
+    try:
+        p = srv.params("mel", sr=dst)
+        rid = srv.submit_pcm(np.frombuffer(raw, np.uint8), ch, code, src, p, mono=mono)
+        got, _ = srv.wait(rid)
+    finally:
+        srv.close()
+    clip = g.to_mono(g.pcm_to_float(raw, bits, flt), ch, mono).astype(np.float32)
+    alone = mm.spectral(mm.resample(clip, src, dst), kind="mel", sr=dst, n_fft=2048, hop_length=512)
+    assert np.array_equal(got, alone)
