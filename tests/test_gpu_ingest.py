@@ -119,3 +119,16 @@ def test_server_pcm_formats(fmt, mono):
     clip = g.to_mono(g.pcm_to_float(raw, bits, flt), ch, mono).astype(np.float32)
     alone = mm.spectral(mm.resample(clip, src, dst), kind="mel", sr=dst, n_fft=2048, hop_length=512)
     assert np.array_equal(got, alone)
+
+
+def test_load_slice_then_resample(tmp_path):
+    """offset / duration pick the frames first (AudioIO.swift:31-45); the
+    slice is then converted and resampled on the GPU."""
+    import metalmom_amd as mm
+    rng = np.random.default_rng(12)
+    x = rng.integers(-20000, 20000, size=(96000, 2)).astype(np.int16)
+    wavfile.write(str(tmp_path / "s.wav"), 48000, x)
+    native, _ = mm.load(str(tmp_path / "s.wav"), sr=None, offset=0.3, duration=0.75)
+    assert native.size == int(0.75 * 48000)
+    y, sr = mm.load(str(tmp_path / "s.wav"), sr=22050, offset=0.3, duration=0.75)
+    assert sr == 22050 and np.array_equal(y, mm.resample(native, 48000, 22050))
