@@ -85,3 +85,38 @@ def test_chroma_batch_equals_single_clips_and_bf16_error():
             else:
                 assert (np.abs(r - single) <= 1e-2 * single.max(axis=0, keepdims=True) + 1e-30).all()
         B.close()
+
+
+def _sine(f, sr=22050, seconds=1.0):
+    i = np.arange(int(sr * seconds), dtype=np.float32)
+    return np.sin(np.float32(2 * np.pi * f) * i / np.float32(sr)).astype(np.float32)
+
+
+@pytest.mark.parametrize("f,pc", [(440.0, 9), (261.63, 0)], ids=["A4", "C4"])
+def test_pure_tone_peaks_at_its_pitch_class(f, pc):
+    """Known answers of ChromaTests.swift:134-176: one second of a sine, the
+    frame-averaged chroma peaks at its pitch class (C = 0, A = 9)."""
+    import metalmom_amd as mm
+    C = mm.chroma_stft(y=_sine(f), sr=22050)
+    assert int(np.argmax(C.mean(axis=1))) == pc
+
+
+def test_silence_gives_zero_chroma():
+    """ChromaTests.swift:180-188: a second of silence, every value 0, with
+    and without frame normalisation (a silent frame stays zero)."""
+    import metalmom_amd as mm
+    y = np.zeros(22050, np.float32)
+    for norm in (None, np.inf, 2.0):
+        C = mm.chroma_stft(y=y, sr=22050, norm=norm)
+        assert C.shape == (12, 44) and not C.any(), norm
+
+
+def test_l2_normalised_frames_have_unit_norm():
+    """ChromaTests.swift:192-209: norm=2 leaves every non-silent frame with
+    L2 norm 1 within 1e-5."""
+    import metalmom_amd as mm
+    C = mm.chroma_stft(y=_sine(440.0), sr=22050, norm=2.0)
+    l2 = np.sqrt((C.astype(np.float64) ** 2).sum(axis=0))
+    live = l2 > 1e-10
+    assert live.all()
+    assert np.abs(l2[live] - 1.0).max() <= 1e-5

@@ -199,3 +199,39 @@ def test_descriptors_on_tiny_clips(n):
     got = mm.feature.spectral_contrast(y=y, sr=22050).astype(np.float64)
     ref = o.spectral_contrast(y.astype(np.float64), 22050)
     assert got.shape == ref.shape and np.abs(got - ref).max() <= 0.05
+
+
+def _sine(f, sr=22050, seconds=1.0):
+    i = np.arange(int(sr * seconds), dtype=np.float32)
+    return np.sin(np.float32(2 * np.pi * f) * i / np.float32(sr)).astype(np.float32)
+
+
+def _lcg_noise(n=22050, seed=42):
+    """The white noise of SpectralDescriptorsTests.swift:289-300: a 64-bit
+    LCG, each state read as a signed integer over Int64.max."""
+    s, out = seed, np.empty(n, np.float32)
+    m = (1 << 64) - 1
+    for i in range(n):
+        s = (s * 6364136223846793005 + 1442695040888963407) & m
+        out[i] = np.float32(np.float64(s - (1 << 64) if s >> 63 else s) / np.float64((1 << 63) - 1))
+    return out
+
+
+def test_descriptor_known_answers_of_the_reference_tests():
+    """The tone and noise known answers of SpectralDescriptorsTests.swift
+    (58-77, 113-131, 212-230, 269-314, 329-343), through the drop-in entries
+    at the defaults: a second of 440 Hz has centroid 440 +- 50 Hz, bandwidth
+    under 500 Hz, 85% rolloff 440 +- 200 Hz and flatness under 0.1 (frames
+    2..n-3 averaged); LCG white noise has mean flatness above 0.3; a 1 kHz
+    tone's middle-frame centroid is above 440 Hz's."""
+    import metalmom_amd as mm
+    y = _sine(440.0)
+    mid = slice(2, -2)
+    cen = mm.spectral_centroid(y=y, sr=22050)[0]
+    assert abs(cen[mid].mean() - 440.0) <= 50.0
+    assert mm.spectral_bandwidth(y=y, sr=22050)[0][mid].mean() < 500.0
+    assert abs(mm.spectral_rolloff(y=y, sr=22050)[0][mid].mean() - 440.0) <= 200.0
+    assert mm.spectral_flatness(y=y)[0][mid].mean() < 0.1
+    assert mm.spectral_flatness(y=_lcg_noise())[0].mean() > 0.3
+    c1k = mm.spectral_centroid(y=_sine(1000.0), sr=22050)[0]
+    assert c1k[len(c1k) // 2] > cen[len(cen) // 2]
