@@ -31,8 +31,10 @@
 // fit in LDS they are computed once per rate pair (k_poly_table, from the same
 // FP32 table and interpolation, the tap distance taken exactly from integers)
 // and each tap is then one LDS weight read, one LDS sample read and one FMA;
-// the sums run over the same taps in the same order. Otherwise the kernel
-// above forms the weights per tap.
+// the sums run over the same taps, split over two or four accumulators. With
+// one phase (target divides source) the row is read with scalar loads and,
+// for an even decimation factor, the samples as aligned float2 / float4.
+// Otherwise the kernel above forms the weights per tap.
 #include "mm_common.h"
 #include "mm_kernels.h"
 
@@ -178,11 +180,42 @@ __global__ __launch_bounds__(kRsThreads) void k_resample_poly(const float* __res
     const int64_t i = i0 + threadIdx.x;
     if (uniform_row) {
       if (i <= i1) {
+        // the output is a whole multiple D of the input rate and lo is not
+        // clamped, so this lane's window starts at X[D * threadIdx.x]; with
+        // D even (and taps a multiple of the width) the window is read as
+        // aligned float2 or float4, so the lanes of a wave read one
+        // contiguous run of LDS
+        const int D = source / target;
         const int64_t k = (i * source) / target;
         const float* src = X + (k - support + 1 - lo);
-        float sum = 0.0f;
+        float sum;
+        if ((D & 3) == 0 && (taps & 3) == 0) {
+          const float4* s4 = reinterpret_cast<const float4*>(src);
+          float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
+#pragma unroll 4
+          for (int m = 0; m < taps; m += 4) {
+            const float4 v = s4[m >> 2];
+            a0 = fmaf(v.x, wtab[m], a0);
+            a1 = fmaf(v.y, wtab[m + 1], a1);
+            a2 = fmaf(v.z, wtab[m + 2], a2);
+            a3 = fmaf(v.w, wtab[m + 3], a3);
+          }
+          sum = (a0 + a1) + (a2 + a3);
+        } else if ((D & 1) == 0) {
+          const float2* s2 = reinterpret_cast<const float2*>(src);
+          float a0 = 0.0f, a1 = 0.0f;
+#pragma unroll 8
+          for (int m = 0; m < taps; m += 2) {
+            const float2 v = s2[m >> 1];
+            a0 = fmaf(v.x, wtab[m], a0);
+            a1 = fmaf(v.y, wtab[m + 1], a1);
+          }
+          sum = a0 + a1;
+        } else {
+          sum = 0.0f;
 #pragma unroll 16
-        for (int m = 0; m < taps; ++m) sum = fmaf(src[m], wtab[m], sum);
+          for (int m = 0; m < taps; ++m) sum = fmaf(src[m], wtab[m], sum);
+        }
         y[i] = wsum1 > 0.0f ? sum * scale : 0.0f;
       }
       continue;
@@ -198,12 +231,37 @@ __global__ __launch_bounds__(kRsThreads) void k_resample_poly(const float* __res
       const bool mir = ph > phases / 2;
       const float* w = mir ? Wt + (phases - ph) * stride + (taps - 1) : Wt + ph * stride;
       const int dw = mir ? -1 : 1;
-      const float* src = staged ? X + (j0 - lo) : x + j0;
       float sum = 0.0f, wsum = 0.0f;
-      for (int m = m0; m <= m1; ++m) {
-        const float wm = w[dw * m];
-        sum += src[m] * wm;
-        wsum += wm;
+      if (staged && m0 == 0 && m1 == taps - 1) {
+        // every tap inside the signal (all but the first and last few
+        // outputs): two accumulator chains over tap pairs (taps is even),
+        // unrolled so that many LDS reads are in flight
+        const float* src = X + (j0 - lo);
+        float s0 = 0.0f, s1 = 0.0f, w0 = 0.0f, w1 = 0.0f;
+#pragma unroll 4
+        for (int m = 0; m < taps; m += 2) {
+          const float a = w[dw * m], b = w[dw * (m + 1)];
+          s0 = fmaf(src[m], a, s0);
+          s1 = fmaf(src[m + 1], b, s1);
+          w0 += a;
+          w1 += b;
+        }
+        sum = s0 + s1;
+        wsum = w0 + w1;
+      } else if (staged) {
+        const float* src = X + (j0 - lo);
+        for (int m = m0; m <= m1; ++m) {
+          const float wm = w[dw * m];
+          sum += src[m] * wm;
+          wsum += wm;
+        }
+      } else {
+        const float* src = x + j0;
+        for (int m = m0; m <= m1; ++m) {
+          const float wm = w[dw * m];
+          sum += src[m] * wm;
+          wsum += wm;
+        }
       }
       y[i] = wsum > 0.0f ? sum * scale : 0.0f;
     }
@@ -265,12 +323,15 @@ hipError_t launch_resample_poly(const float* x, int64_t n, float* y, int64_t n_o
   const int wf = (P / 2 + 1) * stride;
   const int64_t n_chunks = (n_out + kRsThreads - 1) / kRsThreads;
   if (n_chunks <= 0) return hipSuccess;
-  const int gr = (int)std::min<int64_t>(std::max(grid, 1), n_chunks);
+  // two workgroups per CU when both fit in LDS (one phase: 33 KB each), so
+  // each SIMD holds 8 waves rather than 4 while the other waits on LDS
+  const size_t lds = (size_t)(((wf + 3) & ~3) + kWinCapPoly) * sizeof(float);
+  const int per_cu = 2 * lds <= lds_limit_bytes() ? 2 : 1;
+  const int gr = (int)std::min<int64_t>((int64_t)std::max(grid, 1) * per_cu, n_chunks);
   static const hipError_t attr = hipFuncSetAttribute(reinterpret_cast<const void*>(k_resample_poly),
                                                      hipFuncAttributeMaxDynamicSharedMemorySize,
                                                      (int)lds_limit_bytes());
   if (attr != hipSuccess) return attr;
-  const size_t lds = (size_t)(((wf + 3) & ~3) + kWinCapPoly) * sizeof(float);
   const double ratio = (double)target_sr / (double)source_sr;
   const float scale = (float)(ratio < 1.0 ? ratio : 1.0);
   hipLaunchKernelGGL(k_resample_poly, dim3(gr), dim3(kRsThreads), lds, st, x, n, y, n_out, source_sr, target_sr,
