@@ -329,7 +329,7 @@ int32_t Engine::resample_weights(int source_sr, int target_sr, hipStream_t st, c
   if (s != OK) return s;
   std::lock_guard<std::mutex> g(mu_);
   for (auto& e : poly_)
-    if (e->src == source_sr && e->dst == target_sr) {
+    if (e->src == source_sr && e->dst == target_sr && !e->full) {
       *out = static_cast<const float*>(e->w.p);
       *wsum1 = e->wsum1;
       return OK;
@@ -337,6 +337,7 @@ int32_t Engine::resample_weights(int source_sr, int target_sr, hipStream_t st, c
   auto e = std::make_unique<PolyEntry>();
   e->src = source_sr;
   e->dst = target_sr;
+  e->full = false;
   MM_HIP(e->w.ensure((size_t)(P / 2 + 1) * stride * sizeof(float)));   // rows 0 .. P/2
   MM_HIP(launch_poly_table(source_sr, target_sr, table, static_cast<float*>(e->w.p), st));
   MM_HIP(hipStreamSynchronize(st));   // shared by every stream from here on
@@ -348,6 +349,32 @@ int32_t Engine::resample_weights(int source_sr, int target_sr, hipStream_t st, c
   }
   *out = static_cast<const float*>(e->w.p);
   *wsum1 = e->wsum1;
+  poly_.push_back(std::move(e));
+  return OK;
+}
+
+int32_t Engine::resample_phase_weights(int source_sr, int target_sr, hipStream_t st, const float** out) {
+  *out = nullptr;
+  PhaseShape sh;
+  if (!resample_phase_shape(source_sr, target_sr, &sh)) return OK;
+  const float* table = nullptr;
+  int32_t s = resample_table(st, &table);
+  if (s != OK) return s;
+  std::lock_guard<std::mutex> g(mu_);
+  for (auto& e : poly_)
+    if (e->src == source_sr && e->dst == target_sr && e->full) {
+      *out = static_cast<const float*>(e->w.p);
+      return OK;
+    }
+  auto e = std::make_unique<PolyEntry>();
+  e->src = source_sr;
+  e->dst = target_sr;
+  e->full = true;
+  e->wsum1 = 0.0f;
+  MM_HIP(e->w.ensure((size_t)sh.phases * (sh.stride + 1) * sizeof(float)));   // rows, then row sums
+  MM_HIP(launch_phase_table(source_sr, target_sr, table, static_cast<float*>(e->w.p), st));
+  MM_HIP(hipStreamSynchronize(st));   // shared by every stream from here on
+  *out = static_cast<const float*>(e->w.p);
   poly_.push_back(std::move(e));
   return OK;
 }
@@ -825,8 +852,14 @@ int32_t enqueue_resample(const float* d_x, int64_t n, int source_sr, int target_
   if (!e) return ERR_NO_GPU;
   const int64_t m = resample_length(n, source_sr, target_sr);
   const float* w = nullptr;
+  int32_t s = e->resample_phase_weights(source_sr, target_sr, st, &w);
+  if (s != OK) return s;
+  if (w) {   // two or more phases: one phase per wave, weight rows by scalar loads
+    MM_HIP(launch_resample_phase(d_x, n, d_y, m, source_sr, target_sr, w, e->num_cus(), st));
+    return OK;
+  }
   float wsum1 = 0.0f;
-  int32_t s = e->resample_weights(source_sr, target_sr, st, &w, &wsum1);
+  s = e->resample_weights(source_sr, target_sr, st, &w, &wsum1);
   if (s != OK) return s;
   if (w) {   // polyphase: weight rows in LDS
     MM_HIP(launch_resample_poly(d_x, n, d_y, m, source_sr, target_sr, w, wsum1, e->num_cus(), st));

@@ -89,10 +89,12 @@ class Engine {
   // Filter table of the sample-rate converter (generated on first use).
   int32_t resample_table(hipStream_t st, const float** out);
   // Polyphase weight rows of a rate pair (built on first use; nullptr when
-  // they do not fit the polyphase kernel).
   // they do not fit the polyphase kernel); *wsum1: the sum of the row when
   // there is one phase.
   int32_t resample_weights(int source_sr, int target_sr, hipStream_t st, const float** out, float* wsum1);
+  // Full weight table and row sums of the phase-major kernel (built on first
+  // use; nullptr when the rate pair does not fit it).
+  int32_t resample_phase_weights(int source_sr, int target_sr, hipStream_t st, const float** out);
 
   // Persistent grid size: autotuned if known, else a default.
   int grid_for(const SpecRequest& r, int n_tiles, const TileLayout& L);
@@ -113,6 +115,7 @@ class Engine {
   DevBuf rs_table_;
   struct PolyEntry {
     int src, dst;
+    bool full;   // phase-major table (all rows and their sums)
     float wsum1;
     DevBuf w;
   };

@@ -110,6 +110,17 @@ hipError_t launch_poly_table(int source_sr, int target_sr, const float* table, f
 // whether interior outputs are 0, as the per-output weight sum would).
 hipError_t launch_resample_poly(const float* x, int64_t n, float* y, int64_t n_out, int source_sr, int target_sr,
                                 const float* wtab, float wsum1, int grid, hipStream_t st);
+// Phase-major form for rate pairs with 2 .. 4096 phases whose transposed
+// input window (64 outputs of each phase per wave) fits in LDS: the full
+// table of phases rows of stride floats, then one row sum per phase
+// (phases * (stride + 1) floats, launch_phase_table), read by scalar loads.
+struct PhaseShape {
+  int phases, step, support, taps, stride, groups, ld;
+};
+bool resample_phase_shape(int source_sr, int target_sr, PhaseShape* sh);
+hipError_t launch_phase_table(int source_sr, int target_sr, const float* table, float* wfull, hipStream_t st);
+hipError_t launch_resample_phase(const float* x, int64_t n, float* y, int64_t n_out, int source_sr, int target_sr,
+                                 const float* wfull, int grid, hipStream_t st);
 
 // ---- inverse STFT ----
 struct IstftArgs {

@@ -34,7 +34,10 @@
 // the sums run over the same taps, split over two or four accumulators. With
 // one phase (target divides source) the row is read with scalar loads and,
 // for an even decimation factor, the samples as aligned float2 / float4.
-// Otherwise the kernel above forms the weights per tap.
+// Pairs with 2 .. 4096 phases take the phase-major kernel (k_resample_phase:
+// one phase per wave, all P rows in HBM read by scalar loads, the input
+// window transposed in LDS). Otherwise the kernel above forms the weights per
+// tap.
 #include "mm_common.h"
 #include "mm_kernels.h"
 
@@ -50,6 +53,8 @@ constexpr int kRsThreads = 1024;
 constexpr int kWinCap = 6144;                 // staged input samples (24 KB)
 constexpr int kWinCapPoly = 8192;             // staged input samples of the polyphase path (32 KB)
 constexpr int64_t kPolyMaxFloats = 30000;     // polyphase weight rows in LDS (117 KB)
+constexpr int kPhaseMaxFloats = 36864;        // transposed input window of the phase-major path (144 KB)
+constexpr int kPhaseMaxRows = 4096;           // phases of the phase-major path (full weight table in HBM)
 
 __device__ double bessel_i0(double x) {
   double sum = 1.0, term = 1.0;
@@ -268,6 +273,117 @@ __global__ __launch_bounds__(kRsThreads) void k_resample_poly(const float* __res
   }
 }
 
+// Sum of each full weight row, in tap order, after the P rows: the sign that
+// the per-output weight sum has for every output whose taps all lie inside
+// the signal.
+__global__ void k_row_sums(float* __restrict__ W, int rows, int taps, int stride) {
+  const int ph = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ph >= rows) return;
+  const float* w = W + (int64_t)ph * stride;
+  float s = 0.0f;
+  for (int m = 0; m < taps; ++m) s += w[m];
+  W[(int64_t)rows * stride + ph] = s;
+}
+
+// Phase-major form of the polyphase path. With P phases and input step
+// Dp = source / g per P outputs, output i = P q + r has phase (r Dp) mod P
+// and first tap j0 = q Dp + b_r - support + 1, b_r = floor(r Dp / P) < Dp.
+// A workgroup takes 64 G consecutive q and every r; a wave takes one (r,
+// group of 64 q) task, so its weight row is wave-uniform (scalar loads from
+// the full P-row table in HBM) and its 64 lanes read the staged window at
+// a = 64 Dp gq + Dp lane + b_r + m. The window is stored transposed,
+// a -> Xt[(a mod Dp) * LD + a div Dp], so those 64 reads are 64 consecutive
+// words: no bank conflicts, for any Dp. Taps run in segments of constant
+// a div Dp (column steps of LD words). Workgroups whose window leaves the
+// signal sum the clipped taps exactly from HBM instead.
+__global__ __launch_bounds__(kRsThreads) void k_resample_phase(const float* __restrict__ x, int64_t n,
+                                                              float* __restrict__ y, int64_t n_out, int P, int Dp,
+                                                              int support, int taps, int stride, int G, int LD,
+                                                              float scale, const float* __restrict__ wfull) {
+  extern __shared__ float lds[];
+  float* Xt = lds;
+  const float* rowsum = wfull + (int64_t)P * stride;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int nq = 64 * G;
+  const int L = nq * Dp + taps - 1;   // window samples of one workgroup
+  const int64_t q_total = (n_out + P - 1) / P;
+  const int64_t n_blocks = (q_total + nq - 1) / nq;
+  const int tasks = P * G;
+  const int sd = kRsThreads / Dp, sm = kRsThreads - sd * Dp;
+  for (int64_t blk = blockIdx.x; blk < n_blocks; blk += gridDim.x) {
+    const int64_t q0 = blk * nq;
+    const int64_t lo = q0 * Dp - support + 1;
+    const bool interior = lo >= 0 && lo + L - 1 <= n - 1;
+    __syncthreads();   // the previous block's reads of Xt are done
+    if (interior) {
+      int kd = threadIdx.x / Dp, km = threadIdx.x - kd * Dp;
+      for (int k = threadIdx.x; k < L; k += kRsThreads) {
+        Xt[km * LD + kd] = x[lo + k];
+        kd += sd;
+        km += sm;
+        if (km >= Dp) {
+          km -= Dp;
+          ++kd;
+        }
+      }
+    }
+    __syncthreads();
+    for (int t = wave; t < tasks; t += kRsThreads / 64) {
+      const int r = t % P, gq = t / P;
+      const int64_t q = q0 + 64 * gq + lane;
+      const int64_t i = q * P + r;
+      const int64_t rd = (int64_t)r * Dp;
+      const int ph = (int)(rd % P), b = (int)(rd / P);
+      const float* w = wfull + (int64_t)ph * stride;
+      float sum;
+      bool pos;
+      if (interior) {
+        const float* xl = Xt + 64 * gq + lane;
+        float a0 = 0.0f, a1 = 0.0f;
+        int m = 0, col = b, drow = 0;
+        if (Dp == 1) {   // upsampling by a whole factor: b = 0, the window is contiguous
+#pragma unroll 8
+          for (; m < taps; m += 2) {
+            a0 = fmaf(xl[m], w[m], a0);
+            a1 = fmaf(xl[m + 1], w[m + 1], a1);
+          }
+        }
+        while (m < taps) {
+          const int len = min(taps - m, Dp - col);   // taps of constant a div Dp
+          const float* p = xl + col * LD + drow;
+          int k = 0;
+#pragma unroll 4
+          for (; k + 1 < len; k += 2) {
+            a0 = fmaf(p[k * LD], w[m + k], a0);
+            a1 = fmaf(p[(k + 1) * LD], w[m + k + 1], a1);
+          }
+          if (k < len) a0 = fmaf(p[k * LD], w[m + k], a0);
+          m += len;
+          col = 0;
+          ++drow;
+        }
+        sum = a0 + a1;
+        pos = rowsum[ph] > 0.0f;
+      } else {
+        float s = 0.0f, ws = 0.0f;
+        if (i < n_out) {
+          const int64_t j0 = q * Dp + b - support + 1;
+          const int m0 = j0 < 0 ? (int)-j0 : 0;
+          const int m1 = (j0 + taps - 1 > n - 1) ? (int)(n - 1 - j0) : taps - 1;
+          for (int m = m0; m <= m1; ++m) {
+            s += x[j0 + m] * w[m];
+            ws += w[m];
+          }
+        }
+        sum = s;
+        pos = ws > 0.0f;
+      }
+      if (i < n_out) y[i] = pos ? sum * scale : 0.0f;
+    }
+  }
+}
+
 hipError_t launch_tab_resample(float* table, hipStream_t st) {
   hipLaunchKernelGGL(k_tab_resample, dim3((kTabLen + 255) / 256), dim3(256), 0, st, table);
   return hipGetLastError();
@@ -336,6 +452,68 @@ hipError_t launch_resample_poly(const float* x, int64_t n, float* y, int64_t n_o
   const float scale = (float)(ratio < 1.0 ? ratio : 1.0);
   hipLaunchKernelGGL(k_resample_poly, dim3(gr), dim3(kRsThreads), lds, st, x, n, y, n_out, source_sr, target_sr,
                      g, support, taps, stride, P, wf, kWinCapPoly, scale, wsum1, wtab);
+  return hipGetLastError();
+}
+
+bool resample_phase_shape(int source_sr, int target_sr, PhaseShape* sh) {
+  const double ratio = (double)target_sr / (double)source_sr;
+  const double scale = ratio < 1.0 ? ratio : 1.0;
+  const int support = (int)std::ceil((double)kZeros / scale);
+  const int64_t g = gcd64(source_sr, target_sr);
+  const int64_t P = target_sr / g, Dp = source_sr / g;
+  if (P < 2 || P > kPhaseMaxRows || Dp > kRsThreads) return false;
+  const int taps = 2 * support;
+  auto rows_of = [&](int G) {   // LD: rows of the transposed window, odd
+    const int64_t L = 64 * (int64_t)G * Dp + taps - 1;
+    return ((L + Dp - 1) / Dp) | 1;
+  };
+  // enough (r, group) tasks for 32 waves, as far as the window fits
+  int G = (int)std::max<int64_t>(1, std::min<int64_t>(16, (32 + P - 1) / P));
+  while (G > 1 && Dp * rows_of(G) > kPhaseMaxFloats) --G;
+  const int64_t LD = rows_of(G);
+  if (Dp * LD > kPhaseMaxFloats || (size_t)(Dp * LD) * sizeof(float) > lds_limit_bytes()) return false;
+  sh->phases = (int)P;
+  sh->step = (int)Dp;
+  sh->support = support;
+  sh->taps = taps;
+  sh->stride = taps | 1;
+  sh->groups = G;
+  sh->ld = (int)LD;
+  return true;
+}
+
+hipError_t launch_phase_table(int source_sr, int target_sr, const float* table, float* wfull, hipStream_t st) {
+  PhaseShape sh;
+  if (!resample_phase_shape(source_sr, target_sr, &sh)) return hipErrorInvalidConfiguration;
+  const int g = (int)gcd64(source_sr, target_sr);
+  const int D = std::max(source_sr, target_sr);
+  const int wf = sh.phases * sh.stride;
+  hipLaunchKernelGGL(k_poly_table, dim3((wf + 255) / 256), dim3(256), 0, st, table, target_sr, g, D, sh.support,
+                     sh.taps, sh.stride, sh.phases, wfull);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_row_sums, dim3((sh.phases + 63) / 64), dim3(64), 0, st, wfull, sh.phases, sh.taps, sh.stride);
+  return hipGetLastError();
+}
+
+hipError_t launch_resample_phase(const float* x, int64_t n, float* y, int64_t n_out, int source_sr, int target_sr,
+                                 const float* wfull, int grid, hipStream_t st) {
+  PhaseShape sh;
+  if (!resample_phase_shape(source_sr, target_sr, &sh)) return hipErrorInvalidConfiguration;
+  const int64_t nq = 64 * (int64_t)sh.groups;
+  const int64_t n_blocks = ((n_out + sh.phases - 1) / sh.phases + nq - 1) / nq;
+  if (n_blocks <= 0) return hipSuccess;
+  const size_t lds = (size_t)sh.step * sh.ld * sizeof(float);
+  const int per_cu = 2 * lds <= lds_limit_bytes() ? 2 : 1;
+  const int gr = (int)std::min<int64_t>((int64_t)std::max(grid, 1) * per_cu, n_blocks);
+  static const hipError_t attr = hipFuncSetAttribute(reinterpret_cast<const void*>(k_resample_phase),
+                                                     hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                     (int)lds_limit_bytes());
+  if (attr != hipSuccess) return attr;
+  const double ratio = (double)target_sr / (double)source_sr;
+  const float scale = (float)(ratio < 1.0 ? ratio : 1.0);
+  hipLaunchKernelGGL(k_resample_phase, dim3(gr), dim3(kRsThreads), lds, st, x, n, y, n_out, sh.phases, sh.step,
+                     sh.support, sh.taps, sh.stride, sh.groups, sh.ld, scale, wfull);
   return hipGetLastError();
 }
 

@@ -46,12 +46,18 @@ def _check(got, y, src, dst):
     assert err.max() <= tol, (float(err.max()), tol, int(np.argmax(err)))
 
 
-# (source, target, samples): most pairs take the polyphase kernel (weight rows
-# in LDS, the upper half of the phases read mirrored); 44.1 -> 32.001 kHz has
+# (source, target, samples): one-phase pairs (44.1 -> 22.05 kHz) take the
+# polyphase kernel with the row in scalar registers; pairs with 2 .. 4096
+# phases take the phase-major kernel, whose workgroups cover 64 outputs of
+# every phase (64 x 320 input samples at 48 -> 22.05 kHz): the short clips
+# reach only the clipped first and last workgroups, the 150,000-300,000-sample
+# ones the interior workgroups (transposed LDS window) of every layout, input
+# step 1 (8 -> 48 kHz), 147, 320 and 441 included; 44.1 -> 32.001 kHz has
 # too many phases and takes the per-tap kernel
 CASES = [(44100, 22050, 50000), (22050, 44100, 30000), (48000, 22050, 40000), (16000, 22050, 20000),
          (22050, 16000, 20000), (44100, 8000, 30000), (8000, 48000, 5000), (44100, 48000, 30000),
          (44100, 22050, 1), (44100, 22050, 2), (22050, 44100, 3), (48000, 22050, 129),
+         (48000, 22050, 300000), (16000, 22050, 150000), (22050, 16000, 150000), (44100, 8000, 300000),
          (44100, 32001, 20000)]   # 10,667 phases: too many rows for LDS, the per-tap kernel
 
 
@@ -94,6 +100,22 @@ def test_long_signal_and_repeatability():
     ref = g.resample(seg.astype(np.float64), 44100, 22050, table=_table32())
     off = (2 * i - lo) // 2
     assert np.abs(a[i:i + 2000] - ref[off:off + 2000]).max() <= 2e-6 * 128 * np.abs(y).max()
+
+
+def test_long_many_phase_signal_and_repeatability():
+    """48 -> 22.05 kHz (147 phases, 320 input samples per 147 outputs) over
+    3 M samples: two calls bitwise identical; 1,000 outputs from the middle
+    against the oracle run on the inputs they reach (a segment starting on a
+    multiple of 320 samples, so its outputs keep their phases)."""
+    import metalmom_amd as mm
+    y = _signal(3_000_000, 48000, 6)
+    a = mm.resample(y, 48000, 22050)
+    assert np.array_equal(a, mm.resample(y, 48000, 22050))
+    k = int(np.random.default_rng(2).integers(10, len(a) // 147 - 20))
+    lo = 320 * (k - 4)                  # 1,280 samples before output 147 k's window
+    ref = g.resample(y[lo:320 * k + 3000].astype(np.float64), 48000, 22050, table=_table32())
+    got = a[147 * k:147 * k + 1000]
+    assert np.abs(got - ref[588:1588]).max() <= 2e-6 * 140 * np.abs(y).max()
 
 
 def test_device_entry_matches_host_entry():
