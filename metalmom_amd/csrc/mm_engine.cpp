@@ -861,7 +861,11 @@ int32_t enqueue_resample(const float* d_x, int64_t n, int source_sr, int target_
   float wsum1 = 0.0f;
   s = e->resample_weights(source_sr, target_sr, st, &w, &wsum1);
   if (s != OK) return s;
-  if (w) {   // polyphase: weight rows in LDS
+  if (w) {   // polyphase: weight rows in LDS (one phase: the row by scalar loads)
+    if (source_sr == 2 * target_sr) {   // 256 taps: always fits the two-outputs-per-lane kernel
+      MM_HIP(launch_resample_dec2(d_x, n, d_y, m, source_sr, target_sr, w, wsum1, e->num_cus(), st));
+      return OK;
+    }
     MM_HIP(launch_resample_poly(d_x, n, d_y, m, source_sr, target_sr, w, wsum1, e->num_cus(), st));
     return OK;
   }

@@ -110,6 +110,10 @@ hipError_t launch_poly_table(int source_sr, int target_sr, const float* table, f
 // whether interior outputs are 0, as the per-output weight sum would).
 hipError_t launch_resample_poly(const float* x, int64_t n, float* y, int64_t n_out, int source_sr, int target_sr,
                                 const float* wtab, float wsum1, int grid, hipStream_t st);
+// One phase with source = 2 target: two outputs per lane from float4 reads
+// of the staged window (wtab, wsum1 as for launch_resample_poly).
+hipError_t launch_resample_dec2(const float* x, int64_t n, float* y, int64_t n_out, int source_sr, int target_sr,
+                                const float* wtab, float wsum1, int grid, hipStream_t st);
 // Phase-major form for rate pairs with 2 .. 4096 phases whose transposed
 // input window (64 outputs of each phase per wave) fits in LDS: the full
 // table of phases rows of stride floats, then one row sum per phase

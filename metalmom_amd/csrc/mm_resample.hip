@@ -33,7 +33,8 @@
 // and each tap is then one LDS weight read, one LDS sample read and one FMA;
 // the sums run over the same taps, split over two or four accumulators. With
 // one phase (target divides source) the row is read with scalar loads and,
-// for an even decimation factor, the samples as aligned float2 / float4.
+// for an even decimation factor, the samples as aligned float2 / float4;
+// decimation by exactly 2 takes k_resample_dec2 (two outputs per lane).
 // Pairs with 2 .. 4096 phases take the phase-major kernel (k_resample_phase:
 // one phase per wave, all P rows in HBM read by scalar loads, the input
 // window transposed in LDS). Otherwise the kernel above forms the weights per
@@ -273,6 +274,77 @@ __global__ __launch_bounds__(kRsThreads) void k_resample_poly(const float* __res
   }
 }
 
+// One phase, decimation by exactly 2 (44.1 -> 22.05, 48 -> 24, 32 -> 16 kHz):
+// output i = scale * sum_m w[m] x[2 i - support + 1 + m]. A workgroup takes
+// chunks of 2048 outputs; lane t computes outputs i0 + 2t and i0 + 2t + 1,
+// whose windows start at samples 4t and 4t + 2 of the chunk's staged window,
+// so every aligned float4 read (one contiguous run of LDS per wave) serves
+// four taps of each output. The weight row is wave-uniform (scalar loads).
+// Chunks whose window leaves the signal, or that reach past the last output,
+// sum the clipped taps exactly from HBM.
+constexpr int kDec2Out = 2 * kRsThreads;
+__global__ __launch_bounds__(kRsThreads) void k_resample_dec2(const float* __restrict__ x, int64_t n,
+                                                             float* __restrict__ y, int64_t n_out, int support,
+                                                             int taps, float scale, float wsum1,
+                                                             const float* __restrict__ w) {
+  extern __shared__ float lds[];
+  float* X = lds;
+  const int L = 2 * kDec2Out + taps;   // staged samples: the last lane reads up to 4 * 1023 + taps + 3
+  const int64_t n_chunks = (n_out + kDec2Out - 1) / kDec2Out;
+  const int t = threadIdx.x;
+  for (int64_t c = blockIdx.x; c < n_chunks; c += gridDim.x) {
+    const int64_t i0 = c * kDec2Out;
+    const int64_t lo = 2 * i0 - support + 1;
+    const bool interior = lo >= 0 && lo + L - 1 <= n - 1 && i0 + kDec2Out <= n_out;
+    __syncthreads();   // the previous chunk's reads of X are done
+    if (interior)
+      for (int k = t; k < L; k += kRsThreads) X[k] = x[lo + k];
+    __syncthreads();
+    const int64_t i = i0 + 2 * t;
+    if (interior) {
+      const float4* s4 = reinterpret_cast<const float4*>(X + 4 * t);
+      float4 v = s4[0];
+      float p0 = v.x * w[0], p1 = v.y * w[1];
+      p0 = fmaf(v.z, w[2], p0);
+      p1 = fmaf(v.w, w[3], p1);
+      float q0 = v.z * w[0], q1 = v.w * w[1];
+      const int nq = taps >> 2;
+#pragma unroll 4
+      for (int k = 1; k < nq; ++k) {
+        v = s4[k];
+        const int m = 4 * k;
+        p0 = fmaf(v.x, w[m], p0);
+        p1 = fmaf(v.y, w[m + 1], p1);
+        p0 = fmaf(v.z, w[m + 2], p0);
+        p1 = fmaf(v.w, w[m + 3], p1);
+        q0 = fmaf(v.x, w[m - 2], q0);
+        q1 = fmaf(v.y, w[m - 1], q1);
+        q0 = fmaf(v.z, w[m], q0);
+        q1 = fmaf(v.w, w[m + 1], q1);
+      }
+      v = s4[nq];
+      q0 = fmaf(v.x, w[taps - 2], q0);
+      q1 = fmaf(v.y, w[taps - 1], q1);
+      y[i] = wsum1 > 0.0f ? (p0 + p1) * scale : 0.0f;
+      y[i + 1] = wsum1 > 0.0f ? (q0 + q1) * scale : 0.0f;
+    } else {
+      for (int o = 0; o < 2; ++o) {
+        const int64_t io = i + o;
+        if (io >= n_out) break;
+        const int64_t j0 = 2 * io - support + 1;
+        const int m0 = j0 < 0 ? (int)-j0 : 0;
+        const int m1 = (j0 + taps - 1 > n - 1) ? (int)(n - 1 - j0) : taps - 1;
+        float s = 0.0f, ws = 0.0f;
+        for (int m = m0; m <= m1; ++m) {
+          s += x[j0 + m] * w[m];
+          ws += w[m];
+        }
+        y[io] = ws > 0.0f ? s * scale : 0.0f;
+      }
+    }
+  }
+}
+
 // Sum of each full weight row, in tap order, after the P rows: the sign that
 // the per-output weight sum has for every output whose taps all lie inside
 // the signal.
@@ -452,6 +524,23 @@ hipError_t launch_resample_poly(const float* x, int64_t n, float* y, int64_t n_o
   const float scale = (float)(ratio < 1.0 ? ratio : 1.0);
   hipLaunchKernelGGL(k_resample_poly, dim3(gr), dim3(kRsThreads), lds, st, x, n, y, n_out, source_sr, target_sr,
                      g, support, taps, stride, P, wf, kWinCapPoly, scale, wsum1, wtab);
+  return hipGetLastError();
+}
+
+hipError_t launch_resample_dec2(const float* x, int64_t n, float* y, int64_t n_out, int source_sr, int target_sr,
+                                const float* wtab, float wsum1, int grid, hipStream_t st) {
+  int P, taps, stride;
+  if (source_sr != 2 * target_sr || !resample_poly_shape(source_sr, target_sr, &P, &taps, &stride) || P != 1 ||
+      (taps & 3) != 0)
+    return hipErrorInvalidConfiguration;
+  const int64_t n_chunks = (n_out + kDec2Out - 1) / kDec2Out;
+  if (n_chunks <= 0) return hipSuccess;
+  const size_t lds = (size_t)(2 * kDec2Out + taps) * sizeof(float);
+  if (lds > lds_limit_bytes()) return hipErrorInvalidConfiguration;
+  const int per_cu = 2 * lds <= lds_limit_bytes() ? 2 : 1;
+  const int gr = (int)std::min<int64_t>((int64_t)std::max(grid, 1) * per_cu, n_chunks);
+  hipLaunchKernelGGL(k_resample_dec2, dim3(gr), dim3(kRsThreads), lds, st, x, n, y, n_out, taps / 2, taps,
+                     0.5f, wsum1, wtab);
   return hipGetLastError();
 }
 
