@@ -361,17 +361,22 @@ __global__ void k_row_sums(float* __restrict__ W, int rows, int taps, int stride
 // Dp = source / g per P outputs, output i = P q + r has phase (r Dp) mod P
 // and first tap j0 = q Dp + b_r - support + 1, b_r = floor(r Dp / P) < Dp.
 // A workgroup takes 64 G consecutive q and every r; a wave takes one (r,
-// group of 64 q) task, so its weight row is wave-uniform (scalar loads from
-// the full P-row table in HBM) and its 64 lanes read the staged window at
-// a = 64 Dp gq + Dp lane + b_r + m. The window is stored transposed,
+// r + 1, group of 64 q) task, so its two weight rows are wave-uniform
+// (scalar loads from the full P-row table in HBM) and its 64 lanes read the
+// staged window at a = 64 Dp gq + Dp lane + b_r + m, each sample feeding
+// both phases (output r + 1's window starts b_{r+1} - b_r samples later). The window is stored transposed,
 // a -> Xt[(a mod Dp) * LD + a div Dp], so those 64 reads are 64 consecutive
 // words: no bank conflicts, for any Dp. Taps run in segments of constant
 // a div Dp (column steps of LD words). Workgroups whose window leaves the
-// signal sum the clipped taps exactly from HBM instead.
+// signal sum the clipped taps exactly from HBM instead. kLD > 0 fixes LD at
+// compile time (65: 64 lanes and taps <= Dp, e.g. 48 -> 22.05 kHz), so the
+// column steps are immediate offsets rather than one address add per tap.
+template <int kLD>
 __global__ __launch_bounds__(kRsThreads) void k_resample_phase(const float* __restrict__ x, int64_t n,
                                                               float* __restrict__ y, int64_t n_out, int P, int Dp,
-                                                              int support, int taps, int stride, int G, int LD,
+                                                              int support, int taps, int stride, int G, int LD_,
                                                               float scale, const float* __restrict__ wfull) {
+  const int LD = kLD > 0 ? kLD : LD_;
   extern __shared__ float lds[];
   float* Xt = lds;
   const float* rowsum = wfull + (int64_t)P * stride;
@@ -381,7 +386,8 @@ __global__ __launch_bounds__(kRsThreads) void k_resample_phase(const float* __re
   const int L = nq * Dp + taps - 1;   // window samples of one workgroup
   const int64_t q_total = (n_out + P - 1) / P;
   const int64_t n_blocks = (q_total + nq - 1) / nq;
-  const int tasks = P * G;
+  const int Ph = (P + 1) / 2;   // phase pairs
+  const int tasks = Ph * G;
   const int sd = kRsThreads / Dp, sm = kRsThreads - sd * Dp;
   for (int64_t blk = blockIdx.x; blk < n_blocks; blk += gridDim.x) {
     const int64_t q0 = blk * nq;
@@ -402,56 +408,89 @@ __global__ __launch_bounds__(kRsThreads) void k_resample_phase(const float* __re
     }
     __syncthreads();
     for (int t = wave; t < tasks; t += kRsThreads / 64) {
-      const int r = t % P, gq = t / P;
+      // phases r and r + 1 (r + 1 = P: r again, not stored): their windows
+      // start dl = b1 - b samples apart, so each sample read feeds both
+      const int r = 2 * (t % Ph), gq = t / Ph;
+      const bool two = r + 1 < P;
       const int64_t q = q0 + 64 * gq + lane;
       const int64_t i = q * P + r;
-      const int64_t rd = (int64_t)r * Dp;
+      const int64_t rd = (int64_t)r * Dp, rd1 = two ? rd + Dp : rd;
       const int ph = (int)(rd % P), b = (int)(rd / P);
+      const int ph1 = (int)(rd1 % P), dl = (int)(rd1 / P) - b;
       const float* w = wfull + (int64_t)ph * stride;
-      float sum;
-      bool pos;
+      const float* w1 = wfull + (int64_t)ph1 * stride;
+      float sum, sum1;
+      bool pos, pos1;
       if (interior) {
         const float* xl = Xt + 64 * gq + lane;
-        float a0 = 0.0f, a1 = 0.0f;
-        int m = 0, col = b, drow = 0;
-        if (Dp == 1) {   // upsampling by a whole factor: b = 0, the window is contiguous
+        float a0 = 0.0f, a1 = 0.0f, c0 = 0.0f, c1 = 0.0f;
+        // window samples s = 0 .. taps + dl - 1 from b: tap s of output r
+        // (s < taps), tap s - dl of output r + 1 (s >= dl)
+        const int s_end = taps + dl;
+        int s = 0, col = b, drow = 0;
+        if (Dp == 1) {   // upsampling by a whole factor: b = dl = 0, the window is contiguous
 #pragma unroll 8
-          for (; m < taps; m += 2) {
-            a0 = fmaf(xl[m], w[m], a0);
-            a1 = fmaf(xl[m + 1], w[m + 1], a1);
+          for (; s < taps; s += 2) {
+            const float v0 = xl[s], v1 = xl[s + 1];
+            a0 = fmaf(v0, w[s], a0);
+            a1 = fmaf(v1, w[s + 1], a1);
+            c0 = fmaf(v0, w1[s], c0);
+            c1 = fmaf(v1, w1[s + 1], c1);
           }
         }
-        while (m < taps) {
-          const int len = min(taps - m, Dp - col);   // taps of constant a div Dp
+        while (s < s_end) {
+          const int len = min(s_end - s, Dp - col);   // samples of constant a div Dp
           const float* p = xl + col * LD + drow;
+          const int e1 = min(len, max(0, dl - s));     // [0, e1): output r only
+          const int e2 = min(len, max(0, taps - s));   // [e1, e2): both; [e2, len): output r + 1 only
           int k = 0;
-#pragma unroll 4
-          for (; k + 1 < len; k += 2) {
-            a0 = fmaf(p[k * LD], w[m + k], a0);
-            a1 = fmaf(p[(k + 1) * LD], w[m + k + 1], a1);
+          for (; k < e1; ++k) a0 = fmaf(p[k * LD], w[s + k], a0);
+#pragma unroll 8
+          for (; k + 1 < e2; k += 2) {
+            const float v0 = p[k * LD], v1 = p[(k + 1) * LD];
+            a0 = fmaf(v0, w[s + k], a0);
+            a1 = fmaf(v1, w[s + k + 1], a1);
+            c0 = fmaf(v0, w1[s + k - dl], c0);
+            c1 = fmaf(v1, w1[s + k + 1 - dl], c1);
           }
-          if (k < len) a0 = fmaf(p[k * LD], w[m + k], a0);
-          m += len;
+          if (k < e2) {
+            const float v0 = p[k * LD];
+            a0 = fmaf(v0, w[s + k], a0);
+            c0 = fmaf(v0, w1[s + k - dl], c0);
+            ++k;
+          }
+          for (; k < len; ++k) c0 = fmaf(p[k * LD], w1[s + k - dl], c0);
+          s += len;
           col = 0;
           ++drow;
         }
         sum = a0 + a1;
+        sum1 = c0 + c1;
         pos = rowsum[ph] > 0.0f;
+        pos1 = rowsum[ph1] > 0.0f;
       } else {
-        float s = 0.0f, ws = 0.0f;
-        if (i < n_out) {
-          const int64_t j0 = q * Dp + b - support + 1;
+        float sv[2] = {0.0f, 0.0f}, wv[2] = {0.0f, 0.0f};
+        for (int o = 0; o < 2; ++o) {
+          if (i + o >= n_out || (o == 1 && !two)) break;
+          const float* wo = o ? w1 : w;
+          const int64_t j0 = q * Dp + b + (o ? dl : 0) - support + 1;
           const int m0 = j0 < 0 ? (int)-j0 : 0;
           const int m1 = (j0 + taps - 1 > n - 1) ? (int)(n - 1 - j0) : taps - 1;
+          float s_ = 0.0f, ws = 0.0f;
           for (int m = m0; m <= m1; ++m) {
-            s += x[j0 + m] * w[m];
-            ws += w[m];
+            s_ += x[j0 + m] * wo[m];
+            ws += wo[m];
           }
+          sv[o] = s_;
+          wv[o] = ws;
         }
-        sum = s;
-        pos = ws > 0.0f;
+        sum = sv[0];
+        sum1 = sv[1];
+        pos = wv[0] > 0.0f;
+        pos1 = wv[1] > 0.0f;
       }
       if (i < n_out) y[i] = pos ? sum * scale : 0.0f;
+      if (two && i + 1 < n_out) y[i + 1] = pos1 ? sum1 * scale : 0.0f;
     }
   }
 }
@@ -556,8 +595,9 @@ bool resample_phase_shape(int source_sr, int target_sr, PhaseShape* sh) {
     const int64_t L = 64 * (int64_t)G * Dp + taps - 1;
     return ((L + Dp - 1) / Dp) | 1;
   };
-  // enough (r, group) tasks for 32 waves, as far as the window fits
-  int G = (int)std::max<int64_t>(1, std::min<int64_t>(16, (32 + P - 1) / P));
+  // enough (phase pair, group) tasks for 32 waves, as far as the window fits
+  const int64_t Ph = (P + 1) / 2;
+  int G = (int)std::max<int64_t>(1, std::min<int64_t>(16, (32 + Ph - 1) / Ph));
   while (G > 1 && Dp * rows_of(G) > kPhaseMaxFloats) --G;
   const int64_t LD = rows_of(G);
   if (Dp * LD > kPhaseMaxFloats || (size_t)(Dp * LD) * sizeof(float) > lds_limit_bytes()) return false;
@@ -595,14 +635,22 @@ hipError_t launch_resample_phase(const float* x, int64_t n, float* y, int64_t n_
   const size_t lds = (size_t)sh.step * sh.ld * sizeof(float);
   const int per_cu = 2 * lds <= lds_limit_bytes() ? 2 : 1;
   const int gr = (int)std::min<int64_t>((int64_t)std::max(grid, 1) * per_cu, n_blocks);
-  static const hipError_t attr = hipFuncSetAttribute(reinterpret_cast<const void*>(k_resample_phase),
-                                                     hipFuncAttributeMaxDynamicSharedMemorySize,
-                                                     (int)lds_limit_bytes());
-  if (attr != hipSuccess) return attr;
+  static const hipError_t attr0 = hipFuncSetAttribute(reinterpret_cast<const void*>(k_resample_phase<0>),
+                                                      hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                      (int)lds_limit_bytes());
+  static const hipError_t attr65 = hipFuncSetAttribute(reinterpret_cast<const void*>(k_resample_phase<65>),
+                                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                       (int)lds_limit_bytes());
+  if (attr0 != hipSuccess) return attr0;
+  if (attr65 != hipSuccess) return attr65;
   const double ratio = (double)target_sr / (double)source_sr;
   const float scale = (float)(ratio < 1.0 ? ratio : 1.0);
-  hipLaunchKernelGGL(k_resample_phase, dim3(gr), dim3(kRsThreads), lds, st, x, n, y, n_out, sh.phases, sh.step,
-                     sh.support, sh.taps, sh.stride, sh.groups, sh.ld, scale, wfull);
+  if (sh.ld == 65)
+    hipLaunchKernelGGL(k_resample_phase<65>, dim3(gr), dim3(kRsThreads), lds, st, x, n, y, n_out, sh.phases,
+                       sh.step, sh.support, sh.taps, sh.stride, sh.groups, sh.ld, scale, wfull);
+  else
+    hipLaunchKernelGGL(k_resample_phase<0>, dim3(gr), dim3(kRsThreads), lds, st, x, n, y, n_out, sh.phases,
+                       sh.step, sh.support, sh.taps, sh.stride, sh.groups, sh.ld, scale, wfull);
   return hipGetLastError();
 }
 
