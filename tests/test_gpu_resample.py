@@ -46,9 +46,10 @@ def _check(got, y, src, dst):
     assert err.max() <= tol, (float(err.max()), tol, int(np.argmax(err)))
 
 
-# (source, target, samples): one-phase pairs (44.1 -> 22.05 kHz) take the
-# polyphase kernel with the row in scalar registers; pairs with 2 .. 4096
-# phases take the phase-major kernel, whose workgroups cover 64 outputs of
+# (source, target, samples): one-phase pairs take the polyphase kernel with
+# the row in scalar registers (decimation by 2: two outputs per lane); pairs
+# with 2 .. 4096 phases take the phase-major kernel (two phases per wave),
+# whose workgroups cover 64 outputs of
 # every phase (64 x 320 input samples at 48 -> 22.05 kHz): the short clips
 # reach only the clipped first and last workgroups, the 150,000-300,000-sample
 # ones the interior workgroups (transposed LDS window) of every layout, input
@@ -58,6 +59,9 @@ CASES = [(44100, 22050, 50000), (22050, 44100, 30000), (48000, 22050, 40000), (1
          (22050, 16000, 20000), (44100, 8000, 30000), (8000, 48000, 5000), (44100, 48000, 30000),
          (44100, 22050, 1), (44100, 22050, 2), (22050, 44100, 3), (48000, 22050, 129),
          (48000, 22050, 300000), (16000, 22050, 150000), (22050, 16000, 150000), (44100, 8000, 300000),
+         # one phase: decimation by 2 (two outputs per lane), 3 (scalar-index
+         # LDS reads) and 4 (float4 reads), long enough for interior chunks
+         (48000, 24000, 100000), (32000, 16000, 70000), (48000, 16000, 90000), (44100, 11025, 90000),
          (44100, 32001, 20000)]   # 10,667 phases: too many rows for LDS, the per-tap kernel
 
 
