@@ -370,13 +370,25 @@ def descriptor_leg(torch, n_samples, iters, with_cpu=False):
     return res
 
 
+FP32_VECTOR_PEAK_TFLOPS = 157.3
+
+
+def _fma_roofline(taps, ms):
+    """FP32 FMA roofline of a launch of `taps` multiply-adds in `ms`."""
+    tf = 2.0 * taps / (ms * 1e-3) / 1e12
+    return {"achieved": tf, "peak": FP32_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": tf / FP32_VECTOR_PEAK_TFLOPS,
+            "frac_of_unpacked_peak": tf / (FP32_VECTOR_PEAK_TFLOPS / 2), "flops_per_launch": 2.0 * taps}
+
+
 def resample_leg(torch, n_samples, iters, with_cpu=False):
     """Sample-rate conversion (mm_resample_device) of the uneven batch joined
     into one clip, taken as 44.1 kHz audio and converted to 22.05 kHz (the
     server's two rates): 256 taps per output. Kernel time from HIP events on
     the launch stream; algorithmic HBM bytes = 4 per input + 4 per output
-    sample (the kernel is bound by its FP64 tap positions and LDS reads, not
-    by HBM)."""
+    sample, algorithmic FLOPs = 2 per tap (one FMA). The kernels are bound by
+    LDS reads and FMA issue, not by HBM: "fma" prices the taps against the
+    FP32 vector peak (157.3 TFLOP/s counts packed FP32, which this build does
+    not emit; one FMA per lane per cycle is half of it)."""
     import ctypes as C
     from metalmom_amd._native import lib
     from metalmom_amd.workloads import uneven_batch
@@ -409,9 +421,10 @@ def resample_leg(torch, n_samples, iters, with_cpu=False):
            "kernel_ms": ms, "out_samples_per_s": m / (ms * 1e-3), "in_samples_per_s": len(y) / (ms * 1e-3),
            "taps_per_s": m * taps / (ms * 1e-3),
            "hbm": {"achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
-                   "bytes_per_launch": bytes_launch}}
+                   "bytes_per_launch": bytes_launch},
+           "fma": _fma_roofline(m * taps, ms)}
     # a pair with many phases: 48 kHz -> 22.05 kHz, 147 phases of 280 taps
-    # (rows 0..73 in LDS, the others read mirrored)
+    # (the phase-major kernel: two phases per wave)
     src2, dst2 = 48000, 22050
     m2 = lib.mm_resample_length(len(y), src2, dst2)
     Y2 = torch.empty(m2, device="cuda")
@@ -431,7 +444,8 @@ def resample_leg(torch, n_samples, iters, with_cpu=False):
     ms2 = e0.elapsed_time(e1) / iters
     taps2 = 2 * int(np.ceil(64 / (dst2 / src2)))
     res["48k_to_22k"] = {"kernel_ms": ms2, "out_samples_per_s": m2 / (ms2 * 1e-3),
-                         "taps_per_s": m2 * taps2 / (ms2 * 1e-3), "taps": taps2, "phases": 147}
+                         "taps_per_s": m2 * taps2 / (ms2 * 1e-3), "taps": taps2, "phases": 147,
+                         "fma": _fma_roofline(m2 * taps2, ms2)}
     del X, Y, Y2
     if with_cpu:
         from oracle.cpu_baseline import ResampleF32
@@ -651,9 +665,10 @@ def main():
                           not args.no_cpu and not args.profile)
         extra["resample"] = rs
         log(f"resample 44.1 -> 22.05 kHz: {rs['out_samples_per_s'] / 1e9:.2f} G output samples/s, "
-            f"kernel {rs['kernel_ms']:.3f} ms, {rs['taps_per_s'] / 1e12:.2f} T taps/s; 48 -> 22.05 kHz (147 phases) "
+            f"kernel {rs['kernel_ms']:.3f} ms, {rs['taps_per_s'] / 1e12:.2f} T taps/s "
+            f"({100 * rs['fma']['frac']:.1f} % of FP32 FMA peak); 48 -> 22.05 kHz (147 phases) "
             f"{rs['48k_to_22k']['out_samples_per_s'] / 1e9:.2f} G output samples/s, "
-            f"{rs['48k_to_22k']['kernel_ms']:.3f} ms"
+            f"{rs['48k_to_22k']['kernel_ms']:.3f} ms ({100 * rs['48k_to_22k']['fma']['frac']:.1f} %)"
             + (f"; cpu {rs['cpu_baseline']['value'] / 1e6:.2f} M samples/s" if "cpu_baseline" in rs else ""))
 
     cpu = None
