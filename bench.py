@@ -39,6 +39,8 @@ def log(*a):
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_tile_mfcc_bf16_uneven_batch_traffic.json")
 ISTFT_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_istft_tile_2048_long_clip_traffic.json")
+RESAMPLE_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_resample_dec2_44k_to_22k_traffic.json")
+RESAMPLE_PHASE_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_resample_phase_48k_to_22k_traffic.json")
 
 
 def measured_traffic(bytes_launch, path=TRAFFIC_FILE):
@@ -421,7 +423,7 @@ def resample_leg(torch, n_samples, iters, with_cpu=False):
            "kernel_ms": ms, "out_samples_per_s": m / (ms * 1e-3), "in_samples_per_s": len(y) / (ms * 1e-3),
            "taps_per_s": m * taps / (ms * 1e-3),
            "hbm": {"achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
-                   "bytes_per_launch": bytes_launch},
+                   "traffic": measured_traffic(bytes_launch, RESAMPLE_TRAFFIC_FILE), "bytes_per_launch": bytes_launch},
            "fma": _fma_roofline(m * taps, ms)}
     # a pair with many phases: 48 kHz -> 22.05 kHz, 147 phases of 280 taps
     # (the phase-major kernel: two phases per wave)
@@ -446,6 +448,11 @@ def resample_leg(torch, n_samples, iters, with_cpu=False):
     res["48k_to_22k"] = {"kernel_ms": ms2, "out_samples_per_s": m2 / (ms2 * 1e-3),
                          "taps_per_s": m2 * taps2 / (ms2 * 1e-3), "taps": taps2, "phases": 147,
                          "fma": _fma_roofline(m2 * taps2, ms2)}
+    bytes2 = 4 * (len(y) + m2)
+    gbs2 = bytes2 / (ms2 * 1e-3) / 1e9
+    res["48k_to_22k"]["hbm"] = {"achieved": gbs2, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs2 / HBM_PEAK_GBS,
+                                "traffic": measured_traffic(bytes2, RESAMPLE_PHASE_TRAFFIC_FILE),
+                                "bytes_per_launch": bytes2}
     del X, Y, Y2
     if with_cpu:
         from oracle.cpu_baseline import ResampleF32
