@@ -338,8 +338,28 @@ int32_t mm_batch_run(mm_batch b, const void* d_signal, float* d_out, void* hip_s
 int32_t mm_batch_run_graph(mm_batch b, const void* d_signal, float* d_out, void* hip_stream);
 /* Persistent grid: workgroups per CU (0: autotuned or default). */
 int32_t mm_batch_set_grid(mm_batch b, int32_t wg_per_cu);
-/* Times each candidate grid on this batch, keeps the fastest and records it
- * in the tuning file ($METALMOM_AMD_TUNE_FILE or ~/.cache/metalmom_amd_tuning.json). */
+/* Workgroup shapes of the fused kernel: MM_SHAPE_W16 = 16 wavefronts, one
+ * frame each, 16-frame tiles (n_fft <= 2048); MM_SHAPE_W8 = 8 wavefronts,
+ * 8-frame tiles; MM_SHAPE_W8F2 = 8 wavefronts with two frames each, 16-frame
+ * tiles, two workgroups per CU (n_fft 1024 and 2048). Results do not depend
+ * on the shape. */
+#define MM_SHAPE_DEFAULT 0
+#define MM_SHAPE_W16 1
+#define MM_SHAPE_W8 2
+#define MM_SHAPE_W8F2 3
+/* Rebuilds the plan's tile list for `shape` (synchronises the plan's
+ * stream); -1 for a shape the configuration does not offer. */
+int32_t mm_batch_set_shape(mm_batch b, int32_t shape);
+/* What the next step launches: shape, wavefronts per workgroup, frames per
+ * tile, persistent workgroups and the contiguous tiles each one processes
+ * (all 0 for configurations outside the fused kernel). Any pointer may be
+ * NULL. */
+int32_t mm_batch_launch_shape(mm_batch b, int32_t* shape, int32_t* waves_per_wg, int32_t* tile_frames,
+                              int32_t* grid, int32_t* tiles_per_wg);
+/* Times every (workgroup shape, persistent grid of 1..8 workgroups per CU)
+ * on this batch, keeps the fastest and records it in the tuning file
+ * ($METALMOM_AMD_TUNE_FILE or ~/.cache/metalmom_amd_tuning.json); new plans
+ * of the same configuration start from the recorded shape and grid. */
 int32_t mm_batch_autotune(mm_batch b, const void* d_signal, float* d_out, int32_t iters,
                           int32_t* best_wg_per_cu, float* best_ms);
 /* Average device time of the fused phase-1 kernel and of the whole step over
@@ -383,11 +403,6 @@ int32_t mm_server_stats(mm_server s, int64_t* submitted, int64_t* completed, int
                         int64_t* frames);
 /* Finishes queued work, then releases the server. */
 int32_t mm_server_destroy(mm_server s);
-
-/* Instrumented builds (-DMM_PHASE_TIMING) only: device buffer receiving
- * per-wave cycle sums of the fused kernel's phases, [grid][waves][6] int64.
- * A no-op for the kernels of a normal build. */
-int32_t mm_debug_set_phase_buffer(void* d_buf);
 
 /* Device description: name, gfx arch and compute-unit count. */
 int32_t mm_device_info(char* name, int32_t name_len, char* arch, int32_t arch_len, int32_t* n_cu);

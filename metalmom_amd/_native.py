@@ -101,11 +101,13 @@ SIGNATURES = {
     "mm_batch_run": (_I32, [_P, _P, _P, _P]),
     "mm_batch_run_graph": (_I32, [_P, _P, _P, _P]),
     "mm_batch_set_grid": (_I32, [_P, _I32]),
+    "mm_batch_set_shape": (_I32, [_P, _I32]),
+    "mm_batch_launch_shape": (_I32, [_P, C.POINTER(_I32), C.POINTER(_I32), C.POINTER(_I32), C.POINTER(_I32),
+                                     C.POINTER(_I32)]),
     "mm_batch_autotune": (_I32, [_P, _P, _P, _I32, C.POINTER(_I32), C.POINTER(C.c_float)]),
     "mm_batch_time": (_I32, [_P, _P, _P, _P, _I32, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "mm_batch_destroy": (_I32, [_P]),
     "mm_device_info": (_I32, [C.c_char_p, _I32, C.c_char_p, _I32, C.POINTER(_I32)]),
-    "mm_debug_set_phase_buffer": (_I32, [_P]),
     "mm_server_create": (_I32, [_P, _I64, _I32, C.POINTER(_P)]),
     "mm_server_submit": (_I64, [_P, C.POINTER(MMSpecParams), _F, _I64]),
     "mm_server_submit_pcm": (_I64, [_P, C.POINTER(MMSpecParams), _P, _I64, _I32, _I32, _I32, _I32]),

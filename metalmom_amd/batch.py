@@ -67,8 +67,27 @@ class SpectralBatch:
         if lib.mm_batch_set_grid(self._h, int(wg_per_cu)) != 0:
             raise RuntimeError("mm_batch_set_grid failed")
 
+    def set_shape(self, shape: int):
+        """Workgroup shape of the fused kernel (1: 16 waves x 1 frame, 2: 8 x 1,
+        3: 8 x 2 frames, two workgroups per CU)."""
+        s = lib.mm_batch_set_shape(self._h, int(shape))
+        if s != 0:
+            raise RuntimeError(f"mm_batch_set_shape failed with status {s}")
+
+    def launch_shape(self) -> dict:
+        """What the next step launches: shape, waves per workgroup, frames per
+        tile, persistent workgroups and tiles per workgroup."""
+        v = [C.c_int32() for _ in range(5)]
+        s = lib.mm_batch_launch_shape(self._h, *[C.byref(x) for x in v])
+        if s != 0:
+            raise RuntimeError(f"mm_batch_launch_shape failed with status {s}")
+        keys = ("shape", "waves_per_wg", "tile_frames", "grid", "tiles_per_wg")
+        return dict(zip(keys, (x.value for x in v)))
+
     def autotune(self, d_signal, d_out, iters=3):
-        """Project autotuner: times candidate persistent grids, keeps the best."""
+        """Project autotuner: times every (workgroup shape, persistent grid)
+        pair on this batch and keeps the fastest; returns (workgroups per CU,
+        step ms)."""
         best, ms = C.c_int32(), C.c_float()
         s = lib.mm_batch_autotune(self._h, self._ptr(d_signal), self._ptr(d_out), int(iters),
                                   C.byref(best), C.byref(ms))

@@ -29,22 +29,51 @@ struct Batch {
   Lane* lane = nullptr;  // own stream, metadata and scratch
   MetaPtrs mp{};
   int wg_per_cu = 0;
+  int shape = SHAPE_DEFAULT;   // workgroup shape the tile list was built for
+  std::vector<int64_t> lens;
   std::vector<GraphEntry> graphs;
   ~Batch() {
-    for (auto& g : graphs) {
-      (void)hipGraphExecDestroy(g.exec);
-      (void)hipGraphDestroy(g.graph);
-    }
+    drop_graphs();
     if (lane) {
       (void)hipStreamSynchronize(lane->stream);
       Engine::get()->release_lane(lane);
     }
+  }
+  void drop_graphs() {   // captured launches carry the old grid and tile list
+    for (auto& g : graphs) {
+      (void)hipGraphExecDestroy(g.exec);
+      (void)hipGraphDestroy(g.graph);
+    }
+    graphs.clear();
   }
   int grid() const {
     if (wg_per_cu <= 0) return 0;
     return Engine::get()->num_cus() * wg_per_cu;
   }
 };
+
+// Builds and uploads the tile list for `shape` (synchronous on the lane's
+// stream, so no earlier step still reads the old list).
+int32_t set_batch_shape(Batch* b, int shape) {
+  TileLayout L;
+  const bool fused = tile_layout(b->r.n_fft, b->r.out_kind, b->r.precision, b->r.n_mels,
+                                 std::min(b->r.n_mfcc, b->r.n_mels), &L, shape);
+  if (!fused && shape != SHAPE_DEFAULT) return ERR_INVALID;
+  if (hipStreamSynchronize(b->lane->stream) != hipSuccess) return ERR_INTERNAL;
+  b->drop_graphs();
+  build_meta(b->r, b->lens.data(), (int)b->lens.size(), fused ? L.tile_frames : kTileFrames, &b->m);
+  const size_t mb = meta_bytes(b->m);
+  if (b->lane->d_meta.ensure(mb) != hipSuccess || b->lane->h_meta.ensure(mb) != hipSuccess) return ERR_ALLOC;
+  b->lane->meta_key.clear();
+  pack_meta(b->m, b->lane->h_meta.p);
+  if (hipMemcpyAsync(b->lane->d_meta.p, b->lane->h_meta.p, mb, hipMemcpyHostToDevice, b->lane->stream) !=
+          hipSuccess ||
+      hipStreamSynchronize(b->lane->stream) != hipSuccess)
+    return ERR_INTERNAL;
+  b->mp = meta_ptrs(b->m, b->lane->d_meta.p);
+  b->shape = fused ? L.shape : SHAPE_DEFAULT;
+  return prepare_batch(b->r, b->shape);   // kernel attributes, ahead of any stream capture
+}
 
 
 int32_t create_batch(const MMSpecParams* params, const int64_t* lengths, int32_t n_clips, Batch** out) {
@@ -61,20 +90,14 @@ int32_t create_batch(const MMSpecParams* params, const int64_t* lengths, int32_t
   if (!b->lane) return ERR_INTERNAL;
   s = e->tables_for(b->r, b->lane->stream, &b->tb);
   if (s != OK) return s;
+  b->lens.assign(lengths, lengths + n_clips);
+  // the autotuned shape for this configuration, if there is one
+  int shape = e->tuned_shape(b->r);
   TileLayout L;
-  const bool fused =
-      tile_layout(b->r.n_fft, b->r.out_kind, b->r.precision, b->r.n_mels, std::min(b->r.n_mfcc, b->r.n_mels), &L);
-  build_meta(b->r, lengths, n_clips, fused ? L.tile_frames : kTileFrames, &b->m);
-  const size_t mb = meta_bytes(b->m);
-  if (b->lane->d_meta.ensure(mb) != hipSuccess || b->lane->h_meta.ensure(mb) != hipSuccess) return ERR_ALLOC;
-  b->lane->meta_key.clear();
-  pack_meta(b->m, b->lane->h_meta.p);
-  if (hipMemcpyAsync(b->lane->d_meta.p, b->lane->h_meta.p, mb, hipMemcpyHostToDevice, b->lane->stream) !=
-          hipSuccess ||
-      hipStreamSynchronize(b->lane->stream) != hipSuccess)
-    return ERR_INTERNAL;
-  b->mp = meta_ptrs(b->m, b->lane->d_meta.p);
-  s = prepare_batch(b->r);
+  if (shape != SHAPE_DEFAULT && !tile_layout(b->r.n_fft, b->r.out_kind, b->r.precision, b->r.n_mels,
+                                             std::min(b->r.n_mfcc, b->r.n_mels), &L, shape))
+    shape = SHAPE_DEFAULT;
+  s = set_batch_shape(b.get(), shape);
   if (s != OK) return s;
   *out = b.release();
   return OK;
@@ -83,7 +106,7 @@ int32_t create_batch(const MMSpecParams* params, const int64_t* lengths, int32_t
 int32_t run_batch(Batch* b, const void* d_signal, float* d_out, hipStream_t st, hipEvent_t e0 = nullptr,
                   hipEvent_t e1 = nullptr) {
   return enqueue_batch(Engine::get(), b->r, b->tb, b->m, b->mp, d_signal, d_out, &b->lane->d_scratch,
-                       &b->lane->d_scratch2, b->grid(), st, e0, e1);
+                       &b->lane->d_scratch2, b->grid(), st, e0, e1, b->shape);
 }
 
 }  // namespace
@@ -160,7 +183,8 @@ int32_t mm_batch_run_graph(mm_batch bh, const void* d_signal, float* d_out, void
   // the generic (non-fused) path grows scratch buffers on the fly, which
   // stream capture does not allow: it runs eagerly
   TileLayout L;
-  if (!tile_layout(b->r.n_fft, b->r.out_kind, b->r.precision, b->r.n_mels, std::min(b->r.n_mfcc, b->r.n_mels), &L))
+  if (!tile_layout(b->r.n_fft, b->r.out_kind, b->r.precision, b->r.n_mels, std::min(b->r.n_mfcc, b->r.n_mels), &L,
+                   b->shape))
     return run_batch(b, d_signal, d_out, st);
   GraphEntry g{d_signal, d_out, st, nullptr, nullptr};
   if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return MM_ERR_INTERNAL;
@@ -191,11 +215,7 @@ int32_t mm_batch_set_grid(mm_batch bh, int32_t wg_per_cu) try {
   Batch* b = static_cast<Batch*>(bh);
   if (!b || wg_per_cu < 0 || wg_per_cu > 64) return MM_ERR_INVALID_INPUT;
   b->wg_per_cu = wg_per_cu;
-  for (auto& g : b->graphs) {  // captured launches carry the old grid
-    (void)hipGraphExecDestroy(g.exec);
-    (void)hipGraphDestroy(g.graph);
-  }
-  b->graphs.clear();
+  b->drop_graphs();
   return MM_OK;
 } catch (const std::bad_alloc&) {   // no exception leaves the C ABI
   return MM_ERR_ALLOC_FAILED;
@@ -242,26 +262,51 @@ int32_t mm_batch_autotune(mm_batch bh, const void* d_signal, float* d_out, int32
                           float* best_ms) try {
   Batch* b = static_cast<Batch*>(bh);
   if (!b || iters <= 0) return MM_ERR_INVALID_INPUT;
+  // Candidates: every workgroup shape the fused kernel offers for this FFT
+  // size (tile frames and waves per workgroup), each with persistent grids
+  // of 1..8 workgroups per CU, timed as whole steps on this batch. Replaces
+  // fixed per-device thresholds with a measurement on the device at hand.
+  int shapes[4];
+  int ns = tile_shapes(b->r.n_fft, shapes);
+  TileLayout L;
+  if (!tile_layout(b->r.n_fft, b->r.out_kind, b->r.precision, b->r.n_mels, std::min(b->r.n_mfcc, b->r.n_mels), &L))
+    ns = 0;   // generic path: only the grid matters (and it is not used)
+  if (ns == 0) {
+    shapes[0] = SHAPE_DEFAULT;
+    ns = 1;
+  }
   const int cands[] = {1, 2, 3, 4, 6, 8};
-  const int saved = b->wg_per_cu;
-  int best = 0;
+  const int saved_wg = b->wg_per_cu, saved_shape = b->shape;
+  int best = 0, best_shape = SHAPE_DEFAULT;
   float best_t = 1e30f;
-  for (int c : cands) {
-    b->wg_per_cu = c;
-    float k = 0.f, t = 0.f;
-    int32_t s = mm_batch_time(b, d_signal, d_out, nullptr, 1, &k, &t);  // warm-up
-    if (s == OK) s = mm_batch_time(b, d_signal, d_out, nullptr, iters, &k, &t);
+  for (int si = 0; si < ns; ++si) {
+    if (!tile_layout(b->r.n_fft, b->r.out_kind, b->r.precision, b->r.n_mels, std::min(b->r.n_mfcc, b->r.n_mels),
+                     &L, shapes[si]) &&
+        shapes[si] != SHAPE_DEFAULT)
+      continue;   // this kind or precision does not fit the shape's LDS layout
+    int32_t s = set_batch_shape(b, shapes[si]);
+    for (int c : cands) {
+      if (s != OK) break;
+      b->wg_per_cu = c;
+      float k = 0.f, t = 0.f;
+      s = mm_batch_time(b, d_signal, d_out, nullptr, 1, &k, &t);  // warm-up
+      if (s == OK) s = mm_batch_time(b, d_signal, d_out, nullptr, iters, &k, &t);
+      if (s == OK && t < best_t) {
+        best_t = t;
+        best = c;
+        best_shape = b->shape;
+      }
+    }
     if (s != OK) {
-      b->wg_per_cu = saved;
+      b->wg_per_cu = saved_wg;
+      (void)set_batch_shape(b, saved_shape);
       return s;
     }
-    if (t < best_t) {
-      best_t = t;
-      best = c;
-    }
   }
+  int32_t s = set_batch_shape(b, best_shape);
+  if (s != OK) return s;
   mm_batch_set_grid(b, best);
-  Engine::get()->record_tuning(b->r, best);
+  Engine::get()->record_tuning(b->r, best_shape, best);
   if (best_wg_per_cu) *best_wg_per_cu = best;
   if (best_ms) *best_ms = best_t;
   return MM_OK;
@@ -271,6 +316,39 @@ int32_t mm_batch_autotune(mm_batch bh, const void* d_signal, float* d_out, int32
   return MM_ERR_INTERNAL;
 }
 
+int32_t mm_batch_set_shape(mm_batch bh, int32_t shape) try {
+  Batch* b = static_cast<Batch*>(bh);
+  if (!b || shape < 0) return MM_ERR_INVALID_INPUT;
+  if (shape == b->shape) return MM_OK;
+  return set_batch_shape(b, shape);
+} catch (const std::bad_alloc&) {   // no exception leaves the C ABI
+  return MM_ERR_ALLOC_FAILED;
+} catch (...) {
+  return MM_ERR_INTERNAL;
+}
+
+int32_t mm_batch_launch_shape(mm_batch bh, int32_t* shape, int32_t* waves_per_wg, int32_t* tile_frames,
+                              int32_t* grid, int32_t* tiles_per_wg) try {
+  Batch* b = static_cast<Batch*>(bh);
+  if (!b) return MM_ERR_INVALID_INPUT;
+  TileLayout L;
+  const bool fused = tile_layout(b->r.n_fft, b->r.out_kind, b->r.precision, b->r.n_mels,
+                                 std::min(b->r.n_mfcc, b->r.n_mels), &L, b->shape);
+  int tpw = 0, g = 0;
+  if (fused) g = launch_grid(Engine::get(), b->r, b->m.n_tiles, L, b->grid(), &tpw);
+  if (shape) *shape = fused ? L.shape : SHAPE_DEFAULT;
+  if (waves_per_wg) *waves_per_wg = fused ? L.nw : 0;
+  if (tile_frames) *tile_frames = fused ? L.tile_frames : 0;
+  if (grid) *grid = g;
+  if (tiles_per_wg) *tiles_per_wg = tpw;
+  return MM_OK;
+} catch (const std::bad_alloc&) {   // no exception leaves the C ABI
+  return MM_ERR_ALLOC_FAILED;
+} catch (...) {
+  return MM_ERR_INTERNAL;
+}
+
+#ifdef MM_PHASE_TIMING
 // Instrumented builds only: device buffer for per-wave phase cycle sums.
 int32_t mm_debug_set_phase_buffer(void* d_buf) try {
   set_phase_cycles_buffer(static_cast<long long*>(d_buf));
@@ -280,6 +358,7 @@ int32_t mm_debug_set_phase_buffer(void* d_buf) try {
 } catch (...) {
   return MM_ERR_INTERNAL;
 }
+#endif
 
 int32_t mm_batch_destroy(mm_batch bh) try {
   Batch* b = static_cast<Batch*>(bh);

@@ -38,6 +38,11 @@ __host__ __device__ inline int pcm_bytes(int fmt) {
 inline bool pcm_valid(int fmt) { return fmt >= PCM_U8 && fmt <= PCM_F64; }
 
 constexpr int kTileFrames = 16;   // frames per tile = MFMA N dimension
+// LDS copy of the banded filterbank: per chunk 8 rows of 64 bytes of weights,
+// each row padded to 80 bytes so the 16-byte operand reads of lanes 0..7
+// (rows 0..7) fall on distinct banks
+constexpr int kFbRowBytes = 80;
+constexpr int kFbChunkBytes = 8 * kFbRowBytes;
 constexpr int kMelBlock = 8;      // mel rows per filterbank block of the fused contraction
 constexpr int kMaxMelBlocks = 32; // n_mels <= 256 on the fused path
 
@@ -105,6 +110,7 @@ struct TileArgs {
   int ndt, n_mfcc;
   // value transforms
   float power, ref, amin, top_db;
+  float db_off;                // 10 log10(ref): dB = 10 log10(2) log2(max(mel, amin)) - db_off
   float post_scale;            // 2^-power: the fused kernel contracts (2|X|)^power
   int use_top_db;
   int phase;                   // 1: unclamped pass, 2: recompute tiles whose clamp is active

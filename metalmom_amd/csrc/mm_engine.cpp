@@ -394,7 +394,7 @@ std::string Engine::tuning_path() const {
   return std::string(home ? home : "/tmp") + "/.cache/metalmom_amd_tuning.json";
 }
 
-// The file is a flat JSON object {"key": wg_per_cu, ...}.
+// The file is a flat JSON object {"key": integer, ...}.
 void Engine::load_tuning() {
   if (tune_loaded_) return;
   tune_loaded_ = true;
@@ -424,35 +424,49 @@ void Engine::save_tuning() {
   std::ofstream f(path);
   if (!f) return;
   f << "{";
-  for (size_t i = 0; i < tune_.size(); ++i) f << (i ? ", " : "") << "\"" << tune_[i].key << "\": " << tune_[i].wg_per_cu;
+  for (size_t i = 0; i < tune_.size(); ++i) f << (i ? ", " : "") << "\"" << tune_[i].first << "\": " << tune_[i].second;
   f << "}\n";
 }
 
-int Engine::tuned_wg_per_cu(const SpecRequest& r) {
-  std::lock_guard<std::mutex> g(mu_);
-  load_tuning();
-  const std::string k = tune_key(r, arch_);
-  for (auto& e : tune_)
-    if (e.key == k) return e.wg_per_cu;
+// Tuning entries: "<key>:shape" -> the winning TileShape, "<key>:s<shape>"
+// -> workgroups per CU for that shape.
+static int tune_lookup(const std::vector<std::pair<std::string, int>>& t, const std::string& k) {
+  for (auto& e : t)
+    if (e.first == k) return e.second;
   return 0;
 }
 
-void Engine::record_tuning(const SpecRequest& r, int wg_per_cu) {
+int Engine::tuned_wg_per_cu(const SpecRequest& r, int shape) {
   std::lock_guard<std::mutex> g(mu_);
   load_tuning();
-  const std::string k = tune_key(r, arch_);
-  bool found = false;
-  for (auto& e : tune_)
-    if (e.key == k) {
-      e.wg_per_cu = wg_per_cu;
-      found = true;
-    }
-  if (!found) tune_.push_back({k, wg_per_cu});
+  return tune_lookup(tune_, tune_key(r, arch_) + ":s" + std::to_string(shape));
+}
+
+int Engine::tuned_shape(const SpecRequest& r) {
+  std::lock_guard<std::mutex> g(mu_);
+  load_tuning();
+  return tune_lookup(tune_, tune_key(r, arch_) + ":shape");
+}
+
+void Engine::record_tuning(const SpecRequest& r, int shape, int wg_per_cu) {
+  std::lock_guard<std::mutex> g(mu_);
+  load_tuning();
+  const std::string base = tune_key(r, arch_);
+  auto put = [&](const std::string& k, int v) {
+    for (auto& e : tune_)
+      if (e.first == k) {
+        e.second = v;
+        return;
+      }
+    tune_.push_back({k, v});
+  };
+  put(base + ":shape", shape);
+  put(base + ":s" + std::to_string(shape), wg_per_cu);
   save_tuning();
 }
 
 int Engine::grid_for(const SpecRequest& r, int n_tiles, const TileLayout& L) {
-  int per_cu = tuned_wg_per_cu(r);
+  int per_cu = tuned_wg_per_cu(r, L.shape);
   if (per_cu <= 0) {
     // default: as many workgroups as fit by LDS (at most 4 per CU)
     per_cu = (int)std::max<size_t>(1, std::min<size_t>(4, lds_limit_bytes() / std::max<size_t>(L.lds_bytes, 1)));
@@ -561,8 +575,19 @@ static long long* g_phase_cycles = nullptr;
 long long* phase_cycles_buffer() { return g_phase_cycles; }
 void set_phase_cycles_buffer(long long* p) { g_phase_cycles = p; }
 
+int launch_grid(Engine* eng, const SpecRequest& r, int n_tiles, const TileLayout& L, int grid_override,
+                int* tiles_per_wg) {
+  if (n_tiles <= 0) {
+    *tiles_per_wg = 0;
+    return 0;
+  }
+  const int grid = grid_override > 0 ? std::min(grid_override, n_tiles) : eng->grid_for(r, n_tiles, L);
+  *tiles_per_wg = (n_tiles + grid - 1) / std::max(grid, 1);
+  return (n_tiles + *tiles_per_wg - 1) / *tiles_per_wg;
+}
+
 static TileArgs make_args(const SpecRequest& r, const Tables* tb, const BatchMeta& m, const MetaPtrs& mp,
-                          const void* d_signal, void* d_out, const TileLayout& L, int grid) {
+                          const void* d_signal, void* d_out, const TileLayout& L, int tiles_per_wg) {
   const DeviceTables& t = tb->t;
   TileArgs a;
   std::memset(&a, 0, sizeof(a));
@@ -575,7 +600,7 @@ static TileArgs make_args(const SpecRequest& r, const Tables* tb, const BatchMet
   a.tile_f0 = mp.tile_f0;
   a.tile_pf = mp.tile_pf;
   a.n_tiles = m.n_tiles;
-  a.tiles_per_wg = (m.n_tiles + grid - 1) / std::max(grid, 1);
+  a.tiles_per_wg = tiles_per_wg;
   a.n_fft = r.n_fft;
   a.hop = r.hop;
   a.center = r.center;
@@ -602,6 +627,7 @@ static TileArgs make_args(const SpecRequest& r, const Tables* tb, const BatchMet
   a.power = r.power;
   a.post_scale = std::exp2(-r.power);
   a.ref = r.ref;
+  a.db_off = (float)(10.0 * std::log10((double)r.ref));
   a.amin = r.amin;
   a.top_db = r.top_db;
   a.use_top_db = (r.out_kind == OUT_MFCC && r.top_db > 0.0f) ? 1 : 0;
@@ -654,9 +680,9 @@ static int32_t enqueue_generic(const SpecRequest& r, const Tables* tb, const Bat
   return OK;
 }
 
-int32_t prepare_batch(const SpecRequest& r) {
+int32_t prepare_batch(const SpecRequest& r, int shape) {
   TileLayout L;
-  if (!tile_layout(r.n_fft, r.out_kind, r.precision, r.n_mels, std::min(r.n_mfcc, r.n_mels), &L)) return OK;
+  if (!tile_layout(r.n_fft, r.out_kind, r.precision, r.n_mels, std::min(r.n_mfcc, r.n_mels), &L, shape)) return OK;
   TileArgs a;
   std::memset(&a, 0, sizeof(a));
   a.n_fft = r.n_fft;
@@ -668,17 +694,22 @@ int32_t prepare_batch(const SpecRequest& r) {
 
 int32_t enqueue_batch(Engine* eng, const SpecRequest& r, const Tables* tb, const BatchMeta& m, const MetaPtrs& mp,
                       const void* d_signal, void* d_out, DevBuf* scratch, DevBuf* scratch2, int grid_override,
-                      hipStream_t st, hipEvent_t ev_p1_begin, hipEvent_t ev_p1_end) {
+                      hipStream_t st, hipEvent_t ev_p1_begin, hipEvent_t ev_p1_end, int shape) {
   if (m.n_tiles == 0) return OK;
   TileLayout L;
-  const bool fused = tile_layout(r.n_fft, r.out_kind, r.precision, r.n_mels, std::min(r.n_mfcc, r.n_mels), &L);
+  const bool fused =
+      tile_layout(r.n_fft, r.out_kind, r.precision, r.n_mels, std::min(r.n_mfcc, r.n_mels), &L, shape);
   if (!fused) {
+    // a shape the fused kernel does not offer for this configuration
+    if (shape != SHAPE_DEFAULT && tile_layout(r.n_fft, r.out_kind, r.precision, r.n_mels,
+                                              std::min(r.n_mfcc, r.n_mels), &L))
+      return ERR_INVALID;
     if (r.precision != PREC_F32) return ERR_INVALID;
     return enqueue_generic(r, tb, m, d_signal, d_out, scratch, scratch2, st);
   }
-  const int grid = grid_override > 0 ? std::min(grid_override, m.n_tiles) : eng->grid_for(r, m.n_tiles, L);
-  TileArgs a = make_args(r, tb, m, mp, d_signal, d_out, L, grid);
-  const int g = (m.n_tiles + a.tiles_per_wg - 1) / a.tiles_per_wg;
+  int tpw = 0;
+  const int g = launch_grid(eng, r, m.n_tiles, L, grid_override, &tpw);
+  TileArgs a = make_args(r, tb, m, mp, d_signal, d_out, L, tpw);
   // LDS copies of the filterbank band and the DCT operands, when they fit
   // beside the kernel's own layout
   TileLayout LL = L;
@@ -686,9 +717,10 @@ int32_t enqueue_batch(Engine* eng, const SpecRequest& r, const Tables* tb, const
     const int pc = (r.precision == PREC_F32) ? 0 : 1;
     const int nch = tb->cache_chunks[pc];
     const size_t off = (L.lds_bytes + 255) / 256 * 256;
-    const size_t fbb = (size_t)nch * 512;
+    const size_t fbb = (size_t)nch * kFbChunkBytes;
     const size_t dcb = (r.out_kind == OUT_MFCC) ? (size_t)tb->t.ndt * tb->t.nmh * 1024 : 0;
-    if (nch > 0 && off + fbb + dcb <= lds_limit_bytes()) {
+    // (within the LDS share of the workgroups per CU the shape is sized for)
+    if (nch > 0 && off + fbb + dcb <= lds_limit_bytes() / std::max(L.occ, 1)) {
       a.fb_lds_off = (int)off;
       a.dct_lds_off = dcb ? (int)(off + fbb) : 0;
       a.cache_src = tb->t.cache_src + pc * kMaxCacheChunks;
@@ -788,7 +820,7 @@ int32_t enqueue_descriptor(Lane* lane, const SpecRequest& r, const float* d_x, i
     if (s != OK) return s;
     const MetaPtrs mp = meta_ptrs(m, lane->d_meta.p);
     const int grid = std::max(1, std::min(m.n_tiles, eng->num_cus()));
-    const TileArgs a = make_args(r, tb, m, mp, d_x, d_res, L, grid);
+    const TileArgs a = make_args(r, tb, m, mp, d_x, d_res, L, (m.n_tiles + grid - 1) / grid);
     const int g = (m.n_tiles + a.tiles_per_wg - 1) / a.tiles_per_wg;
     MM_HIP(launch_desc_fused(a, kind, (float)r.sr, (float)r.n_fft, param, d_res, g, st));
     return OK;

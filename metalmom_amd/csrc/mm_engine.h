@@ -96,10 +96,13 @@ class Engine {
   // use; nullptr when the rate pair does not fit it).
   int32_t resample_phase_weights(int source_sr, int target_sr, hipStream_t st, const float** out);
 
-  // Persistent grid size: autotuned if known, else a default.
+  // Persistent grid size: autotuned for this shape if known, else a default.
   int grid_for(const SpecRequest& r, int n_tiles, const TileLayout& L);
-  void record_tuning(const SpecRequest& r, int wg_per_cu);
-  int tuned_wg_per_cu(const SpecRequest& r);
+  // Records the autotuner's winner: its workgroup shape and, for that shape,
+  // the workgroups per CU.
+  void record_tuning(const SpecRequest& r, int shape, int wg_per_cu);
+  int tuned_wg_per_cu(const SpecRequest& r, int shape);
+  int tuned_shape(const SpecRequest& r);   // SHAPE_DEFAULT when not tuned
   std::string tuning_path() const;
   void load_tuning();
   void save_tuning();
@@ -120,11 +123,7 @@ class Engine {
     DevBuf w;
   };
   std::vector<std::unique_ptr<PolyEntry>> poly_;
-  struct TuneEntry {
-    std::string key;
-    int wg_per_cu;
-  };
-  std::vector<TuneEntry> tune_;
+  std::vector<std::pair<std::string, int>> tune_;   // tuning file entries (key, value)
   bool tune_loaded_ = false;
 };
 
@@ -159,16 +158,23 @@ struct MetaPtrs {
 };
 MetaPtrs meta_ptrs(const BatchMeta& m, void* dev);
 
+// Workgroups of a fused launch over n_tiles tiles (grid_override > 0:
+// workgroups requested, else the tuned or default grid) and the contiguous
+// tiles each one processes.
+int launch_grid(Engine* eng, const SpecRequest& r, int n_tiles, const TileLayout& L, int grid_override,
+                int* tiles_per_wg);
+// `shape`: the TileShape the metadata's tiles were built for.
 int32_t enqueue_batch(Engine* eng, const SpecRequest& r, const Tables* t, const BatchMeta& m, const MetaPtrs& mp,
                       const void* d_signal, void* d_out, DevBuf* scratch, DevBuf* scratch2, int grid_override,
-                      hipStream_t st, hipEvent_t ev_p1_begin = nullptr, hipEvent_t ev_p1_end = nullptr);
+                      hipStream_t st, hipEvent_t ev_p1_begin = nullptr, hipEvent_t ev_p1_end = nullptr,
+                      int shape = SHAPE_DEFAULT);
 
 // Device buffer for the phase timing of instrumented builds (null: off).
 long long* phase_cycles_buffer();
 void set_phase_cycles_buffer(long long* p);
 
 // Sets the fused kernel's LDS attribute ahead of any stream capture.
-int32_t prepare_batch(const SpecRequest& r);
+int32_t prepare_batch(const SpecRequest& r, int shape = SHAPE_DEFAULT);
 
 // Single clip through pinned staging on the lane's stream (synchronous).
 // On success *out is malloc'd (caller frees) and rows/cols are set.

@@ -8,6 +8,7 @@
 
 #include "mm_common.h"
 
+#include <climits>
 #include <type_traits>
 
 namespace mm {
@@ -134,6 +135,27 @@ __device__ __forceinline__ TilePf ldc_pf(const TilePf* p, int64_t i) {
   r.mask = ldc(&p[i].mask, 0);
   r.pad = 0;
   return r;
+}
+
+// Two floats -> their 16-bit (bf16 / f16, round to nearest even) hi parts
+// and the 16-bit rounding of what each hi part leaves (lo), packed in pairs
+// (element 0 in the low half): d ~ hi + lo to about twice the 16-bit
+// precision.
+template <int PREC>
+__device__ __forceinline__ void split16(float d0, float d1, unsigned& hi, unsigned& lo) {
+  typedef float f2v_ __attribute__((ext_vector_type(2)));
+  if constexpr (PREC == PREC_BF16) {
+    typedef __bf16 b2v __attribute__((ext_vector_type(2)));
+    const b2v h = __builtin_convertvector((f2v_){d0, d1}, b2v);
+    hi = __builtin_bit_cast(unsigned, h);
+    const float h0 = __uint_as_float(hi << 16), h1 = __uint_as_float(hi & 0xffff0000u);
+    lo = __builtin_bit_cast(unsigned, __builtin_convertvector((f2v_){d0 - h0, d1 - h1}, b2v));
+  } else {
+    typedef _Float16 h2v_ __attribute__((ext_vector_type(2)));
+    const h2v_ h = __builtin_convertvector((f2v_){d0, d1}, h2v_);
+    hi = __builtin_bit_cast(unsigned, h);
+    lo = __builtin_bit_cast(unsigned, __builtin_convertvector((f2v_){d0 - (float)h.x, d1 - (float)h.y}, h2v_));
+  }
 }
 
 // Waits for every vector memory operation of this wave (s_waitcnt vmcnt(0);
@@ -782,5 +804,34 @@ __device__ __forceinline__ float wave_sum(float v) {
 }
 __device__ __forceinline__ float wave_max(float v) { return wave_reduce<true>(v); }
 __device__ __forceinline__ float wave_min(float v) { return wave_reduce<false>(v); }
+
+// Order-preserving integer image of a float (x < y <=> fkey(x) < fkey(y) for
+// non-NaN x, y; -0 sorts just below +0) and its inverse. Integer max / min
+// need no NaN canonicalisation, so a DPP step is one v_max_i32_dpp.
+__device__ __forceinline__ int fkey(float x) {
+  const int i = __float_as_int(x);
+  return i ^ ((i >> 31) & 0x7fffffff);
+}
+__device__ __forceinline__ float fkey_inv(int k) { return __int_as_float(k ^ ((k >> 31) & 0x7fffffff)); }
+template <bool MAX, int CTRL>
+__device__ __forceinline__ int dpp_step_i(int v) {
+  const int o = __builtin_amdgcn_mov_dpp(v, CTRL, 0xF, 0xF, true);
+  return MAX ? max(v, o) : min(v, o);
+}
+// Max / min of an int over the 64 lanes of a wave, wave-uniform result.
+template <bool MAX>
+__device__ __forceinline__ int wave_reduce_i(int v) {
+  v = dpp_step_i<MAX, 0xB1>(v);    // quad_perm [1, 0, 3, 2]
+  v = dpp_step_i<MAX, 0x4E>(v);    // quad_perm [2, 3, 0, 1]
+  v = dpp_step_i<MAX, 0x141>(v);   // row_half_mirror
+  v = dpp_step_i<MAX, 0x140>(v);   // row_mirror: every lane holds its row's result
+  const int r0 = __builtin_amdgcn_readlane(v, 0), r1 = __builtin_amdgcn_readlane(v, 16);
+  const int r2 = __builtin_amdgcn_readlane(v, 32), r3 = __builtin_amdgcn_readlane(v, 48);
+  return MAX ? max(max(r0, r1), max(r2, r3)) : min(min(r0, r1), min(r2, r3));
+}
+// Maximum of non-negative floats (their bit patterns order like the values).
+__device__ __forceinline__ float wave_max_nonneg(float v) {
+  return __int_as_float(wave_reduce_i<true>(__float_as_int(v)));
+}
 
 }  // namespace mm

@@ -26,7 +26,22 @@ struct TileLayout {
   size_t lds_bytes;  // dynamic LDS per workgroup
   int part_off;      // LDS offset of the DCT partial area (0: reuse the staging regions)
   int tab_off;       // LDS offset of the FFT table copies (16-bit mel / MFCC, n_fft 1024, 2048)
+  int shape;         // TileShape this layout was made for (never SHAPE_DEFAULT)
+  int occ;           // workgroups per CU the layout is sized for (LDS copies of tables only within this)
 };
+
+// Workgroup shapes of the fused kernel, chosen per batch by the autotuner.
+enum TileShape : int {
+  SHAPE_DEFAULT = 0,   // 16 waves x 1 frame up to n_fft 2048, 8 x 1 at 4096 (or set_tile_waves)
+  SHAPE_W16 = 1,       // 16 waves, one frame each: 16-frame tiles (n_fft <= 2048)
+  SHAPE_W8 = 2,        // 8 waves, one frame each: 8-frame tiles
+  SHAPE_W8F2 = 3,      // 8 waves, two frames each in turn: 16-frame tiles, two workgroups per CU
+                       // (n_fft 1024 and 2048), so one workgroup's contraction and barriers
+                       // overlap the other's FFTs
+};
+// Shapes the fused kernel offers for this FFT size (SHAPE_W16 first when it
+// exists); returns their number (at most 4).
+int tile_shapes(int n_fft, int* out);
 
 void set_lds_limit_bytes(size_t n);
 // Workgroup shape of the fused kernel for n_fft <= 2048: 16 or 8 waves (one
@@ -39,7 +54,7 @@ hipError_t gen_tables(const TableKey& key, const DeviceTables& t, hipStream_t st
 
 // Returns false when the fused kernel does not cover this configuration
 // (then the generic kernels are used).
-bool tile_layout(int n_fft, int out_kind, int prec, int n_mels, int n_mfcc, TileLayout* L);
+bool tile_layout(int n_fft, int out_kind, int prec, int n_mels, int n_mfcc, TileLayout* L, int shape = SHAPE_DEFAULT);
 hipError_t launch_tile(const TileArgs& a, int prec, const TileLayout& L, int grid, hipStream_t st);
 
 hipError_t launch_fill_u32(unsigned* p, int64_t n, unsigned v, hipStream_t st);

@@ -17,9 +17,11 @@
 
 #include <algorithm>
 
-#ifndef MM_TILE_OCC
-#define MM_TILE_OCC
-#endif
+// Waves per SIMD the fused kernel is compiled for: 8-wave workgroups up to
+// n_fft 2048 run two per CU (4 waves per SIMD), so their registers are
+// capped at 128.
+template <int LOG2M, int NW>
+constexpr int tile_waves_per_eu() { return (NW == 8 && LOG2M <= 10) ? 4 : 1; }
 
 // Optional phase timing of the fused kernel (build with -DMM_PHASE_TIMING):
 // lane 0 of every wave accumulates shader-clock cycles spent in each tile
@@ -57,7 +59,8 @@ namespace mm {
 //     K order), partial sums reduced across waves in LDS in fixed order.
 // Persistent: workgroup g processes tiles [g*C, (g+1)*C).
 template <int LOG2M, int MODE, int PREC, int NW, int FPW, typename TIn>
-__global__ __launch_bounds__(NW * 64) MM_TILE_OCC void k_tile(TileArgs a) {
+__global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(tile_waves_per_eu<LOG2M, NW>())))
+void k_tile(TileArgs a) {
   extern __shared__ __align__(16) unsigned char lds[];
   constexpr int M = 1 << LOG2M;
   constexpr int NI = (M / 2 + 1 + 63) / 64;       // split iterations per lane
@@ -80,8 +83,14 @@ __global__ __launch_bounds__(NW * 64) MM_TILE_OCC void k_tile(TileArgs a) {
   float* red_min = red_max + NW;                              // [NW]
   constexpr int ESZ = SPEC ? (CPLX ? 8 : 4) : (PREC == PREC_F32 ? 4 : 2);
   const int rowb = S * ESZ;
+  // A wave's region: its FFT scratch with its last frame's staging row over
+  // it (written once that frame's FFT is done), then the rows of its other
+  // frames from max(scratch, row) on; FPW = 1 needs only the first part.
+  constexpr int SCRB = ((M + M / 16) * (M >= 512 ? 4 : 8) + 15) / 16 * 16;
+  const int row0 = max(SCRB, (rowb + 15) / 16 * 16);
   auto row_ptr = [&](int f) -> unsigned char* {   // staging row of tile frame f
-    return lds + (f / FPW) * RB + (f % FPW) * rowb;
+    const int q = f % FPW;
+    return lds + (f / FPW) * RB + ((q == FPW - 1) ? 0 : row0 + q * rowb);
   };
 
   const int t_begin = blockIdx.x * a.tiles_per_wg;
@@ -123,10 +132,15 @@ __global__ __launch_bounds__(NW * 64) MM_TILE_OCC void k_tile(TileArgs a) {
           }
         }
       } else {
+        // nothing to fetch: pf is unused on the next tile; an empty asm
+        // definition ends the old values' lifetime without an instruction
 #pragma unroll
         for (int q = 0; q < SHP::Q0; ++q)
 #pragma unroll
-          for (int r2 = 0; r2 < SHP::R0; ++r2) pf[q][r2] = make_float2(0.0f, 0.0f);
+          for (int r2 = 0; r2 < SHP::R0; ++r2) {
+            __asm__ volatile("" : "=v"(pf[q][r2].x));
+            __asm__ volatile("" : "=v"(pf[q][r2].y));
+          }
       }
     }
   };
@@ -163,7 +177,7 @@ __global__ __launch_bounds__(NW * 64) MM_TILE_OCC void k_tile(TileArgs a) {
     twp = a.twp;
     twn = a.twN;
   }
-  // LDS copies of the banded filterbank ([chunk][8 rows][64 B]) and of the
+  // LDS copies of the banded filterbank ([chunk][8 rows][80 B]: 64 B of weights per row) and of the
   // DCT operand tables; the first tile's barrier publishes them
   if constexpr (!SPEC) {
     if (a.fb_lds_off && t_begin < t_end) {
@@ -175,7 +189,7 @@ __global__ __launch_bounds__(NW * 64) MM_TILE_OCC void k_tile(TileArgs a) {
       for (int p = threadIdx.x; p < a.cache_chunks * 32; p += NT) {
         const int c = p >> 5, r = (p >> 2) & 7, q = p & 3;
         const FE* src = tab + ldc(a.cache_src, c) + r * a.fbs + q * EPP;
-        *reinterpret_cast<uint4*>(lds + a.fb_lds_off + c * 512 + r * 64 + q * 16) =
+        *reinterpret_cast<uint4*>(lds + a.fb_lds_off + c * kFbChunkBytes + r * kFbRowBytes + q * 16) =
             *reinterpret_cast<const uint4*>(src);
       }
     }
@@ -220,13 +234,64 @@ __global__ __launch_bounds__(NW * 64) MM_TILE_OCC void k_tile(TileArgs a) {
     constexpr int EC = CPLX ? 2 : 1;
     float hk[NH][NI][EC];
     float hm[NH][NI][EC];
-    float hv[FPW][QL][RL][EC];
-    float hn[FPW][EC];
-    int fexp[FPW];
+    // register-split frames of a multi-frame wave are staged as soon as
+    // their FFT is done (the next frame's FFT reuses the registers and the
+    // scratch; earlier frames' rows lie outside the scratch)
+    constexpr bool STAGE_EACH = RS && FPW > 1;
+    constexpr int NQ = STAGE_EACH ? 1 : FPW;       // frames whose spectra are held at once
+    float hv[NQ][QL][RL][EC];
+    float hn[NQ][EC];
+    int fexp[NQ];
+    // Writes frame q's spectrum (registers) into its staging row, and its
+    // operand scale exponent for the contraction.
+    auto put = [&](unsigned char* row, int k, const float* v, int e) {
+      if constexpr (CPLX) {
+        reinterpret_cast<float2*>(row)[k] = make_float2(v[0], v[EC - 1]);
+      } else if constexpr (SPEC || PREC == PREC_F32) {
+        reinterpret_cast<float*>(row)[k] = v[0];
+      } else {
+        const float sv = ldexpf(v[0], -e);
+        reinterpret_cast<uint16_t*>(row)[k] = (PREC == PREC_BF16) ? f2bf(sv) : f2h(sv);
+      }
+    };
+    auto stage = [&](int q) {
+      const int qh = STAGE_EACH ? 0 : q;
+      const int fl = wv * FPW + q;
+      const int fr = f0 + fl;
+      unsigned char* row = row_ptr(fl);
+      if (fr < T) {
+        if constexpr (RS) {
 #pragma unroll
-    for (int q = 0; q < FPW; ++q) {
-      fexp[q] = 0;
-      const int fr = f0 + wave * FPW + q;
+          for (int b = 0; b < QL; ++b)
+#pragma unroll
+            for (int r = 0; r < RL; ++r) put(row, lane + 64 * b + SH::BL * r, hv[qh][b][r], fexp[qh]);
+          if (lane == 0) put(row, M, hn[qh], fexp[qh]);
+        } else {
+#pragma unroll
+          for (int i = 0; i < NI; ++i) {
+            const int k = lane + 64 * i;
+            if ((M / 2 + 1) % 64 == 0 || k <= M / 2) {
+              put(row, k, hk[qh][i], fexp[qh]);
+              put(row, M - k, hm[qh][i], fexp[qh]);
+            }
+          }
+        }
+        if constexpr (!SPEC) {
+          // bins F .. S-1 must read as zero in the banded contraction
+          for (int k = F + lane; k < S; k += 64) {
+            if constexpr (PREC == PREC_F32) reinterpret_cast<float*>(row)[k] = 0.0f;
+            else reinterpret_cast<uint16_t*>(row)[k] = 0;
+          }
+        }
+      }
+      if constexpr (!SPEC) {
+        if (lane == 0) sc_exp[fl] = (fr < T) ? fexp[qh] : 0;
+      }
+    };
+    auto frame_body = [&](int q) {
+      const int qh = STAGE_EACH ? 0 : q;
+      fexp[qh] = 0;
+      const int fr = f0 + wv * FPW + q;   // wave-uniform: framing math stays scalar
       if (fr < T) {
         float vmax = 0.0f;
         if constexpr (RS) {
@@ -242,25 +307,25 @@ __global__ __launch_bounds__(NW * 64) MM_TILE_OCC void k_tile(TileArgs a) {
             for (int r = 0; r < RL; ++r) {
               const float2 X = x[b][r];   // 2 X[k]
               if constexpr (CPLX) {
-                hv[q][b][r][0] = 0.5f * X.x;
-                hv[q][b][r][EC - 1] = 0.5f * X.y;
+                hv[qh][b][r][0] = 0.5f * X.x;
+                hv[qh][b][r][EC - 1] = 0.5f * X.y;
               } else {
                 const float p2 = fmaf(X.x, X.x, X.y * X.y);   // 4 |X[k]|^2
                 float v = p2;
                 if constexpr (MODE == OUT_STFT_MAG) v = 0.5f * sqrtf(p2);
                 else if constexpr (MODE == OUT_STFT_POWER) v = 0.25f * p2;
-                hv[q][b][r][0] = v;
+                hv[qh][b][r][0] = v;
               }
             }
           if constexpr (CPLX) {
-            hn[q][0] = 0.5f * nyq;
-            hn[q][EC - 1] = 0.0f;
+            hn[qh][0] = 0.5f * nyq;
+            hn[qh][EC - 1] = 0.0f;
           } else {
             const float p2 = nyq * nyq;
             float v = p2;
             if constexpr (MODE == OUT_STFT_MAG) v = 0.5f * fabsf(nyq);
             else if constexpr (MODE == OUT_STFT_POWER) v = 0.25f * p2;
-            hn[q][0] = v;
+            hn[qh][0] = v;
           }
           if constexpr (!SPEC) {
             // (2|X|)^p from (2|X|)^2 (the 2^p is removed after the mel
@@ -271,21 +336,21 @@ __global__ __launch_bounds__(NW * 64) MM_TILE_OCC void k_tile(TileArgs a) {
 #pragma unroll
                 for (int b = 0; b < QL; ++b)
 #pragma unroll
-                  for (int r = 0; r < RL; ++r) hv[q][b][r][0] = sqrtf(hv[q][b][r][0]);
-                hn[q][0] = sqrtf(hn[q][0]);
+                  for (int r = 0; r < RL; ++r) hv[qh][b][r][0] = sqrtf(hv[qh][b][r][0]);
+                hn[qh][0] = sqrtf(hn[qh][0]);
               } else {
 #pragma unroll
                 for (int b = 0; b < QL; ++b)
 #pragma unroll
-                  for (int r = 0; r < RL; ++r) hv[q][b][r][0] = pow_from_p2(hv[q][b][r][0], a.power);
-                hn[q][0] = pow_from_p2(hn[q][0], a.power);
+                  for (int r = 0; r < RL; ++r) hv[qh][b][r][0] = pow_from_p2(hv[qh][b][r][0], a.power);
+                hn[qh][0] = pow_from_p2(hn[qh][0], a.power);
               }
             }
 #pragma unroll
             for (int b = 0; b < QL; ++b)
 #pragma unroll
-              for (int r = 0; r < RL; ++r) vmax = fmaxf(vmax, hv[q][b][r][0]);
-            if (lane == 0) vmax = fmaxf(vmax, hn[q][0]);
+              for (int r = 0; r < RL; ++r) vmax = fmaxf(vmax, hv[qh][b][r][0]);
+            if (lane == 0) vmax = fmaxf(vmax, hn[qh][0]);
           }
         } else {
           frame_fft<LOG2M, TIn>(a, clip, fr, scr, lane);
@@ -296,22 +361,22 @@ __global__ __launch_bounds__(NW * 64) MM_TILE_OCC void k_tile(TileArgs a) {
               float2 xk, xmk;
               real_split<LOG2M>(scr, a.twN, k, xk, xmk);
               if constexpr (CPLX) {
-                hk[q][i][0] = xk.x; hk[q][i][EC - 1] = xk.y;
-                hm[q][i][0] = xmk.x; hm[q][i][EC - 1] = xmk.y;
+                hk[qh][i][0] = xk.x; hk[qh][i][EC - 1] = xk.y;
+                hm[qh][i][0] = xmk.x; hm[qh][i][EC - 1] = xmk.y;
               } else {
                 const float pk = fmaf(xk.x, xk.x, xk.y * xk.y);
                 const float pm = fmaf(xmk.x, xmk.x, xmk.y * xmk.y);
                 if constexpr (MODE == OUT_STFT_MAG) {
-                  hk[q][i][0] = sqrtf(pk);
-                  hm[q][i][0] = sqrtf(pm);
+                  hk[qh][i][0] = sqrtf(pk);
+                  hm[qh][i][0] = sqrtf(pm);
                 } else if constexpr (MODE == OUT_STFT_POWER) {
-                  hk[q][i][0] = pk;
-                  hm[q][i][0] = pm;
+                  hk[qh][i][0] = pk;
+                  hm[qh][i][0] = pm;
                 } else {
                   // same 2^p scale as the register split (removed after the mel product)
-                  hk[q][i][0] = pow_from_p2(4.0f * pk, a.power);
-                  hm[q][i][0] = pow_from_p2(4.0f * pm, a.power);
-                  vmax = fmaxf(vmax, fmaxf(hk[q][i][0], hm[q][i][0]));
+                  hk[qh][i][0] = pow_from_p2(4.0f * pk, a.power);
+                  hm[qh][i][0] = pow_from_p2(4.0f * pm, a.power);
+                  vmax = fmaxf(vmax, fmaxf(hk[qh][i][0], hm[qh][i][0]));
                 }
               }
             }
@@ -323,12 +388,26 @@ __global__ __launch_bounds__(NW * 64) MM_TILE_OCC void k_tile(TileArgs a) {
           // frame's top range (exactly undone after the contraction): bf16
           // keeps the FP32 exponent range, FP16 maps the frame maximum into
           // [2^14, 2^15) so ~84 dB below it stay normal numbers
-          vmax = wave_max(vmax);
+          vmax = wave_max_nonneg(vmax);
           int e = 0;
           if (vmax > 0.0f && vmax <= 3.0e38f) e = __builtin_amdgcn_frexp_expf(vmax) - 1 - (PREC == PREC_F16 ? 14 : 0);
-          fexp[q] = e;
+          fexp[qh] = e;
         }
       }
+      if constexpr (STAGE_EACH) {
+        wave_lds_fence();
+        stage(q);
+        wave_lds_fence();
+      }
+    };
+    if constexpr (STAGE_EACH) {
+      // one frame at a time (a rolled loop: nothing of one frame's FFT is
+      // scheduled into the other's)
+#pragma unroll 1
+      for (int q = 0; q < FPW; ++q) frame_body(q);
+    } else {
+#pragma unroll
+      for (int q = 0; q < FPW; ++q) frame_body(q);
     }
 
     MM_PT(0);
@@ -349,49 +428,9 @@ __global__ __launch_bounds__(NW * 64) MM_TILE_OCC void k_tile(TileArgs a) {
       for (int bi = 0; bi < MAXB; ++bi) slot[bi] = ldc(a.assign, wq * MAXB + bi);
     }
     // ---- staging rows (this wave's region; its scratch is no longer used) ----
-    auto put = [&](unsigned char* row, int k, const float* v, int e) {
-      if constexpr (CPLX) {
-        reinterpret_cast<float2*>(row)[k] = make_float2(v[0], v[EC - 1]);
-      } else if constexpr (SPEC || PREC == PREC_F32) {
-        reinterpret_cast<float*>(row)[k] = v[0];
-      } else {
-        const float sv = ldexpf(v[0], -e);
-        reinterpret_cast<uint16_t*>(row)[k] = (PREC == PREC_BF16) ? f2bf(sv) : f2h(sv);
-      }
-    };
+    if constexpr (!STAGE_EACH) {
 #pragma unroll
-    for (int q = 0; q < FPW; ++q) {
-      const int fl = wave * FPW + q;
-      const int fr = f0 + fl;
-      unsigned char* row = row_ptr(fl);
-      if (fr < T) {
-        if constexpr (RS) {
-#pragma unroll
-          for (int b = 0; b < QL; ++b)
-#pragma unroll
-            for (int r = 0; r < RL; ++r) put(row, lane + 64 * b + SH::BL * r, hv[q][b][r], fexp[q]);
-          if (lane == 0) put(row, M, hn[q], fexp[q]);
-        } else {
-#pragma unroll
-          for (int i = 0; i < NI; ++i) {
-            const int k = lane + 64 * i;
-            if ((M / 2 + 1) % 64 == 0 || k <= M / 2) {
-              put(row, k, hk[q][i], fexp[q]);
-              put(row, M - k, hm[q][i], fexp[q]);
-            }
-          }
-        }
-        if constexpr (!SPEC) {
-          // bins F .. S-1 must read as zero in the banded contraction
-          for (int k = F + lane; k < S; k += 64) {
-            if constexpr (PREC == PREC_F32) reinterpret_cast<float*>(row)[k] = 0.0f;
-            else reinterpret_cast<uint16_t*>(row)[k] = 0;
-          }
-        }
-      }
-      if constexpr (!SPEC) {
-        if (lane == 0) sc_exp[fl] = (fr < T) ? fexp[q] : 0;
-      }
+      for (int q = 0; q < FPW; ++q) stage(q);
     }
     MM_PT(1);
     lds_barrier();
@@ -432,8 +471,10 @@ __global__ __launch_bounds__(NW * 64) MM_TILE_OCC void k_tile(TileArgs a) {
       const unsigned char* prow = row_ptr(ri < TF ? ri : 0);
       const int myexp = (ri < TF) ? sc_exp[ri] : 0;
       const bool frame_ok = (ri < TF) && (f0 + ri) < T;
+      const float vscale = (PREC == PREC_F32) ? a.post_scale : ldexpf(a.post_scale, myexp);
+      const bool track = (MODE == OUT_MFCC) && a.use_top_db && a.phase == 1;   // phase 1: tile extrema for the clamp
       f32x4 dbv[MAXB];
-      float lmax = -INFINITY, lmin = INFINITY;
+      int kmax = INT_MIN, kmin = INT_MAX;   // fkey of this lane's dB extrema
       int nb = 0;   // slots in use (filled as a prefix)
 #pragma unroll
       for (int bi = 0; bi < MAXB; ++bi)
@@ -490,7 +531,7 @@ __global__ __launch_bounds__(NW * 64) MM_TILE_OCC void k_tile(TileArgs a) {
         const FE* pr = reinterpret_cast<const FE*>(prow) + x0 + (CH / 4) * g;
         // this lane's filterbank operand: row ri & 7, bins (CH/4) g .. of a chunk
         const bool cached = a.fb_lds_off != 0;
-        const unsigned char* fbl = lds + a.fb_lds_off + slot_cache_first(v) * 512 + (ri & 7) * 64 + g * 16;
+        const unsigned char* fbl = lds + a.fb_lds_off + slot_cache_first(v) * kFbChunkBytes + (ri & 7) * kFbRowBytes + g * 16;
         const FE* fbtab;
         if constexpr (PREC == PREC_F32) fbtab = a.fb32;
         else fbtab = a.fb16;
@@ -499,9 +540,9 @@ __global__ __launch_bounds__(NW * 64) MM_TILE_OCC void k_tile(TileArgs a) {
         auto round = [&](auto kc, int c) {
           constexpr int K = decltype(kc)::value;
           AV av[K], bv[K];
-          if (cached) {   // LDS copy: [chunk][8 rows][64 B]
+          if (cached) {   // LDS copy: [chunk][8 rows][80 B]
 #pragma unroll
-            for (int u = 0; u < K; ++u) av[u] = *reinterpret_cast<const AV*>(fbl + 512 * min(c + u, n - 1));
+            for (int u = 0; u < K; ++u) av[u] = *reinterpret_cast<const AV*>(fbl + kFbChunkBytes * min(c + u, n - 1));
           } else {
 #pragma unroll
             for (int u = 0; u < K; ++u) av[u] = ldg<AV>(fbb + CH * min(c + u, n - 1), fo);
@@ -525,13 +566,11 @@ __global__ __launch_bounds__(NW * 64) MM_TILE_OCC void k_tile(TileArgs a) {
       };
       // acc (of block h) -> mel values, then the MEL store or dB
       auto finish_block = [&](int h, int bi, f32x4 acc) {
-        if constexpr (PREC != PREC_F32) {
+        // acc[j] = mel[8h + 4(g & 1) + j] of tile frame ri (lane groups 2, 3
+        // repeat groups 0, 1); vscale undoes the frame's power-of-two operand
+        // scale and the 2^p of the contraction in one exact multiply
 #pragma unroll
-          for (int j = 0; j < 4; ++j) acc[j] = ldexpf(acc[j], myexp);
-        }
-#pragma unroll
-        for (int j = 0; j < 4; ++j) acc[j] *= a.post_scale;   // 2^-power
-        // acc[j] = mel[8h + 4g + j] of tile frame ri (g < 2; groups 2, 3 hold zeros)
+        for (int j = 0; j < 4; ++j) acc[j] *= vscale;
         if constexpr (MODE == OUT_MEL) {
           if (frame_ok && g < 2) {
             float* o = reinterpret_cast<float*>(a.out) + ldc(a.out_off, clip);
@@ -542,17 +581,31 @@ __global__ __launch_bounds__(NW * 64) MM_TILE_OCC void k_tile(TileArgs a) {
             }
           }
         } else {
+          // 10 log10(max(mel, amin) / ref) = 10 log10(2) log2(max(mel, amin))
+          // - 10 log10(ref) (db_off = 0 for the default ref = 1, where this
+          // is the same float as the quotient form). The DCT table is zero in
+          // the K slots of lane groups 2, 3 and of mels past n_mels, so those
+          // slots need no masking.
 #pragma unroll
           for (int j = 0; j < 4; ++j) {
-            const int m = kMelBlock * h + 4 * g + j;
-            const bool valid = (g < 2) && (m < a.n_mels);
-            float d = db10(fmaxf(acc[j], a.amin) / a.ref);
-            d = fmaxf(d, thr);
-            if (valid && frame_ok) {
-              lmax = fmaxf(lmax, d);
-              lmin = fminf(lmin, d);
+            const float d = fmaf(3.0102999566398120f, __builtin_amdgcn_logf(fmaxf(acc[j], a.amin)), -a.db_off);
+            dbv[bi][j] = fmaxf(d, thr);   // thr = -inf except in phase 2 (floor at the clip max - top_db)
+          }
+          if (track) {
+            if (kMelBlock * h + kMelBlock <= a.n_mels) {
+#pragma unroll
+              for (int j = 0; j < 4; ++j) {
+                kmax = max(kmax, fkey(dbv[bi][j]));
+                kmin = min(kmin, fkey(dbv[bi][j]));
+              }
+            } else {   // the last block of an n_mels that is not a multiple of 8
+#pragma unroll
+              for (int j = 0; j < 4; ++j)
+                if (kMelBlock * h + 4 * (g & 1) + j < a.n_mels) {
+                  kmax = max(kmax, fkey(dbv[bi][j]));
+                  kmin = min(kmin, fkey(dbv[bi][j]));
+                }
             }
-            dbv[bi][j] = valid ? d : 0.0f;
           }
         }
       };
@@ -568,90 +621,94 @@ __global__ __launch_bounds__(NW * 64) MM_TILE_OCC void k_tile(TileArgs a) {
       if constexpr (MODE == OUT_MEL) prefetch_next(tile, nrec);
 
       if constexpr (MODE == OUT_MFCC) {
-        const int nact = min(NW, a.nmh);   // waves holding mel blocks: 0 .. nact-1
-        lmax = wave_max(lmax);
-        lmin = wave_min(lmin);
-        if (lane == 0) { red_max[wave] = lmax; red_min[wave] = lmin; }
-        // DCT partials [ndt][nact][64 lanes][4]: in their own LDS area when
+        if (track) {
+          // lanes of frames past the clip end hold stale rows: neutral values
+          const int tmax = wave_reduce_i<true>(frame_ok ? kmax : INT_MIN);
+          const int tmin = wave_reduce_i<false>(frame_ok ? kmin : INT_MAX);
+          if (lane == 0) {
+            reinterpret_cast<int*>(red_max)[wave] = tmax;
+            reinterpret_cast<int*>(red_min)[wave] = tmin;
+          }
+        }
+        // DCT partials, one per (row tile ct, mel block h): [ndt][nmh][64
+        // lanes][4], summed over h in increasing order below, so the result
+        // does not depend on which wave contracted which block (any
+        // workgroup shape gives the same bits). In their own LDS area when
         // the layout has one (2 barriers per tile), else over the staging
-        // regions once every wave has read them (4 barriers)
+        // regions once every wave has read them (4 barriers).
         const bool sep = a.lds_part_off > 0;
         float* red = reinterpret_cast<float*>(lds + (sep ? a.lds_part_off : 0));
         if (!sep) lds_barrier();   // staging reads done; region memory is reused below
-        if (nb > 0) {
-          for (int ct = 0; ct < a.ndt; ++ct) {
-            f32x4 accd = {0.0f, 0.0f, 0.0f, 0.0f};
+        for (int ct = 0; ct < a.ndt; ++ct) {
 #pragma unroll
-            for (int bi = 0; bi < MAXB; ++bi) {
-              if (bi < nb) {
-                const DctOp op = load_dct(ct, slot_block(slot[bi]));
-                if constexpr (PREC == PREC_F32) {
+          for (int bi = 0; bi < MAXB; ++bi) {
+            if (bi < nb) {
+              const int h = slot_block(slot[bi]);
+              const DctOp op = load_dct(ct, h);
+              f32x4 accd = {0.0f, 0.0f, 0.0f, 0.0f};
+              if constexpr (PREC == PREC_F32) {
 #pragma unroll
-                  for (int j = 0; j < 4; ++j)
-                    accd = __builtin_amdgcn_mfma_f32_16x16x4f32(op.f[j], dbv[bi][j], accd, 0, 0, 0);
+                for (int j = 0; j < 4; ++j)
+                  accd = __builtin_amdgcn_mfma_f32_16x16x4f32(op.f[j], dbv[bi][j], accd, 0, 0, 0);
+              } else {
+                // hi / lo split of the dB values (two conversions per pair)
+                uint2 bhi2, blo2;
+                split16<PREC>(dbv[bi][0], dbv[bi][1], bhi2.x, blo2.x);
+                split16<PREC>(dbv[bi][2], dbv[bi][3], bhi2.y, blo2.y);
+                if constexpr (PREC == PREC_BF16) {
+                  const s16x4 Ah = __builtin_bit_cast(s16x4, op.hi), Al = __builtin_bit_cast(s16x4, op.lo);
+                  const s16x4 Bh = __builtin_bit_cast(s16x4, bhi2), Bl = __builtin_bit_cast(s16x4, blo2);
+                  accd = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Ah, Bh, accd, 0, 0, 0);
+                  accd = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Al, Bh, accd, 0, 0, 0);
+                  accd = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Ah, Bl, accd, 0, 0, 0);
                 } else {
-                  uint16_t bh[4], bl[4];
-#pragma unroll
-                  for (int j = 0; j < 4; ++j) {
-                    const float d = dbv[bi][j];
-                    if constexpr (PREC == PREC_BF16) {
-                      bh[j] = f2bf(d);
-                      bl[j] = f2bf(d - bf2f(bh[j]));
-                    } else {
-                      bh[j] = f2h(d);
-                      bl[j] = f2h(d - h2f(bh[j]));
-                    }
-                  }
-                  const uint2 bhi2 = make_uint2(bh[0] | ((unsigned)bh[1] << 16), bh[2] | ((unsigned)bh[3] << 16));
-                  const uint2 blo2 = make_uint2(bl[0] | ((unsigned)bl[1] << 16), bl[2] | ((unsigned)bl[3] << 16));
-                  if constexpr (PREC == PREC_BF16) {
-                    const s16x4 Ah = __builtin_bit_cast(s16x4, op.hi), Al = __builtin_bit_cast(s16x4, op.lo);
-                    const s16x4 Bh = __builtin_bit_cast(s16x4, bhi2), Bl = __builtin_bit_cast(s16x4, blo2);
-                    accd = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Ah, Bh, accd, 0, 0, 0);
-                    accd = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Al, Bh, accd, 0, 0, 0);
-                    accd = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Ah, Bl, accd, 0, 0, 0);
-                  } else {
-                    const f16x4 Ah = __builtin_bit_cast(f16x4, op.hi), Al = __builtin_bit_cast(f16x4, op.lo);
-                    const f16x4 Bh = __builtin_bit_cast(f16x4, bhi2), Bl = __builtin_bit_cast(f16x4, blo2);
-                    accd = __builtin_amdgcn_mfma_f32_16x16x16f16(Ah, Bh, accd, 0, 0, 0);
-                    accd = __builtin_amdgcn_mfma_f32_16x16x16f16(Al, Bh, accd, 0, 0, 0);
-                    accd = __builtin_amdgcn_mfma_f32_16x16x16f16(Ah, Bl, accd, 0, 0, 0);
-                  }
+                  const f16x4 Ah = __builtin_bit_cast(f16x4, op.hi), Al = __builtin_bit_cast(f16x4, op.lo);
+                  const f16x4 Bh = __builtin_bit_cast(f16x4, bhi2), Bl = __builtin_bit_cast(f16x4, blo2);
+                  accd = __builtin_amdgcn_mfma_f32_16x16x16f16(Ah, Bh, accd, 0, 0, 0);
+                  accd = __builtin_amdgcn_mfma_f32_16x16x16f16(Al, Bh, accd, 0, 0, 0);
+                  accd = __builtin_amdgcn_mfma_f32_16x16x16f16(Ah, Bl, accd, 0, 0, 0);
                 }
               }
+              mfma_settle(accd);
+              float* dst = red + ((int64_t)(ct * a.nmh + h) * 64 + lane) * 4;
+              *reinterpret_cast<float4*>(dst) = make_float4(accd[0], accd[1], accd[2], accd[3]);
             }
-            mfma_settle(accd);
-            float* dst = red + ((int64_t)(ct * nact + wave) * 64 + lane) * 4;
-            *reinterpret_cast<float4*>(dst) = make_float4(accd[0], accd[1], accd[2], accd[3]);
           }
         }
         prefetch_next(tile, nrec);
         MM_PT(3);
         lds_barrier();   // partials and per-wave extrema are in LDS
         MM_PT(4);
-        if (threadIdx.x == 0 && a.phase == 1 && a.use_top_db) {
-          float bm = -INFINITY, bn = INFINITY;
-          for (int w = 0; w < NW; ++w) { bm = fmaxf(bm, red_max[w]); bn = fminf(bn, red_min[w]); }
-          if (bm > -INFINITY) atomicMax(&a.clip_max_key[clip], float_to_key(bm));
-          a.tile_min[tile] = bn;
+        if (track && threadIdx.x == 0) {
+          int bm = INT_MIN, bn = INT_MAX;
+          for (int w = 0; w < NW; ++w) {
+            bm = max(bm, reinterpret_cast<const int*>(red_max)[w]);
+            bn = min(bn, reinterpret_cast<const int*>(red_min)[w]);
+          }
+          if (bm > INT_MIN) atomicMax(&a.clip_max_key[clip], float_to_key(fkey_inv(bm)));
+          a.tile_min[tile] = (bn < INT_MAX) ? fkey_inv(bn) : INFINITY;
         }
-        // fixed-order cross-wave sum; wave ct writes DCT rows 16ct .. 16ct+15
-        // (with a separate area, the other waves already start the next tile)
-        for (int ct = wave; ct < a.ndt; ct += NW) {
-          float4 sm = make_float4(0.f, 0.f, 0.f, 0.f);
-          for (int w = 0; w < nact; ++w) {
-            const float4 p = *reinterpret_cast<const float4*>(red + ((int64_t)(ct * nact + w) * 64 + lane) * 4);
-            sm.x += p.x; sm.y += p.y; sm.z += p.z; sm.w += p.w;
-          }
-          if (frame_ok) {
-            float* o = reinterpret_cast<float*>(a.out) + ldc(a.out_off, clip);
-            const float sv[4] = {sm.x, sm.y, sm.z, sm.w};
+        // fixed-order sum over the blocks: thread t of the workgroup owns
+        // output (coefficient 16 ct + (t >> 4) % 16, frame t & 15), so 16
+        // consecutive threads store 16 consecutive frames of one coefficient
+        const int nout = a.ndt * 256;
+        for (int t = threadIdx.x; t < nout; t += NT) {
+          const int ct = t >> 8, cl = (t >> 4) & 15, fi = t & 15;
+          const int e = ((cl >> 2) * 16 + fi) * 4 + (cl & 3);   // MFMA layout: lane (cl / 4) * 16 + fi, item cl % 4
+          const float* src = red + (int64_t)ct * a.nmh * 256 + e;
+          float sm = 0.0f;
+          int h = 0;
+          for (; h + 8 <= a.nmh; h += 8) {   // 8 reads in flight, added in block order
+            float v[8];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-              const int c = 16 * ct + 4 * g + j;
-              if (c < a.n_mfcc) o[(int64_t)c * T + f0 + ri] = sv[j];
-            }
+            for (int u = 0; u < 8; ++u) v[u] = src[(h + u) * 256];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) sm += v[u];
           }
+          for (; h < a.nmh; ++h) sm += src[h * 256];
+          const int c = 16 * ct + cl;
+          if (fi < TF && f0 + fi < T && c < a.n_mfcc)
+            reinterpret_cast<float*>(a.out)[ldc(a.out_off, clip) + (int64_t)c * T + f0 + fi] = sm;
         }
         MM_PT(5);
         if (sep) continue;
@@ -1050,8 +1107,12 @@ static hipError_t launch_tile_t(const TileArgs& a, int grid, size_t lds, hipStre
 template <int LOG2M, int MODE, int PREC, typename TIn>
 static hipError_t launch_tile_s(const TileArgs& a, const TileLayout& L, int grid, hipStream_t st) {
   if constexpr (LOG2M <= 10) {
-    if (L.nw == 16) return launch_tile_t<LOG2M, MODE, PREC, 16, 1, TIn>(a, grid, L.lds_bytes, st);
+    if (L.shape == SHAPE_W16) return launch_tile_t<LOG2M, MODE, PREC, 16, 1, TIn>(a, grid, L.lds_bytes, st);
   }
+  if constexpr (LOG2M == 9 || LOG2M == 10) {
+    if (L.shape == SHAPE_W8F2) return launch_tile_t<LOG2M, MODE, PREC, 8, 2, TIn>(a, grid, L.lds_bytes, st);
+  }
+  if (L.shape != SHAPE_W8) return hipErrorInvalidValue;
   return launch_tile_t<LOG2M, MODE, PREC, 8, 1, TIn>(a, grid, L.lds_bytes, st);
 }
 
@@ -1091,15 +1152,28 @@ static size_t g_lds_limit = 64 * 1024;
 void set_lds_limit_bytes(size_t n) { g_lds_limit = n; }
 size_t lds_limit_bytes() { return g_lds_limit; }
 
-bool tile_layout(int n_fft, int out_kind, int prec, int n_mels, int n_mfcc, TileLayout* L) {
+int tile_shapes(int n_fft, int* out) {
+  if (n_fft < 256 || n_fft > 4096 || (n_fft & (n_fft - 1))) return 0;
+  int n = 0;
+  if (n_fft <= 2048) out[n++] = SHAPE_W16;
+  out[n++] = SHAPE_W8;
+  if (n_fft == 1024 || n_fft == 2048) out[n++] = SHAPE_W8F2;
+  return n;
+}
+
+bool tile_layout(int n_fft, int out_kind, int prec, int n_mels, int n_mfcc, TileLayout* L, int shape) {
   if (n_fft < 256 || n_fft > 4096 || (n_fft & (n_fft - 1))) return false;
   const int M = n_fft / 2, F = M + 1;
+  if (shape == SHAPE_DEFAULT) shape = (M > 1024) ? SHAPE_W8 : (g_tile_waves == 8 ? SHAPE_W8 : SHAPE_W16);
+  if (shape != SHAPE_W16 && shape != SHAPE_W8 && shape != SHAPE_W8F2) return false;
+  if (shape == SHAPE_W16 && M > 1024) return false;   // 16 waves need n_fft <= 2048
+  if (shape == SHAPE_W8F2 && M != 512 && M != 1024) return false;   // register-split FFT sizes
   const bool spec = out_kind <= OUT_STFT_COMPLEX;
   if (!spec) {
     if (n_mels < 1 || n_mels > kMaxMelBlocks * kMelBlock) return false;   // larger: generic path
     if (out_kind == OUT_MFCC && (n_mfcc < 1 || n_mfcc > 128)) return false;
   }
-  const int nw = (M > 1024) ? 8 : (g_tile_waves ? g_tile_waves : 16), fpw = 1;
+  const int nw = (shape == SHAPE_W16) ? 16 : 8, fpw = (shape == SHAPE_W8F2) ? 2 : 1;
   // FFT scratch: float2 exchanges for the small sizes, half-size float
   // exchanges for the register-split sizes (n_fft >= 1024)
   const int scr = (M + M / 16) * ((M >= 512) ? 4 : 8);
@@ -1116,8 +1190,10 @@ bool tile_layout(int n_fft, int out_kind, int prec, int n_mels, int n_mfcc, Tile
     const int need = ((F + 31) / 32) * 32 + 64;
     S = need + ((8 - need % 128) + 128) % 128;       // S/2 = 4 (mod 64) dwords
   }
-  // region: FFT scratch, or staging row(s) + a 1 KiB hand-over slot
-  int rb = std::max(fpw * S * esz, scr);
+  // region: the FFT scratch with the wave's last staging row over it, then
+  // the rows of its other frames
+  const int first = std::max((scr + 15) / 16 * 16, (S * esz + 15) / 16 * 16);
+  int rb = (fpw > 1) ? first + (fpw - 1) * S * esz : std::max(S * esz, scr);
   rb = (rb + 15) / 16 * 16;
   const size_t extras = 16 * 4 + 2 * nw * 4;
   size_t total = (size_t)nw * rb + extras;
@@ -1132,8 +1208,8 @@ bool tile_layout(int n_fft, int out_kind, int prec, int n_mels, int n_mfcc, Tile
   int part_off = 0;
   if (out_kind == OUT_MFCC) {
     const int ndt = (n_mfcc + 15) / 16;
-    const int nact = std::min(nw, (n_mels + kMelBlock - 1) / kMelBlock);
-    const size_t part = (size_t)ndt * nact * 256 * 4;
+    const int nmh = (n_mels + kMelBlock - 1) / kMelBlock;   // one partial per mel block
+    const size_t part = (size_t)ndt * nmh * 256 * 4;
     if (total + part + 16 <= lds_limit_bytes()) {
       // DCT partials in their own area: two barriers per tile
       part_off = (int)(((total + 15) / 16) * 16);
@@ -1150,6 +1226,9 @@ bool tile_layout(int n_fft, int out_kind, int prec, int n_mels, int n_mfcc, Tile
   L->part_off = part_off;
   L->tab_off = tab_off;
   L->tile_frames = nw * fpw;
+  L->shape = shape;
+  // the two-frame shape exists to run two workgroups per CU
+  L->occ = (shape == SHAPE_W16) ? 1 : (int)std::max<size_t>(1, std::min<size_t>(2, lds_limit_bytes() / total));
   return true;
 }
 
