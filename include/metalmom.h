@@ -315,6 +315,23 @@ int32_t mm_istft_device(mm_context ctx, const float* d_stft, int32_t ld_frames, 
 int32_t mm_spectral(mm_context ctx, const MMSpecParams* params, const float* signal_data,
                     int64_t signal_length, MMBuffer* out);
 
+/* mm_spectral on device memory that allocates nothing: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under
+ * hipGraph capture, tight serving loops). d_signal holds signal_length
+ * float32 samples, d_out receives the [rows, frames] result (both reported
+ * through rows_out / frames_out, which may be NULL); enqueued on `stream`
+ * (NULL = the null stream), kernel launches only. The clip's work list and
+ * top_db scratch live in one 1 MiB device array per GPU, shared by every
+ * call on that GPU and keeping the last call's contents: calls that overlap
+ * on one GPU race. Limits: configurations of the fused kernel only (n_fft
+ * 256..4096, n_mels <= 256, float32 input), and at most ~37,000 tiles of 16
+ * frames (about 600,000 frames) per clip; beyond them -1. The
+ * configuration's tables must already be resident (the first mm_spectral,
+ * mm_batch_create or server request of the same configuration builds them);
+ * otherwise -3 and nothing is enqueued. */
+int32_t mm_spectral_device_noalloc(const MMSpecParams* params, const float* d_signal, int64_t signal_length,
+                                   float* d_out, int64_t* rows_out, int64_t* frames_out, void* stream);
+
 /* Host-buffer batch: clip i -> outs[i] (each released with mm_buffer_free).
  * All clips run in one persistent launch. */
 int32_t mm_spectral_batch(mm_context ctx, const MMSpecParams* params, const float* const* clips,

@@ -92,6 +92,8 @@ SIGNATURES = {
                                         _P, C.POINTER(_I64), C.POINTER(_I64), _P]),
     "mm_istft_device": (_I32, [_P, _P, _I32, _I32, _I32, _I32, _I32, _I32, _I32, _I64, _P, _P]),
     "mm_spectral": (_I32, [_P, C.POINTER(MMSpecParams), _F, _I64, _BUF]),
+    "mm_spectral_device_noalloc": (_I32, [C.POINTER(MMSpecParams), _P, _I64, _P, C.POINTER(_I64), C.POINTER(_I64),
+                                          _P]),
     "mm_spectral_batch": (_I32, [_P, C.POINTER(MMSpecParams), C.POINTER(_F), C.POINTER(_I64), _I32,
                                  _BUF]),
     "mm_batch_create": (_I32, [_P, C.POINTER(MMSpecParams), C.POINTER(_I64), _I32, C.POINTER(_P)]),

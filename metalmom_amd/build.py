@@ -19,7 +19,8 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmetalmom.so")
 
-SOURCES = ["mm_kernels.hip", "mm_istft.hip", "mm_onset.hip", "mm_desc.hip", "mm_resample.hip", "mm_pcm.hip", "mm_engine.cpp", "mm_abi.cpp", "mm_batch.cpp", "mm_server.cpp", "mm_audio.cpp"]
+SOURCES = ["mm_kernels.hip", "mm_istft.hip", "mm_onset.hip", "mm_desc.hip", "mm_resample.hip", "mm_pcm.hip",
+           "mm_noalloc.hip", "mm_engine.cpp", "mm_abi.cpp", "mm_batch.cpp", "mm_server.cpp", "mm_audio.cpp"]
 ARCH = os.environ.get("METALMOM_AMD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 

@@ -308,6 +308,14 @@ int32_t Engine::tables_for(const SpecRequest& r, hipStream_t st, const Tables** 
   return OK;
 }
 
+const Tables* Engine::tables_resident(const SpecRequest& r) {
+  const TableKey k = key_of(r);
+  std::lock_guard<std::mutex> g(mu_);
+  for (auto& t : tables_)
+    if (t->key == k) return t.get();
+  return nullptr;
+}
+
 int32_t Engine::resample_table(hipStream_t st, const float** out) {
   std::lock_guard<std::mutex> g(mu_);
   if (!rs_table_.p) {

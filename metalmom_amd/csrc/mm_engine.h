@@ -86,6 +86,8 @@ class Engine {
 
   // Tables for a request (generated on first use, then shared).
   int32_t tables_for(const SpecRequest& r, hipStream_t st, const Tables** out);
+  // The request's tables if they are already resident, else nullptr (never allocates).
+  const Tables* tables_resident(const SpecRequest& r);
   // Filter table of the sample-rate converter (generated on first use).
   int32_t resample_table(hipStream_t st, const float** out);
   // Polyphase weight rows of a rate pair (built on first use; nullptr when
