@@ -30,6 +30,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP32_VECTOR_PEAK_TFLOPS = 157.3   # published FP32 vector peak (counts packed FP32, which this build does not emit)
 METRIC = "MFCC frames/s batched+BF16 vs plain FP32 on 1 GPU; p50/p99 latency @ fixed req/s"
 
 
@@ -69,11 +70,12 @@ def cpu_model():
     return "unknown"
 
 
-def cpu_baseline(clips, max_frames=400000, workers=0):
+def cpu_baseline(clips, max_frames=400000, workers=0, iters=5):
     """Float32 NumPy restatement of the reference CPU path on a bounded sample
     of the same batch (whole clips until max_frames; the default takes the
-    whole 336,013-frame batch), BLAS limited to one thread. workers = 0: one
-    core, clips in sequence; workers > 0: that many threads over the clips
+    whole 336,013-frame batch), BLAS limited to one thread, timed over
+    ``iters`` passes after a warm-up (mean, min and median pass). workers = 0:
+    one core, clips in sequence; workers > 0: that many threads over the clips
     (numpy's FFT, BLAS and elementwise kernels run without the GIL)."""
     from oracle.cpu_baseline import MfccF32
     try:
@@ -89,22 +91,192 @@ def cpu_baseline(clips, max_frames=400000, workers=0):
         sample.append(c)
         frames += 1 + len(c) // 512
     f(sample[0][:22050])  # warm-up
-    t0 = time.perf_counter()
-    if workers > 0:
-        from concurrent.futures import ThreadPoolExecutor
-        with ThreadPoolExecutor(max_workers=workers) as ex:
-            list(ex.map(f, sample))
-    else:
-        for c in sample:
-            f(c)
-    dt = time.perf_counter() - t0
+    times = []
+    for _ in range(iters):
+        t0 = time.perf_counter()
+        if workers > 0:
+            from concurrent.futures import ThreadPoolExecutor
+            with ThreadPoolExecutor(max_workers=workers) as ex:
+                list(ex.map(f, sample))
+        else:
+            for c in sample:
+                f(c)
+        times.append(time.perf_counter() - t0)
     if limiter is not None:
         limiter.unregister() if hasattr(limiter, "unregister") else None
+    dt = float(np.mean(times))
     return {"value": frames / dt, "unit": "frames/s", "cores": max(workers, 1), "kind": "port",
-            "sample": f"{len(sample)} whole clips of the uneven batch, {frames} frames, "
-                      f"float32 numpy rfft + dense FB@P + dB + DCT, {dt:.1f} s"
+            "sample": f"{len(sample)} whole clips of the uneven batch, {frames} frames per pass, "
+                      f"float32 numpy rfft + dense FB@P + dB + DCT, {iters} passes "
+                      f"(mean {dt:.2f} s, min {min(times):.2f} s, median {float(np.median(times)):.2f} s)"
                       + (f", {workers} threads" if workers else ", 1 thread"),
+            "pass_s": {"mean": dt, "min": float(min(times)), "p50": float(np.median(times)), "n": iters},
             "host_cpus": os.cpu_count(), "cpu_model": cpu_model()}
+
+
+def timing_stats(ms):
+    ms = np.asarray(ms, dtype=np.float64)
+    return {"mean": float(ms.mean()), "min": float(ms.min()), "p50": float(np.median(ms)), "n": int(ms.size)}
+
+
+def fft_flops(n_fft):
+    """Algorithmic FLOPs of one real FFT frame: 2.5 N log2 N."""
+    return 2.5 * n_fft * np.log2(n_fft)
+
+
+def mfma_flops_per_tile(sr, n_fft, n_mels, n_mfcc, prec):
+    """Executed MFMA FLOPs of one tile of the fused kernel (16 output columns
+    whatever the tile's frame count): each block of 8 mels contracts its bin
+    band in chunks of 32 bins (16-bit) or 16 bins (FP32) as 16x16 outputs
+    (rows 8-15 repeat rows 0-7), first bin rounded down to a multiple of 4;
+    the DCT takes 3 16x16x16 MFMAs (hi/lo split) or 4 16x16x4 FP32 MFMAs per
+    (block, 16 coefficients)."""
+    from metalmom_amd._mel_fb import mel_filterbank
+    fb = mel_filterbank(sr, n_fft, n_mels)
+    ch = 16 if prec == "f32" else 32
+    chunks = 0
+    for b in range((n_mels + 7) // 8):
+        nz = np.nonzero(fb[8 * b:8 * b + 8].any(axis=0))[0]
+        if nz.size:
+            x0 = int(nz[0]) & ~3
+            chunks += (int(nz[-1]) + 1 - x0 + ch - 1) // ch
+    nmh, ndt = (n_mels + 7) // 8, (n_mfcc + 15) // 16
+    dct = nmh * ndt * (4 * 2 * 16 * 16 * 4 if prec == "f32" else 3 * 2 * 16 * 16 * 16)
+    return chunks * 2 * 16 * 16 * ch + dct
+
+
+def useful_mfma_flops_per_frame(sr, n_fft, n_mels, n_mfcc):
+    """FLOPs of the contractions' nonzero work per frame: 2 x filterbank
+    nonzeros + 2 x n_mels x n_mfcc (DCT)."""
+    from metalmom_amd._mel_fb import mel_filterbank
+    return 2 * int(np.count_nonzero(mel_filterbank(sr, n_fft, n_mels))) + 2 * n_mels * n_mfcc
+
+
+MFMA_PEAK = {"bf16": 2500.0, "f16": 2500.0, "f32": 157.3}   # dense TFLOP/s, published
+
+
+def kernel_roofline(frames, bytes_launch, n_fft, n_tiles, mfma_tile_flops, useful_mfma_frame, prec, k_ms,
+                    traffic=None):
+    """Position of one fused launch against the three ceilings: HBM (algorithmic
+    bytes), FP32 vector (the FFT's 2.5 N log2 N per frame: the binding unit,
+    as the kernel's time is VALU issue of the FFT) and MFMA (executed and
+    useful contraction FLOPs)."""
+    s = k_ms * 1e-3
+    gbs = bytes_launch / s / 1e9
+    fft_tf = frames * fft_flops(n_fft) / s / 1e12
+    mf_tf = n_tiles * mfma_tile_flops / s / 1e12
+    return {"bound": "valu", "achieved": fft_tf, "peak": FP32_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": fft_tf / FP32_VECTOR_PEAK_TFLOPS, "frac_of_unpacked_fp32": fft_tf / (FP32_VECTOR_PEAK_TFLOPS / 2),
+            "traffic": traffic, "kernel_ms": k_ms, "fft_flops_per_frame": fft_flops(n_fft),
+            "hbm": {"achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+                    "bytes_per_launch": bytes_launch, "traffic": traffic},
+            "mfma": {"achieved": mf_tf, "peak": MFMA_PEAK[prec], "unit": "TFLOP/s", "frac": mf_tf / MFMA_PEAK[prec],
+                     "executed_flops_per_launch": n_tiles * mfma_tile_flops,
+                     "useful_flops_per_launch": frames * useful_mfma_frame, "precision": prec}}
+
+
+def fp64_reference(y, sr=22050, n_fft=2048, hop=512, n_mels=128, n_mfcc=13, top_db=80.0):
+    """Float64 STFT magnitude [F, T], mel power [n_mels, T] and MFCC of one
+    clip with the reference's semantics (centre zero padding, periodic Hann,
+    Slaney mel weights, whole-clip
+    top_db, orthonormal DCT-II), for the bench's error report."""
+    import scipy.fft
+    from metalmom_amd._mel_fb import mel_filterbank
+    n = np.arange(n_fft)
+    w = 0.5 - 0.5 * np.cos(2 * np.pi * n / n_fft)
+    x = np.pad(np.asarray(y, dtype=np.float64), (n_fft // 2, n_fft // 2))
+    T = 1 + (len(x) - n_fft) // hop
+    fr = np.lib.stride_tricks.as_strided(x, shape=(T, n_fft), strides=(x.strides[0] * hop, x.strides[0]))
+    X = np.fft.rfft(fr * w, axis=1).T
+    mel = mel_filterbank(sr, n_fft, n_mels, dtype=np.float64) @ (X.real ** 2 + X.imag ** 2)
+    db = 10.0 * np.log10(np.maximum(mel, 1e-10))
+    db = np.maximum(db, db.max() - top_db)
+    return np.abs(X), mel, scipy.fft.dct(db, type=2, norm="ortho", axis=0)[:n_mfcc]
+
+
+def _db(m, top_db=80.0):
+    d = 10.0 * np.log10(np.maximum(np.asarray(m, dtype=np.float64), 1e-10))
+    return np.maximum(d, d.max() - top_db)
+
+
+def error_report(torch, clips, d_in, B, idx):
+    """FP32 and BF16 outputs of sampled clips of the batch against the float64
+    reference: STFT magnitude max-abs / max, mel max-abs dB, MFCC max-abs /
+    max |MFCC|. Each clip runs through a one-clip plan on the batch's own
+    device samples (bitwise equal to its block of the batch result)."""
+    from metalmom_amd.batch import SpectralBatch
+    common = dict(sr=22050, n_fft=2048, hop_length=512, n_mels=128, n_mfcc=13, top_db=80.0)
+    res = {"f32": {"stft_rel": 0.0, "mel_db_abs": 0.0, "mfcc_rel": 0.0}, "bf16": {"mel_db_abs": 0.0, "mfcc_rel": 0.0}}
+
+    def run(kind, prec, i):
+        ioff = B.clips[i][0]
+        n = len(clips[i])
+        S = SpectralBatch([n], kind=kind, precision=prec, **common)
+        o = S.empty_output()
+        S.run(d_in[ioff:ioff + n], o)
+        torch.cuda.synchronize()
+        _, _, rows, frames = S.clips[0]
+        a = o[:rows * frames].cpu().numpy().reshape(rows, frames).astype(np.float64)
+        S.close()
+        return a
+
+    for i in idx:
+        mag, mel, mf = fp64_reference(clips[i])
+        res["f32"]["stft_rel"] = max(res["f32"]["stft_rel"], float(np.abs(run("stft", "f32", i) - mag).max() / mag.max()))
+        for prec in ("f32", "bf16"):
+            r = res[prec]
+            r["mel_db_abs"] = max(r["mel_db_abs"], float(np.abs(_db(run("mel", prec, i)) - _db(mel)).max()))
+            r["mfcc_rel"] = max(r["mfcc_rel"], float(np.abs(run("mfcc", prec, i) - mf).max() / np.abs(mf).max()))
+    res["f32"]["within_targets"] = bool(res["f32"]["stft_rel"] <= 1e-5 and res["f32"]["mel_db_abs"] <= 1e-3
+                                        and res["f32"]["mfcc_rel"] <= 1e-4)
+    res["bf16"]["within_0.05_db"] = bool(res["bf16"]["mel_db_abs"] <= 0.05)
+    res["sample"] = (f"clips {list(map(int, idx))} of the uneven batch (shortest, longest and four between), "
+                     f"vs float64 numpy (rfft, Slaney mel, dB with top_db 80, DCT-II ortho)")
+    return res
+
+
+def large_fft_leg(torch, steps):
+    """The large-FFT batch: 1024 clips x 10 s at 44.1 kHz, FP16 input, n_fft
+    4096, hop 1024, 256 mels, 20 MFCC, FP16 mel / DCT contractions on MFMA
+    (FP32 FFT): one persistent launch per step from a hipGraph, grid and shape
+    autotuned."""
+    from metalmom_amd.batch import SpectralBatch
+    from metalmom_amd.workloads import large_fft_batch
+    t0 = time.perf_counter()
+    clips = large_fft_batch()
+    gen_s = time.perf_counter() - t0
+    lens = [len(c) for c in clips]
+    B = SpectralBatch(lens, kind="mfcc", sr=44100, n_fft=4096, hop_length=1024, n_mels=256, n_mfcc=20,
+                      precision="f16", input_dtype="f16")
+    d_in = B.pack(clips)
+    del clips
+    out = B.empty_output()
+    st = torch.cuda.Stream()
+    wpc, _ = B.autotune(d_in, out, iters=2)
+    for _ in range(2):
+        B.run(d_in, out, stream=st, graph=True)
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    t0 = time.perf_counter()
+    for a, b in ev:
+        a.record(st)
+        B.run(d_in, out, stream=st, graph=True)
+        b.record(st)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    step_ms = [a.elapsed_time(b) for a, b in ev]
+    k_ms, _ = B.time(d_in, out, iters=10)
+    frames = B.total_frames
+    bytes_launch = 2 * B.total_input + 4 * 20 * frames
+    rl = kernel_roofline(frames, bytes_launch, 4096, B.n_tiles, mfma_flops_per_tile(44100, 4096, 256, 20, "f16"),
+                         useful_mfma_flops_per_frame(44100, 4096, 256, 20), "f16", k_ms)
+    sh = B.launch_shape()
+    B.close()
+    return {"workload": "1024 clips x 10 s @44.1 kHz, FP16 input, n_fft 4096 hop 1024, 256 mels, 20 MFCC, "
+                        "FP16 MFMA mel/DCT (FP32 FFT), one persistent hipGraph launch per step",
+            "frames": frames, "frames_per_s": frames * steps / wall, "step_ms": timing_stats(step_ms),
+            "kernel_ms": k_ms, "persistent_wg_per_cu": wpc, "launch": sh, "roofline": rl,
+            "input_generation_s": gen_s}
 
 
 def istft_leg(torch, n_samples, iters, with_cpu):
@@ -372,7 +544,6 @@ def descriptor_leg(torch, n_samples, iters, with_cpu=False):
     return res
 
 
-FP32_VECTOR_PEAK_TFLOPS = 157.3
 
 
 def _fma_roofline(taps, ms):
@@ -465,6 +636,41 @@ def resample_leg(torch, n_samples, iters, with_cpu=False):
     return res
 
 
+def serve_leg(rates, seconds, sources, pcm=None, kinds=None, precision="f32", backlog=0):
+    """Open-loop Poisson serving at each rate (after a 0.5 s warm-up at that
+    rate) and, with backlog > 0, a saturation point: that many requests
+    queued at t = 0. Latency runs from each request's scheduled arrival to
+    its result on the host."""
+    from metalmom_amd.serve import KINDS, SpectralServer, open_loop
+    kinds = kinds or KINDS
+    out = []
+    for rate in list(rates) + ([float("inf")] if backlog else []):
+        srv = SpectralServer(hbm_budget_bytes=32 << 30)
+        kw = dict(sources=sources, pcm=pcm, kinds=kinds, precision=precision)
+        if np.isfinite(rate):
+            open_loop(srv, rate=rate, duration=0.5, seed=99, **kw)
+            r = open_loop(srv, rate=rate, duration=seconds, seed=4, **kw)
+        else:
+            open_loop(srv, rate=float("inf"), n_requests=max(1, backlog // 4), seed=99, **kw)
+            r = open_loop(srv, rate=float("inf"), n_requests=backlog, seed=4, **kw)
+        st = srv.stats()
+        srv.close()
+        e = {"rate_req_s": rate if np.isfinite(rate) else "saturation",
+             "achieved_rate": r["achieved_rate"], "requests": r["n_requests"],
+             "p50_ms": r["p50"] * 1e3, "p99_ms": r["p99"] * 1e3,
+             "server_p50_ms": r["server_p50"] * 1e3, "server_p99_ms": r["server_p99"] * 1e3,
+             "frames_per_s": r["frames_per_s"], "device_steps": st["steps"],
+             "types": "+".join(kinds), "precision": precision}
+        if pcm is not None:
+            e["input"] = "int16 stereo PCM, resampled to 22.05 kHz on the GPU"
+        out.append(e)
+        log(f"serving {e['types']} {precision}{' PCM' if pcm is not None else ''} @{e['rate_req_s']}: "
+            f"submitted {r['achieved_rate']:.0f} req/s, p50 {e['p50_ms']:.2f} ms, p99 {e['p99_ms']:.2f} ms "
+            f"(server {e['server_p50_ms']:.2f} / {e['server_p99_ms']:.2f}), {r['frames_per_s'] / 1e6:.2f} M frames/s, "
+            f"{st['steps']} steps for {r['n_requests']} requests")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -473,14 +679,22 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain single-clip leg")
     ap.add_argument("--profile", action="store_true", help="short run for a profiler")
-    ap.add_argument("--serve-rates", default="1000,5000,8000",
-                    help="Poisson request rates (req/s) for the serving leg; empty to skip")
+    ap.add_argument("--serve-rates", default="200,500,1000,2000,5000",
+                    help="Poisson request rates (req/s) for the serving legs; empty to skip")
     ap.add_argument("--serve-seconds", type=float, default=2.0)
+    ap.add_argument("--serve-backlog", type=int, default=4000,
+                    help="requests queued at once for the saturation point (0: none)")
+    ap.add_argument("--serve-bf16-rates", default="1000,5000",
+                    help="rates of the BF16 MFCC-only serving leg; empty to skip")
     ap.add_argument("--serve-pcm-rates", default="1000,5000",
                     help="rates for the raw-PCM serving leg (int16 stereo at 22.05 / 44.1 kHz, analysed at "
                          "22.05 kHz); empty to skip")
     ap.add_argument("--grid", type=int, default=0,
                     help="persistent workgroups per CU (skips the autotuner; 0 = autotune)")
+    ap.add_argument("--shape", type=int, default=0,
+                    help="fused-kernel workgroup shape with --grid (1: 16x1, 2: 8x1, 3: 8x2; 0: default)")
+    ap.add_argument("--no-large-fft", action="store_true", help="skip the large-FFT FP16 batch leg")
+    ap.add_argument("--no-errors", action="store_true", help="skip the FP64 error report")
     ap.add_argument("--no-istft", action="store_true", help="skip the inverse-STFT leg")
     ap.add_argument("--no-onset", action="store_true", help="skip the onset-strength leg")
     ap.add_argument("--no-chroma", action="store_true", help="skip the chroma leg")
@@ -517,27 +731,35 @@ def main():
     log(f"batch: {len(clips)} clips, {B.total_input} samples, {frames} frames, {B.n_tiles} tiles")
 
     if args.grid > 0:
+        if args.shape > 0:
+            B.set_shape(args.shape)
         best_grid = args.grid
         B.set_grid(best_grid)
-        log(f"persistent grid: {best_grid} workgroups/CU (fixed)")
+        log(f"persistent grid: {best_grid} workgroups/CU (fixed), launch {B.launch_shape()}")
     else:
         best_grid, tune_ms = B.autotune(d_in, out, iters=2)
-        log(f"autotuned persistent grid: {best_grid} workgroups/CU ({tune_ms:.3f} ms/step)")
+        log(f"autotuned: {B.launch_shape()} ({tune_ms:.3f} ms/step)")
+    launch = B.launch_shape()
 
     steps = 3 if args.profile else args.steps
     warm = 1 if args.profile else args.warmup
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
     with torch.cuda.stream(stream):
         for _ in range(warm):
             B.run(d_in, out, stream=stream, graph=True)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        for _ in range(steps):
+        for a, b in ev:
+            a.record(stream)
             B.run(d_in, out, stream=stream, graph=True)
+            b.record(stream)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
     value = frames * steps / dt
     ms_step = dt * 1e3 / steps
-    log(f"bf16 batched: {value / 1e6:.1f} M frames/s ({ms_step:.3f} ms/step)")
+    step_stats = timing_stats([a.elapsed_time(b) for a, b in ev])
+    log(f"bf16 batched: {value / 1e6:.1f} M frames/s ({ms_step:.3f} ms/step; events: mean {step_stats['mean']:.3f}, "
+        f"min {step_stats['min']:.3f}, p50 {step_stats['p50']:.3f} ms)")
 
     if args.copy_bytes > 0:
         src = torch.empty(args.copy_bytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
@@ -551,15 +773,18 @@ def main():
     # fused kernel time from HIP events on its own stream (the plan's stream)
     k_ms, s_ms = B.time(d_in, out, iters=5 if args.profile else 20)
     bytes_launch = 4 * B.total_input + 4 * n_mfcc * frames
-    achieved = bytes_launch / (k_ms * 1e-3) / 1e9
-    log(f"fused kernel {k_ms:.3f} ms, step {s_ms:.3f} ms, {achieved:.0f} GB/s algorithmic")
-
     traffic = measured_traffic(bytes_launch)
+    roofline = kernel_roofline(frames, bytes_launch, 2048, B.n_tiles, mfma_flops_per_tile(22050, 2048, 128, n_mfcc, "bf16"),
+                               useful_mfma_flops_per_frame(22050, 2048, 128, n_mfcc), "bf16", k_ms, traffic)
+    log(f"fused kernel {k_ms:.3f} ms, step {s_ms:.3f} ms: FFT {roofline['achieved']:.1f} TFLOP/s "
+        f"({100 * roofline['frac']:.1f} % of {FP32_VECTOR_PEAK_TFLOPS} FP32 vector), HBM {roofline['hbm']['achieved']:.0f} GB/s "
+        f"({100 * roofline['hbm']['frac']:.1f} %), MFMA {roofline['mfma']['achieved']:.1f} TFLOP/s executed "
+        f"({100 * roofline['mfma']['frac']:.2f} % of {MFMA_PEAK['bf16']:.0f})")
     if traffic is not None:
         log(f"HBM traffic per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated): {traffic / 1e6:.1f} MB "
             f"= {traffic / bytes_launch:.2f}x algorithmic")
 
-    extra = {}
+    extra = {"step_ms_events": step_stats}
     if not args.profile:
         # FP32 batched path on the same batch
         B32 = make("f32")
@@ -567,21 +792,32 @@ def main():
         B32.autotune(d_in, out32, iters=2)
         B32.run(d_in, out32, graph=True)
         torch.cuda.synchronize()
+        n32 = max(5, steps // 2)
         t0 = time.perf_counter()
-        for _ in range(max(3, steps // 2)):
+        for _ in range(n32):
             B32.run(d_in, out32, graph=True)
         torch.cuda.synchronize()
-        f32_fps = frames * max(3, steps // 2) / (time.perf_counter() - t0)
+        f32_fps = frames * n32 / (time.perf_counter() - t0)
+        k32, _ = B32.time(d_in, out32, iters=10)
         extra["batched_fp32_frames_per_s"] = f32_fps
-        # BF16 vs FP32 GPU agreement on the MFCCs (the FP64-oracle errors are in tests/)
-        a, b = out.cpu().numpy(), out32.cpu().numpy()
-        extra["bf16_vs_fp32_mfcc_max_rel"] = float(np.abs(a - b).max() / np.abs(b).max())
-        log(f"fp32 batched: {f32_fps / 1e6:.1f} M frames/s")
+        extra["batched_fp32_kernel_ms"] = k32
+        extra["batched_fp32_launch"] = B32.launch_shape()
+        log(f"fp32 batched: {f32_fps / 1e6:.1f} M frames/s (kernel {k32:.3f} ms)")
+        B32.close()
+
+        if not args.no_errors:
+            order = np.argsort(lens)
+            idx = sorted({int(order[0]), int(order[-1])} | {int(order[len(order) * q // 5]) for q in (1, 2, 3, 4)})
+            err = error_report(torch, clips, d_in, B, idx)
+            extra["error_vs_fp64"] = err
+            log(f"error vs fp64 on {len(idx)} clips: FP32 STFT {err['f32']['stft_rel']:.2e} rel, mel "
+                f"{err['f32']['mel_db_abs']:.2e} dB, MFCC {err['f32']['mfcc_rel']:.2e} rel; BF16 mel "
+                f"{err['bf16']['mel_db_abs']:.2e} dB, MFCC {err['bf16']['mfcc_rel']:.2e} rel")
 
         if not args.no_plain:
             # plain path: one clip per launch, FP32, no graph, in sequence on
             # one stream (each plan would otherwise use its own pooled stream
-            # and clips could overlap across hardware queues)
+            # and clips could overlap across hardware queues); 5 timed passes
             singles = [SpectralBatch([n], kind="mfcc", n_fft=2048, hop_length=512, n_mels=128,
                                      n_mfcc=n_mfcc, precision="f32") for n in lens]
             ins = [d_in[B.clips[i][0]:B.clips[i][0] + lens[i]] for i in range(len(lens))]
@@ -590,54 +826,42 @@ def main():
             for s, i_, o_ in zip(singles, ins, outs):
                 s.run(i_, o_, stream=plain_stream)
             torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            for s, i_, o_ in zip(singles, ins, outs):
-                s.run(i_, o_, stream=plain_stream)
-            torch.cuda.synchronize()
-            plain_fps = frames / (time.perf_counter() - t0)
+            passes = []
+            for _ in range(5):
+                t0 = time.perf_counter()
+                for s, i_, o_ in zip(singles, ins, outs):
+                    s.run(i_, o_, stream=plain_stream)
+                torch.cuda.synchronize()
+                passes.append((time.perf_counter() - t0) * 1e3)
+            pst = timing_stats(passes)
+            plain_fps = frames / (pst["mean"] * 1e-3)
             extra["plain_fp32_frames_per_s"] = plain_fps
+            extra["plain_fp32_pass_ms"] = pst
             extra["speedup_vs_plain_fp32"] = value / plain_fps
-            log(f"plain fp32 single-clip: {plain_fps / 1e6:.2f} M frames/s; batched bf16 is "
-                f"{value / plain_fps:.1f}x")
+            log(f"plain fp32 single-clip: {plain_fps / 1e6:.2f} M frames/s (pass mean {pst['mean']:.2f} ms, min "
+                f"{pst['min']:.2f}, p50 {pst['p50']:.2f}); batched bf16 is {value / plain_fps:.2f}x")
             for s in singles:
                 s.close()
 
+    if not args.no_large_fft and not args.profile:
+        lf = large_fft_leg(torch, max(5, steps // 2))
+        extra["large_fft_f16"] = lf
+        log(f"large-FFT FP16 batch: {lf['frames_per_s'] / 1e6:.1f} M frames/s, kernel {lf['kernel_ms']:.3f} ms, "
+            f"{lf['roofline']['hbm']['achieved']:.0f} GB/s, FFT {lf['roofline']['achieved']:.1f} TFLOP/s "
+            f"(inputs generated in {lf['input_generation_s']:.1f} s)")
+
     serving = []
     if args.serve_rates and not args.profile:
-        # continuous batching server: open-loop Poisson arrivals, mixed request
-        # types (STFT / mel 128 / MFCC 13), sr in {22050, 44100}, log-normal
-        # lengths (median 5 s); latency = submission to result on the host
-        from metalmom_amd.serve import SpectralServer, make_sources, open_loop
+        from metalmom_amd.serve import make_sources, pcm_sources
         sources = make_sources()
-        for rate in [float(r) for r in args.serve_rates.split(",") if r]:
-            srv = SpectralServer(hbm_budget_bytes=32 << 30)
-            # warm-up at the measured rate: grows the lane buffers and the
-            # process heap that results are copied into to their steady size
-            open_loop(srv, rate=rate, duration=0.5, seed=99, sources=sources)
-            r = open_loop(srv, rate=rate, duration=args.serve_seconds, seed=4, sources=sources)
-            st = srv.stats()
-            srv.close()
-            serving.append({"rate_req_s": rate, "achieved_rate": r["achieved_rate"],
-                            "requests": r["n_requests"], "p50_ms": r["p50"] * 1e3, "p99_ms": r["p99"] * 1e3,
-                            "frames_per_s": r["frames_per_s"], "device_steps": st["steps"]})
-            log(f"serving @{rate:.0f} req/s: p50 {r['p50'] * 1e3:.2f} ms, p99 {r['p99'] * 1e3:.2f} ms, "
-                f"{r['frames_per_s'] / 1e6:.2f} M frames/s, {st['steps']} steps for {r['n_requests']} requests")
-        # the same schedule as raw int16 stereo PCM at the source rates, mixed
-        # down and resampled to 22.05 kHz on the GPU inside each step
-        from metalmom_amd.serve import pcm_sources
-        psrc = pcm_sources(sources) if args.serve_pcm_rates else None
-        for rate in [float(r) for r in args.serve_pcm_rates.split(",") if r]:
-            srv = SpectralServer(hbm_budget_bytes=32 << 30)
-            open_loop(srv, rate=rate, duration=0.5, seed=99, sources=sources, pcm=psrc)
-            r = open_loop(srv, rate=rate, duration=args.serve_seconds, seed=4, sources=sources, pcm=psrc)
-            st = srv.stats()
-            srv.close()
-            serving.append({"rate_req_s": rate, "input": "int16 stereo PCM, resampled to 22.05 kHz on the GPU",
-                            "achieved_rate": r["achieved_rate"], "requests": r["n_requests"],
-                            "p50_ms": r["p50"] * 1e3, "p99_ms": r["p99"] * 1e3,
-                            "frames_per_s": r["frames_per_s"], "device_steps": st["steps"]})
-            log(f"serving PCM @{rate:.0f} req/s: p50 {r['p50'] * 1e3:.2f} ms, p99 {r['p99'] * 1e3:.2f} ms, "
-                f"{r['frames_per_s'] / 1e6:.2f} M frames/s, {st['steps']} steps for {r['n_requests']} requests")
+        rates = [float(r) for r in args.serve_rates.split(",") if r]
+        serving += serve_leg(rates, args.serve_seconds, sources, backlog=args.serve_backlog)
+        if args.serve_bf16_rates:
+            serving += serve_leg([float(r) for r in args.serve_bf16_rates.split(",") if r], args.serve_seconds,
+                                 sources, kinds=("mfcc",), precision="bf16", backlog=args.serve_backlog)
+        if args.serve_pcm_rates:
+            serving += serve_leg([float(r) for r in args.serve_pcm_rates.split(",") if r], args.serve_seconds,
+                                 sources, pcm=pcm_sources(sources))
 
     if not args.no_istft:
         ist = istft_leg(torch, B.total_input, 3 if args.profile else 20, not args.no_cpu and not args.profile)
@@ -706,10 +930,8 @@ def main():
                                "n_fft 2048 hop 512, 128 mels, 13 MFCC, top_db 80, BF16 mel/DCT MFMA "
                                "(FP32 FFT), one persistent hipGraph launch per step",
                    "clips": len(clips), "frames": frames, "samples": B.total_input,
-                   "persistent_wg_per_cu": best_grid},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel_ms": k_ms, "bytes_per_launch": bytes_launch},
+                   "persistent_wg_per_cu": best_grid, "launch": launch},
+        "roofline": roofline,
         "cpu_baseline": cpu,
     }
     rec.update(extra)

@@ -256,8 +256,9 @@ int32_t mm_stft_complex(mm_context ctx, const float* signal_data, int64_t signal
  * signal_length samples, d_out receives one value per STFT frame (count
  * through out_frames, which may be NULL). Enqueued on `stream` without a host
  * synchronisation after the first call for a clip length and parameter set.
- * Needs a live context (it holds the magnitude spectrogram in between); calls
- * on one context must not overlap. A non-power-of-two n_fft is rejected. */
+ * Needs a live context (it holds the magnitude spectrogram in between); a
+ * later call on the same context from another stream (or a host-buffer call)
+ * first waits for this call's work. A non-power-of-two n_fft is rejected. */
 int32_t mm_spectral_descriptor_device(mm_context ctx, const float* d_signal, int64_t signal_length,
                                       int32_t sample_rate, int32_t n_fft, int32_t hop_length,
                                       int32_t win_length, int32_t center, int32_t kind, float param,
@@ -265,8 +266,9 @@ int32_t mm_spectral_descriptor_device(mm_context ctx, const float* d_signal, int
 /* mm_spectral_contrast on device memory: d_out receives [n_bands + 1,
  * frames] (frames through out_frames, which may be NULL). Enqueued on
  * `stream` without a host synchronisation after the first call for a clip
- * length and parameter set. Needs a live context; calls on one context must
- * not overlap. A non-power-of-two n_fft is rejected. */
+ * length and parameter set. Needs a live context; a later call on the same
+ * context from another stream (or a host-buffer call) first waits for this
+ * call's work. A non-power-of-two n_fft is rejected. */
 int32_t mm_spectral_contrast_device(mm_context ctx, const float* d_signal, int64_t signal_length,
                                     int32_t sample_rate, int32_t n_fft, int32_t hop_length,
                                     int32_t win_length, int32_t center, int32_t n_bands, float f_min,
@@ -278,7 +280,8 @@ int32_t mm_spectral_contrast_device(mm_context ctx, const float* d_signal, int64
  * without a host synchronisation after the first call for a clip length and
  * parameter set (which uploads the clip's work list and, on first use of a
  * configuration, builds its tables). Needs a live context (it holds the mel
- * spectrogram in between); calls on one context must not overlap. A
+ * spectrogram in between); a later call on the same context from another
+ * stream (or a host-buffer call) first waits for this call's work. A
  * non-power-of-two n_fft is rejected (-1). */
 int32_t mm_onset_strength_device(mm_context ctx, const float* d_signal, int64_t signal_length,
                                  int32_t sample_rate, int32_t n_fft, int32_t hop_length,
@@ -306,7 +309,8 @@ int64_t mm_istft_length(int32_t n_freqs, int32_t n_frames, int32_t hop_length, i
  * multiple of 16 lets the kernel read whole cache lines), d_out
  * mm_istft_length(...) floats; enqueued on `stream` (a hipStream_t; NULL =
  * the null stream) without a host synchronisation. The context provides the
- * scratch of the generic path; calls on one context must not overlap. */
+ * scratch of the generic path; a later call on the same context from another
+ * stream (or a host-buffer call) first waits for this call's work. */
 int32_t mm_istft_device(mm_context ctx, const float* d_stft, int32_t ld_frames, int32_t n_freqs,
                         int32_t n_frames, int32_t sample_rate, int32_t hop_length, int32_t win_length,
                         int32_t center, int64_t output_length, float* d_out, void* stream);

@@ -25,8 +25,9 @@ def _mel_to_hz(mel):
     return np.where(mel < min_log_mel, mel * f_sp, min_log_hz * np.exp((mel - min_log_mel) * logstep))
 
 
-def mel_filterbank(sr=22050, n_fft=2048, n_mels=128, fmin=0.0, fmax=None) -> np.ndarray:
-    """``[n_mels, n_fft//2+1]`` float32 triangles with Slaney area normalisation."""
+def mel_filterbank(sr=22050, n_fft=2048, n_mels=128, fmin=0.0, fmax=None, dtype=np.float32) -> np.ndarray:
+    """``[n_mels, n_fft//2+1]`` triangles with Slaney area normalisation,
+    computed in float64 and returned as ``dtype`` (float32 by default)."""
     fmax = sr / 2.0 if fmax is None else float(fmax)
     n_freqs = n_fft // 2 + 1
     pts = _mel_to_hz(np.linspace(_hz_to_mel(fmin), _hz_to_mel(fmax), n_mels + 2))
@@ -36,4 +37,4 @@ def mel_filterbank(sr=22050, n_fft=2048, n_mels=128, fmin=0.0, fmax=None) -> np.
         up = np.where((freqs >= lo) & (freqs <= ce) & (ce != lo), (freqs - lo) / (ce - lo), 0.0)
         down = np.where((freqs > ce) & (freqs <= hi) & (hi != ce), (hi - freqs) / (hi - ce), 0.0)
     w = np.where(up > 0, up, down) * (2.0 / (hi - lo))
-    return w.astype(np.float32)
+    return w.astype(dtype)

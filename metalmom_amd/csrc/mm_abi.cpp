@@ -31,7 +31,7 @@ namespace {
 
 struct Ctx {
   Lane* lane = nullptr;
-  Lane* get() {
+  Lane* acquire() {
     if (!lane) {
       Engine* e = Engine::get();
       if (!e) return nullptr;
@@ -39,7 +39,21 @@ struct Ctx {
     }
     return lane;
   }
+  // host-buffer paths (work on the lane's own stream, synchronous)
+  Lane* get() { return for_stream(acquire() ? lane->stream : nullptr); }
+  // device-buffer paths: work on the caller's stream `st`
+  Lane* for_stream(hipStream_t st) {
+    Lane* l = acquire();
+    if (!l || l->enter(st) != OK) return nullptr;
+    return l;
+  }
 };
+
+// Records the end of a device-entry call's work on the lane (see Lane).
+int32_t device_call_done(Lane* lane, hipStream_t st, int32_t s) {
+  if (s != OK) return s;
+  return lane->leave(st);
+}
 
 std::mutex g_reg_mu;
 std::unordered_set<void*>& registry() {
@@ -199,9 +213,10 @@ int32_t mm_spectral_descriptor_device(mm_context ctx, const float* d_signal, int
   r.win = win_length > 0 ? win_length : n_fft;
   r.center = center != 0;
   if (out_frames) *out_frames = frame_count(signal_length, n_fft, r.hop, r.center);
-  Lane* lane = static_cast<Ctx*>(ctx)->get();
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  Lane* lane = static_cast<Ctx*>(ctx)->for_stream(st);
   if (!lane) return ERR_NO_GPU;
-  return enqueue_descriptor(lane, r, d_signal, signal_length, kind, param, d_out, static_cast<hipStream_t>(stream));
+  return device_call_done(lane, st, enqueue_descriptor(lane, r, d_signal, signal_length, kind, param, d_out, st));
 } catch (const std::bad_alloc&) {   // no exception leaves the C ABI
   return MM_ERR_ALLOC_FAILED;
 } catch (...) {
@@ -222,9 +237,10 @@ int32_t mm_spectral_contrast_device(mm_context ctx, const float* d_signal, int64
   r.win = win_length > 0 ? win_length : n_fft;
   r.center = center != 0;
   if (out_frames) *out_frames = frame_count(signal_length, n_fft, r.hop, r.center);
-  Lane* lane = static_cast<Ctx*>(ctx)->get();
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  Lane* lane = static_cast<Ctx*>(ctx)->for_stream(st);
   if (!lane) return ERR_NO_GPU;
-  return enqueue_contrast(lane, r, d_signal, signal_length, n_bands, f_min, d_out, static_cast<hipStream_t>(stream));
+  return device_call_done(lane, st, enqueue_contrast(lane, r, d_signal, signal_length, n_bands, f_min, d_out, st));
 } catch (const std::bad_alloc&) {   // no exception leaves the C ABI
   return MM_ERR_ALLOC_FAILED;
 } catch (...) {
@@ -251,9 +267,10 @@ int32_t mm_onset_strength_device(mm_context ctx, const float* d_signal, int64_t 
   const OnsetShape o = onset_shape(r, signal_length, aggregate != 0, 1);
   if (out_rows) *out_rows = o.rows;
   if (out_cols) *out_cols = o.cols;
-  Lane* lane = static_cast<Ctx*>(ctx)->get();
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  Lane* lane = static_cast<Ctx*>(ctx)->for_stream(st);
   if (!lane) return ERR_NO_GPU;
-  return enqueue_onset(lane, r, d_signal, signal_length, aggregate != 0, 1, d_out, static_cast<hipStream_t>(stream));
+  return device_call_done(lane, st, enqueue_onset(lane, r, d_signal, signal_length, aggregate != 0, 1, d_out, st));
 } catch (const std::bad_alloc&) {   // no exception leaves the C ABI
   return MM_ERR_ALLOC_FAILED;
 } catch (...) {
@@ -279,10 +296,10 @@ int32_t mm_istft_device(mm_context ctx, const float* d_stft, int32_t ld_frames, 
   if (istft_shape(n_freqs, n_frames, hop_length, center, output_length, &sh) != OK) return MM_ERR_INVALID_INPUT;
   const int ld = (ld_frames > 0) ? ld_frames : n_frames;
   if (ld < n_frames) return MM_ERR_INVALID_INPUT;
-  Lane* lane = static_cast<Ctx*>(ctx)->get();
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  Lane* lane = static_cast<Ctx*>(ctx)->for_stream(st);
   if (!lane) return ERR_NO_GPU;
-  return enqueue_istft(sh, sample_rate, win_length, d_stft, ld, d_out, &lane->d_scratch,
-                       static_cast<hipStream_t>(stream));
+  return device_call_done(lane, st, enqueue_istft(sh, sample_rate, win_length, d_stft, ld, d_out, &lane->d_scratch, st));
 } catch (const std::bad_alloc&) {   // no exception leaves the C ABI
   return MM_ERR_ALLOC_FAILED;
 } catch (...) {

@@ -98,7 +98,30 @@ void HostBuf::release() {
   p = nullptr;
   n = 0;
 }
+int32_t Lane::settle() {
+  if (busy_set) {
+    MM_HIP(hipEventSynchronize(busy));
+    busy_set = false;
+  }
+  return OK;
+}
+
+int32_t Lane::enter(hipStream_t st) {
+  if (busy_set && st != busy_stream) return settle();
+  return OK;
+}
+
+int32_t Lane::leave(hipStream_t st) {
+  if (!busy) MM_HIP(hipEventCreateWithFlags(&busy, hipEventDisableTiming));
+  MM_HIP(hipEventRecord(busy, st));
+  busy_stream = st;
+  busy_set = true;
+  return OK;
+}
+
 Lane::~Lane() {
+  (void)settle();
+  if (busy) (void)hipEventDestroy(busy);
   d_small.release();
   d_in.release();
   d_out.release();
@@ -288,6 +311,7 @@ Lane* Engine::acquire_lane() {
 
 void Engine::release_lane(Lane* l) {
   if (!l) return;
+  (void)l->settle();   // the next borrower may rewrite its buffers at once
   std::lock_guard<std::mutex> g(mu_);
   free_lanes_.push_back(l);
 }

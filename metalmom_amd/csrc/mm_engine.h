@@ -63,12 +63,28 @@ struct HostBuf {
 };
 
 // One stream plus its workspaces. Pooled: contexts borrow one on first use.
+//
+// The *_device entry points enqueue work that reads the lane's buffers on the
+// caller's stream and return before it runs. `busy` is recorded after such
+// work; any later use of the lane from another stream (or from the host
+// paths on `stream`, or the lane's return to the pool) first waits for it,
+// so buffers are neither rewritten nor reallocated under a running kernel.
+// Calls on the stream that recorded it are ordered by the stream itself.
 struct Lane {
   hipStream_t stream = nullptr;
   DevBuf d_in, d_out, d_meta, d_scratch, d_scratch2, d_small;
   HostBuf h_in, h_out, h_meta, h_small;
   std::string meta_key;   // work list now in d_meta for device-resident single-clip calls ("" = none)
   std::string bands_key;  // spectral contrast band table now in d_small (at kBandsOff), "" = none
+  hipEvent_t busy = nullptr;          // recorded after the last device-entry work
+  hipStream_t busy_stream = nullptr;  // the stream it was recorded on
+  bool busy_set = false;
+  // before using the lane's buffers for work on `st`
+  int32_t enter(hipStream_t st);
+  // after enqueuing device-entry work on `st` that uses the lane's buffers
+  int32_t leave(hipStream_t st);
+  // waits for every recorded device-entry work
+  int32_t settle();
   ~Lane();
 };
 

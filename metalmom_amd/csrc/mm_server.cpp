@@ -153,9 +153,18 @@ bool same_plan(const SpecRequest& a, const SpecRequest& b) {
 
 size_t a256(size_t x) { return (x + 255) & ~(size_t)255; }
 
+// Device bytes a request holds while in a step: its input and output, plus
+// for configurations outside the fused kernel the generic path's complex
+// spectrogram scratch (T x F x 8 bytes) and, for MFCC, its mel scratch.
 size_t request_bytes(const SpecRequest& r, int64_t len) {
   const int64_t T = frame_count(len, r.n_fft, r.hop, r.center);
-  return a256((size_t)len * 4) + a256((size_t)(out_rows(r) * T * out_elem_floats(r)) * 4) + 1024;
+  size_t b = a256((size_t)len * 4) + a256((size_t)(out_rows(r) * T * out_elem_floats(r)) * 4) + 1024;
+  TileLayout L;
+  if (!tile_layout(r.n_fft, r.out_kind, r.precision, r.n_mels, std::min(r.n_mfcc, r.n_mels), &L)) {
+    b += a256((size_t)T * (size_t)(r.n_fft / 2 + 1) * 8);
+    if (r.out_kind == OUT_MFCC) b += a256((size_t)r.n_mels * (size_t)T * 4 + 256);
+  }
+  return b;
 }
 
 // Device staging of a PCM request: its raw bytes, and its converted samples

@@ -115,17 +115,21 @@ def make_sources(seconds: float = 330.0, seed: int = 4):
     return {sr: chirp_noise_clip(int(seconds * sr), sr, rng) for sr in SRS}
 
 
-def make_schedule(rate: float, duration: float, seed: int = 4):
-    """Poisson arrivals: (t_arrival, kind, sr, n_samples, source offset)."""
+def make_schedule(rate: float, duration: float, seed: int = 4, kinds=KINDS, n_requests: int = 0):
+    """Poisson arrivals: (t_arrival, kind, sr, n_samples, source offset).
+
+    ``rate`` = inf (or <= 0) gives a backlog: ``n_requests`` requests all
+    arriving at t = 0 (the saturation point)."""
     rng = np.random.default_rng(seed)
-    n = max(1, int(rate * duration))
-    t = np.cumsum(rng.exponential(1.0 / rate, n))
+    backlog = not (rate > 0 and np.isfinite(rate))
+    n = max(1, int(n_requests) if backlog else int(rate * duration))
+    t = np.zeros(n) if backlog else np.cumsum(rng.exponential(1.0 / rate, n))
     secs = np.clip(np.exp(np.log(5.0) + rng.standard_normal(n)), 0.5, 300.0)
     sr = np.array(SRS)[rng.integers(0, 2, n)]
-    kind = rng.integers(0, 3, n)
+    kind = rng.integers(0, len(kinds), n)
     lens = np.round(secs * sr).astype(np.int64)
     offs = np.array([rng.integers(0, int(330 * s) - L + 1) for s, L in zip(sr, lens)])
-    return [(float(t[i]), KINDS[kind[i]], int(sr[i]), int(lens[i]), int(offs[i])) for i in range(n)]
+    return [(float(t[i]), kinds[kind[i]], int(sr[i]), int(lens[i]), int(offs[i])) for i in range(n)]
 
 
 def pcm_sources(sources):
@@ -139,26 +143,38 @@ def pcm_sources(sources):
 
 
 def open_loop(server: SpectralServer, rate: float, duration: float = 2.0, seed: int = 4,
-              sources=None, keep_outputs: bool = False, pcm=None, analysis_sr: int = 22050):
+              sources=None, keep_outputs: bool = False, pcm=None, analysis_sr: int = 22050,
+              kinds=KINDS, precision: str = "f32", n_requests: int = 0):
     """Submits a Poisson schedule in real time and collects every result.
 
+    Latency is measured from each request's scheduled arrival (t0 + its
+    Poisson time), not from when the submitting thread got to it: a late
+    submitter's lag, the copy of the clip into the server and the time to
+    the result on the host all count (no coordinated omission). The server's
+    own submit-to-completion time is reported beside it. ``rate`` = inf
+    submits a backlog of ``n_requests`` at t = 0 (the saturation point).
+
+    ``kinds`` restricts the request types (e.g. ``("mfcc",)``) and
+    ``precision`` sets the mel / DCT contraction arithmetic (f32, bf16, f16).
     With ``pcm`` (from ``pcm_sources``) every clip is submitted as int16
     stereo PCM at its source rate and analysed at ``analysis_sr``: the GPU
     mixes it to mono and resamples it inside the request's step.
 
-    Returns dict(latencies, p50, p99, frames, wall, frames_per_s, achieved_rate,
-    outputs (if kept), schedule).
+    Returns dict(latencies (arrival to result), p50, p99, server_p50,
+    server_p99, frames, wall, frames_per_s, achieved_rate (measured
+    submissions per second), n_requests, outputs (if kept), schedule).
     """
     sources = sources or make_sources(seed=seed)
-    sched = make_schedule(rate, duration, seed)
+    sched = make_schedule(rate, duration, seed, kinds=kinds, n_requests=n_requests)
     if pcm is not None:
         from ._native import PCM_S16
-        params = {(k, sr): server.params(k, sr=analysis_sr) for k in KINDS for sr in SRS}
+        params = {(k, sr): server.params(k, sr=analysis_sr, precision=precision) for k in kinds for sr in SRS}
     else:
-        params = {(k, sr): server.params(k, sr=sr) for k in KINDS for sr in SRS}
+        params = {(k, sr): server.params(k, sr=sr, precision=precision) for k in kinds for sr in SRS}
     ids = [None] * len(sched)
     ready = threading.Semaphore(0)
     lat = [0.0] * len(sched)
+    t_sub = [0.0] * len(sched)
     frames = [0] * len(sched)
     outs = [None] * len(sched) if keep_outputs else None
 
@@ -182,11 +198,17 @@ def open_loop(server: SpectralServer, rate: float, duration: float = 2.0, seed: 
             ids[i] = server.submit_pcm(pcm[sr][off:off + n], 2, PCM_S16, sr, params[(kind, sr)])
         else:
             ids[i] = server.submit(sources[sr][off:off + n], params[(kind, sr)])
+        # the server stamps its submit time as submit() returns (same monotonic
+        # clock as perf_counter); the lag since the scheduled arrival is added
+        t_sub[i] = time.perf_counter()
         ready.release()
     th.join()
     wall = time.perf_counter() - t0
-    la = np.array(lat)
+    ls = np.array(lat)                                                          # submit -> result
+    la = ls + np.array([t_sub[i] - (t0 + sched[i][0]) for i in range(len(sched))])   # arrival -> result
+    span = t_sub[-1] - t0
     return {"latencies": la, "p50": float(np.percentile(la, 50)), "p99": float(np.percentile(la, 99)),
+            "server_p50": float(np.percentile(ls, 50)), "server_p99": float(np.percentile(ls, 99)),
             "frames": int(sum(frames)), "wall": wall, "frames_per_s": sum(frames) / wall,
-            "achieved_rate": len(sched) / sched[-1][0] if sched[-1][0] > 0 else float("nan"),
+            "achieved_rate": len(sched) / span if span > 0 else float("inf"),
             "n_requests": len(sched), "outputs": outs, "schedule": sched}
