@@ -736,6 +736,46 @@ __device__ __forceinline__ void real_split_regs(const float2 (&z)[FftShape<LOG2M
   nyq_re = 2.0f * (z[0][0].x - z[0][0].y);                   // 2 X[M]
 }
 
+// Real-FFT split in registers, one mirror pair per computation. Lane l
+// takes the pairs (k, M-k) with k = l + 64 q + BL r for r < RL/2 (each pair
+// of the spectrum exactly once: the mirror of such a k sits at r' >= RL/2)
+// and returns 2 X[k] in xk[q][r] and 2 X[M-k] = 2 conj(E + i W_N^k O) in
+// xm[q][r] (lane 0, q = r = 0: 2 X[0] and the Nyquist 2 X[M]); lane 0 also
+// returns xh = 2 X[M/2] = 2 conj(Z[M/2]), the one bin that is its own mirror
+// besides DC. Half the mirror fetches, twiddle loads and arithmetic of
+// computing every bin on its own lane.
+template <int LOG2M>
+__device__ __forceinline__ void real_split_pairs(const float2 (&z)[FftShape<LOG2M>::QL][FftShape<LOG2M>::RL],
+                                                 const float2* __restrict__ twN, int lane_in,
+                                                 float2 (&xk)[FftShape<LOG2M>::QL][FftShape<LOG2M>::RL / 2],
+                                                 float2 (&xm)[FftShape<LOG2M>::QL][FftShape<LOG2M>::RL / 2],
+                                                 float2& xh) {
+  const int lane = opaque(lane_in);
+  using S = FftShape<LOG2M>;
+  constexpr int QL = S::QL, RL = S::RL, RH = RL / 2, BL = S::BL;
+  static_assert(RL >= 2 && RL % 2 == 0, "pairing needs an even last radix");
+  const int paddr = ((64 - lane) & 63) << 2;
+#pragma unroll
+  for (int q = 0; q < QL; ++q) {
+#pragma unroll
+    for (int r = 0; r < RH; ++r) {
+      const float2 src = z[QL - 1 - q][RL - 1 - r];
+      float2 bz = make_float2(__int_as_float(__builtin_amdgcn_ds_bpermute(paddr, __float_as_int(src.x))),
+                              __int_as_float(__builtin_amdgcn_ds_bpermute(paddr, __float_as_int(src.y))));
+      const float2 own = (q == 0) ? z[0][(RL - r) % RL] : z[(QL - q) % QL][RL - 1 - r];
+      if (lane == 0) bz = own;
+      const float2 A = z[q][r];
+      const float2 E = make_float2(A.x + bz.x, A.y - bz.y);   // 2E
+      const float2 O = make_float2(A.x - bz.x, A.y + bz.y);   // 2O
+      // W_N^k, k = lane + 64 q + BL r
+      const float2 T = cmul(O, ldg<float2>(twN + 64 * q + BL * r, (unsigned)lane * 8u));
+      xk[q][r] = make_float2(E.x + T.y, E.y - T.x);            // 2 X[k]
+      xm[q][r] = make_float2(E.x - T.y, -(E.y + T.x));         // 2 X[M-k]
+    }
+  }
+  xh = make_float2(2.0f * z[0][RH].x, -2.0f * z[0][RH].y);    // lane 0: 2 X[M/2]
+}
+
 // Real-FFT split: from Z (M complex) to X[k] and X[M-k] for k = lane + 64 i.
 // X[k] = E - i W_N^k O,  X[M-k] = conj(E + i W_N^k O),
 // E = (Z[k] + conj Z[M-k]) / 2,  O = (Z[k] - conj Z[M-k]) / 2.

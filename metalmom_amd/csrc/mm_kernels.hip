@@ -238,6 +238,13 @@ void k_tile(TileArgs a) {
     // their FFT is done (the next frame's FFT reuses the registers and the
     // scratch; earlier frames' rows lie outside the scratch)
     constexpr bool STAGE_EACH = RS && FPW > 1;
+    // spectrum bin of register (b, r) after the paired real split: k for
+    // r < RL/2, M - k of the pair (b, r - RL/2) above (lane 0, b = 0, r =
+    // RL/2: M, the Nyquist bin)
+    auto bin_of = [&](int b, int r) -> int {
+      if (r < RL / 2) return lane + 64 * b + SH::BL * r;
+      return M - (lane + 64 * b + SH::BL * (r - RL / 2));
+    };
     constexpr int NQ = STAGE_EACH ? 1 : FPW;       // frames whose spectra are held at once
     float hv[NQ][QL][RL][EC];
     float hn[NQ][EC];
@@ -264,8 +271,8 @@ void k_tile(TileArgs a) {
 #pragma unroll
           for (int b = 0; b < QL; ++b)
 #pragma unroll
-            for (int r = 0; r < RL; ++r) put(row, lane + 64 * b + SH::BL * r, hv[qh][b][r], fexp[qh]);
-          if (lane == 0) put(row, M, hn[qh], fexp[qh]);
+            for (int r = 0; r < RL; ++r) put(row, bin_of(b, r), hv[qh][b][r], fexp[qh]);
+          if (lane == 0) put(row, M / 2, hn[qh], fexp[qh]);
         } else {
 #pragma unroll
           for (int i = 0; i < NI; ++i) {
@@ -298,35 +305,31 @@ void k_tile(TileArgs a) {
           float2 z[QL][RL];
           frame_fft_split<LOG2M, TIn>(a, clip, fr, reinterpret_cast<float*>(scr), lane, win, twp, z, pf,
                                       PF && pf_tile == tile);
-          float2 x[QL][RL];
-          float nyq;
-          real_split_regs<LOG2M>(z, twn, lane, x, nyq);
+          // hv[b][r]: 2 X[k] of pair (b, r) for r < RL/2, 2 X[M-k] of pair
+          // (b, r - RL/2) for r >= RL/2 (see bin_of); hn: lane 0's X[M/2]
+          float2 xk[QL][RL / 2], xm[QL][RL / 2];
+          float2 xh;
+          real_split_pairs<LOG2M>(z, twn, lane, xk, xm, xh);
+          auto set = [&](float* h, const float2 X) {   // X = 2 X[bin]
+            if constexpr (CPLX) {
+              h[0] = 0.5f * X.x;
+              h[EC - 1] = 0.5f * X.y;
+            } else {
+              const float p2 = fmaf(X.x, X.x, X.y * X.y);   // 4 |X|^2
+              float v = p2;
+              if constexpr (MODE == OUT_STFT_MAG) v = 0.5f * sqrtf(p2);
+              else if constexpr (MODE == OUT_STFT_POWER) v = 0.25f * p2;
+              h[0] = v;
+            }
+          };
 #pragma unroll
           for (int b = 0; b < QL; ++b)
 #pragma unroll
-            for (int r = 0; r < RL; ++r) {
-              const float2 X = x[b][r];   // 2 X[k]
-              if constexpr (CPLX) {
-                hv[qh][b][r][0] = 0.5f * X.x;
-                hv[qh][b][r][EC - 1] = 0.5f * X.y;
-              } else {
-                const float p2 = fmaf(X.x, X.x, X.y * X.y);   // 4 |X[k]|^2
-                float v = p2;
-                if constexpr (MODE == OUT_STFT_MAG) v = 0.5f * sqrtf(p2);
-                else if constexpr (MODE == OUT_STFT_POWER) v = 0.25f * p2;
-                hv[qh][b][r][0] = v;
-              }
+            for (int r = 0; r < RL / 2; ++r) {
+              set(hv[qh][b][r], xk[b][r]);
+              set(hv[qh][b][r + RL / 2], xm[b][r]);
             }
-          if constexpr (CPLX) {
-            hn[qh][0] = 0.5f * nyq;
-            hn[qh][EC - 1] = 0.0f;
-          } else {
-            const float p2 = nyq * nyq;
-            float v = p2;
-            if constexpr (MODE == OUT_STFT_MAG) v = 0.5f * fabsf(nyq);
-            else if constexpr (MODE == OUT_STFT_POWER) v = 0.25f * p2;
-            hn[qh][0] = v;
-          }
+          set(hn[qh], xh);
           if constexpr (!SPEC) {
             // (2|X|)^p from (2|X|)^2 (the 2^p is removed after the mel
             // product): one wave-uniform branch per frame, so the default
