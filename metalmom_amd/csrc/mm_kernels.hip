@@ -95,6 +95,23 @@ void k_tile(TileArgs a) {
 
   const int t_begin = blockIdx.x * a.tiles_per_wg;
   const int t_end = min(t_begin + a.tiles_per_wg, a.n_tiles);
+  if constexpr (MODE == OUT_MFCC) {
+    // phase 2 recomputes only tiles whose top_db floor is active: a
+    // workgroup with none of them leaves before filling its LDS tables
+    if (a.phase == 2) {
+      bool act = false;
+      if (a.use_top_db)
+        for (int t = t_begin + (int)threadIdx.x; t < t_end; t += NW * 64)
+          act |= a.tile_min[t] < key_to_float(a.clip_max_key[ldc(a.tile_clip, t)]) - a.top_db;
+      // workgroup OR through one LDS flag per wave (its own slots, never reused)
+      int* any_act = reinterpret_cast<int*>(lds + NW * a.lds_region_bytes) + 16 + 2 * NW;
+      if ((threadIdx.x & 63) == 0) any_act[threadIdx.x >> 6] = __builtin_amdgcn_ballot_w64(act) != 0;
+      lds_barrier();
+      int any = 0;
+      for (int w = 0; w < NW; ++w) any |= any_act[w];
+      if (!any) return;
+    }
+  }
   MM_PT_DECL
   // Samples of this wave's frame in the next tile, loaded while the current
   // tile's contraction and barriers run (register-split FFTs, one frame per
@@ -1198,7 +1215,7 @@ bool tile_layout(int n_fft, int out_kind, int prec, int n_mels, int n_mfcc, Tile
   const int first = std::max((scr + 15) / 16 * 16, (S * esz + 15) / 16 * 16);
   int rb = (fpw > 1) ? first + (fpw - 1) * S * esz : std::max(S * esz, scr);
   rb = (rb + 15) / 16 * 16;
-  const size_t extras = 16 * 4 + 2 * nw * 4;
+  const size_t extras = 16 * 4 + 2 * nw * 4 + nw * 4;   // frame exponents, per-wave extrema, phase-2 flags
   size_t total = (size_t)nw * rb + extras;
   if (total > lds_limit_bytes()) return false;
   int tab_off = 0;
