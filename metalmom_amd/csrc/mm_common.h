@@ -60,6 +60,7 @@ struct DeviceTables {
   float2* tw = nullptr;         // [M] W_M^t, M = n_fft/2
   float2* twN = nullptr;        // [M] W_N^k, N = n_fft
   float2* twp = nullptr;        // per-pass Stockham twiddles of the fused FFT (< M entries)
+  unsigned char* mft = nullptr; // matrix-core FFT table (n_fft 2048 only; mfft2048)
   float* fb32 = nullptr;        // [nmb*16][fbs] mel weights, zero padded
   uint16_t* fb16 = nullptr;     // same in bf16 or f16 (see fb16_kind)
   uint16_t* fbh = nullptr;      // f16 copy
@@ -78,6 +79,29 @@ struct DeviceTables {
   int nmh = 0;                  // mel blocks of 8 (the contraction's unit of work)
   int ndt = 0;                  // DCT row tiles of 16
 };
+
+// Table of the matrix-core FFT of the fused kernel (mm_fft_mfma.h), built
+// once per configuration (it holds the analysis window) and copied into LDS
+// by each workgroup:
+//   [0, 8 KiB)        window pairs of each lane's input points, [pair 8][lane 64] x 4 floats
+//   [8 KiB, 24 KiB)   constant operand, [tile t 4][block b 2][hi, lo][lane 64] x 8 FP16
+//   [24 KiB, 32 KiB)  stage twiddles W_1024^(n1 k2), [pair 8][lane 64] x 2 float2
+//   [32 KiB, 36 KiB)  split twiddles W_2048^k, [pair 4][lane 64] x 2 float2
+constexpr int kMfWinBytes = 8 * 64 * 16;
+constexpr int kMfOpBytes = 4 * 2 * 2 * 64 * 16;
+constexpr int kMfTw1Bytes = 8 * 64 * 16;
+constexpr int kMfTw2Bytes = 4 * 64 * 16;
+constexpr int kMfOpOff = kMfWinBytes;
+constexpr int kMfTw1Off = kMfOpOff + kMfOpBytes;
+constexpr int kMfTw2Off = kMfTw1Off + kMfTw1Bytes;
+constexpr int kMfTabBytes = kMfTw2Off + kMfTw2Bytes;
+
+// Whether the fused kernel of an output kind (spec: STFT outputs), operand
+// precision and FFT size 2^(log2m+1) runs the matrix-core FFT: the 16-bit mel
+// and MFCC kernels at n_fft 2048 (the FP32 kernels keep the FP32 FFT).
+__host__ __device__ constexpr bool mfft_kernel(bool spec, int prec, int log2m) {
+  return !spec && prec != PREC_F32 && log2m == 10;
+}
 
 // Everything one fused-tile launch needs. Passed by value as a kernel argument.
 struct TileArgs {
@@ -99,6 +123,7 @@ struct TileArgs {
   const float2* tw;
   const float2* twN;
   const float2* twp;
+  const unsigned char* mft;    // matrix-core FFT table (16-bit mel / MFCC at n_fft 2048)
   const float* fb32;
   const uint16_t* fb16;
   const int2* band;            // per block of 8 mels

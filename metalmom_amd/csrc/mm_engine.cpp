@@ -186,6 +186,7 @@ static int32_t make_tables(const TableKey& key, hipStream_t st, std::unique_ptr<
   const size_t o_tw = take((size_t)M * 8);
   const size_t o_twn = take((size_t)M * 8);
   const size_t o_twp = take((size_t)M * 8);   // per-pass tables total < M entries
+  const size_t o_mft = take(key.n_fft == 2048 ? (size_t)kMfTabBytes : 0);
   const size_t o_fb32 = take((size_t)rows * t.fbs * 4);
   const size_t o_fb16 = take((size_t)rows * t.fbs * 2);
   const size_t o_fbh = take((size_t)rows * t.fbs * 2);
@@ -207,6 +208,7 @@ static int32_t make_tables(const TableKey& key, hipStream_t st, std::unique_ptr<
   t.tw = reinterpret_cast<float2*>(b + o_tw);
   t.twN = reinterpret_cast<float2*>(b + o_twn);
   t.twp = reinterpret_cast<float2*>(b + o_twp);
+  t.mft = (key.n_fft == 2048) ? reinterpret_cast<unsigned char*>(b + o_mft) : nullptr;
   t.fb32 = reinterpret_cast<float*>(b + o_fb32);
   t.fb16 = reinterpret_cast<uint16_t*>(b + o_fb16);
   t.fbh = reinterpret_cast<uint16_t*>(b + o_fbh);
@@ -642,6 +644,7 @@ static TileArgs make_args(const SpecRequest& r, const Tables* tb, const BatchMet
   a.tw = t.tw;
   a.twN = t.twN;
   a.twp = t.twp;
+  a.mft = t.mft;
   a.fb32 = t.fb32;
   a.fb16 = (r.precision == PREC_F16) ? t.fbh : t.fb16;
   a.band = (r.precision == PREC_F32) ? t.band32 : t.band16;
@@ -749,15 +752,23 @@ int32_t enqueue_batch(Engine* eng, const SpecRequest& r, const Tables* tb, const
     const int pc = (r.precision == PREC_F32) ? 0 : 1;
     const int nch = tb->cache_chunks[pc];
     const size_t off = (L.lds_bytes + 255) / 256 * 256;
-    const size_t fbb = (size_t)nch * kFbChunkBytes;
     const size_t dcb = (r.out_kind == OUT_MFCC) ? (size_t)tb->t.ndt * tb->t.nmh * 1024 : 0;
-    // (within the LDS share of the workgroups per CU the shape is sized for)
-    if (nch > 0 && off + fbb + dcb <= lds_limit_bytes() / std::max(L.occ, 1)) {
-      a.fb_lds_off = (int)off;
-      a.dct_lds_off = dcb ? (int)(off + fbb) : 0;
-      a.cache_src = tb->t.cache_src + pc * kMaxCacheChunks;
-      a.cache_chunks = nch;
-      LL.lds_bytes = off + fbb + dcb;
+    // (within the LDS share of the workgroups per CU the shape is sized for):
+    // the DCT operands (read by every wave in every tile) when they fit, and
+    // beside them the filterbank chunks in block order, all of them or as
+    // many whole blocks' worth as fit (the kernel reads a block from LDS when
+    // all of its chunks are cached)
+    const size_t lim = lds_limit_bytes() / std::max(L.occ, 1);
+    if (off + dcb <= lim && (nch > 0 || dcb > 0)) {
+      const int nc = (int)std::min<size_t>((size_t)nch, (lim - off - dcb) / kFbChunkBytes);
+      if (nc > 0) {
+        a.fb_lds_off = (int)off;
+        a.cache_src = tb->t.cache_src + pc * kMaxCacheChunks;
+        a.cache_chunks = nc;
+      }
+      const size_t fbc = (size_t)nc * kFbChunkBytes;
+      a.dct_lds_off = dcb ? (int)(off + fbc) : 0;
+      LL.lds_bytes = off + fbc + dcb;
     }
   }
   if (a.use_top_db) MM_HIP(launch_fill_u32(mp.clip_max_key, m.n_clips, float_to_key(-INFINITY), st));

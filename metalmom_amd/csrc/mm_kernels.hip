@@ -13,6 +13,7 @@
 //   DCT-II ortho basis product               Features/MFCC.swift:79-106
 #include "mm_common.h"
 #include "mm_fft_dev.h"
+#include "mm_fft_mfma.h"
 #include "mm_kernels.h"
 
 #include <algorithm>
@@ -86,7 +87,10 @@ void k_tile(TileArgs a) {
   // A wave's region: its FFT scratch with its last frame's staging row over
   // it (written once that frame's FFT is done), then the rows of its other
   // frames from max(scratch, row) on; FPW = 1 needs only the first part.
-  constexpr int SCRB = ((M + M / 16) * (M >= 512 ? 4 : 8) + 15) / 16 * 16;
+  // 16-bit mel / MFCC at n_fft 2048: the FFT runs on the matrix cores from
+  // registers (mfft2048) and needs no LDS scratch
+  constexpr bool MF = mfft_kernel(SPEC, PREC, LOG2M);
+  constexpr int SCRB = MF ? 0 : ((M + M / 16) * (M >= 512 ? 4 : 8) + 15) / 16 * 16;
   const int row0 = max(SCRB, (rowb + 15) / 16 * 16);
   auto row_ptr = [&](int f) -> unsigned char* {   // staging row of tile frame f
     const int q = f % FPW;
@@ -135,17 +139,23 @@ void k_tile(TileArgs a) {
         const TIn* xf = wave_uniform(static_cast<const TIn*>(a.signal) + r.base + (int64_t)wave * a.hop);
         const int l = opaque(lane0);
         const unsigned lo = (unsigned)l * (2u * sizeof(TIn));
+        if constexpr (MF) {   // the transform's pairs: mfft_point(lane, i)
+          const unsigned lp = (unsigned)((l & 15) + 128 * (l >> 4)) * (2u * sizeof(TIn));
 #pragma unroll
-        for (int q = 0; q < SHP::Q0; ++q) {
-          // lanes past the pass-0 butterflies (B0 < 64) hold no sample pairs:
-          // their indices would run past the frame
-          if ((SHP::B0 % 64 == 0) || l + 64 * q < SHP::B0) {
+          for (int i = 0; i < 16; ++i) pf[0][i] = SamplePair<TIn>::load(xf + 2 * mfft_pair_off(i), lp);
+        } else {
 #pragma unroll
-            for (int r2 = 0; r2 < SHP::R0; ++r2)   // sample pair n = lane + 64 q + r2 B0
-              pf[q][r2] = SamplePair<TIn>::load(xf + 2 * (64 * q + r2 * SHP::B0), lo);
-          } else {
+          for (int q = 0; q < SHP::Q0; ++q) {
+            // lanes past the pass-0 butterflies (B0 < 64) hold no sample
+            // pairs: their indices would run past the frame
+            if ((SHP::B0 % 64 == 0) || l + 64 * q < SHP::B0) {
 #pragma unroll
-            for (int r2 = 0; r2 < SHP::R0; ++r2) pf[q][r2] = make_float2(0.0f, 0.0f);
+              for (int r2 = 0; r2 < SHP::R0; ++r2)   // sample pair n = lane + 64 q + r2 B0
+                pf[q][r2] = SamplePair<TIn>::load(xf + 2 * (64 * q + r2 * SHP::B0), lo);
+            } else {
+#pragma unroll
+              for (int r2 = 0; r2 < SHP::R0; ++r2) pf[q][r2] = make_float2(0.0f, 0.0f);
+            }
           }
         }
       } else {
@@ -167,13 +177,26 @@ void k_tile(TileArgs a) {
   // LDS copies of the FFT tables (16-bit mel / MFCC, n_fft 1024 and 2048):
   // [window: n_fft floats][per-pass twiddles: M float2][W_N^k: M float2],
   // published by a barrier before the first FFT
-  constexpr bool TABL = !SPEC && PREC != PREC_F32 && NW == 16 && SHP::REG_SPLIT && LOG2M <= 10;
+  constexpr bool TABL = !MF && !SPEC && PREC != PREC_F32 && NW == 16 && SHP::REG_SPLIT && LOG2M <= 10;
+  // the transform's table in LDS (any shape; mfft2048, with the window)
+  const unsigned char* mft = nullptr;
   // (the pointers are set in a compile-time branch: a pointer that may be
   // either global or LDS would compile to flat loads)
   const float* win;
   const float2* twp;
   const float2* twn;
-  if constexpr (TABL) {
+  if constexpr (MF) {
+    unsigned char* tb = lds + a.tab_lds_off;
+    if (t_begin < t_end) {
+      for (int p = threadIdx.x; p < kMfTabBytes / 16; p += NT)
+        reinterpret_cast<uint4*>(tb)[p] = reinterpret_cast<const uint4*>(a.mft)[p];
+      lds_barrier();
+    }
+    win = a.window;
+    mft = tb;
+    twp = a.twp;
+    twn = a.twN;
+  } else if constexpr (TABL) {
     unsigned char* tb = lds + a.tab_lds_off;
     if (t_begin < t_end) {
       const int nw4 = a.n_fft / 4, nt = M / 2;   // 16-byte pieces of the window / of each twiddle table
@@ -259,6 +282,10 @@ void k_tile(TileArgs a) {
     // r < RL/2, M - k of the pair (b, r - RL/2) above (lane 0, b = 0, r =
     // RL/2: M, the Nyquist bin)
     auto bin_of = [&](int b, int r) -> int {
+      if constexpr (MF) {   // slot 2b + r of the transform (mfft_bin)
+        if (r < RL / 2) return mfft_bin(lane, 2 * b + r);
+        return M - mfft_bin(lane, 2 * b + r - RL / 2);
+      }
       if (r < RL / 2) return lane + 64 * b + SH::BL * r;
       return M - (lane + 64 * b + SH::BL * (r - RL / 2));
     };
@@ -266,6 +293,7 @@ void k_tile(TileArgs a) {
     float hv[NQ][QL][RL][EC];
     float hn[NQ][EC];
     int fexp[NQ];
+    int fadj[NQ];   // added to fexp for the contraction: undoes the transform's input scale
     // Writes frame q's spectrum (registers) into its staging row, and its
     // operand scale exponent for the contraction.
     auto put = [&](unsigned char* row, int k, const float* v, int e) {
@@ -309,24 +337,39 @@ void k_tile(TileArgs a) {
         }
       }
       if constexpr (!SPEC) {
-        if (lane == 0) sc_exp[fl] = (fr < T) ? fexp[qh] : 0;
+        if (lane == 0) sc_exp[fl] = (fr < T) ? fexp[qh] + fadj[qh] : 0;
       }
     };
     auto frame_body = [&](int q) {
       const int qh = STAGE_EACH ? 0 : q;
       fexp[qh] = 0;
+      fadj[qh] = 0;
       const int fr = f0 + wv * FPW + q;   // wave-uniform: framing math stays scalar
       if (fr < T) {
         float vmax = 0.0f;
         if constexpr (RS) {
-          float2 z[QL][RL];
-          frame_fft_split<LOG2M, TIn>(a, clip, fr, reinterpret_cast<float*>(scr), lane, win, twp, z, pf,
-                                      PF && pf_tile == tile);
           // hv[b][r]: 2 X[k] of pair (b, r) for r < RL/2, 2 X[M-k] of pair
           // (b, r - RL/2) for r >= RL/2 (see bin_of); hn: lane 0's X[M/2]
           float2 xk[QL][RL / 2], xm[QL][RL / 2];
           float2 xh;
-          real_split_pairs<LOG2M>(z, twn, lane, xk, xm, xh);
+          int es = 0;   // the transform's input scale (2^es)
+          if constexpr (MF) {
+            float2 v[16], k8[8], m8[8];
+            mfft_load<TIn>(a, clip, fr, lane, mft, v, pf[0], PF && pf_tile == tile);
+            es = mfft_frame(v, mft, lane, k8, m8, xh);
+#pragma unroll
+            for (int b = 0; b < QL; ++b)
+#pragma unroll
+              for (int r = 0; r < RL / 2; ++r) {
+                xk[b][r] = k8[2 * b + r];
+                xm[b][r] = m8[2 * b + r];
+              }
+          } else {
+            float2 z[QL][RL];
+            frame_fft_split<LOG2M, TIn>(a, clip, fr, reinterpret_cast<float*>(scr), lane, win, twp, z, pf,
+                                        PF && pf_tile == tile);
+            real_split_pairs<LOG2M>(z, twn, lane, xk, xm, xh);
+          }
           auto set = [&](float* h, const float2 X) {   // X = 2 X[bin]
             if constexpr (CPLX) {
               h[0] = 0.5f * X.x;
@@ -348,6 +391,23 @@ void k_tile(TileArgs a) {
             }
           set(hn[qh], xh);
           if constexpr (!SPEC) {
+            if constexpr (MF) {
+              // the input scale 2^es: folded into the frame exponent for
+              // powers 2 and 1, removed before the power otherwise
+              if (es != 0) {
+                if (a.power == 2.0f) {
+                  fadj[qh] = -2 * es;
+                } else if (a.power == 1.0f) {
+                  fadj[qh] = -es;
+                } else {
+#pragma unroll
+                  for (int b = 0; b < QL; ++b)
+#pragma unroll
+                    for (int r = 0; r < RL; ++r) hv[qh][b][r][0] = ldexpf(hv[qh][b][r][0], -2 * es);
+                  hn[qh][0] = ldexpf(hn[qh][0], -2 * es);
+                }
+              }
+            }
             // (2|X|)^p from (2|X|)^2 (the 2^p is removed after the mel
             // product): one wave-uniform branch per frame, so the default
             // power 2 runs no transcendental instruction at all
@@ -550,7 +610,7 @@ void k_tile(TileArgs a) {
         const int h = slot_block(v), x0 = slot_first_bin(v), n = slot_chunks(v);
         const FE* pr = reinterpret_cast<const FE*>(prow) + x0 + (CH / 4) * g;
         // this lane's filterbank operand: row ri & 7, bins (CH/4) g .. of a chunk
-        const bool cached = a.fb_lds_off != 0;
+        const bool cached = a.fb_lds_off != 0 && slot_cache_first(v) + n <= a.cache_chunks;
         const unsigned char* fbl = lds + a.fb_lds_off + slot_cache_first(v) * kFbChunkBytes + (ri & 7) * kFbRowBytes + g * 16;
         const FE* fbtab;
         if constexpr (PREC == PREC_F32) fbtab = a.fb32;
@@ -915,6 +975,12 @@ __global__ void k_tab_twiddle(float2* tw, int M, float2* twN, int n_fft) {
   if (t < M || t == 0) twN[t] = make_float2((float)cospi(2.0 * t / n_fft), (float)(-sinpi(2.0 * t / n_fft)));
 }
 
+// the matrix-core FFT table (mfft_table_piece), one 16-byte piece per thread
+__global__ void k_tab_mfft(const float* win, unsigned char* tab) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < kMfTabBytes / 16) mfft_table_store(p, win, tab);
+}
+
 // entry [(r-1) NS + k] = W_{NS R}^{k r} of one Stockham pass
 __global__ void k_tab_twiddle_pass(float2* twp, int NS, int R) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1086,6 +1152,7 @@ hipError_t gen_tables(const TableKey& key, const DeviceTables& t, hipStream_t st
       off += (R - 1) * NS;
     }
   }
+  if (t.mft) hipLaunchKernelGGL(k_tab_mfft, dim3(nblk(kMfTabBytes / 16, 256)), dim3(256), 0, st, t.window, t.mft);
   const int rows = t.nmb * 16;
   if (rows > 0) {
     if (key.fb_kind == FB_CHROMA)
@@ -1195,8 +1262,10 @@ bool tile_layout(int n_fft, int out_kind, int prec, int n_mels, int n_mfcc, Tile
   }
   const int nw = (shape == SHAPE_W16) ? 16 : 8, fpw = (shape == SHAPE_W8F2) ? 2 : 1;
   // FFT scratch: float2 exchanges for the small sizes, half-size float
-  // exchanges for the register-split sizes (n_fft >= 1024)
-  const int scr = (M + M / 16) * ((M >= 512) ? 4 : 8);
+  // exchanges for the register-split sizes (n_fft >= 1024), none for the
+  // matrix-core transform (registers only)
+  const bool mf = mfft_kernel(spec, prec, __builtin_ctz(M));
+  const int scr = mf ? 0 : (M + M / 16) * ((M >= 512) ? 4 : 8);
   int esz, S;
   if (spec) {
     esz = (out_kind == OUT_STFT_COMPLEX) ? 8 : 4;
@@ -1219,7 +1288,12 @@ bool tile_layout(int n_fft, int out_kind, int prec, int n_mels, int n_mfcc, Tile
   size_t total = (size_t)nw * rb + extras;
   if (total > lds_limit_bytes()) return false;
   int tab_off = 0;
-  if (!spec && prec != PREC_F32 && nw == 16 && (M == 512 || M == 1024)) {
+  if (mf) {
+    // the matrix-core transform's table in LDS (any shape; it holds the window)
+    tab_off = (int)((total + 15) / 16 * 16);
+    total = (size_t)tab_off + (size_t)kMfTabBytes;
+    if (total > lds_limit_bytes()) return false;
+  } else if (!spec && prec != PREC_F32 && nw == 16 && (M == 512 || M == 1024)) {
     // FFT tables in LDS (the kernel's TABL case): window, per-pass twiddles, W_N^k
     tab_off = (int)((total + 15) / 16 * 16);
     total = (size_t)tab_off + (size_t)n_fft * 4 + (size_t)M * 8 * 2;
