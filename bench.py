@@ -674,8 +674,10 @@ def serve_leg(rates, seconds, sources, pcm=None, kinds=None, precision="f32", ba
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    # defaults: the GPU ramps up over ~40 steps after any idle period (even
+    # 50 ms), so 50 warm-up steps reach the sustained rate before the timed ones
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain single-clip leg")
     ap.add_argument("--profile", action="store_true", help="short run for a profiler")
@@ -770,8 +772,11 @@ def main():
         log(f"calibration copy: {args.copy_bytes} bytes read, {args.copy_bytes} bytes written")
         del src, dst
 
-    # fused kernel time from HIP events on its own stream (the plan's stream)
-    k_ms, s_ms = B.time(d_in, out, iters=5 if args.profile else 20)
+    # fused kernel time from HIP events on its own stream (the plan's stream),
+    # after the same warm-up
+    if not args.profile:
+        B.time(d_in, out, iters=warm)
+    k_ms, s_ms = B.time(d_in, out, iters=5 if args.profile else 50)
     bytes_launch = 4 * B.total_input + 4 * n_mfcc * frames
     traffic = measured_traffic(bytes_launch)
     roofline = kernel_roofline(frames, bytes_launch, 2048, B.n_tiles, mfma_flops_per_tile(22050, 2048, 128, n_mfcc, "bf16"),
@@ -790,7 +795,8 @@ def main():
         B32 = make("f32")
         out32 = B32.empty_output()
         B32.autotune(d_in, out32, iters=2)
-        B32.run(d_in, out32, graph=True)
+        for _ in range(warm):
+            B32.run(d_in, out32, graph=True)
         torch.cuda.synchronize()
         n32 = max(5, steps // 2)
         t0 = time.perf_counter()
@@ -798,7 +804,7 @@ def main():
             B32.run(d_in, out32, graph=True)
         torch.cuda.synchronize()
         f32_fps = frames * n32 / (time.perf_counter() - t0)
-        k32, _ = B32.time(d_in, out32, iters=10)
+        k32, _ = B32.time(d_in, out32, iters=50)
         extra["batched_fp32_frames_per_s"] = f32_fps
         extra["batched_fp32_kernel_ms"] = k32
         extra["batched_fp32_launch"] = B32.launch_shape()
