@@ -746,8 +746,10 @@ def main():
         log(f"autotuned: {B.launch_shape()} ({tune_ms:.3f} ms/step)")
     launch = B.launch_shape()
 
-    steps = 3 if args.profile else args.steps
-    warm = 1 if args.profile else args.warmup
+    # a profiler run keeps the warm-up (its trace is read at steady state)
+    # and times few steps
+    steps = 10 if args.profile else args.steps
+    warm = args.warmup
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
     with torch.cuda.stream(stream):
         for _ in range(warm):
@@ -777,9 +779,8 @@ def main():
 
     # fused kernel time from HIP events on its own stream (the plan's stream),
     # after the same warm-up
-    if not args.profile:
-        B.time(d_in, out, iters=warm)
-    k_ms, s_ms = B.time(d_in, out, iters=5 if args.profile else 50)
+    B.time(d_in, out, iters=warm)
+    k_ms, s_ms = B.time(d_in, out, iters=20 if args.profile else 50)
     bytes_launch = 4 * B.total_input + 4 * n_mfcc * frames
     traffic = measured_traffic(bytes_launch)
     roofline = kernel_roofline(frames, bytes_launch, 2048, B.n_tiles, mfma_flops_per_tile(22050, 2048, 128, n_mfcc, "bf16"),
