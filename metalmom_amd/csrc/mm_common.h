@@ -152,7 +152,7 @@ struct TileArgs {
   int dct_lds_off;             // LDS copy of the DCT operand tables (0: global memory)
   const int* cache_src;        // [cache_chunks] sources of the filterbank cache (cache_sources)
   int cache_chunks;
-  long long* phase_cycles;     // instrumented builds only: [grid][waves][6] cycle sums
+  long long* phase_cycles;     // instrumented builds only: [grid][waves][9] cycle sums
 };
 
 // A wave's mel block slot, packed in 32 bits: bit 31 set for a used slot,

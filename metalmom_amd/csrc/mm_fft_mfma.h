@@ -381,13 +381,21 @@ __device__ __forceinline__ void mfft_load(const TileArgs& a, int clip, int fr, i
   }
 }
 
+// m = max(m, |a|, |b|) in one instruction (fmaxf over fabsf compiles to
+// three per pair: in IEEE mode each operand is first quieted by a max with
+// itself). A NaN operand may leave m anything; mf_scale_exp then picks no
+// scaling or some power of two, either of which the caller undoes exactly.
+__device__ __forceinline__ void mf_absmax2(float& m, float a, float b) {
+  __asm__("v_max3_f32 %0, |%1|, |%2|, %0" : "+v"(m) : "v"(a), "v"(b));
+}
+
 // The whole frame: windowed pairs -> power-of-two scale -> transform.
 // Returns the scale exponent es (the spectrum is 2^es times the frame's).
 __device__ __forceinline__ int mfft_frame(float2 (&v)[16], const unsigned char* tab, int lane, float2 (&xk)[8],
                                           float2 (&xm)[8], float2& xh) {
   float m = 0.0f;
 #pragma unroll
-  for (int i = 0; i < 16; ++i) m = fmaxf(m, fmaxf(fabsf(v[i].x), fabsf(v[i].y)));
+  for (int i = 0; i < 16; ++i) mf_absmax2(m, v[i].x, v[i].y);
   const int es = mf_scale_exp(wave_max_nonneg(m));
   if (es != 0) {
 #pragma unroll

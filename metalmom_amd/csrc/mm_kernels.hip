@@ -26,11 +26,14 @@ constexpr int tile_waves_per_eu() { return (NW == 8 && LOG2M <= 10) ? 4 : 1; }
 
 // Optional phase timing of the fused kernel (build with -DMM_PHASE_TIMING):
 // lane 0 of every wave accumulates shader-clock cycles spent in each tile
-// phase and stores the sums to TileArgs::phase_cycles at exit.
+// phase and stores the sums to TileArgs::phase_cycles at exit. Slots: FFT +
+// power, staging, first barrier, mel + dB, extrema reduction, DCT, next-tile
+// prefetch, second barrier, cross-block sum.
+constexpr int kPhaseSlots = 9;
 #ifdef MM_PHASE_TIMING
-#define MM_PT_DECL long long pt_acc[6] = {0, 0, 0, 0, 0, 0}; long long pt_last = __builtin_amdgcn_s_memtime();
+#define MM_PT_DECL long long pt_acc[kPhaseSlots] = {}; long long pt_last = __builtin_amdgcn_s_memtime();
 #define MM_PT(i) do { const long long now_ = __builtin_amdgcn_s_memtime(); pt_acc[i] += now_ - pt_last; pt_last = now_; } while (0)
-#define MM_PT_STORE() do { if (a.phase_cycles && lane0 == 0) for (int i_ = 0; i_ < 6; ++i_) a.phase_cycles[(blockIdx.x * NW + wave) * 6 + i_] += pt_acc[i_]; } while (0)
+#define MM_PT_STORE() do { if (a.phase_cycles && lane0 == 0) for (int i_ = 0; i_ < kPhaseSlots; ++i_) a.phase_cycles[(blockIdx.x * NW + wave) * kPhaseSlots + i_] += pt_acc[i_]; } while (0)
 #else
 #define MM_PT_DECL
 #define MM_PT(i) do { } while (0)
@@ -294,6 +297,13 @@ void k_tile(TileArgs a) {
     float hn[NQ][EC];
     int fexp[NQ];
     int fadj[NQ];   // added to fexp for the contraction: undoes the transform's input scale
+    // Per-frame power-of-two operand scale (16-bit contractions). A BF16
+    // value has the FP32 exponent range, so scaling its operand by 2^-e and
+    // the result by 2^e changes no bit unless a value leaves the normal
+    // range; in the MFCC kernel anything that small lies below the amin floor
+    // (1e-10) of the dB, so the BF16 MFCC kernel stages unscaled values. The
+    // FP16 kernels (65504 at most) and the BF16 mel / chroma output keep it.
+    constexpr bool FSCALE = !SPEC && (PREC == PREC_F16 || (PREC == PREC_BF16 && MODE != OUT_MFCC));
     // Writes frame q's spectrum (registers) into its staging row, and its
     // operand scale exponent for the contraction.
     auto put = [&](unsigned char* row, int k, const float* v, int e) {
@@ -302,7 +312,7 @@ void k_tile(TileArgs a) {
       } else if constexpr (SPEC || PREC == PREC_F32) {
         reinterpret_cast<float*>(row)[k] = v[0];
       } else {
-        const float sv = ldexpf(v[0], -e);
+        const float sv = FSCALE ? ldexpf(v[0], -e) : v[0];
         reinterpret_cast<uint16_t*>(row)[k] = (PREC == PREC_BF16) ? f2bf(sv) : f2h(sv);
       }
     };
@@ -463,7 +473,7 @@ void k_tile(TileArgs a) {
           }
         }
         wave_lds_fence();
-        if constexpr (!SPEC && PREC != PREC_F32) {
+        if constexpr (FSCALE) {
           // per-frame power-of-two scale so the 16-bit operand keeps the
           // frame's top range (exactly undone after the contraction): bf16
           // keeps the FP32 exponent range, FP16 maps the frame maximum into
@@ -699,6 +709,7 @@ void k_tile(TileArgs a) {
       // waves without mel blocks reach this point at once: their prefetch
       // overlaps the other waves' contraction
       if constexpr (MODE == OUT_MEL) prefetch_next(tile, nrec);
+      MM_PT(3);
 
       if constexpr (MODE == OUT_MFCC) {
         if (track) {
@@ -710,6 +721,7 @@ void k_tile(TileArgs a) {
             reinterpret_cast<int*>(red_min)[wave] = tmin;
           }
         }
+        MM_PT(4);
         // DCT partials, one per (row tile ct, mel block h): [ndt][nmh][64
         // lanes][4], summed over h in increasing order below, so the result
         // does not depend on which wave contracted which block (any
@@ -755,10 +767,11 @@ void k_tile(TileArgs a) {
             }
           }
         }
+        MM_PT(5);
         prefetch_next(tile, nrec);
-        MM_PT(3);
+        MM_PT(6);
         lds_barrier();   // partials and per-wave extrema are in LDS
-        MM_PT(4);
+        MM_PT(7);
         if (track && threadIdx.x == 0) {
           int bm = INT_MIN, bn = INT_MAX;
           for (int w = 0; w < NW; ++w) {
@@ -790,7 +803,7 @@ void k_tile(TileArgs a) {
           if (fi < TF && f0 + fi < T && c < a.n_mfcc)
             reinterpret_cast<float*>(a.out)[ldc(a.out_off, clip) + (int64_t)c * T + f0 + fi] = sm;
         }
-        MM_PT(5);
+        MM_PT(8);
         if (sep) continue;
       }
       lds_barrier();
