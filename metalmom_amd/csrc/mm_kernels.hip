@@ -522,6 +522,13 @@ void k_tile(TileArgs a) {
 #pragma unroll
       for (int q = 0; q < FPW; ++q) stage(q);
     }
+    // Matrix-core FFT kernels: the next tile's samples are requested as soon
+    // as this wave's frame is staged. Waves on one SIMD finish their FFTs
+    // thousands of cycles apart, so the workgroup's 16 requests reach the
+    // vector memory path spread out instead of in one burst after the
+    // contraction (where each wave stalled issuing them); the samples stay in
+    // registers through the contraction, which needs few.
+    if constexpr (MF) prefetch_next(tile, nrec);
     MM_PT(1);
     lds_barrier();
     MM_PT(2);
@@ -643,11 +650,16 @@ void k_tile(TileArgs a) {
           for (int u = 0; u < K; ++u)
             if (u == 0 || c + u < n) acc = mfma_chunk(av[u], bv[u], acc);
           mfma_settle(acc);
-          wait_vmem_all();   // duplicate loads whose MFMA was skipped end here
+          // duplicate loads whose MFMA was skipped end here (operands from
+          // L2 only: an unconditional wait would also hold the LDS-operand
+          // rounds until the next tile's sample prefetch has landed)
+          if (!cached) wait_vmem_all();
         };
         for (int c = 0; c < n;) {
           const int rem = n - c;
-          if (rem > 4) { round(std::integral_constant<int, 8>{}, c); c += 8; }
+          // rounds of at most 4 chunks in the matrix-core FFT kernels: they
+          // hold the next tile's samples in registers through the contraction
+          if (!MF && rem > 4) { round(std::integral_constant<int, 8>{}, c); c += 8; }
           else if (rem > 2) { round(std::integral_constant<int, 4>{}, c); c += 4; }
           else if (rem == 2) { round(std::integral_constant<int, 2>{}, c); c += 2; }
           else { round(std::integral_constant<int, 1>{}, c); c += 1; }
@@ -708,7 +720,7 @@ void k_tile(TileArgs a) {
       }
       // waves without mel blocks reach this point at once: their prefetch
       // overlaps the other waves' contraction
-      if constexpr (MODE == OUT_MEL) prefetch_next(tile, nrec);
+      if constexpr (MODE == OUT_MEL && !MF) prefetch_next(tile, nrec);
       MM_PT(3);
 
       if constexpr (MODE == OUT_MFCC) {
@@ -768,7 +780,7 @@ void k_tile(TileArgs a) {
           }
         }
         MM_PT(5);
-        prefetch_next(tile, nrec);
+        if constexpr (!MF) prefetch_next(tile, nrec);
         MM_PT(6);
         lds_barrier();   // partials and per-wave extrema are in LDS
         MM_PT(7);
