@@ -84,11 +84,11 @@ struct DeviceTables {
 // once per configuration (it holds the analysis window) and copied into LDS
 // by each workgroup:
 //   [0, 8 KiB)        window pairs of each lane's input points, [pair 8][lane 64] x 4 floats
-//   [8 KiB, 24 KiB)   constant operand, [tile t 4][block b 2][hi, lo][lane 64] x 8 FP16
-//   [24 KiB, 32 KiB)  stage twiddles W_1024^(n1 k2), [pair 8][lane 64] x 2 float2
-//   [32 KiB, 36 KiB)  split twiddles W_2048^k, [pair 4][lane 64] x 2 float2
+//   [8 KiB, 16 KiB)   constant operand, [parity p 2][part t 2][hi, lo][lane 64] x 8 FP16
+//   [16 KiB, 24 KiB)  stage twiddles W_1024^(n1 k2), [p 2][mt 2][pair 2][lane 64] x 2 float2
+//   [24 KiB, 28 KiB)  split twiddles W_2048^k, [p 2][pair 2][lane 64] x 2 float2
 constexpr int kMfWinBytes = 8 * 64 * 16;
-constexpr int kMfOpBytes = 4 * 2 * 2 * 64 * 16;
+constexpr int kMfOpBytes = 2 * 2 * 2 * 64 * 16;
 constexpr int kMfTw1Bytes = 8 * 64 * 16;
 constexpr int kMfTw2Bytes = 4 * 64 * 16;
 constexpr int kMfOpOff = kMfWinBytes;

@@ -294,7 +294,8 @@ void k_tile(TileArgs a) {
     };
     constexpr int NQ = STAGE_EACH ? 1 : FPW;       // frames whose spectra are held at once
     float hv[NQ][QL][RL][EC];
-    float hn[NQ][EC];
+    float hn[NQ][EC];                              // bin M/2 (valid in lane HL)
+    constexpr int HL = MF ? 32 : 0;
     int fexp[NQ];
     int fadj[NQ];   // added to fexp for the contraction: undoes the transform's input scale
     // Per-frame power-of-two operand scale (16-bit contractions). A BF16
@@ -327,7 +328,7 @@ void k_tile(TileArgs a) {
           for (int b = 0; b < QL; ++b)
 #pragma unroll
             for (int r = 0; r < RL; ++r) put(row, bin_of(b, r), hv[qh][b][r], fexp[qh]);
-          if (lane == 0) put(row, M / 2, hn[qh], fexp[qh]);
+          if (lane == HL) put(row, M / 2, hn[qh], fexp[qh]);
         } else {
 #pragma unroll
           for (int i = 0; i < NI; ++i) {
@@ -359,7 +360,7 @@ void k_tile(TileArgs a) {
         float vmax = 0.0f;
         if constexpr (RS) {
           // hv[b][r]: 2 X[k] of pair (b, r) for r < RL/2, 2 X[M-k] of pair
-          // (b, r - RL/2) for r >= RL/2 (see bin_of); hn: lane 0's X[M/2]
+          // (b, r - RL/2) for r >= RL/2 (see bin_of); hn: X[M/2] in lane HL
           float2 xk[QL][RL / 2], xm[QL][RL / 2];
           float2 xh;
           int es = 0;   // the transform's input scale (2^es)
@@ -440,7 +441,7 @@ void k_tile(TileArgs a) {
             for (int b = 0; b < QL; ++b)
 #pragma unroll
               for (int r = 0; r < RL; ++r) vmax = fmaxf(vmax, hv[qh][b][r][0]);
-            if (lane == 0) vmax = fmaxf(vmax, hn[qh][0]);
+            if (lane == HL) vmax = fmaxf(vmax, hn[qh][0]);
           }
         } else {
           frame_fft<LOG2M, TIn>(a, clip, fr, scr, lane);
