@@ -1,9 +1,10 @@
 """The matrix-core FFT of the 16-bit mel / MFCC kernels at n_fft 2048
-(mfft2048: FP16 hi/lo operands on v_mfma_f32_16x16x32_f16, per-frame
-power-of-two input scale).
+(mfft2048: a radix-2 step and two 16-point contractions per 32-point DFT,
+FP16 hi/lo operands on v_mfma_f32_16x16x32_f16, per-frame power-of-two
+input scale).
 
 * Exactness of the input scale: frames whose largest sample lies outside
-  [2^-4, 2^10) are normalised by a power of two before the transform and the
+  [2^-4, 2^9) are normalised by a power of two before the transform and the
   factor is removed from the frame's exponent afterwards. Two inputs that
   differ by a power of two and are both normalised therefore run the very
   same transform, so their mel spectra agree bit for bit after the exact
