@@ -697,7 +697,7 @@ def main():
     ap.add_argument("--grid", type=int, default=0,
                     help="persistent workgroups per CU (skips the autotuner; 0 = autotune)")
     ap.add_argument("--shape", type=int, default=0,
-                    help="fused-kernel workgroup shape with --grid (1: 16x1, 2: 8x1, 3: 8x2; 0: default)")
+                    help="fused-kernel workgroup shape with --grid (1: 16x1, 2: 8x1, 3: 8x2, 4: 16x1 pipelined; 0: default)")
     ap.add_argument("--no-large-fft", action="store_true", help="skip the large-FFT FP16 batch leg")
     ap.add_argument("--no-errors", action="store_true", help="skip the FP64 error report")
     ap.add_argument("--no-istft", action="store_true", help="skip the inverse-STFT leg")

@@ -69,7 +69,8 @@ class SpectralBatch:
 
     def set_shape(self, shape: int):
         """Workgroup shape of the fused kernel (1: 16 waves x 1 frame, 2: 8 x 1,
-        3: 8 x 2 frames, two workgroups per CU)."""
+        3: 8 x 2 frames, two workgroups per CU; 4: 16 x 1 pipelined, one
+        barrier per tile, 16-bit MFCC at n_fft 2048 only)."""
         s = lib.mm_batch_set_shape(self._h, int(shape))
         if s != 0:
             raise RuntimeError(f"mm_batch_set_shape failed with status {s}")

@@ -147,6 +147,8 @@ struct TileArgs {
   int lds_row_stride;          // staging row stride in elements
   int lds_region_bytes;        // per-wave LDS region
   int lds_part_off;            // MFCC: LDS offset of the DCT partials (0: over the regions)
+  int lds_stage2_off;          // pipelined shape: second staging regions (odd processed tiles)
+  int lds_part2_off;           // pipelined shape: second DCT partial area
   int tab_lds_off;             // LDS copies of window and twiddles (16-bit mel / MFCC kernels)
   int fb_lds_off;              // LDS copy of the banded filterbank (0: read it from global memory)
   int dct_lds_off;             // LDS copy of the DCT operand tables (0: global memory)

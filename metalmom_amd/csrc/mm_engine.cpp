@@ -674,6 +674,8 @@ static TileArgs make_args(const SpecRequest& r, const Tables* tb, const BatchMet
   a.lds_row_stride = L.row_stride;
   a.lds_region_bytes = L.region_bytes;
   a.lds_part_off = L.part_off;
+  a.lds_stage2_off = L.stage2_off;
+  a.lds_part2_off = L.part2_off;
   a.tab_lds_off = L.tab_off;
   a.phase_cycles = phase_cycles_buffer();
   return a;
@@ -759,7 +761,20 @@ int32_t enqueue_batch(Engine* eng, const SpecRequest& r, const Tables* tb, const
     // many whole blocks' worth as fit (the kernel reads a block from LDS when
     // all of its chunks are cached)
     const size_t lim = lds_limit_bytes() / std::max(L.occ, 1);
-    if (off + dcb <= lim && (nch > 0 || dcb > 0)) {
+    if (L.stage2_off > 0) {
+      // pipelined layout: the filterbank chunks first (every wave reads its
+      // band in every tile), the DCT operands only if they still fit
+      const int nc = (int)std::min<size_t>((size_t)nch, off < lim ? (lim - off) / kFbChunkBytes : 0);
+      if (nc > 0) {
+        a.fb_lds_off = (int)off;
+        a.cache_src = tb->t.cache_src + pc * kMaxCacheChunks;
+        a.cache_chunks = nc;
+      }
+      const size_t fbc = (size_t)nc * kFbChunkBytes;
+      const bool dfit = dcb > 0 && off + fbc + dcb <= lim;
+      a.dct_lds_off = dfit ? (int)(off + fbc) : 0;
+      LL.lds_bytes = off + fbc + (dfit ? dcb : 0);
+    } else if (off + dcb <= lim && (nch > 0 || dcb > 0)) {
       const int nc = (int)std::min<size_t>((size_t)nch, (lim - off - dcb) / kFbChunkBytes);
       if (nc > 0) {
         a.fb_lds_off = (int)off;

@@ -27,6 +27,8 @@ struct TileLayout {
   int part_off;      // LDS offset of the DCT partial area (0: reuse the staging regions)
   int tab_off;       // LDS offset of the FFT table copies (16-bit mel / MFCC, n_fft 1024, 2048)
   int shape;         // TileShape this layout was made for (never SHAPE_DEFAULT)
+  int stage2_off;    // SHAPE_W16P: LDS offset of the second staging regions (0: single-buffered)
+  int part2_off;     // SHAPE_W16P: LDS offset of the second DCT partial area
   int occ;           // workgroups per CU the layout is sized for (LDS copies of tables only within this)
 };
 
@@ -38,6 +40,10 @@ enum TileShape : int {
   SHAPE_W8F2 = 3,      // 8 waves, two frames each in turn: 16-frame tiles, two workgroups per CU
                        // (n_fft 1024 and 2048), so one workgroup's contraction and barriers
                        // overlap the other's FFTs
+  SHAPE_W16P = 4,      // 16 waves, one frame each, pipelined: staging rows and DCT partials
+                       // double-buffered, one barrier per tile, so waves that finish a tile's
+                       // contraction start the next FFT while others still contract
+                       // (16-bit MFCC at n_fft 2048: the matrix-core FFT needs no LDS scratch)
 };
 // Shapes the fused kernel offers for this FFT size (SHAPE_W16 first when it
 // exists); returns their number (at most 4).

@@ -62,7 +62,14 @@ namespace mm {
 //     straight in as the B operand (the DCT table is stored in the matching
 //     K order), partial sums reduced across waves in LDS in fixed order.
 // Persistent: workgroup g processes tiles [g*C, (g+1)*C).
-template <int LOG2M, int MODE, int PREC, int NW, int FPW, typename TIn>
+//
+// PIPE (SHAPE_W16P, 16-bit MFCC with the matrix-core FFT): the staging rows,
+// frame exponents, per-wave extrema and DCT partials exist twice and
+// processed tiles alternate between the two sets, so one barrier per tile
+// suffices: after it a wave first finishes the previous tile (top_db
+// bookkeeping and block sum, waves 0-3) and contracts this one, then goes on
+// to the next FFT while other waves are still contracting.
+template <int LOG2M, int MODE, int PREC, int NW, int FPW, typename TIn, bool PIPE>
 __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(tile_waves_per_eu<LOG2M, NW>())))
 void k_tile(TileArgs a) {
   extern __shared__ __align__(16) unsigned char lds[];
@@ -82,9 +89,16 @@ void k_tile(TileArgs a) {
   const int S = a.lds_row_stride;                  // staging row stride (elements)
   unsigned char* region = lds + wave * RB;
   float2* scr = reinterpret_cast<float2*>(region);
-  int* sc_exp = reinterpret_cast<int*>(lds + NW * RB);       // [16]
-  float* red_max = reinterpret_cast<float*>(sc_exp + 16);    // [NW]
-  float* red_min = red_max + NW;                              // [NW]
+  constexpr int NB = PIPE ? 2 : 1;                           // staging / partial buffers
+  int* sc_exp0 = reinterpret_cast<int*>(lds + NW * RB);      // [NB][16]
+  float* red_max0 = reinterpret_cast<float*>(sc_exp0 + 16 * NB);   // [NB][NW]
+  float* red_min0 = red_max0 + NW * NB;                       // [NB][NW]
+  // the set of the tile being processed (PIPE: alternating per processed tile)
+  int buf = 0;
+  int* sc_exp = sc_exp0;
+  float* red_max = red_max0;
+  float* red_min = red_min0;
+  unsigned char* stg = lds;                                   // its staging regions
   constexpr int ESZ = SPEC ? (CPLX ? 8 : 4) : (PREC == PREC_F32 ? 4 : 2);
   const int rowb = S * ESZ;
   // A wave's region: its FFT scratch with its last frame's staging row over
@@ -97,7 +111,7 @@ void k_tile(TileArgs a) {
   const int row0 = max(SCRB, (rowb + 15) / 16 * 16);
   auto row_ptr = [&](int f) -> unsigned char* {   // staging row of tile frame f
     const int q = f % FPW;
-    return lds + (f / FPW) * RB + ((q == FPW - 1) ? 0 : row0 + q * rowb);
+    return stg + (f / FPW) * RB + ((q == FPW - 1) ? 0 : row0 + q * rowb);
   };
 
   const int t_begin = blockIdx.x * a.tiles_per_wg;
@@ -110,9 +124,12 @@ void k_tile(TileArgs a) {
       if (a.use_top_db)
         for (int t = t_begin + (int)threadIdx.x; t < t_end; t += NW * 64)
           act |= a.tile_min[t] < key_to_float(a.clip_max_key[ldc(a.tile_clip, t)]) - a.top_db;
-      // workgroup OR through one LDS flag per wave (its own slots, never reused)
-      int* any_act = reinterpret_cast<int*>(lds + NW * a.lds_region_bytes) + 16 + 2 * NW;
-      if ((threadIdx.x & 63) == 0) any_act[threadIdx.x >> 6] = __builtin_amdgcn_ballot_w64(act) != 0;
+      // workgroup OR through one LDS flag per wave (its own slots, never
+      // reused); the wave vote is taken with every lane active (a ballot
+      // inside the lane-0 branch would see lane 0's tile only)
+      int* any_act = reinterpret_cast<int*>(red_min0 + NW * NB);
+      const bool wave_act = __builtin_amdgcn_ballot_w64(act) != 0;
+      if ((threadIdx.x & 63) == 0) any_act[threadIdx.x >> 6] = wave_act;
       lds_barrier();
       int any = 0;
       for (int w = 0; w < NW; ++w) any |= any_act[w];
@@ -249,6 +266,45 @@ void k_tile(TileArgs a) {
       }
     }
   }
+  // Top_db bookkeeping (thread 0) and the fixed-order block sum of a
+  // finished MFCC tile whose DCT partials and per-wave extrema (set pb) a
+  // barrier has published in LDS: thread t of the workgroup owns output
+  // (coefficient 16 ct + (t >> 4) % 16, frame t & 15), so 16 consecutive
+  // threads store 16 consecutive frames of one coefficient.
+  auto finish_tile = [&](int ft, int pb) {
+    const int fclip = ldc(a.tile_clip, ft), ff0 = ldc(a.tile_f0, ft), fT = ldc(a.clip_frames, fclip);
+    const int po = PIPE ? (pb ? a.lds_part2_off : a.lds_part_off) : (a.lds_part_off > 0 ? a.lds_part_off : 0);
+    const float* red = reinterpret_cast<const float*>(lds + po);
+    if (a.use_top_db && a.phase == 1 && threadIdx.x == 0) {
+      int bm = INT_MIN, bn = INT_MAX;
+      for (int w = 0; w < NW; ++w) {
+        bm = max(bm, reinterpret_cast<const int*>(red_max0 + NW * pb)[w]);
+        bn = min(bn, reinterpret_cast<const int*>(red_min0 + NW * pb)[w]);
+      }
+      if (bm > INT_MIN) atomicMax(&a.clip_max_key[fclip], float_to_key(fkey_inv(bm)));
+      a.tile_min[ft] = (bn < INT_MAX) ? fkey_inv(bn) : INFINITY;
+    }
+    const int nout = a.ndt * 256;
+    for (int t = threadIdx.x; t < nout; t += NT) {
+      const int ct = t >> 8, cl = (t >> 4) & 15, fi = t & 15;
+      const int e = ((cl >> 2) * 16 + fi) * 4 + (cl & 3);   // MFMA layout: lane (cl / 4) * 16 + fi, item cl % 4
+      const float* src = red + (int64_t)ct * a.nmh * 256 + e;
+      float sm = 0.0f;
+      int h = 0;
+      for (; h + 8 <= a.nmh; h += 8) {   // 8 reads in flight, added in block order
+        float v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = src[(h + u) * 256];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) sm += v[u];
+      }
+      for (; h < a.nmh; ++h) sm += src[h * 256];
+      const int c = 16 * ct + cl;
+      if (fi < TF && ff0 + fi < fT && c < a.n_mfcc)
+        reinterpret_cast<float*>(a.out)[ldc(a.out_off, fclip) + (int64_t)c * fT + ff0 + fi] = sm;
+    }
+  };
+  int pend = -1, pbuf = 0;   // PIPE: the processed tile still to finish, and its set
   const int wv = __builtin_amdgcn_readfirstlane(wave);
   for (int tile = t_begin; tile < t_end; ++tile) {
     // the prefetched samples are used first thing; on tiles that do not use
@@ -267,6 +323,12 @@ void k_tile(TileArgs a) {
         thr = key_to_float(a.clip_max_key[clip]) - a.top_db;
         if (!(a.tile_min[tile] < thr)) continue;   // clamp inactive: phase 1 result stands
       }
+    }
+    if constexpr (PIPE) {
+      sc_exp = sc_exp0 + 16 * buf;
+      red_max = red_max0 + NW * buf;
+      red_min = red_min0 + NW * buf;
+      stg = lds + (buf ? a.lds_stage2_off : 0);
     }
 
     // ---- FFT of this wave's frames; spectra kept in registers ----
@@ -533,6 +595,9 @@ void k_tile(TileArgs a) {
     MM_PT(1);
     lds_barrier();
     MM_PT(2);
+    if constexpr (PIPE) {
+      if (pend >= 0) finish_tile(pend, pbuf);   // its partials were published by this barrier
+    }
 
     if constexpr (SPEC) {
       // ---- transposed write-out: [F, T], runs of TF frames per bin ----
@@ -742,7 +807,8 @@ void k_tile(TileArgs a) {
         // the layout has one (2 barriers per tile), else over the staging
         // regions once every wave has read them (4 barriers).
         const bool sep = a.lds_part_off > 0;
-        float* red = reinterpret_cast<float*>(lds + (sep ? a.lds_part_off : 0));
+        float* red = reinterpret_cast<float*>(
+            lds + (PIPE ? (buf ? a.lds_part2_off : a.lds_part_off) : (sep ? a.lds_part_off : 0)));
         if (!sep) lds_barrier();   // staging reads done; region memory is reused below
         for (int ct = 0; ct < a.ndt; ++ct) {
 #pragma unroll
@@ -783,43 +849,25 @@ void k_tile(TileArgs a) {
         MM_PT(5);
         if constexpr (!MF) prefetch_next(tile, nrec);
         MM_PT(6);
+        if constexpr (PIPE) {
+          pend = tile;   // finished after the next tile's barrier
+          pbuf = buf;
+          buf ^= 1;
+          continue;
+        }
         lds_barrier();   // partials and per-wave extrema are in LDS
         MM_PT(7);
-        if (track && threadIdx.x == 0) {
-          int bm = INT_MIN, bn = INT_MAX;
-          for (int w = 0; w < NW; ++w) {
-            bm = max(bm, reinterpret_cast<const int*>(red_max)[w]);
-            bn = min(bn, reinterpret_cast<const int*>(red_min)[w]);
-          }
-          if (bm > INT_MIN) atomicMax(&a.clip_max_key[clip], float_to_key(fkey_inv(bm)));
-          a.tile_min[tile] = (bn < INT_MAX) ? fkey_inv(bn) : INFINITY;
-        }
-        // fixed-order sum over the blocks: thread t of the workgroup owns
-        // output (coefficient 16 ct + (t >> 4) % 16, frame t & 15), so 16
-        // consecutive threads store 16 consecutive frames of one coefficient
-        const int nout = a.ndt * 256;
-        for (int t = threadIdx.x; t < nout; t += NT) {
-          const int ct = t >> 8, cl = (t >> 4) & 15, fi = t & 15;
-          const int e = ((cl >> 2) * 16 + fi) * 4 + (cl & 3);   // MFMA layout: lane (cl / 4) * 16 + fi, item cl % 4
-          const float* src = red + (int64_t)ct * a.nmh * 256 + e;
-          float sm = 0.0f;
-          int h = 0;
-          for (; h + 8 <= a.nmh; h += 8) {   // 8 reads in flight, added in block order
-            float v[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) v[u] = src[(h + u) * 256];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) sm += v[u];
-          }
-          for (; h < a.nmh; ++h) sm += src[h * 256];
-          const int c = 16 * ct + cl;
-          if (fi < TF && f0 + fi < T && c < a.n_mfcc)
-            reinterpret_cast<float*>(a.out)[ldc(a.out_off, clip) + (int64_t)c * T + f0 + fi] = sm;
-        }
+        finish_tile(tile, 0);
         MM_PT(8);
         if (sep) continue;
       }
       lds_barrier();
+    }
+  }
+  if constexpr (PIPE) {
+    if (pend >= 0) {
+      lds_barrier();
+      finish_tile(pend, pbuf);
     }
   }
   MM_PT_STORE();
@@ -1203,9 +1251,9 @@ hipError_t gen_tables(const TableKey& key, const DeviceTables& t, hipStream_t st
 
 static int g_tile_waves;
 
-template <int LOG2M, int MODE, int PREC, int NW, int FPW, typename TIn>
+template <int LOG2M, int MODE, int PREC, int NW, int FPW, typename TIn, bool PIPE = false>
 static hipError_t launch_tile_t(const TileArgs& a, int grid, size_t lds, hipStream_t st) {
-  auto fn = k_tile<LOG2M, MODE, PREC, NW, FPW, TIn>;
+  auto fn = k_tile<LOG2M, MODE, PREC, NW, FPW, TIn, PIPE>;
   // set once per instantiation (thread-safe static init); grid == 0 only
   // prepares the attribute, so stream capture never sees the first call
   static const hipError_t attr = hipFuncSetAttribute(
@@ -1221,6 +1269,9 @@ template <int LOG2M, int MODE, int PREC, typename TIn>
 static hipError_t launch_tile_s(const TileArgs& a, const TileLayout& L, int grid, hipStream_t st) {
   if constexpr (LOG2M <= 10) {
     if (L.shape == SHAPE_W16) return launch_tile_t<LOG2M, MODE, PREC, 16, 1, TIn>(a, grid, L.lds_bytes, st);
+  }
+  if constexpr (mfft_kernel(false, PREC, LOG2M) && MODE == OUT_MFCC) {
+    if (L.shape == SHAPE_W16P) return launch_tile_t<LOG2M, MODE, PREC, 16, 1, TIn, true>(a, grid, L.lds_bytes, st);
   }
   if constexpr (LOG2M == 9 || LOG2M == 10) {
     if (L.shape == SHAPE_W8F2) return launch_tile_t<LOG2M, MODE, PREC, 8, 2, TIn>(a, grid, L.lds_bytes, st);
@@ -1271,6 +1322,7 @@ int tile_shapes(int n_fft, int* out) {
   if (n_fft <= 2048) out[n++] = SHAPE_W16;
   out[n++] = SHAPE_W8;
   if (n_fft == 1024 || n_fft == 2048) out[n++] = SHAPE_W8F2;
+  if (n_fft == 2048) out[n++] = SHAPE_W16P;   // its layout exists for the 16-bit MFCC kernels
   return n;
 }
 
@@ -1278,7 +1330,7 @@ bool tile_layout(int n_fft, int out_kind, int prec, int n_mels, int n_mfcc, Tile
   if (n_fft < 256 || n_fft > 4096 || (n_fft & (n_fft - 1))) return false;
   const int M = n_fft / 2, F = M + 1;
   if (shape == SHAPE_DEFAULT) shape = (M > 1024) ? SHAPE_W8 : (g_tile_waves == 8 ? SHAPE_W8 : SHAPE_W16);
-  if (shape != SHAPE_W16 && shape != SHAPE_W8 && shape != SHAPE_W8F2) return false;
+  if (shape != SHAPE_W16 && shape != SHAPE_W8 && shape != SHAPE_W8F2 && shape != SHAPE_W16P) return false;
   if (shape == SHAPE_W16 && M > 1024) return false;   // 16 waves need n_fft <= 2048
   if (shape == SHAPE_W8F2 && M != 512 && M != 1024) return false;   // register-split FFT sizes
   const bool spec = out_kind <= OUT_STFT_COMPLEX;
@@ -1286,11 +1338,13 @@ bool tile_layout(int n_fft, int out_kind, int prec, int n_mels, int n_mfcc, Tile
     if (n_mels < 1 || n_mels > kMaxMelBlocks * kMelBlock) return false;   // larger: generic path
     if (out_kind == OUT_MFCC && (n_mfcc < 1 || n_mfcc > 128)) return false;
   }
-  const int nw = (shape == SHAPE_W16) ? 16 : 8, fpw = (shape == SHAPE_W8F2) ? 2 : 1;
+  const bool pipe = (shape == SHAPE_W16P);
+  const int nw = (shape == SHAPE_W16 || pipe) ? 16 : 8, fpw = (shape == SHAPE_W8F2) ? 2 : 1;
   // FFT scratch: float2 exchanges for the small sizes, half-size float
   // exchanges for the register-split sizes (n_fft >= 1024), none for the
   // matrix-core transform (registers only)
   const bool mf = mfft_kernel(spec, prec, __builtin_ctz(M));
+  if (pipe && !(mf && out_kind == OUT_MFCC)) return false;
   const int scr = mf ? 0 : (M + M / 16) * ((M >= 512) ? 4 : 8);
   int esz, S;
   if (spec) {
@@ -1310,7 +1364,8 @@ bool tile_layout(int n_fft, int out_kind, int prec, int n_mels, int n_mfcc, Tile
   const int first = std::max((scr + 15) / 16 * 16, (S * esz + 15) / 16 * 16);
   int rb = (fpw > 1) ? first + (fpw - 1) * S * esz : std::max(S * esz, scr);
   rb = (rb + 15) / 16 * 16;
-  const size_t extras = 16 * 4 + 2 * nw * 4 + nw * 4;   // frame exponents, per-wave extrema, phase-2 flags
+  // frame exponents and per-wave extrema (one set per staging buffer), phase-2 flags
+  const size_t extras = (16 * 4 + 2 * nw * 4) * (pipe ? 2 : 1) + nw * 4;
   size_t total = (size_t)nw * rb + extras;
   if (total > lds_limit_bytes()) return false;
   int tab_off = 0;
@@ -1337,6 +1392,19 @@ bool tile_layout(int n_fft, int out_kind, int prec, int n_mels, int n_mfcc, Tile
     } else if (part > (size_t)nw * rb) {
       return false;   // partials would not even fit over the staging regions
     }
+    if (pipe) {
+      // the second staging regions and partial area
+      if (part_off == 0) return false;
+      L->stage2_off = (int)((total + 15) / 16 * 16);
+      total = (size_t)L->stage2_off + (size_t)nw * rb;
+      L->part2_off = (int)((total + 15) / 16 * 16);
+      total = (size_t)L->part2_off + part;
+      if (total > lds_limit_bytes()) return false;
+    }
+  }
+  if (!pipe) {
+    L->stage2_off = 0;
+    L->part2_off = 0;
   }
   L->nw = nw;
   L->fpw = fpw;
@@ -1348,7 +1416,7 @@ bool tile_layout(int n_fft, int out_kind, int prec, int n_mels, int n_mfcc, Tile
   L->tile_frames = nw * fpw;
   L->shape = shape;
   // the two-frame shape exists to run two workgroups per CU
-  L->occ = (shape == SHAPE_W16) ? 1 : (int)std::max<size_t>(1, std::min<size_t>(2, lds_limit_bytes() / total));
+  L->occ = (shape == SHAPE_W16 || pipe) ? 1 : (int)std::max<size_t>(1, std::min<size_t>(2, lds_limit_bytes() / total));
   return true;
 }
 

@@ -95,6 +95,73 @@ def assert_single_clip_bitwise(B, d_in, res, lens, make, stride=1):
         S.close()
 
 
+# -- top_db clamping inside persistent batches, and the pipelined shape ------------
+
+def quiet_section_clips(n=64, seed=7, sr=22050):
+    """n clips of up to 8 s of noise (every 9th shorter than 3000 samples);
+    every 3rd has its middle third 120 dB down, so the whole-clip top_db
+    floor clamps those frames."""
+    rng = np.random.default_rng(seed)
+    clips = []
+    for i in range(n):
+        m = int(rng.integers(200, 8 * sr)) if i % 9 else int(rng.integers(1, 3000))
+        x = (0.1 * rng.standard_normal(m)).astype(np.float32)
+        if i % 3 == 0:
+            x[m // 3:2 * m // 3] *= np.float32(1e-6)
+        clips.append(x)
+    return clips
+
+
+@pytest.mark.parametrize("prec,tol", [("f32", 1e-4), ("bf16", 2e-3), ("f16", 2e-3)])
+def test_batched_top_db_clamp(prec, tol):
+    """The second (clamping) pass of a persistent batch recomputes every tile
+    whose frames fall below its clip's top_db floor, also when a workgroup's
+    other tiles need nothing (regression: a workgroup whose first tile was
+    unclamped used to leave without recomputing the rest). Every clip with a
+    quiet section matches the float64 oracle and its own single-clip run; for
+    the 16-bit contractions the pipelined 16-wave shape (staging rows and DCT
+    partials double-buffered, one barrier per tile, each tile's block sum done
+    after the next tile's barrier or after the loop) gives the bits of the
+    plain 16-wave shape, with long persistent loops (one workgroup per CU) and
+    short ones, graph or direct launches, and clips shorter than one tile."""
+    from metalmom_amd.batch import SpectralBatch
+    SR = 22050
+    clips = quiet_section_clips(sr=SR)
+    lens = np.array([len(c) for c in clips], dtype=np.int64)
+
+    def make(ls):
+        return SpectralBatch(ls, kind="mfcc", sr=SR, n_fft=2048, hop_length=512, n_mels=128, n_mfcc=13,
+                             top_db=80.0, precision=prec)
+
+    B = make(lens)
+    d_in = B.pack(clips)
+    ref = None
+    for shape in ((1, 4) if prec != "f32" else (1,)):
+        B.set_shape(shape)
+        for g in (0, 1):
+            B.set_grid(g)
+            sh = B.launch_shape()
+            assert (sh["shape"], sh["waves_per_wg"], sh["tile_frames"]) == (shape, 16, 16), sh
+            if g == 1:
+                assert sh["tiles_per_wg"] >= 3, sh
+            for graph in (False, True):
+                r = run_plan(B, d_in, graph=graph).cpu().numpy()
+                if ref is None:
+                    ref = r
+                    assert np.isfinite(ref).all()
+                else:
+                    assert np.array_equal(r, ref), (shape, g, graph)
+    quiet = [i for i in range(len(clips)) if i % 3 == 0]
+    for i in quiet:
+        _, off, rows, frames = B.clips[i]
+        got = ref[off:off + rows * frames].reshape(rows, frames)
+        want = o.mfcc(clips[i], SR, 13, 2048, 512)
+        assert got.shape == want.shape
+        assert rel_err(got, want) <= tol, i
+    B.set_shape(1)
+    assert_single_clip_bitwise(B, d_in, ref, lens, make, stride=3)
+
+
 # -- the 256-clip uneven batch ---------------------------------------------------
 
 @pytest.fixture(scope="module")
@@ -128,13 +195,18 @@ def test_uneven_batch_mfcc_persistent(uneven, prec, tol):
     one = B.launch_shape()
     assert one["tiles_per_wg"] >= 40, one
     assert np.array_equal(run_plan(B, d_in).cpu().numpy(), res)
-    # every workgroup shape gives the same bits
-    for shape, waves, frames in ((1, 16, 16), (2, 8, 8), (3, 8, 16)):
+    # every workgroup shape gives the same bits (the pipelined 16-wave shape
+    # exists for the 16-bit contractions only)
+    shapes = [(1, 16, 16), (2, 8, 8), (3, 8, 16)] + ([(4, 16, 16)] if prec != "f32" else [])
+    for shape, waves, frames in shapes:
         B.set_shape(shape)
         B.set_grid(0)
         sh = B.launch_shape()
         assert (sh["shape"], sh["waves_per_wg"], sh["tile_frames"]) == (shape, waves, frames), sh
         assert np.array_equal(run_plan(B, d_in, graph=False).cpu().numpy(), res), shape
+    if prec == "f32":
+        with pytest.raises(RuntimeError):
+            B.set_shape(4)
 
     # every clip alone
     B.set_shape(tuned["shape"])
