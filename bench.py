@@ -131,8 +131,9 @@ def mfma_flops_per_tile(sr, n_fft, n_mels, n_mfcc, prec):
     (rows 8-15 repeat rows 0-7), first bin rounded down to a multiple of 4;
     the DCT takes 3 16x16x16 MFMAs (hi/lo split) or 4 16x16x4 FP32 MFMAs per
     (block, 16 coefficients). The 16-bit kernels at n_fft 2048 also run each
-    frame's FFT on the matrix cores: two 64x64x32 real products per frame,
-    each as 8 output tiles x 2 K blocks x 3 (hi/lo) 16x16x32 FP16 MFMAs."""
+    frame's FFT on the matrix cores: per stage two 32x32x32 real products per
+    parity (a radix-2 step halves each 32-point DFT), each as 4 output tiles x
+    3 (hi/lo) 16x16x32 FP16 MFMAs: 48 MFMAs per frame."""
     from metalmom_amd._mel_fb import mel_filterbank
     fb = mel_filterbank(sr, n_fft, n_mels)
     ch = 16 if prec == "f32" else 32
@@ -144,7 +145,7 @@ def mfma_flops_per_tile(sr, n_fft, n_mels, n_mfcc, prec):
             chunks += (int(nz[-1]) + 1 - x0 + ch - 1) // ch
     nmh, ndt = (n_mels + 7) // 8, (n_mfcc + 15) // 16
     dct = nmh * ndt * (4 * 2 * 16 * 16 * 4 if prec == "f32" else 3 * 2 * 16 * 16 * 16)
-    fft = 16 * 2 * 8 * 2 * 3 * (2 * 16 * 16 * 32) if (prec != "f32" and n_fft == 2048) else 0
+    fft = 16 * 48 * (2 * 16 * 16 * 32) if (prec != "f32" and n_fft == 2048) else 0
     return chunks * 2 * 16 * 16 * ch + dct + fft
 
 

@@ -39,12 +39,12 @@ def test_fma_roofline_prices_two_flops_per_tap():
 def test_contraction_flop_counts():
     """Executed MFMA FLOPs of one n_fft 2048 / 128-mel / 13-MFCC BF16 tile:
     42 chunks of 32 bins (16x16x32 MFMAs), 16 blocks x 3 DCT MFMAs
-    (16x16x16) and the matrix-core FFT's 96 16x16x32 MFMAs per frame for 16
+    (16x16x16) and the matrix-core FFT's 48 16x16x32 MFMAs per frame for 16
     frames (the FP32 kernel and n_fft 4096 keep the FFT on the VALU); the
     FFT's 2.5 N log2 N; useful FLOPs from the filterbank's 2,018 nonzeros
     and the 128 x 13 DCT."""
     import bench
-    fft_tile = 16 * 96 * 16384
+    fft_tile = 16 * 48 * 16384
     assert bench.mfma_flops_per_tile(22050, 2048, 128, 13, "bf16") == 42 * 16384 + 16 * 3 * 8192 + fft_tile
     assert bench.mfma_flops_per_tile(22050, 2048, 128, 13, "f16") == 42 * 16384 + 16 * 3 * 8192 + fft_tile
     assert bench.mfma_flops_per_tile(22050, 2048, 128, 13, "f32") == 80 * 8192 + 16 * 4 * 2048
