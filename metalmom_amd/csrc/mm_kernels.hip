@@ -210,6 +210,16 @@ void k_tile(TileArgs a) {
     if (t_begin < t_end) {
       for (int p = threadIdx.x; p < kMfTabBytes / 16; p += NT)
         reinterpret_cast<uint4*>(tb)[p] = reinterpret_cast<const uint4*>(a.mft)[p];
+      // bins F .. S-1 of every staging row (both sets) read as zero in the
+      // banded contraction; with no FFT scratch under the rows, staging
+      // (bins 0 .. F-1) never touches them again
+      const int npad = S - F;
+      for (int idx = threadIdx.x; idx < NB * TF * npad; idx += NT) {
+        const int rr = idx / npad, k = F + idx % npad;
+        stg = lds + ((rr >= TF) ? a.lds_stage2_off : 0);
+        reinterpret_cast<uint16_t*>(row_ptr(rr % TF))[k] = 0;
+      }
+      stg = lds;
       lds_barrier();
     }
     win = a.window;
@@ -385,7 +395,36 @@ void k_tile(TileArgs a) {
       const int fr = f0 + fl;
       unsigned char* row = row_ptr(fl);
       if (fr < T) {
-        if constexpr (RS) {
+        if constexpr (MF) {
+          // the transform's bins (mfft_bin): slot (p, j) of this lane at
+          // byte ob[p] + 128 j of the row and its mirror at 2048 - ob[p] -
+          // 128 j; p = 0 and p = 1 of one slot share a packed conversion
+          const int lb = 4 * (lane & 15) + 512 * (lane >> 4);
+          const int ob0 = lb + (mfft_own_odd(lane, 0) ? 64 : 0), ob1 = lb + 2;
+          auto cvt2 = [&](float x0, float x1) -> unsigned {
+            if constexpr (FSCALE) {
+              x0 = ldexpf(x0, -fexp[qh]);
+              x1 = ldexpf(x1, -fexp[qh]);
+            }
+            if constexpr (PREC == PREC_BF16) {
+              typedef __bf16 b2v_ __attribute__((ext_vector_type(2)));
+              return __builtin_bit_cast(unsigned, __builtin_convertvector((f2v){x0, x1}, b2v_));
+            } else {
+              typedef _Float16 h2v_ __attribute__((ext_vector_type(2)));
+              return __builtin_bit_cast(unsigned, __builtin_convertvector((f2v){x0, x1}, h2v_));
+            }
+          };
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {   // register (b, r) of slot 4p + j: b = 2p + j / 2, r = j % 2 (+2: mirror)
+            const unsigned o = cvt2(hv[qh][j >> 1][j & 1][0], hv[qh][2 + (j >> 1)][j & 1][0]);
+            const unsigned m = cvt2(hv[qh][j >> 1][2 + (j & 1)][0], hv[qh][2 + (j >> 1)][2 + (j & 1)][0]);
+            *reinterpret_cast<uint16_t*>(row + ob0 + 128 * j) = (uint16_t)o;
+            *reinterpret_cast<uint16_t*>(row + ob1 + 128 * j) = (uint16_t)(o >> 16);
+            *reinterpret_cast<uint16_t*>(row + 2048 - ob0 - 128 * j) = (uint16_t)m;
+            *reinterpret_cast<uint16_t*>(row + 2048 - ob1 - 128 * j) = (uint16_t)(m >> 16);
+          }
+          if (lane == HL) put(row, M / 2, hn[qh], fexp[qh]);
+        } else if constexpr (RS) {
 #pragma unroll
           for (int b = 0; b < QL; ++b)
 #pragma unroll
@@ -401,8 +440,10 @@ void k_tile(TileArgs a) {
             }
           }
         }
-        if constexpr (!SPEC) {
-          // bins F .. S-1 must read as zero in the banded contraction
+        if constexpr (!SPEC && !MF) {
+          // bins F .. S-1 must read as zero in the banded contraction (the
+          // FFT scratch under the row overwrites them every tile; the
+          // matrix-core kernels zero them once)
           for (int k = F + lane; k < S; k += 64) {
             if constexpr (PREC == PREC_F32) reinterpret_cast<float*>(row)[k] = 0.0f;
             else reinterpret_cast<uint16_t*>(row)[k] = 0;
