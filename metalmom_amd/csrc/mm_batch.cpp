@@ -14,12 +14,14 @@ using namespace mm;
 
 namespace {
 
+// Captured steps of one (input, output, stream), one per set of top_db clip
+// maxima (steps alternate between the two).
 struct GraphEntry {
   const void* in;
   float* out;
   hipStream_t st;
-  hipGraph_t graph;
-  hipGraphExec_t exec;
+  hipGraph_t graph[2];
+  hipGraphExec_t exec[2];
 };
 
 struct Batch {
@@ -32,6 +34,10 @@ struct Batch {
   int shape = SHAPE_DEFAULT;   // workgroup shape the tile list was built for
   std::vector<int64_t> lens;
   std::vector<GraphEntry> graphs;
+  // Steps so far: step s uses the clip-maximum set s % 2, which the previous
+  // step's phase 1 cleared, and clears the other set for the next step
+  // (both sets are uploaded at -inf with the work list).
+  unsigned step = 0;
   ~Batch() {
     drop_graphs();
     if (lane) {
@@ -40,10 +46,11 @@ struct Batch {
     }
   }
   void drop_graphs() {   // captured launches carry the old grid and tile list
-    for (auto& g : graphs) {
-      (void)hipGraphExecDestroy(g.exec);
-      (void)hipGraphDestroy(g.graph);
-    }
+    for (auto& g : graphs)
+      for (int p = 0; p < 2; ++p) {
+        if (g.exec[p]) (void)hipGraphExecDestroy(g.exec[p]);
+        if (g.graph[p]) (void)hipGraphDestroy(g.graph[p]);
+      }
     graphs.clear();
   }
   int grid() const {
@@ -71,6 +78,7 @@ int32_t set_batch_shape(Batch* b, int shape) {
       hipStreamSynchronize(b->lane->stream) != hipSuccess)
     return ERR_INTERNAL;
   b->mp = meta_ptrs(b->m, b->lane->d_meta.p);
+  b->step = 0;
   b->shape = fused ? L.shape : SHAPE_DEFAULT;
   return prepare_batch(b->r, b->shape);   // kernel attributes, ahead of any stream capture
 }
@@ -103,10 +111,14 @@ int32_t create_batch(const MMSpecParams* params, const int64_t* lengths, int32_t
   return OK;
 }
 
+// One step with clip-maximum set `set` (0 or 1; -1: the next in turn).
 int32_t run_batch(Batch* b, const void* d_signal, float* d_out, hipStream_t st, hipEvent_t e0 = nullptr,
-                  hipEvent_t e1 = nullptr) {
-  return enqueue_batch(Engine::get(), b->r, b->tb, b->m, b->mp, d_signal, d_out, &b->lane->d_scratch,
-                       &b->lane->d_scratch2, b->grid(), st, e0, e1, b->shape);
+                  hipEvent_t e1 = nullptr, int set = -1) {
+  if (set < 0) set = (int)(b->step++ & 1u);
+  MetaPtrs mp = b->mp;
+  if (set) std::swap(mp.clip_max_key, mp.clip_max_key2);
+  return enqueue_batch(Engine::get(), b->r, b->tb, b->m, mp, d_signal, d_out, &b->lane->d_scratch,
+                       &b->lane->d_scratch2, b->grid(), st, e0, e1, b->shape, mp.clip_max_key2);
 }
 
 }  // namespace
@@ -177,34 +189,40 @@ int32_t mm_batch_run_graph(mm_batch bh, const void* d_signal, float* d_out, void
   Batch* b = static_cast<Batch*>(bh);
   if (!b || (b->m.total_in > 0 && !d_signal) || (b->m.total_out > 0 && !d_out)) return MM_ERR_INVALID_INPUT;
   hipStream_t st = hip_stream ? static_cast<hipStream_t>(hip_stream) : b->lane->stream;
-  for (auto& g : b->graphs)
-    if (g.in == d_signal && g.out == d_out && g.st == st)
-      return hipGraphLaunch(g.exec, st) == hipSuccess ? MM_OK : MM_ERR_INTERNAL;
   // the generic (non-fused) path grows scratch buffers on the fly, which
   // stream capture does not allow: it runs eagerly
   TileLayout L;
   if (!tile_layout(b->r.n_fft, b->r.out_kind, b->r.precision, b->r.n_mels, std::min(b->r.n_mfcc, b->r.n_mels), &L,
                    b->shape))
     return run_batch(b, d_signal, d_out, st);
-  GraphEntry g{d_signal, d_out, st, nullptr, nullptr};
-  if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return MM_ERR_INTERNAL;
-  const int32_t s = run_batch(b, d_signal, d_out, st);
-  hipGraph_t graph = nullptr;
-  const hipError_t ec = hipStreamEndCapture(st, &graph);
-  if (s != OK) {
-    if (graph) (void)hipGraphDestroy(graph);
-    return s;
+  const int set = (int)(b->step & 1u);
+  GraphEntry* g = nullptr;
+  for (auto& e : b->graphs)
+    if (e.in == d_signal && e.out == d_out && e.st == st) g = &e;
+  if (!g) {
+    b->graphs.push_back(GraphEntry{d_signal, d_out, st, {nullptr, nullptr}, {nullptr, nullptr}});
+    g = &b->graphs.back();
   }
-  if (ec != hipSuccess || !graph) return MM_ERR_INTERNAL;
-  hipGraphExec_t exec = nullptr;
-  if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
-    (void)hipGraphDestroy(graph);
-    return MM_ERR_INTERNAL;
+  if (!g->exec[set]) {
+    if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return MM_ERR_INTERNAL;
+    const int32_t s = run_batch(b, d_signal, d_out, st, nullptr, nullptr, set);
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(st, &graph);
+    if (s != OK) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return s;
+    }
+    if (ec != hipSuccess || !graph) return MM_ERR_INTERNAL;
+    hipGraphExec_t exec = nullptr;
+    if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+      (void)hipGraphDestroy(graph);
+      return MM_ERR_INTERNAL;
+    }
+    g->graph[set] = graph;
+    g->exec[set] = exec;
   }
-  g.graph = graph;
-  g.exec = exec;
-  b->graphs.push_back(g);
-  return hipGraphLaunch(exec, st) == hipSuccess ? MM_OK : MM_ERR_INTERNAL;
+  ++b->step;
+  return hipGraphLaunch(g->exec[set], st) == hipSuccess ? MM_OK : MM_ERR_INTERNAL;
 } catch (const std::bad_alloc&) {   // no exception leaves the C ABI
   return MM_ERR_ALLOC_FAILED;
 } catch (...) {

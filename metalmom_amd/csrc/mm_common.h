@@ -140,6 +140,11 @@ struct TileArgs {
   int use_top_db;
   int phase;                   // 1: unclamped pass, 2: recompute tiles whose clamp is active
   unsigned* clip_max_key;      // per clip, ordered-int encoding of the max dB
+  // phase 1, when set: the clip maxima of the next step, which workgroup 0
+  // sets to -inf (a plan alternates two sets, so its steps need no fill
+  // launch; nullptr: the caller clears clip_max_key before phase 1)
+  unsigned* clip_max_clear;
+  int n_clips;
   float* tile_min;             // per tile min dB (phase 1 writes, phase 2 reads)
   // output
   void* out;

@@ -557,7 +557,7 @@ static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 size_t meta_bytes(const BatchMeta& m) {
   const size_t c = (size_t)std::max(m.n_clips, 1), t = (size_t)std::max(m.n_tiles, 1);
-  return align256(c * 8) * 3 + align256(c * 4) + align256(t * 4) * 2 + align256(c * 4) + align256(t * 4) +
+  return align256(c * 8) * 3 + align256(c * 4) + align256(t * 4) * 2 + align256(c * 4) * 2 + align256(t * 4) +
          align256(t * sizeof(TilePf));
 }
 
@@ -573,6 +573,7 @@ MetaPtrs meta_ptrs(const BatchMeta& m, void* dev) {
   p.tile_clip = reinterpret_cast<const int32_t*>(b + o); o += align256(t * 4);
   p.tile_f0 = reinterpret_cast<const int32_t*>(b + o); o += align256(t * 4);
   p.clip_max_key = reinterpret_cast<unsigned*>(b + o); o += align256(c * 4);
+  p.clip_max_key2 = reinterpret_cast<unsigned*>(b + o); o += align256(c * 4);
   p.tile_min = reinterpret_cast<float*>(b + o); o += align256(t * 4);
   p.tile_pf = reinterpret_cast<const TilePf*>(b + o);
   return p;
@@ -595,7 +596,11 @@ void pack_meta(const BatchMeta& m, void* host) {
   o += align256(t * 4);
   if (m.n_tiles) std::memcpy(b + o, m.tile_f0.data(), (size_t)m.n_tiles * 4);
   o += align256(t * 4);
-  o += align256(c * 4);   // clip_max_key (device-written)
+  // the two sets of clip maxima (device-written): -inf, as phase 1 expects them
+  for (int k = 0; k < 2; ++k) {
+    for (int i = 0; i < m.n_clips; ++i) reinterpret_cast<unsigned*>(b + o)[i] = float_to_key(-INFINITY);
+    o += align256(c * 4);
+  }
   o += align256(t * 4);   // tile_min (device-written)
   if (m.n_tiles) std::memcpy(b + o, m.tile_pf.data(), (size_t)m.n_tiles * sizeof(TilePf));
 }
@@ -668,6 +673,7 @@ static TileArgs make_args(const SpecRequest& r, const Tables* tb, const BatchMet
   a.use_top_db = (r.out_kind == OUT_MFCC && r.top_db > 0.0f) ? 1 : 0;
   a.phase = 1;
   a.clip_max_key = mp.clip_max_key;
+  a.n_clips = m.n_clips;
   a.tile_min = mp.tile_min;
   a.out = d_out;
   a.out_kind = r.out_kind;
@@ -731,7 +737,8 @@ int32_t prepare_batch(const SpecRequest& r, int shape) {
 
 int32_t enqueue_batch(Engine* eng, const SpecRequest& r, const Tables* tb, const BatchMeta& m, const MetaPtrs& mp,
                       const void* d_signal, void* d_out, DevBuf* scratch, DevBuf* scratch2, int grid_override,
-                      hipStream_t st, hipEvent_t ev_p1_begin, hipEvent_t ev_p1_end, int shape) {
+                      hipStream_t st, hipEvent_t ev_p1_begin, hipEvent_t ev_p1_end, int shape,
+                      unsigned* clear_next) {
   if (m.n_tiles == 0) return OK;
   TileLayout L;
   const bool fused =
@@ -786,7 +793,13 @@ int32_t enqueue_batch(Engine* eng, const SpecRequest& r, const Tables* tb, const
       LL.lds_bytes = off + fbc + dcb;
     }
   }
-  if (a.use_top_db) MM_HIP(launch_fill_u32(mp.clip_max_key, m.n_clips, float_to_key(-INFINITY), st));
+  // clear_next: mp.clip_max_key is at -inf already (a plan's alternating
+  // sets: the previous step's phase 1 cleared it) and this phase 1 clears
+  // the other set for the next step; otherwise a fill launch clears it
+  if (a.use_top_db) {
+    if (clear_next) a.clip_max_clear = clear_next;
+    else MM_HIP(launch_fill_u32(mp.clip_max_key, m.n_clips, float_to_key(-INFINITY), st));
+  }
   if (ev_p1_begin) MM_HIP(hipEventRecord(ev_p1_begin, st));
   MM_HIP(launch_tile(a, r.precision, LL, g, st));
   if (ev_p1_end) MM_HIP(hipEventRecord(ev_p1_end, st));

@@ -171,6 +171,7 @@ struct MetaPtrs {
   const int32_t* tile_clip;
   const int32_t* tile_f0;
   unsigned* clip_max_key;
+  unsigned* clip_max_key2;   // the second set of a plan's alternating steps
   float* tile_min;
   const TilePf* tile_pf;
 };
@@ -185,7 +186,7 @@ int launch_grid(Engine* eng, const SpecRequest& r, int n_tiles, const TileLayout
 int32_t enqueue_batch(Engine* eng, const SpecRequest& r, const Tables* t, const BatchMeta& m, const MetaPtrs& mp,
                       const void* d_signal, void* d_out, DevBuf* scratch, DevBuf* scratch2, int grid_override,
                       hipStream_t st, hipEvent_t ev_p1_begin = nullptr, hipEvent_t ev_p1_end = nullptr,
-                      int shape = SHAPE_DEFAULT);
+                      int shape = SHAPE_DEFAULT, unsigned* clear_next = nullptr);
 
 // Device buffer for the phase timing of instrumented builds (null: off).
 long long* phase_cycles_buffer();

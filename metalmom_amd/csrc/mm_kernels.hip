@@ -117,6 +117,11 @@ void k_tile(TileArgs a) {
   const int t_begin = blockIdx.x * a.tiles_per_wg;
   const int t_end = min(t_begin + a.tiles_per_wg, a.n_tiles);
   if constexpr (MODE == OUT_MFCC) {
+    // the next step's clip maxima (nobody reads them in this step)
+    if (a.phase == 1 && a.clip_max_clear && blockIdx.x == 0)
+      for (int c = threadIdx.x; c < a.n_clips; c += NT) a.clip_max_clear[c] = float_to_key(-INFINITY);
+  }
+  if constexpr (MODE == OUT_MFCC) {
     // phase 2 recomputes only tiles whose top_db floor is active: a
     // workgroup with none of them leaves before filling its LDS tables
     if (a.phase == 2) {

@@ -162,6 +162,53 @@ def test_batched_top_db_clamp(prec, tol):
     assert_single_clip_bitwise(B, d_in, ref, lens, make, stride=3)
 
 
+@pytest.mark.parametrize("prec", ["f32", "bf16"])
+def test_plan_steps_restore_clip_maxima(prec):
+    """A plan's steps launch no fill of the top_db clip maxima: they
+    alternate between two sets, and each step's first pass sets the other one
+    to -inf for the next step. Steps alternate between two inputs whose clip
+    maxima differ by 60 dB (the loud parts of the second are 1000x quieter,
+    so its floor sits 60 dB lower and clamps other frames), through graph
+    replays of both sets and direct launches, in an order that gives each
+    input both sets; every step must equal a fresh plan's first step on the
+    same input bit for bit (a maximum left over from an earlier step would
+    raise the floor)."""
+    from metalmom_amd.batch import SpectralBatch
+    SR = 22050
+    clips = quiet_section_clips(n=48, seed=11, sr=SR)
+    quiet = []
+    for i, c in enumerate(clips):
+        q = c.copy()
+        if i % 3 == 0:   # loud thirds 60 dB down, the quiet middle unchanged
+            m = len(q)
+            q[:m // 3] *= np.float32(1e-3)
+            q[2 * m // 3:] *= np.float32(1e-3)
+        quiet.append(q)
+    lens = np.array([len(c) for c in clips], dtype=np.int64)
+
+    def make():
+        return SpectralBatch(lens, kind="mfcc", sr=SR, n_fft=2048, hop_length=512, n_mels=128, n_mfcc=13,
+                             top_db=80.0, precision=prec)
+
+    refs = []
+    for data in (clips, quiet):
+        F = make()
+        refs.append(run_plan(F, F.pack(data)).cpu().numpy())
+        F.close()
+    assert not np.array_equal(refs[0], refs[1])
+    B = make()
+    B.set_grid(1)   # long persistent loops
+    d = [B.pack(clips), B.pack(quiet)]
+    outs = [B.empty_output(), B.empty_output()]   # one graph per (input, output): replays
+    for step, (k, graph) in enumerate([(0, True), (1, True), (0, True), (1, False), (1, True), (0, False),
+                                       (0, True), (1, True)]):
+        outs[k].fill_(float("nan"))
+        B.run(d[k], outs[k], graph=graph)
+        sync()
+        assert np.array_equal(outs[k].cpu().numpy(), refs[k]), (step, k, graph)
+    B.close()
+
+
 # -- the 256-clip uneven batch ---------------------------------------------------
 
 @pytest.fixture(scope="module")
