@@ -250,9 +250,11 @@ def large_fft_leg(torch, steps):
     clips = large_fft_batch()
     gen_s = time.perf_counter() - t0
     lens = [len(c) for c in clips]
-    B = SpectralBatch(lens, kind="mfcc", sr=44100, n_fft=4096, hop_length=1024, n_mels=256, n_mfcc=20,
-                      precision="f16", input_dtype="f16")
+    common = dict(sr=44100, n_fft=4096, hop_length=1024, n_mels=256, n_mfcc=20, precision="f16", input_dtype="f16")
+    B = SpectralBatch(lens, kind="mfcc", **common)
     d_in = B.pack(clips)
+    err_idx = [0, 341, 682, 1023]
+    err_clips = {i: np.asarray(clips[i], dtype=np.float64) for i in err_idx}
     del clips
     out = B.empty_output()
     st = torch.cuda.Stream()
@@ -275,12 +277,35 @@ def large_fft_leg(torch, steps):
     rl = kernel_roofline(frames, bytes_launch, 4096, B.n_tiles, mfma_flops_per_tile(44100, 4096, 256, 20, "f16"),
                          useful_mfma_flops_per_frame(44100, 4096, 256, 20), "f16", k_ms)
     sh = B.launch_shape()
+    # FP64 error of sampled clips (outside the timed region): the batch's own
+    # MFCC blocks, and the mel of one-clip plans on the same device samples
+    B.run(d_in, out, stream=st, graph=True)
+    torch.cuda.synchronize()
+    res = out.cpu().numpy()
+    err = {"mel_db_abs": 0.0, "mfcc_rel": 0.0}
+    for i in err_idx:
+        _, mel, mf = fp64_reference(err_clips[i], sr=44100, n_fft=4096, hop=1024, n_mels=256, n_mfcc=20)
+        _, ooff, rows, nfr = B.clips[i]
+        got = res[ooff:ooff + rows * nfr].reshape(rows, nfr).astype(np.float64)
+        err["mfcc_rel"] = max(err["mfcc_rel"], float(np.abs(got - mf).max() / np.abs(mf).max()))
+        ioff, n = B.clips[i][0], lens[i]
+        S = SpectralBatch([n], kind="mel", **common)
+        o = S.empty_output()
+        S.run(d_in[ioff:ioff + n], o)
+        torch.cuda.synchronize()
+        _, _, mrows, mfr = S.clips[0]
+        gm = o[:mrows * mfr].cpu().numpy().reshape(mrows, mfr)
+        S.close()
+        err["mel_db_abs"] = max(err["mel_db_abs"], float(np.abs(_db(gm) - _db(mel)).max()))
+    err["within_0.05_db"] = bool(err["mel_db_abs"] <= 0.05)
+    err["sample"] = (f"clips {err_idx} of the batch vs float64 numpy (rfft, Slaney mel, dB with top_db 80, "
+                     f"DCT-II ortho); MFCC from the batch result, mel from one-clip plans")
     B.close()
     return {"workload": "1024 clips x 10 s @44.1 kHz, FP16 input, n_fft 4096 hop 1024, 256 mels, 20 MFCC, "
                         "FP16 MFMA mel/DCT (FP32 FFT), one persistent hipGraph launch per step",
             "frames": frames, "frames_per_s": frames * steps / wall, "step_ms": timing_stats(step_ms),
             "kernel_ms": k_ms, "persistent_wg_per_cu": wpc, "launch": sh, "roofline": rl,
-            "input_generation_s": gen_s}
+            "error_vs_fp64": err, "input_generation_s": gen_s}
 
 
 def istft_leg(torch, n_samples, iters, with_cpu):

@@ -352,10 +352,13 @@ int32_t mm_batch_info(mm_batch b, int64_t* total_input_samples, int64_t* total_o
 int32_t mm_batch_clip(mm_batch b, int32_t clip, int64_t* input_offset, int64_t* output_offset,
                       int64_t* rows, int64_t* frames);
 /* Enqueue one step on `hip_stream` (NULL: the context's stream). Inputs and
- * outputs are device pointers; nothing is allocated. */
+ * outputs are device pointers; nothing is allocated. Steps of one plan
+ * alternate between two sets of top_db scratch, each step clearing the
+ * other set for the next one: a plan's steps must run in the order they
+ * were enqueued (one stream, or streams ordered by events). */
 int32_t mm_batch_run(mm_batch b, const void* d_signal, float* d_out, void* hip_stream);
 /* Same, replayed from a hipGraph captured on first use for this
- * (d_signal, d_out) pair. */
+ * (d_signal, d_out, stream) and scratch set (two graphs per triple). */
 int32_t mm_batch_run_graph(mm_batch b, const void* d_signal, float* d_out, void* hip_stream);
 /* Persistent grid: workgroups per CU (0: autotuned or default). */
 int32_t mm_batch_set_grid(mm_batch b, int32_t wg_per_cu);
