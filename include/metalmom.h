@@ -428,6 +428,11 @@ int32_t mm_server_poll(mm_server s, int64_t request_id, MMBuffer* out, double* l
 int32_t mm_server_wait(mm_server s, int64_t request_id, MMBuffer* out, double* latency_s);
 int32_t mm_server_stats(mm_server s, int64_t* submitted, int64_t* completed, int64_t* steps,
                         int64_t* frames);
+/* Device memory the server's streams hold for their steps (inputs, outputs,
+ * work lists, scratch; not the shared tables): at most 2.5 x the HBM budget
+ * while steps run, at most the budget once they are done. No counterpart in
+ * the reference (it has no server). */
+int32_t mm_server_device_bytes(mm_server s, int64_t* bytes);
 /* Finishes queued work, then releases the server. */
 int32_t mm_server_destroy(mm_server s);
 

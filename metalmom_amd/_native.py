@@ -116,6 +116,7 @@ SIGNATURES = {
     "mm_server_poll": (_I32, [_P, _I64, _BUF, C.POINTER(C.c_double)]),
     "mm_server_wait": (_I32, [_P, _I64, _BUF, C.POINTER(C.c_double)]),
     "mm_server_stats": (_I32, [_P, C.POINTER(_I64), C.POINTER(_I64), C.POINTER(_I64), C.POINTER(_I64)]),
+    "mm_server_device_bytes": (_I32, [_P, C.POINTER(_I64)]),
     "mm_server_destroy": (_I32, [_P]),
 }
 

@@ -153,18 +153,35 @@ bool same_plan(const SpecRequest& a, const SpecRequest& b) {
 
 size_t a256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-// Device bytes a request holds while in a step: its input and output, plus
-// for configurations outside the fused kernel the generic path's complex
-// spectrogram scratch (T x F x 8 bytes) and, for MFCC, its mel scratch.
+// Device bytes a request holds while in a step: its input and output, its
+// share of the work list (per-clip records and at most T / 8 + 1 tiles of
+// 48 bytes), plus for configurations outside the fused kernel the generic
+// path's complex spectrogram scratch (T x F x 8 bytes) and, for MFCC, its
+// mel scratch.
 size_t request_bytes(const SpecRequest& r, int64_t len) {
   const int64_t T = frame_count(len, r.n_fft, r.hop, r.center);
-  size_t b = a256((size_t)len * 4) + a256((size_t)(out_rows(r) * T * out_elem_floats(r)) * 4) + 1024;
+  size_t b = a256((size_t)len * 4) + a256((size_t)(out_rows(r) * T * out_elem_floats(r)) * 4) + 1024 +
+             (size_t)(T / 8 + 1) * 48;
   TileLayout L;
   if (!tile_layout(r.n_fft, r.out_kind, r.precision, r.n_mels, std::min(r.n_mfcc, r.n_mels), &L)) {
     b += a256((size_t)T * (size_t)(r.n_fft / 2 + 1) * 8);
     if (r.out_kind == OUT_MFCC) b += a256((size_t)r.n_mels * (size_t)T * 4 + 256);
   }
   return b;
+}
+
+// Device bytes a lane's buffers hold, and their release (the lane is idle).
+size_t lane_dev_bytes(const Lane* L) {
+  return L->d_in.n + L->d_out.n + L->d_meta.n + L->d_scratch.n + L->d_scratch2.n + L->d_small.n;
+}
+void release_dev(Lane* L) {
+  L->d_in.release();
+  L->d_out.release();
+  L->d_meta.release();
+  L->d_scratch.release();
+  L->d_scratch2.release();
+  L->d_small.release();
+  L->meta_key.clear();
 }
 
 // Device staging of a PCM request: its raw bytes, and its converted samples
@@ -190,7 +207,9 @@ class Server {
     for (int i = 0; i < kLanes; ++i) {
       Lane* l = e->acquire_lane();
       if (!l) return ERR_INTERNAL;
+      release_dev(l);   // pooled lanes may hold buffers of earlier users
       free_lanes_.push_back(l);
+      lanes_.push_back(l);
     }
     th_ = std::thread([this] { loop(); });
     return OK;
@@ -295,6 +314,13 @@ class Server {
     delete q;
     return s == OK ? 1 : s;
   }
+
+  // Device bytes the server's lanes hold (inputs, outputs, work lists,
+  // scratch). A lane keeps its buffers after a step only while they hold at
+  // most budget / 3 bytes, and a step's buffers hold at most 1.5 times its
+  // own bytes, which the admission caps at the budget over all steps in
+  // flight: at most 2.5 x the budget in all, at most the budget when idle.
+  size_t device_bytes() const { return held_.load(); }
 
   void stats(int64_t* submitted, int64_t* completed, int64_t* steps, int64_t* frames) {
     std::lock_guard<std::mutex> g(mu_);
@@ -439,6 +465,7 @@ class Server {
     chk(L->d_meta.ensure(s.meta_bytes_));
     L->meta_key.clear();
     chk(L->d_out.ensure(s.out_bytes));
+    update_held();
     if (s.out_bytes > 0) {
       if (char* p = out_ring_.alloc(s.out_bytes)) {
         s.blk = new OutBlock();
@@ -583,6 +610,11 @@ class Server {
         s->blk = nullptr;
       }
       (void)hipEventDestroy(s->done);
+      // the idle lane keeps its buffers only up to a third of the budget
+      if (lane_dev_bytes(s->lane) > budget_ / kLanes) {
+        release_dev(s->lane);
+        update_held();
+      }
       {
         std::lock_guard<std::mutex> g(mu_);
         for (auto& G : s->groups)
@@ -601,8 +633,18 @@ class Server {
     }
   }
 
+  // held_ = device bytes of all lanes; written by the scheduler thread, the
+  // only one that resizes lane buffers
+  void update_held() {
+    size_t b = 0;
+    for (const Lane* l : lanes_) b += lane_dev_bytes(l);
+    held_.store(b);
+  }
+
   size_t budget_;
   int max_batch_;
+  std::vector<Lane*> lanes_;
+  std::atomic<size_t> held_{0};
   std::mutex mu_;
   std::condition_variable cv_, done_cv_;
   std::deque<Req*> queue_;
@@ -692,6 +734,14 @@ int32_t mm_server_stats(mm_server srv, int64_t* submitted, int64_t* completed, i
 } catch (const std::bad_alloc&) {   // no exception leaves the C ABI
   return MM_ERR_ALLOC_FAILED;
 } catch (...) {
+  return MM_ERR_INTERNAL;
+}
+
+int32_t mm_server_device_bytes(mm_server srv, int64_t* bytes) try {
+  if (!srv || !bytes) return MM_ERR_INVALID_INPUT;
+  *bytes = (int64_t)static_cast<Server*>(srv)->device_bytes();
+  return MM_OK;
+} catch (...) {   // no exception leaves the C ABI
   return MM_ERR_INTERNAL;
 }
 

@@ -92,6 +92,14 @@ class SpectralServer:
         lib.mm_server_stats(self._h, C.byref(a), C.byref(b), C.byref(c), C.byref(d))
         return {"submitted": a.value, "completed": b.value, "steps": c.value, "frames": d.value}
 
+    def device_bytes(self) -> int:
+        """Device memory the server's streams hold for their steps."""
+        b = C.c_int64()
+        s = lib.mm_server_device_bytes(self._h, C.byref(b))
+        if s != 0:
+            raise RuntimeError(f"mm_server_device_bytes failed with status {s}")
+        return b.value
+
     def close(self):
         if getattr(self, "_h", None):
             lib.mm_server_destroy(self._h)

@@ -91,6 +91,43 @@ def test_small_budget_serialises_and_rejects_oversized(sources):
         assert np.array_equal(a, ref)
 
 
+def test_device_memory_stays_within_budget_multiple(sources):
+    """The server's streams keep their step buffers only up to a third of
+    the HBM budget each once a step is done, so the device memory they hold
+    stays within 2.5 x the budget while steps run and within the budget once
+    they are done, whatever the mix of small and near-budget requests; the
+    results still equal each clip run alone."""
+    from metalmom_amd.serve import SpectralServer
+    budget = 16 << 20
+    srv = SpectralServer(hbm_budget_bytes=budget, max_batch=8)
+    assert srv.device_bytes() == 0
+    rng = np.random.default_rng(21)
+    y22 = sources[22050]
+    reqs = []
+    for i in range(36):
+        kind = ("stft", "mel", "mfcc")[i % 3]
+        # every 4th request is long (30-38 s): as an STFT magnitude request
+        # ~12 bytes per sample, 8-10 MB, over half the budget
+        n = int(rng.integers(30 * 22050, 38 * 22050)) if i % 4 == 0 else int(rng.integers(2000, 3 * 22050))
+        off = int(rng.integers(0, len(y22) - n))
+        reqs.append((kind, y22[off:off + n]))
+    ids = [srv.submit(y, srv.params(kind, sr=22050)) for kind, y in reqs]
+    peak = 0
+    outs = []
+    for i in ids:
+        peak = max(peak, srv.device_bytes())
+        outs.append(srv.wait(i)[0])
+        peak = max(peak, srv.device_bytes())
+    st = srv.stats()
+    idle = srv.device_bytes()
+    srv.close()
+    assert st["completed"] == len(reqs)
+    assert 0 < peak <= 2.5 * budget, peak
+    assert idle <= budget, idle
+    for (kind, y), a in zip(reqs, outs):
+        assert np.array_equal(a, _alone(kind, 22050, y)), (kind, len(y))
+
+
 def test_open_loop_latency_report(sources):
     from metalmom_amd.serve import SpectralServer, open_loop
     srv = SpectralServer()
