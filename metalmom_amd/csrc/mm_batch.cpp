@@ -247,9 +247,12 @@ int32_t mm_batch_time(mm_batch bh, const void* d_signal, float* d_out, void* hip
   if (!b || iters <= 0) return MM_ERR_INVALID_INPUT;
   hipStream_t st = hip_stream ? static_cast<hipStream_t>(hip_stream) : b->lane->stream;
   std::vector<hipEvent_t> ev(4 * iters, nullptr);
-  for (auto& e : ev)
-    if (hipEventCreate(&e) != hipSuccess) return MM_ERR_INTERNAL;
   int32_t s = OK;
+  for (auto& e : ev)
+    if (s == OK && hipEventCreate(&e) != hipSuccess) {
+      e = nullptr;
+      s = ERR_INTERNAL;
+    }
   for (int i = 0; i < iters && s == OK; ++i) {
     if (hipEventRecord(ev[4 * i], st) != hipSuccess) s = ERR_INTERNAL;
     if (s == OK) s = run_batch(b, d_signal, d_out, st, ev[4 * i + 1], ev[4 * i + 2]);
@@ -265,7 +268,8 @@ int32_t mm_batch_time(mm_batch bh, const void* d_signal, float* d_out, void* hip
     k += a;
     t += c;
   }
-  for (auto& e : ev) (void)hipEventDestroy(e);
+  for (auto& e : ev)
+    if (e) (void)hipEventDestroy(e);
   if (s != OK) return s;
   if (kernel_ms) *kernel_ms = (float)(k / iters);
   if (step_ms) *step_ms = (float)(t / iters);
