@@ -308,6 +308,45 @@ def large_fft_leg(torch, steps):
             "error_vs_fp64": err, "input_generation_s": gen_s}
 
 
+def single_clip_leg(iters=20):
+    """One clip per call through the drop-in Python API (a context per call,
+    host buffers in and out, as the reference's wrappers do): the plain 30 s
+    clip at 22.05 kHz (MFCC 13, zero and reflect padding), and the reference's
+    own benchmark condition, a 30 s 440 Hz sine at 44.1 kHz (STFT magnitude,
+    128-mel spectrogram, MFCC 20), whose published Apple M4 Max timings are
+    7.9 / 7.1 / 7.3 ms. Wall time per call, after 3 warm-up calls."""
+    import metalmom_amd as mm
+    from metalmom_amd import feature
+    from metalmom_amd.workloads import plain_clip
+    y0 = plain_clip()
+    t = np.arange(30 * 44100) / 44100
+    y1 = np.sin(2 * np.pi * 440 * t).astype(np.float32)
+
+    def timed(fn):
+        for _ in range(3):
+            r = fn()
+        ms = []
+        for _ in range(iters):
+            t0 = time.perf_counter()
+            r = fn()
+            ms.append((time.perf_counter() - t0) * 1e3)
+        return timing_stats(ms), r.shape[-1]
+
+    out = {}
+    for name, fn in [
+        ("mfcc_30s_22k_constant", lambda: feature.mfcc(y=y0, sr=22050, n_mfcc=13)),
+        ("mfcc_30s_22k_reflect", lambda: mm.spectral(y0, kind="mfcc", sr=22050, n_mfcc=13, pad_mode="reflect")),
+        ("stft_30s_44k_sine", lambda: mm.stft(y1, n_fft=2048)),
+        ("mel_30s_44k_sine", lambda: feature.melspectrogram(y=y1, sr=44100)),
+        ("mfcc20_30s_44k_sine", lambda: feature.mfcc(y=y1, sr=44100, n_mfcc=20)),
+    ]:
+        st, frames = timed(fn)
+        out[name] = {"ms": st, "frames": int(frames), "frames_per_s": frames / (st["p50"] * 1e-3)}
+    out["reference_published_ms_m4max"] = {"stft_30s_44k_sine": 7.9, "mel_30s_44k_sine": 7.1,
+                                           "mfcc20_30s_44k_sine": 7.3}
+    return out
+
+
 def istft_leg(torch, n_samples, iters, with_cpu):
     """Inverse STFT (mm_istft_device) of one long clip on device-resident data:
     n_fft 2048, hop 512, centred, the spectrogram of n_samples samples
@@ -885,6 +924,11 @@ def main():
         log(f"large-FFT FP16 batch: {lf['frames_per_s'] / 1e6:.1f} M frames/s, kernel {lf['kernel_ms']:.3f} ms, "
             f"{lf['roofline']['hbm']['achieved']:.0f} GB/s, FFT {lf['roofline']['achieved']:.1f} TFLOP/s "
             f"(inputs generated in {lf['input_generation_s']:.1f} s)")
+    if not args.profile:
+        sc = single_clip_leg()
+        extra["single_clip_api"] = sc
+        log("single clip through the Python API (p50 ms): " +
+            ", ".join(f"{k} {v['ms']['p50']:.2f}" for k, v in sc.items() if isinstance(v, dict) and "ms" in v))
 
     serving = []
     if args.serve_rates and not args.profile:
