@@ -284,11 +284,19 @@ class Server {
     std::unique_lock<std::mutex> g(mu_);
     auto it = reqs_.find(id);
     if (it == reqs_.end()) return ERR_INVALID;
+    if (wait) {
+      // another thread may collect the same id meanwhile: look it up again
+      done_cv_.wait(g, [&] {
+        it = reqs_.find(id);
+        return it == reqs_.end() || it->second->state == 2;
+      });
+      if (it == reqs_.end()) return ERR_INVALID;
+    }
     Req* q = it->second;
-    if (wait) done_cv_.wait(g, [&] { return q->state == 2; });
     if (q->state != 2) return 0;
     reqs_.erase(it);
     g.unlock();
+    done_cv_.notify_all();   // a second waiter on this id returns ERR_INVALID
     if (latency) *latency = q->t_done - q->t_submit;
     int32_t s = q->status;
     if (s == OK && q->res) {  // copy the result out of the output ring

@@ -183,3 +183,32 @@ def test_full_rings_fall_back_to_heap_paths():
             assert np.array_equal(got, alone)
     finally:
         srv.close()
+
+
+def test_two_waiters_on_one_request(sources):
+    """Two threads waiting on the same request id: one collects the result,
+    the other returns the invalid-id status instead of touching the
+    collected request or blocking."""
+    import threading
+    from metalmom_amd.serve import SpectralServer
+    y = sources[22050][:4 * 22050]
+    srv = SpectralServer(hbm_budget_bytes=1 << 30)
+    for rep in range(5):
+        rid = srv.submit(y, srv.params("mfcc", sr=22050))
+        got, errs = [], []
+
+        def waiter():
+            try:
+                got.append(srv.wait(rid)[0])
+            except RuntimeError as e:
+                errs.append(str(e))
+
+        ths = [threading.Thread(target=waiter) for _ in range(2)]
+        for t in ths:
+            t.start()
+        for t in ths:
+            t.join(timeout=60)
+        assert not any(t.is_alive() for t in ths), rep
+        assert len(got) == 1 and len(errs) == 1 and "status -1" in errs[0], (rep, errs)
+    srv.close()
+    assert np.array_equal(got[0], _alone("mfcc", 22050, y))
