@@ -889,29 +889,6 @@ def main():
                 f"{err['f32']['mel_db_abs']:.2e} dB, MFCC {err['f32']['mfcc_rel']:.2e} rel; BF16 mel "
                 f"{err['bf16']['mel_db_abs']:.2e} dB, MFCC {err['bf16']['mfcc_rel']:.2e} rel")
 
-        if not args.profile:
-            # end to end from host buffers (mm_spectral_batch: clips copied
-            # into pinned memory, one H2D copy, the persistent launch, one D2H
-            # copy, per-clip results copied out), reported beside the
-            # device-resident figure
-            from metalmom_amd.batch import spectral_batch
-            e2e = []
-            for it in range(4):
-                t0 = time.perf_counter()
-                res = spectral_batch(clips, kind="mfcc", n_fft=2048, hop_length=512, n_mels=128, n_mfcc=n_mfcc,
-                                     precision="bf16")
-                if it:
-                    e2e.append((time.perf_counter() - t0) * 1e3)
-            assert sum(r.shape[1] for r in res) == frames
-            est = timing_stats(e2e)
-            extra["end_to_end_host_batch_bf16"] = {
-                "ms": est, "frames_per_s": frames / (est["p50"] * 1e-3),
-                "input_bytes": int(4 * B.total_input),
-                "path": "host float32 clips -> pinned staging -> one H2D copy -> persistent launch -> one D2H copy "
-                        "-> per-clip host arrays (mm_spectral_batch, plan created per call)"}
-            log(f"end to end from host buffers (BF16): {frames / (est['p50'] * 1e-3) / 1e6:.2f} M frames/s "
-                f"(p50 {est['p50']:.1f} ms per batch)")
-
         if not args.no_plain:
             # plain path: one clip per launch, FP32, no graph, in sequence on
             # one stream (each plan would otherwise use its own pooled stream
@@ -940,6 +917,29 @@ def main():
                 f"{pst['min']:.2f}, p50 {pst['p50']:.2f}); batched bf16 is {value / plain_fps:.2f}x")
             for s in singles:
                 s.close()
+
+        if not args.profile:
+            # end to end from host buffers (mm_spectral_batch: clips copied
+            # into pinned memory, one H2D copy, the persistent launch, one D2H
+            # copy, per-clip results copied out), reported beside the
+            # device-resident figure
+            from metalmom_amd.batch import spectral_batch
+            e2e = []
+            for it in range(4):
+                t0 = time.perf_counter()
+                res = spectral_batch(clips, kind="mfcc", n_fft=2048, hop_length=512, n_mels=128, n_mfcc=n_mfcc,
+                                     precision="bf16")
+                if it:
+                    e2e.append((time.perf_counter() - t0) * 1e3)
+            assert sum(r.shape[1] for r in res) == frames
+            est = timing_stats(e2e)
+            extra["end_to_end_host_batch_bf16"] = {
+                "ms": est, "frames_per_s": frames / (est["p50"] * 1e-3),
+                "input_bytes": int(4 * B.total_input),
+                "path": "host float32 clips -> pinned staging -> one H2D copy -> persistent launch -> one D2H copy "
+                        "-> per-clip host arrays (mm_spectral_batch, plan created per call)"}
+            log(f"end to end from host buffers (BF16): {frames / (est['p50'] * 1e-3) / 1e6:.2f} M frames/s "
+                f"(p50 {est['p50']:.1f} ms per batch)")
 
     if not args.no_large_fft and not args.profile:
         lf = large_fft_leg(torch, max(5, steps // 2))
