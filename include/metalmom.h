@@ -365,10 +365,10 @@ int32_t mm_batch_set_grid(mm_batch b, int32_t wg_per_cu);
 /* Workgroup shapes of the fused kernel: MM_SHAPE_W16 = 16 wavefronts, one
  * frame each, 16-frame tiles (n_fft <= 2048); MM_SHAPE_W8 = 8 wavefronts,
  * 8-frame tiles; MM_SHAPE_W8F2 = 8 wavefronts with two frames each, 16-frame
- * tiles, two workgroups per CU (n_fft 1024 and 2048); MM_SHAPE_W16P = 16
- * wavefronts pipelined (staging rows and DCT partials double-buffered, one
- * barrier per tile; BF16 / FP16 MFCC at n_fft 2048). Results do not depend
- * on the shape. */
+ * tiles (n_fft 1024 to 4096; two workgroups per CU at 1024 and 2048);
+ * MM_SHAPE_W16P = 16 wavefronts pipelined (staging rows and DCT partials
+ * double-buffered, one barrier per tile; BF16 / FP16 MFCC at n_fft 2048).
+ * Results do not depend on the shape. */
 #define MM_SHAPE_DEFAULT 0
 #define MM_SHAPE_W16 1
 #define MM_SHAPE_W8 2

@@ -1310,7 +1310,8 @@ static hipError_t launch_tile_t(const TileArgs& a, int grid, size_t lds, hipStre
   return hipGetLastError();
 }
 
-// (NW, FPW): 16 x 1 or 8 x 1 (8-frame tiles) up to n_fft 2048; 8 x 1 at 4096
+// (NW, FPW): 16 x 1 up to n_fft 2048; 8 x 1 (8-frame tiles) at every size;
+// 8 x 2 (16-frame tiles) from n_fft 1024 to 4096
 template <int LOG2M, int MODE, int PREC, typename TIn>
 static hipError_t launch_tile_s(const TileArgs& a, const TileLayout& L, int grid, hipStream_t st) {
   if constexpr (LOG2M <= 10) {
@@ -1319,7 +1320,7 @@ static hipError_t launch_tile_s(const TileArgs& a, const TileLayout& L, int grid
   if constexpr (mfft_kernel(false, PREC, LOG2M) && MODE == OUT_MFCC) {
     if (L.shape == SHAPE_W16P) return launch_tile_t<LOG2M, MODE, PREC, 16, 1, TIn, true>(a, grid, L.lds_bytes, st);
   }
-  if constexpr (LOG2M == 9 || LOG2M == 10) {
+  if constexpr (LOG2M >= 9 && LOG2M <= 11) {
     if (L.shape == SHAPE_W8F2) return launch_tile_t<LOG2M, MODE, PREC, 8, 2, TIn>(a, grid, L.lds_bytes, st);
   }
   if (L.shape != SHAPE_W8) return hipErrorInvalidValue;
@@ -1367,7 +1368,7 @@ int tile_shapes(int n_fft, int* out) {
   int n = 0;
   if (n_fft <= 2048) out[n++] = SHAPE_W16;
   out[n++] = SHAPE_W8;
-  if (n_fft == 1024 || n_fft == 2048) out[n++] = SHAPE_W8F2;
+  if (n_fft >= 1024) out[n++] = SHAPE_W8F2;
   if (n_fft == 2048) out[n++] = SHAPE_W16P;   // its layout exists for the 16-bit MFCC kernels
   return n;
 }
@@ -1378,7 +1379,7 @@ bool tile_layout(int n_fft, int out_kind, int prec, int n_mels, int n_mfcc, Tile
   if (shape == SHAPE_DEFAULT) shape = (M > 1024) ? SHAPE_W8 : (g_tile_waves == 8 ? SHAPE_W8 : SHAPE_W16);
   if (shape != SHAPE_W16 && shape != SHAPE_W8 && shape != SHAPE_W8F2 && shape != SHAPE_W16P) return false;
   if (shape == SHAPE_W16 && M > 1024) return false;   // 16 waves need n_fft <= 2048
-  if (shape == SHAPE_W8F2 && M != 512 && M != 1024) return false;   // register-split FFT sizes
+  if (shape == SHAPE_W8F2 && M != 512 && M != 1024 && M != 2048) return false;   // register-split FFT sizes
   const bool spec = out_kind <= OUT_STFT_COMPLEX;
   if (!spec) {
     if (n_mels < 1 || n_mels > kMaxMelBlocks * kMelBlock) return false;   // larger: generic path

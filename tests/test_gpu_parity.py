@@ -310,7 +310,10 @@ def test_multi_round_mel_blocks(prec, idt):
     accumulator carried between them. A round whose MFMA result is read too
     early loses one chunk: the top bands come out several dB low in whole
     tiles, depending on timing, so the launches are repeated. Every band is
-    checked in dB (16-bit tier, 0.05 dB) and the MFCC at the 16-bit tier."""
+    checked in dB (16-bit tier, 0.05 dB) and the MFCC at the 16-bit tier, for
+    the default workgroup shape and then each 8-wave shape (8 frames per tile
+    and 16, two per wave), all of which exist at n_fft 4096; the two 8-wave
+    shapes give the same bits."""
     import torch
     from metalmom_amd.batch import SpectralBatch
     sr, n = 44100, 3 * 44100
@@ -334,15 +337,58 @@ def test_multi_round_mel_blocks(prec, idt):
     Bf = SpectralBatch([n] * 4, kind="mfcc", sr=sr, n_fft=4096, hop_length=1024, n_mels=256, n_mfcc=20,
                        precision=prec, input_dtype=idt)
     dm, df = Bm.pack(clips), Bf.pack(clips)
-    for rep in range(3):
-        om, of = Bm.empty_output(), Bf.empty_output()
-        Bm.run(dm, om)
-        Bf.run(df, of)
+    last = {}
+    for shape in (None, 2, 3):
+        if shape is not None:
+            for B in (Bm, Bf):
+                B.set_shape(shape)
+                sh = B.launch_shape()
+                assert (sh["shape"], sh["waves_per_wg"], sh["tile_frames"]) == (shape, 8, 8 * (shape - 1)), sh
+        for rep in range(3):
+            om, of = Bm.empty_output(), Bf.empty_output()
+            Bm.run(dm, om)
+            Bf.run(df, of)
+            torch.cuda.synchronize()
+            for i, (m, f) in enumerate(zip(Bm.unpack(om), Bf.unpack(of))):
+                d = np.abs(o.power_to_db(m.astype(np.float64)) - Om[i])
+                assert d.max() <= 0.05, (shape, rep, i, int(np.argmax(d.max(axis=1))), float(d.max()))
+                assert rel_err(f, Of[i]) <= 2e-3, (shape, rep, i)
+            if shape is not None:
+                last[shape] = (Bm.unpack(om), Bf.unpack(of))
+    # the two 8-wave shapes give the same bits
+    for k in (0, 1):
+        for a, b in zip(last[2][k], last[3][k]):
+            assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("kind", ["stft", "mel", "mfcc"])
+def test_n4096_eight_wave_shapes_f32(kind):
+    """n_fft 4096 in FP32: the 8-frame and the 16-frame (two per wave) tiles
+    give the same bits, and both meet the FP32 targets against the oracle."""
+    import torch
+    from metalmom_amd.batch import SpectralBatch
+    sr = 44100
+    clips = _uneven_clips(7, 5, sr=sr)
+    B = SpectralBatch([len(c) for c in clips], kind=kind, sr=sr, n_fft=4096, hop_length=1024, n_mfcc=13)
+    d_in = B.pack(clips)
+    res = {}
+    for shape in (2, 3):
+        B.set_shape(shape)
+        sh = B.launch_shape()
+        assert (sh["shape"], sh["waves_per_wg"], sh["tile_frames"]) == (shape, 8, 8 * (shape - 1)), sh
+        out = B.empty_output()
+        B.run(d_in, out)
         torch.cuda.synchronize()
-        for i, (m, f) in enumerate(zip(Bm.unpack(om), Bf.unpack(of))):
-            d = np.abs(o.power_to_db(m.astype(np.float64)) - Om[i])
-            assert d.max() <= 0.05, (rep, i, int(np.argmax(d.max(axis=1))), float(d.max()))
-            assert rel_err(f, Of[i]) <= 2e-3, (rep, i)
+        res[shape] = B.unpack(out)
+    B.close()
+    for c, a, b in zip(clips, res[2], res[3]):
+        assert np.array_equal(a, b)
+        if kind == "stft":
+            assert rel_err(a, o.stft_magnitude(c, 4096, 1024)) <= 1e-5
+        elif kind == "mel":
+            assert rel_err(a, o.melspectrogram(c, sr, 4096, 1024, n_mels=128)) <= 1e-5
+        else:
+            assert rel_err(a, o.mfcc(c, sr, 13, 4096, 1024, n_mels=128)) <= 1e-4
 
 
 @pytest.mark.parametrize("n", [1, 2, 3, 5, 7, 9, 1003, 40001])
