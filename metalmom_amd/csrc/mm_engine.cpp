@@ -781,16 +781,20 @@ int32_t enqueue_batch(Engine* eng, const SpecRequest& r, const Tables* tb, const
       const bool dfit = dcb > 0 && off + fbc + dcb <= lim;
       a.dct_lds_off = dfit ? (int)(off + fbc) : 0;
       LL.lds_bytes = off + fbc + (dfit ? dcb : 0);
-    } else if (off + dcb <= lim && (nch > 0 || dcb > 0)) {
-      const int nc = (int)std::min<size_t>((size_t)nch, (lim - off - dcb) / kFbChunkBytes);
+    } else if (off < lim && (nch > 0 || dcb > 0)) {
+      // DCT operands too large for the room left (n_fft 4096 with 256 mels):
+      // the filterbank chunks alone, the DCT operands from L2
+      const bool dfit = off + dcb <= lim;
+      const size_t dct_bytes = dfit ? dcb : 0;
+      const int nc = (int)std::min<size_t>((size_t)nch, (lim - off - dct_bytes) / kFbChunkBytes);
       if (nc > 0) {
         a.fb_lds_off = (int)off;
         a.cache_src = tb->t.cache_src + pc * kMaxCacheChunks;
         a.cache_chunks = nc;
       }
       const size_t fbc = (size_t)nc * kFbChunkBytes;
-      a.dct_lds_off = dcb ? (int)(off + fbc) : 0;
-      LL.lds_bytes = off + fbc + dcb;
+      a.dct_lds_off = (dcb && dfit) ? (int)(off + fbc) : 0;
+      LL.lds_bytes = off + fbc + dct_bytes;
     }
   }
   // clear_next: mp.clip_max_key is at -inf already (a plan's alternating
