@@ -42,6 +42,7 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_tile_mfcc_bf16_uneven_batch_tra
 ISTFT_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_istft_tile_2048_long_clip_traffic.json")
 RESAMPLE_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_resample_dec2_44k_to_22k_traffic.json")
 RESAMPLE_PHASE_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_resample_phase_48k_to_22k_traffic.json")
+LARGE_FFT_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_tile_mfcc_f16_large_fft_traffic.json")
 
 
 def measured_traffic(bytes_launch, path=TRAFFIC_FILE):
@@ -275,7 +276,8 @@ def large_fft_leg(torch, steps):
     frames = B.total_frames
     bytes_launch = 2 * B.total_input + 4 * 20 * frames
     rl = kernel_roofline(frames, bytes_launch, 4096, B.n_tiles, mfma_flops_per_tile(44100, 4096, 256, 20, "f16"),
-                         useful_mfma_flops_per_frame(44100, 4096, 256, 20), "f16", k_ms)
+                         useful_mfma_flops_per_frame(44100, 4096, 256, 20), "f16", k_ms,
+                         measured_traffic(bytes_launch, LARGE_FFT_TRAFFIC_FILE))
     sh = B.launch_shape()
     # FP64 error of sampled clips (outside the timed region): the batch's own
     # MFCC blocks, and the mel of one-clip plans on the same device samples
