@@ -1376,7 +1376,16 @@ int tile_shapes(int n_fft, int* out) {
 bool tile_layout(int n_fft, int out_kind, int prec, int n_mels, int n_mfcc, TileLayout* L, int shape) {
   if (n_fft < 256 || n_fft > 4096 || (n_fft & (n_fft - 1))) return false;
   const int M = n_fft / 2, F = M + 1;
-  if (shape == SHAPE_DEFAULT) shape = (M > 1024) ? SHAPE_W8 : (g_tile_waves == 8 ? SHAPE_W8 : SHAPE_W16);
+  if (shape == SHAPE_DEFAULT) {
+    if (M > 1024) {
+      // n_fft 4096: 16-frame tiles of two frames per wave (faster for every
+      // output kind and precision), 8-frame tiles where their LDS does not fit
+      if (tile_layout(n_fft, out_kind, prec, n_mels, n_mfcc, L, SHAPE_W8F2)) return true;
+      shape = SHAPE_W8;
+    } else {
+      shape = (g_tile_waves == 8) ? SHAPE_W8 : SHAPE_W16;
+    }
+  }
   if (shape != SHAPE_W16 && shape != SHAPE_W8 && shape != SHAPE_W8F2 && shape != SHAPE_W16P) return false;
   if (shape == SHAPE_W16 && M > 1024) return false;   // 16 waves need n_fft <= 2048
   if (shape == SHAPE_W8F2 && M != 512 && M != 1024 && M != 2048) return false;   // register-split FFT sizes
