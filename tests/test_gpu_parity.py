@@ -361,6 +361,22 @@ def test_multi_round_mel_blocks(prec, idt):
             assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("n_fft,kind,prec,shape", [
+    (2048, "stft", "f32", 3), (2048, "complex", "f32", 1), (2048, "mel", "f32", 1), (2048, "mel", "bf16", 1),
+    (2048, "mfcc", "f32", 1), (2048, "mfcc", "bf16", 4), (2048, "mfcc", "f16", 4),
+    (4096, "stft", "f32", 3), (4096, "complex", "f32", 2), (4096, "mel", "bf16", 3), (4096, "mfcc", "f32", 3),
+    (4096, "mfcc", "f16", 3)])
+def test_default_shapes(n_fft, kind, prec, shape):
+    """The untuned workgroup shape (single-clip calls, the server, plans
+    never autotuned): the measured-fastest one per configuration, 8-frame
+    tiles at n_fft 4096 where the two-frame shape's LDS does not fit."""
+    from metalmom_amd.batch import SpectralBatch
+    B = SpectralBatch([3 * 44100, 44100], kind=kind, sr=44100, n_fft=n_fft, hop_length=n_fft // 4, n_mels=128,
+                      n_mfcc=13, precision=prec)
+    B.set_shape(0)
+    assert B.launch_shape()["shape"] == shape
+    B.close()
+
 @pytest.mark.parametrize("kind", ["stft", "mel", "mfcc"])
 def test_n4096_eight_wave_shapes_f32(kind):
     """n_fft 4096 in FP32: the 8-frame and the 16-frame (two per wave) tiles
