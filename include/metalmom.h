@@ -369,7 +369,8 @@ int32_t mm_batch_set_grid(mm_batch b, int32_t wg_per_cu);
  * MM_SHAPE_W16P = 16 wavefronts pipelined (staging rows and DCT partials
  * double-buffered, one barrier per tile; BF16 / FP16 MFCC at n_fft 2048).
  * Results do not depend on the shape. MM_SHAPE_DEFAULT: W16 up to n_fft 2048
- * (at 2048: W8F2 for the STFT magnitude / power, W16P for the 16-bit MFCC),
+ * (W8F2 for the STFT magnitude / power at 1024 and 2048, W16P for the 16-bit
+ * MFCC at 2048),
  * W8F2 at 4096 (W8 where its LDS does not fit). */
 #define MM_SHAPE_DEFAULT 0
 #define MM_SHAPE_W16 1

@@ -1385,10 +1385,10 @@ bool tile_layout(int n_fft, int out_kind, int prec, int n_mels, int n_mfcc, Tile
     } else if (g_tile_waves != 0) {   // METALMOM_AMD_TILE_WAVES: 8 or 16 waves, one frame each
       shape = (g_tile_waves == 8) ? SHAPE_W8 : SHAPE_W16;
     } else {
-      // n_fft 2048: the STFT magnitude / power in 16-frame tiles of two frames
-      // per wave, the 16-bit MFCC pipelined (each measured faster than 16
-      // single-frame waves); W16 for everything else
-      if (M == 1024 && out_kind <= OUT_STFT_POWER &&
+      // n_fft 1024 / 2048: the STFT magnitude / power in 16-frame tiles of
+      // two frames per wave, the 16-bit MFCC at 2048 pipelined (each measured
+      // faster than 16 single-frame waves); W16 for everything else
+      if ((M == 512 || M == 1024) && out_kind <= OUT_STFT_POWER &&
           tile_layout(n_fft, out_kind, prec, n_mels, n_mfcc, L, SHAPE_W8F2))
         return true;
       if (M == 1024 && out_kind == OUT_MFCC && prec != PREC_F32 &&

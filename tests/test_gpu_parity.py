@@ -362,7 +362,7 @@ def test_multi_round_mel_blocks(prec, idt):
 
 
 @pytest.mark.parametrize("n_fft,kind,prec,shape", [
-    (2048, "stft", "f32", 3), (2048, "complex", "f32", 1), (2048, "mel", "f32", 1), (2048, "mel", "bf16", 1),
+    (1024, "stft", "f32", 3), (1024, "mel", "f32", 1), (1024, "mfcc", "bf16", 1), (2048, "stft", "f32", 3), (2048, "complex", "f32", 1), (2048, "mel", "f32", 1), (2048, "mel", "bf16", 1),
     (2048, "mfcc", "f32", 1), (2048, "mfcc", "bf16", 4), (2048, "mfcc", "f16", 4),
     (4096, "stft", "f32", 3), (4096, "complex", "f32", 2), (4096, "mel", "bf16", 3), (4096, "mfcc", "f32", 3),
     (4096, "mfcc", "f16", 3)])
